@@ -1,0 +1,690 @@
+// ORACLE — test infrastructure only, never the product path.
+//
+// A from-scratch CPU restatement of GAMER's MUSCL-Hancock (MHM / MHM_RP) patch-group fluid update,
+// non-MHD, no passive scalars, EOS_GAMMA, PLM reconstruction.  It exists to check the HIP solver:
+// only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+//
+// Every floating-point expression keeps the operand order of the reference so the result can be
+// compared bit-for-bit with the reference's own CPU build (oracle/_ref, made by oracle/Makefile):
+//   driver + half step         src/Model_Hydro/CPU_Hydro/CPU_FluidSolver_MHM.cpp:352-589, 615-949
+//   PLM + limiter + Hancock    src/Model_Hydro/CPU_Hydro/CPU_Shared_DataReconstruction.cpp:188-689, 1917-2032, 2063-2160
+//   full-step fluxes, store    src/Model_Hydro/CPU_Hydro/CPU_Shared_ComputeFlux.cpp:112-450
+//   full-step update + check   src/Model_Hydro/CPU_Hydro/CPU_Shared_FullStepUpdate.cpp:63-271
+//   HLLC / HLLE / Roe          src/Model_Hydro/CPU_Hydro/CPU_Shared_RiemannSolver_{HLLC:391-655, HLLE:301-698, Roe:87-711}.cpp
+//   conversions and checks     src/Model_Hydro/CPU_Hydro/CPU_Shared_FluUtility.cpp:93-551, 681-1330
+//   EOS_GAMMA                  src/EoS/Gamma/CPU_EoS_Gamma.cpp:54-192
+//   CFL time step              src/Model_Hydro/CPU_Hydro/CPU_dtSolver_HydroCFL.cpp:57-209
+//
+// Pinning: tests/test_oracle_ref.py (bit-for-bit against oracle/_ref on seeded inputs),
+// tests/test_oracle_golden.py (committed fixtures made from the reference build) and
+// tests/test_oracle_props.py (uniform state, Riemann consistency, upwinding, limiter closed forms,
+// conservation telescoping, axis-permutation symmetry).
+
+#include "hydro_oracle.h"
+#include <cmath>
+#include <cfloat>
+#include <cstring>
+#include <vector>
+#include <omp.h>
+
+namespace {
+
+constexpr int NV  = 5;      // conserved / primitive fields: rho, mx, my, mz, E
+constexpr int PS  = 16;     // patch-group interior width
+constexpr int PSH = 8;      // patch width
+
+template <typename T> struct Lim;
+template <> struct Lim<double> { static double tiny() { return DBL_MIN; } static double huge() { return DBL_MAX; } static double eps() { return DBL_EPSILON; } };
+template <> struct Lim<float>  { static float  tiny() { return FLT_MIN; } static float  huge() { return FLT_MAX; } static float  eps() { return FLT_EPSILON; } };
+
+template <typename T> inline T vmax( T a, T b ) { return std::fmax( a, b ); }
+template <typename T> inline T vmin( T a, T b ) { return std::fmin( a, b ); }
+template <typename T> inline T sgn ( T a )      { return ( a < (T)0 ) ? (T)-1 : (T)+1; }
+
+// EOS_GAMMA auxiliary constants: computed in double, then narrowed (CPU_EoS_Gamma.cpp:54-66)
+template <typename T>
+struct Gas {
+   T gamma, gm1, inv_gm1, inv_gamma;
+   explicit Gas( double g ) : gamma( (T)g ), gm1( (T)(g - 1.0) ), inv_gm1( (T)(1.0/(g - 1.0)) ), inv_gamma( (T)(1.0/g) ) {}
+};
+
+template <typename T>
+struct Opt {
+   Gas<T> gas;
+   T dt, dh, min_dens, min_pres, min_eint;
+   int scheme, rsolver, limiter;
+   Opt( double g ) : gas( g ) {}
+};
+
+// ---------------- cell conversions ----------------
+// cyclic permutation of the momentum so that component 1 is normal to the interface (FluUtility.cpp:93-176)
+template <typename T> inline void rotate( T u[], int d, bool fwd )
+{
+   if ( d == 0 ) return;
+   const T a = u[1], b = u[2], c = u[3];
+   if ( ( d == 1 ) == fwd ) { u[1] = b; u[2] = c; u[3] = a; }   // (x,y,z) -> (y,z,x)
+   else                     { u[1] = c; u[2] = a; u[3] = b; }   // (x,y,z) -> (z,x,y)
+}
+
+template <typename T> inline T floor_pres( T p, T pmin ) { return ( p == p ) ? vmax( p, pmin ) : p; }
+
+template <typename T> inline T eint_of( T d, T mx, T my, T mz, T e )
+{  return e - (T)0.5*( mx*mx + my*my + mz*mz ) / d; }
+
+template <typename T> inline T pres_of( const Gas<T> &g, T d, T mx, T my, T mz, T e, bool floor, T pmin )
+{
+   T p = eint_of( d, mx, my, mz, e ) * g.gm1;
+   return floor ? floor_pres( p, pmin ) : p;
+}
+
+template <typename T> inline T etot_of( T d, T mx, T my, T mz, T eint )
+{
+   T e = (T)0.5*( mx*mx + my*my + mz*mz ) / d;
+   return e + eint;
+}
+
+// leave E untouched unless the internal-energy floor actually changes Eint (FluUtility.cpp:787-805)
+template <typename T> inline T engy_with_eint_floor( T d, T mx, T my, T mz, T e, T emin )
+{
+   const T ein  = eint_of( d, mx, my, mz, e );
+   const T eout = ( ein == ein ) ? vmax( ein, emin ) : ein;
+   return ( ein == eout ) ? e : e - ein + eout;
+}
+
+template <typename T> inline void con2pri( const Gas<T> &g, const T c[NV], T w[NV], T pmin )
+{
+   const T r = (T)1.0 / c[0];
+   w[0] = c[0];
+   w[1] = c[1]*r;  w[2] = c[2]*r;  w[3] = c[3]*r;
+   w[4] = pres_of( g, c[0], c[1], c[2], c[3], c[4], true, pmin );
+}
+
+template <typename T> inline void pri2con( const Gas<T> &g, const T w[NV], T c[NV] )
+{
+   c[0] = w[0];
+   c[1] = w[0]*w[1];  c[2] = w[0]*w[2];  c[3] = w[0]*w[3];
+   c[4] = etot_of( c[0], c[1], c[2], c[3], w[4]*g.inv_gm1 );
+}
+
+// physical flux along x of a state already rotated to x (FluUtility.cpp:458-551)
+template <typename T> inline void xflux( const T u[NV], T p, T f[NV] )
+{
+   const T r  = (T)1.0 / u[0];
+   const T vx = r*u[1];
+   f[0] = u[1];
+   f[1] = vx*u[1] + p;
+   f[2] = vx*u[2];
+   f[3] = vx*u[3];
+   f[4] = vx*( u[4] + p );
+}
+
+// flux along direction d of an unrotated state; pressure recovered from the rotated state
+template <typename T> inline void dir_flux( const Gas<T> &g, int d, const T in[NV], T pmin, T f[NV] )
+{
+   T u[NV];
+   for (int v=0; v<NV; v++) u[v] = in[v];
+   rotate( u, d, true );
+   const T p = pres_of( g, u[0], u[1], u[2], u[3], u[4], true, pmin );
+   xflux( u, p, f );
+   rotate( f, d, false );
+}
+
+// ---------------- Riemann solvers (hydro, Davis wave speeds) ----------------
+template <typename T> void rs_hlle_x( const Gas<T> &g, const T L[NV], const T R[NV], T pmin, T out[NV] )
+{
+   const T Z = (T)0.0, ONE = (T)1.0;
+   const T rl = ONE / L[0], rr = ONE / R[0];
+   const T ul = rl*L[1], ur = rr*R[1];
+   const T pl = pres_of( g, L[0], L[1], L[2], L[3], L[4], true, pmin );
+   const T pr = pres_of( g, R[0], R[1], R[2], R[3], R[4], true, pmin );
+   const T cl = std::sqrt( g.gamma*pl / L[0] );
+   const T cr = std::sqrt( g.gamma*pr / R[0] );
+   T sl = vmin( ul - cl, ur - cr );
+   T sr = vmax( ul + cl, ur + cr );
+   sl = vmin( sl, Z );
+   sr = vmax( sr, Z );
+   T fl[NV], fr[NV];
+   xflux( L, pl, fl );
+   xflux( R, pr, fr );
+   for (int v=0; v<NV; v++) { fl[v] -= sl*L[v];  fr[v] -= sr*R[v]; }
+   if ( sl == Z && sr == Z )
+      for (int v=0; v<NV; v++) out[v] = fl[v];
+   else
+   {
+      const T inv = ONE / ( sr - sl );
+      for (int v=0; v<NV; v++) out[v] = inv*( sr*fl[v] - sl*fr[v] );
+   }
+}
+
+template <typename T> void rs_hllc_x( const Gas<T> &g, const T L[NV], const T R[NV], T pmin, T out[NV] )
+{
+   const T Z = (T)0.0, ONE = (T)1.0;
+   const T rl = ONE / L[0], rr = ONE / R[0];
+   const T ul = rl*L[1], ur = rr*R[1];
+   const T pl = pres_of( g, L[0], L[1], L[2], L[3], L[4], true, pmin );
+   const T pr = pres_of( g, R[0], R[1], R[2], R[3], R[4], true, pmin );
+   const T cl = std::sqrt( g.gamma*pl / L[0] );
+   const T cr = std::sqrt( g.gamma*pr / R[0] );
+   const T wl1 = ul - cl, wl2 = ur - cr, wr1 = ul + cl, wr2 = ur + cr;
+   const T wl = vmin( wl1, wl2 ), wr = vmax( wr1, wr2 );
+   const T tl = +L[0]*( ( wl1 < wl2 ) ? cl : ( ul - ur ) + cr );
+   const T tr = -R[0]*( ( wr2 > wr1 ) ? cr : ( ul - ur ) + cl );
+   const T t2 = ONE / ( tl - tr );
+   const T vs = t2*( pl - pr + tl*ul - tr*ur );
+   T ps = t2*( tl*( pr + tr*ur ) - tr*( pl + tl*ul ) );
+   ps = floor_pres( ps, pmin );
+
+   T f[NV], cf, cs;
+   if ( vs >= Z )
+   {
+      const T s = vmin( wl, Z );
+      xflux( L, pl, f );
+      for (int v=0; v<NV; v++) f[v] -= s*L[v];
+      if ( vs == Z && s == Z ) { cf = ONE; cs = Z; }
+      else { const T t4 = ONE / ( vs - s );  cf = t4*vs;  cs = -t4*s*ps; }
+   }
+   else
+   {
+      const T s = vmax( wr, Z );
+      xflux( R, pr, f );
+      for (int v=0; v<NV; v++) f[v] -= s*R[v];
+      const T t4 = ONE / ( vs - s );
+      cf = t4*vs;  cs = -t4*s*ps;
+   }
+   for (int v=0; v<NV; v++) out[v] = cf*f[v];
+   out[1] += cs;
+   out[4] += cs*vs;
+}
+
+template <typename T> void rs_roe_x( const Gas<T> &g, const T L[NV], const T R[NV], T pmin, T out[NV] )
+{
+   const T Z = (T)0.0, ONE = (T)1.0, HALF = (T)0.5;
+   const T rl = ONE / L[0], rr = ONE / R[0];
+   const T pl = pres_of( g, L[0], L[1], L[2], L[3], L[4], true, pmin );
+   const T pr = pres_of( g, R[0], R[1], R[2], R[3], R[4], true, pmin );
+   const T hl = rl*( L[4] + pl ), hr = rr*( R[4] + pr );
+   const T sql = std::sqrt( L[0] ), sqr = std::sqrt( R[0] );
+   const T rho = sql*sqr, irho = ONE/rho;
+   const T isql = ONE/sql, isqr = ONE/sqr, isum = ONE/( sql + sqr );
+   const T u = isum*( isql*L[1] + isqr*R[1] );
+   const T v = isum*( isql*L[2] + isqr*R[2] );
+   const T w = isum*( isql*L[3] + isqr*R[3] );
+   const T V2 = u*u + v*v + w*w;
+   const T H  = isum*( sql*hl + sqr*hr );
+   T gpr = g.gm1*( H - HALF*V2 );
+   gpr = g.gamma*irho*floor_pres( gpr*rho/g.gamma, pmin );
+   const T a2 = gpr, a = std::sqrt( a2 );
+   const T ev[NV] = { u - a, u, u, u, u + a };
+
+   T fl[NV], fr[NV];
+   xflux( L, pl, fl );
+   xflux( R, pr, fr );
+   if ( ev[0]    >= Z ) { for (int k=0; k<NV; k++) out[k] = fl[k];  return; }
+   if ( ev[NV-1] <= Z ) { for (int k=0; k<NV; k++) out[k] = fr[k];  return; }
+
+   const T rev[NV][NV] = { { ONE,     ONE,       Z,   Z,     ONE   },
+                           { u - a,   u,         Z,   Z,     u + a },
+                           { v,       v,         ONE, Z,     v     },
+                           { w,       w,         Z,   ONE,   w     },
+                           { H - u*a, HALF*V2,   v,   w,     H + u*a } };
+   T jump[NV], amp[NV];
+   for (int k=0; k<NV; k++) jump[k] = R[k] - L[k];
+   amp[2] = jump[2] - v*jump[0];
+   amp[3] = jump[3] - w*jump[0];
+   amp[1] = g.gm1/a2*( jump[0]*( H - u*u ) + u*jump[1] - jump[4] + v*amp[2] + w*amp[3] );
+   amp[0] = HALF/a*( jump[0]*( u + a ) - jump[1] - a*amp[1] );
+   amp[4] = jump[0] - amp[0] - amp[1];
+
+   // intermediate-state positivity check, falling back to HLLE (Roe.cpp:607-667; CUFLU.h CHECK_INTERMEDIATE)
+   T is[NV];
+   for (int k=0; k<NV; k++) is[k] = L[k];
+   for (int t=0; t<NV-1; t++)
+   {
+      for (int k=0; k<NV; k++) is[k] += amp[t]*rev[k][t];
+      if ( ev[t+1] > ev[t] )
+      {
+         const T ip = pres_of( g, is[0], is[1], is[2], is[3], is[4], false, (T)0 );
+         if ( is[0] <= Z || ip <= Z ) { rs_hlle_x( g, L, R, pmin, out );  return; }
+      }
+   }
+
+   for (int k=0; k<NV; k++) amp[k] *= std::fabs( ev[k] );
+   for (int k=0; k<NV; k++)
+   {
+      out[k] = fl[k] + fr[k];
+      for (int t=0; t<NV; t++) out[k] -= amp[t]*rev[k][t];
+      out[k] *= HALF;
+   }
+}
+
+// Riemann solve across a d-normal interface; HLLC falls back to HLLE on a NaN flux component
+// (RSOLVER_RESCUE, CPU_FluidSolver_MHM.cpp:737-786, CPU_Shared_ComputeFlux.cpp:313-362)
+template <typename T> void riemann( const Gas<T> &g, int rsolver, int d, const T Lin[NV], const T Rin[NV], T pmin, T out[NV] )
+{
+   T L[NV], R[NV];
+   for (int v=0; v<NV; v++) { L[v] = Lin[v];  R[v] = Rin[v]; }
+   rotate( L, d, true );
+   rotate( R, d, true );
+   switch ( rsolver )
+   {
+      case ORC_HLLC: rs_hllc_x( g, L, R, pmin, out ); break;
+      case ORC_HLLE: rs_hlle_x( g, L, R, pmin, out ); break;
+      default:       rs_roe_x ( g, L, R, pmin, out ); break;
+   }
+   rotate( out, d, false );
+
+   if ( rsolver == ORC_HLLC )
+      for (int v=0; v<NV; v++)
+         if ( out[v] != out[v] )
+         {
+            for (int k=0; k<NV; k++) { L[k] = Lin[k];  R[k] = Rin[k]; }
+            rotate( L, d, true );
+            rotate( R, d, true );
+            rs_hlle_x( g, L, R, pmin, out );
+            rotate( out, d, false );
+            break;
+         }
+}
+
+// ---------------- PLM slope limiter (DataReconstruction.cpp:1917-2032) ----------------
+template <typename T> void limit_slope( const T L[NV], const T C[NV], const T R[NV], int lim, T coeff, T s[NV] )
+{
+   T sl[NV], sr[NV], sc[NV], sa[NV];
+   for (int v=0; v<NV; v++)
+   {
+      sl[v] = C[v] - L[v];
+      sr[v] = R[v] - C[v];
+      sc[v] = (T)0.5*( sl[v] + sr[v] );
+   }
+   if ( lim == 4 )
+      for (int v=0; v<NV; v++)
+         sa[v] = ( sl[v]*sr[v] > (T)0.0 ) ? (T)2.0*sl[v]*sr[v]/( sl[v] + sr[v] ) : (T)0.0;
+
+   for (int v=0; v<NV; v++)
+   {
+      const T lr = sl[v]*sr[v];
+      if ( !( lr > (T)0.0 ) ) { s[v] = (T)0.0;  continue; }
+      switch ( lim )
+      {
+         case 6: s[v] = sc[v];  break;                                                 // central
+         case 1: s[v] = (T)2.0*lr/( sl[v] + sr[v] );  break;                           // van Leer
+         case 2: {                                                                     // generalised MinMod
+            const T a = sl[v]*coeff, b = sr[v]*coeff;
+            T m = vmin( std::fabs(a), std::fabs(b) );
+            m = vmin( std::fabs(sc[v]), m );
+            s[v] = m*sgn( sc[v] );
+         } break;
+         case 3: s[v] = lr*( sl[v] + sr[v] ) / ( sl[v]*sl[v] + sr[v]*sr[v] );  break;  // van Albada
+         default: {                                                                    // van Leer + generalised MinMod
+            const T a = sl[v]*coeff, b = sr[v]*coeff;
+            T m = vmin( std::fabs(a), std::fabs(b) );
+            m = vmin( std::fabs(sc[v]), m );
+            m = vmin( std::fabs(sa[v]), m );
+            s[v] = m*sgn( sc[v] );
+         } break;
+      }
+   }
+}
+
+// ---------------- one patch group ----------------
+template <typename T>
+struct Scratch {
+   std::vector<T> pri;     // [5][N^3]  primitive (MHM) / half-step primitive (MHM_RP, stride 20)
+   std::vector<T> hflux;   // [3][5][21^3] half-step fluxes (MHM_RP)
+   std::vector<T> fc;      // [6][5][18^3] face-centred conserved states
+   std::vector<T> flux;    // [3][5][17^3] full-step fluxes
+   Scratch() : pri( NV*22*22*22 ), hflux( 3*NV*21*21*21 ), fc( 6*NV*18*18*18 ), flux( 3*NV*17*17*17 ) {}
+};
+
+inline int idx3( int i, int j, int k, int ni, int nj ) { return ( k*nj + j )*ni + i; }
+
+// MHM_RP phase 1a: first-order Riemann fluxes between input cells (CPU_FluidSolver_MHM.cpp:615-798)
+template <typename T> void half_fluxes( const Opt<T> &o, const T *u, T *hf )
+{
+   const int NX = 22, NF = 21, N3F = NF*NF*NF, N3X = NX*NX*NX;
+   const int dcv[3] = { 1, NX, NX*NX };
+   for (int d=0; d<3; d++)
+   {
+      const int s[3]  = { d == 0 ? 0 : 1, d == 1 ? 0 : 1, d == 2 ? 0 : 1 };
+      const int n[3]  = { d == 0 ? NF : NF-1, d == 1 ? NF : NF-1, d == 2 ? NF : NF-1 };
+      for (int k=0; k<n[2]; k++)
+      for (int j=0; j<n[1]; j++)
+      for (int i=0; i<n[0]; i++)
+      {
+         const int ic = idx3( i+s[0], j+s[1], k+s[2], NX, NX );
+         T L[NV], R[NV], F[NV];
+         for (int v=0; v<NV; v++) { L[v] = u[v*N3X + ic];  R[v] = u[v*N3X + ic + dcv[d]]; }
+         riemann( o.gas, o.rsolver, d, L, R, o.min_pres, F );
+         const int jf = idx3( i, j, k, NF, NF );
+         for (int v=0; v<NV; v++) hf[ ( d*NV + v )*N3F + jf ] = F[v];
+      }
+   }
+}
+
+// MHM_RP phase 1b: half-step update, floors, conversion to primitive (CPU_FluidSolver_MHM.cpp:830-949)
+template <typename T> void half_update( const Opt<T> &o, const T *u, const T *hf, T *pri )
+{
+   const int NX = 22, NF = 21, NH = 20, N3F = NF*NF*NF, N3X = NX*NX*NX, N3H = NH*NH*NH;
+   const int dfl[3] = { 1, NF, NF*NF };
+   const T dt_dh2 = (T)0.5*o.dt/o.dh;
+   for (int k=0; k<NH; k++)
+   for (int j=0; j<NH; j++)
+   for (int i=0; i<NH; i++)
+   {
+      const int jf = idx3( i, j, k, NF, NF );
+      const int ic = idx3( i+1, j+1, k+1, NX, NX );
+      T c[NV], w[NV], df[3][NV];
+      for (int d=0; d<3; d++)
+      for (int v=0; v<NV; v++)
+         df[d][v] = hf[ ( d*NV + v )*N3F + jf + dfl[d] ] - hf[ ( d*NV + v )*N3F + jf ];
+      for (int v=0; v<NV; v++)
+         c[v] = u[v*N3X + ic] - dt_dh2*( df[0][v] + df[1][v] + df[2][v] );
+      c[0] = vmax( c[0], o.min_dens );
+      c[4] = engy_with_eint_floor( c[0], c[1], c[2], c[3], c[4], o.min_eint );
+      con2pri( o.gas, c, w, o.min_pres );
+      const int jo = idx3( i, j, k, NH, NH );
+      for (int v=0; v<NV; v++) pri[v*N3H + jo] = w[v];
+   }
+}
+
+// MHM only: Hancock half-step advance of the six face states of one cell (DataReconstruction.cpp:2063-2160)
+template <typename T> void hancock( const Opt<T> &o, T fcc[6][NV], const T fcp[6][NV], const T *u, int ic, int N3X )
+{
+   const T dt_dh2 = (T)0.5*o.dt/o.dh;
+   T F[6][NV];
+   for (int f=0; f<6; f++) dir_flux( o.gas, f/2, fcc[f], o.min_pres, F[f] );
+   for (int v=0; v<NV; v++)
+   {
+      const T dF = dt_dh2*( F[1][v] - F[0][v] + F[3][v] - F[2][v] + F[5][v] - F[4][v] );
+      for (int f=0; f<6; f++) fcc[f][v] -= dF;
+   }
+   const T tiny = Lim<T>::tiny(), huge = Lim<T>::huge();
+   auto bad = [&]( T x, T lo ) { return ( x < lo || x > huge || x != x ); };
+   for (int f=0; f<6; f++)
+   {
+      if ( bad( fcc[f][0], tiny ) || bad( fcc[f][4], tiny ) || bad( fcp[f][4], (T)0.0 ) )
+      {
+         for (int g=0; g<6; g++)
+         for (int v=0; v<NV; v++) fcc[g][v] = u[v*N3X + ic];
+         break;
+      }
+      fcc[f][0] = vmax( fcc[f][0], o.min_dens );
+      fcc[f][4] = engy_with_eint_floor( fcc[f][0], fcc[f][1], fcc[f][2], fcc[f][3], fcc[f][4], o.min_eint );
+   }
+}
+
+// PLM reconstruction of the 18^3 face-centred states (DataReconstruction.cpp:188-689)
+template <typename T> void reconstruct( const Opt<T> &o, const T *pri, int NIn, const T *u_mhm, T coeff, T *fc )
+{
+   const int NFC = 18, N3FC = NFC*NFC*NFC, N3In = NIn*NIn*NIn;
+   const int dcc[3] = { 1, NIn, NIn*NIn };
+   for (int k=0; k<NFC; k++)
+   for (int j=0; j<NFC; j++)
+   for (int i=0; i<NFC; i++)
+   {
+      const int ic = idx3( i+1, j+1, k+1, NIn, NIn );
+      T C[NV], fcp[6][NV], fcc[6][NV];
+      for (int v=0; v<NV; v++) C[v] = pri[v*N3In + ic];
+      for (int d=0; d<3; d++)
+      {
+         T Lc[NV], Rc[NV], s[NV];
+         for (int v=0; v<NV; v++) { Lc[v] = pri[v*N3In + ic - dcc[d]];  Rc[v] = pri[v*N3In + ic + dcc[d]]; }
+         limit_slope( Lc, C, Rc, o.limiter, coeff, s );
+         T *fl = fcp[2*d], *fr = fcp[2*d+1];
+         for (int v=0; v<NV; v++)
+         {
+            fl[v] = C[v] - (T)0.5*s[v];
+            fr[v] = C[v] + (T)0.5*s[v];
+         }
+         for (int v=0; v<NV; v++)
+         {
+            T lo = ( C[v] < Lc[v] ) ? C[v] : Lc[v];
+            T hi = ( C[v] > Lc[v] ) ? C[v] : Lc[v];
+            fl[v] = ( fl[v] > lo ) ? fl[v] : lo;
+            fl[v] = ( fl[v] < hi ) ? fl[v] : hi;
+            fr[v] = (T)2.0*C[v] - fl[v];
+            lo = ( C[v] < Rc[v] ) ? C[v] : Rc[v];
+            hi = ( C[v] > Rc[v] ) ? C[v] : Rc[v];
+            fr[v] = ( fr[v] > lo ) ? fr[v] : lo;
+            fr[v] = ( fr[v] < hi ) ? fr[v] : hi;
+            fl[v] = (T)2.0*C[v] - fr[v];
+         }
+         pri2con( o.gas, fl, fcc[2*d] );
+         pri2con( o.gas, fr, fcc[2*d+1] );
+      }
+      if ( o.scheme == ORC_MHM ) hancock( o, fcc, fcp, u_mhm, ic, N3In );
+      const int jf = idx3( i, j, k, NFC, NFC );
+      for (int f=0; f<6; f++)
+      for (int v=0; v<NV; v++) fc[ ( f*NV + v )*N3FC + jf ] = fcc[f][v];
+   }
+}
+
+// full-step fluxes between reconstructed faces, NSkip_N=0, NSkip_T=1 (ComputeFlux.cpp:112-376)
+template <typename T> void full_fluxes( const Opt<T> &o, const T *fc, T *fl )
+{
+   const int NFC = 18, NFL = 17, N3FC = NFC*NFC*NFC, N3FL = NFL*NFL*NFL;
+   const int dfc[3] = { 1, NFC, NFC*NFC };
+   for (int d=0; d<3; d++)
+   {
+      const int s[3] = { d == 0 ? 0 : 1, d == 1 ? 0 : 1, d == 2 ? 0 : 1 };
+      const int n[3] = { d == 0 ? 17 : 16, d == 1 ? 17 : 16, d == 2 ? 17 : 16 };
+      for (int k=0; k<n[2]; k++)
+      for (int j=0; j<n[1]; j++)
+      for (int i=0; i<n[0]; i++)
+      {
+         const int jc = idx3( i+s[0], j+s[1], k+s[2], NFC, NFC );
+         T L[NV], R[NV], F[NV];
+         for (int v=0; v<NV; v++)
+         {
+            L[v] = fc[ ( (2*d+1)*NV + v )*N3FC + jc ];
+            R[v] = fc[ ( (2*d  )*NV + v )*N3FC + jc + dfc[d] ];
+         }
+         riemann( o.gas, o.rsolver, d, L, R, o.min_pres, F );
+         const int jf = idx3( i, j, k, NFL, NFL );
+         for (int v=0; v<NV; v++) fl[ ( d*NV + v )*N3FL + jf ] = F[v];
+      }
+   }
+}
+
+// coarse-fine interface fluxes: x/y/z planes at 0, 8, 16 (ComputeFlux.cpp:390-450)
+template <typename T> void store_int_flux( const T *fl, T *out )
+{
+   const int NFL = 17, N3FL = NFL*NFL*NFL;
+   for (int t=0; t<PS*PS; t++)
+   {
+      const int a = t % PS, b = t / PS;
+      for (int v=0; v<NV; v++)
+      for (int p=0; p<3; p++)
+      {
+         const int pos = p*PSH;
+         out[ ( (0+p)*NV + v )*PS*PS + t ] = fl[ ( 0*NV + v )*N3FL + idx3( pos, a, b, NFL, NFL ) ];
+         out[ ( (3+p)*NV + v )*PS*PS + t ] = fl[ ( 1*NV + v )*N3FL + idx3( a, pos, b, NFL, NFL ) ];
+         out[ ( (6+p)*NV + v )*PS*PS + t ] = fl[ ( 2*NV + v )*N3FL + idx3( a, b, pos, NFL, NFL ) ];
+      }
+   }
+}
+
+// CONS-mode unphysical check, EOS_GAMMA (FluUtility.cpp:843-940)
+template <typename T> bool unphysical( const T c[NV] )
+{
+   const T huge = Lim<T>::huge();
+   bool bad = false;
+   for (int v=0; v<NV; v++)
+   {
+      if ( c[v] != c[v] ) bad = true;
+      if ( v >= 1 && v <= 3 ) { if ( c[v] < -huge || c[v] > huge ) bad = true; }
+      else                    { if ( c[v] <= (T)0.0 || c[v] > huge ) bad = true; }
+   }
+   const T e = eint_of( c[0], c[1], c[2], c[3], c[4] );
+   if ( e < (T)-3.0*c[4]*Lim<T>::eps() || e > huge || e != e ) bad = true;
+   return bad;
+}
+
+// full-step update of the 16^3 interior; returns true if any cell is unphysical (FullStepUpdate.cpp:63-271)
+template <typename T> bool full_update( const Opt<T> &o, const T *u, int NX, int ghost, const T *fl, T *out )
+{
+   const int NFL = 17, N3FL = NFL*NFL*NFL, N3X = NX*NX*NX;
+   const int dfl[3] = { 1, NFL, NFL*NFL };
+   const T dt_dh = o.dt/o.dh;
+   bool fail = false;
+   for (int k=0; k<PS; k++)
+   for (int j=0; j<PS; j++)
+   for (int i=0; i<PS; i++)
+   {
+      const int jf = idx3( i, j, k, NFL, NFL );
+      const int ic = idx3( i+ghost, j+ghost, k+ghost, NX, NX );
+      T df[3][NV], c[NV];
+      for (int d=0; d<3; d++)
+      for (int v=0; v<NV; v++)
+         df[d][v] = fl[ ( d*NV + v )*N3FL + jf + dfl[d] ] - fl[ ( d*NV + v )*N3FL + jf ];
+      for (int v=0; v<NV; v++)
+         c[v] = u[v*N3X + ic] - dt_dh*( df[0][v] + df[1][v] + df[2][v] );
+      const int jo = idx3( i, j, k, PS, PS );
+      for (int v=0; v<NV; v++) out[v*PS*PS*PS + jo] = c[v];
+      if ( unphysical( c ) ) fail = true;
+   }
+   return fail;
+}
+
+// the per-PG driver, including the MinMod-coefficient retry loop (CPU_FluidSolver_MHM.cpp:365-587)
+template <typename T> void solve_pg( const Opt<T> &o, const T *u, T *out, T *iflux, bool store_flux,
+                                     T minmod_coeff, int maxiter, Scratch<T> &s )
+{
+   const bool rp = ( o.scheme == ORC_MHM_RP );
+   const int  NX = rp ? 22 : 20;
+   const int  ghost = rp ? 3 : 2;
+
+   if ( rp )
+   {
+      half_fluxes( o, u, s.hflux.data() );
+      half_update( o, u, s.hflux.data(), s.pri.data() );
+   }
+   else
+   {
+      const int N3X = NX*NX*NX;
+      for (int c=0; c<N3X; c++)
+      {
+         T cc[NV], w[NV];
+         for (int v=0; v<NV; v++) cc[v] = u[v*N3X + c];
+         con2pri( o.gas, cc, w, o.min_pres );
+         for (int v=0; v<NV; v++) s.pri[v*N3X + c] = w[v];
+      }
+   }
+
+   int it = 0;
+   bool fail;
+   do {
+      T coeff = ( maxiter == 0 ) ? minmod_coeff : minmod_coeff - (T)it*minmod_coeff/(T)maxiter;
+      coeff = vmax( coeff, (T)0.0 );
+      reconstruct( o, s.pri.data(), rp ? 20 : 20, u, coeff, s.fc.data() );
+      full_fluxes( o, s.fc.data(), s.flux.data() );
+      if ( store_flux ) store_int_flux( s.flux.data(), iflux );
+      fail = full_update( o, u, NX, ghost, s.flux.data(), out );
+      it++;
+   } while ( fail && it <= maxiter );
+}
+
+template <typename T>
+int run( const T *in, T *out, T *flux, int npg, double dt, double dh, double gamma, int scheme, int rsolver,
+         int store_flux, int limiter, double minmod_coeff, int maxiter, double min_dens, double min_pres,
+         double min_eint, int nthreads )
+{
+   if ( npg < 0 || ( npg > 0 && ( in == nullptr || out == nullptr ) ) ) return 1;
+   if ( store_flux && npg > 0 && flux == nullptr ) return 1;
+   if ( scheme != ORC_MHM && scheme != ORC_MHM_RP ) return 2;
+   if ( rsolver != ORC_HLLC && rsolver != ORC_HLLE && rsolver != ORC_ROE ) return 3;
+   if ( limiter != 1 && limiter != 2 && limiter != 3 && limiter != 4 && limiter != 6 ) return 4;
+   if ( maxiter < 0 ) return 5;
+
+   Opt<T> o( gamma );
+   o.dt = (T)dt;  o.dh = (T)dh;
+   o.min_dens = (T)min_dens;  o.min_pres = (T)min_pres;  o.min_eint = (T)min_eint;
+   o.scheme = scheme;  o.rsolver = rsolver;  o.limiter = limiter;
+
+   const int NX = ( scheme == ORC_MHM_RP ) ? 22 : 20;
+   const long in_sz = (long)NV*NX*NX*NX, out_sz = (long)NV*PS*PS*PS, fl_sz = 9L*NV*PS*PS;
+   if ( nthreads > 0 ) omp_set_num_threads( nthreads );
+
+#  pragma omp parallel
+   {
+      Scratch<T> s;
+#     pragma omp for schedule( dynamic, 1 )
+      for (int p=0; p<npg; p++)
+         solve_pg( o, in + p*in_sz, out + p*out_sz, store_flux ? flux + p*fl_sz : nullptr,
+                   store_flux != 0, (T)minmod_coeff, maxiter, s );
+   }
+   return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int oracle_fluid_solver_f64( const double *in, double *out, double *flux, int npg, double dt, double dh,
+                             double gamma, int scheme, int rsolver, int store_flux, int limiter,
+                             double minmod_coeff, int minmod_maxiter, double min_dens, double min_pres,
+                             double min_eint, int nthreads )
+{
+   return run<double>( in, out, flux, npg, dt, dh, gamma, scheme, rsolver, store_flux, limiter, minmod_coeff,
+                       minmod_maxiter, min_dens, min_pres, min_eint, nthreads );
+}
+
+int oracle_fluid_solver_f32( const float *in, float *out, float *flux, int npg, double dt, double dh,
+                             double gamma, int scheme, int rsolver, int store_flux, int limiter,
+                             double minmod_coeff, int minmod_maxiter, double min_dens, double min_pres,
+                             double min_eint, int nthreads )
+{
+   return run<float>( in, out, flux, npg, dt, dh, gamma, scheme, rsolver, store_flux, limiter, minmod_coeff,
+                      minmod_maxiter, min_dens, min_pres, min_eint, nthreads );
+}
+
+int oracle_riemann_f64( int rsolver, int d, const double L[5], const double R[5], double gamma,
+                        double min_pres, double flux[5] )
+{
+   if ( d < 0 || d > 2 ) return 1;
+   if ( rsolver != ORC_HLLC && rsolver != ORC_HLLE && rsolver != ORC_ROE ) return 3;
+   Gas<double> g( gamma );
+   riemann( g, rsolver, d, L, R, min_pres, flux );
+   return 0;
+}
+
+int oracle_limit_slope_f64( const double L[5], const double C[5], const double R[5], int limiter,
+                            double minmod_coeff, double slope[5] )
+{
+   if ( limiter != 1 && limiter != 2 && limiter != 3 && limiter != 4 && limiter != 6 ) return 4;
+   limit_slope( L, C, R, limiter, minmod_coeff, slope );
+   return 0;
+}
+
+// per-patch CFL: dt = Safety*dh / max_cells max_d( |v_d| + c_s ) (CPU_dtSolver_HydroCFL.cpp:57-209)
+int oracle_dt_cfl_f64( const double *in, double *dt, int npatch, double dh, double safety, double gamma,
+                       double min_pres, int nthreads )
+{
+   if ( npatch < 0 || ( npatch > 0 && ( in == nullptr || dt == nullptr ) ) ) return 1;
+   const Gas<double> g( gamma );
+   const int N3 = PSH*PSH*PSH;
+   const double dhs = safety*dh;
+   if ( nthreads > 0 ) omp_set_num_threads( nthreads );
+#  pragma omp parallel for schedule( dynamic, 1 )
+   for (int p=0; p<npatch; p++)
+   {
+      const double *u = in + (long)p*NV*N3;
+      double mx = 0.0;
+      for (int t=0; t<N3; t++)
+      {
+         const double r  = 1.0 / u[t];
+         const double vx = std::fabs( u[1*N3+t] )*r, vy = std::fabs( u[2*N3+t] )*r, vz = std::fabs( u[3*N3+t] )*r;
+         const double p_ = pres_of( g, u[t], u[1*N3+t], u[2*N3+t], u[3*N3+t], u[4*N3+t], true, min_pres );
+         double cx = std::sqrt( g.gamma*p_ / u[t] );
+         const double cy = cx + vy, cz = cx + vz;
+         cx += vx;
+         mx = vmax( cx, mx );
+         mx = vmax( cy, mx );
+         mx = vmax( cz, mx );
+      }
+      dt[p] = dhs/mx;
+   }
+   return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,138 @@
+"""ORACLE bindings — test infrastructure only.
+
+ctypes access to
+  * ``liboracle.so``: the from-scratch CPU restatement of GAMER's MHM/MHM_RP solver (hydro_oracle.cpp), and
+  * ``_ref/libgamer_ref_<scheme>_<rsolver>_<prec>.so``: GAMER's own CPU_FluidSolver_MHM, compiled from the
+    reference sources by oracle/Makefile (absent where the reference tree was never available).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module, and only as
+the checker or the CPU baseline — never as the thing measured or shipped.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SCHEME = {"MHM": 3, "MHM_RP": 4}
+RSOLVER = {"ROE": 2, "HLLE": 3, "HLLC": 4}
+LIMITER = {"VANLEER": 1, "GMINMOD": 2, "ALBADA": 3, "VL_GMINMOD": 4, "CENTRAL": 6}
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_SOLVER_ARGS = [_P, _P, _P, _I, _D, _D, _D, _I, _I, _I, _I, _D, _I, _D, _D, _D, _I]
+_REF_ARGS = [_P, _P, _P, _I, _D, _D, _D, _I, _I, _D, _I, _D, _D, _D, _I]
+
+_lib = None
+
+
+def lib():
+    """The restatement library (built by ``make -C oracle oracle``)."""
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle oracle`")
+        L = ctypes.CDLL(path)
+        for name in ("oracle_fluid_solver_f64", "oracle_fluid_solver_f32"):
+            getattr(L, name).argtypes = _SOLVER_ARGS
+            getattr(L, name).restype = _I
+        L.oracle_riemann_f64.argtypes = [_I, _I, _P, _P, _D, _D, _P]
+        L.oracle_limit_slope_f64.argtypes = [_P, _P, _P, _I, _D, _P]
+        L.oracle_dt_cfl_f64.argtypes = [_P, _P, _I, _D, _D, _D, _D, _I]
+        _lib = L
+    return _lib
+
+
+def ref_path(scheme: str, rsolver: str, prec: str) -> str:
+    return os.path.join(HERE, "_ref", f"libgamer_ref_{scheme.lower()}_{rsolver.lower()}_{prec}.so")
+
+
+def ref_available(scheme="MHM_RP", rsolver="HLLC", prec="f64") -> bool:
+    return os.path.exists(ref_path(scheme, rsolver, prec))
+
+
+_refs = {}
+
+
+def ref_lib(scheme: str, rsolver: str, prec: str):
+    key = (scheme, rsolver, prec)
+    if key not in _refs:
+        L = ctypes.CDLL(ref_path(scheme, rsolver, prec))
+        L.gamer_ref_fluid_solver.argtypes = _REF_ARGS
+        L.gamer_ref_fluid_solver.restype = _I
+        _refs[key] = L
+    return _refs[key]
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _alloc_out(npg, dtype, store_flux):
+    out = np.empty((npg, 5, 16, 16, 16), dtype=dtype)
+    flux = np.empty((npg, 9, 5, 16, 16), dtype=dtype) if store_flux else None
+    return out, flux
+
+
+def fluid_solver(u, dt, dh, gamma=5.0 / 3.0, scheme="MHM_RP", rsolver="HLLC", store_flux=True,
+                 limiter="VL_GMINMOD", minmod_coeff=1.5, minmod_maxiter=0, min_dens=0.0, min_pres=0.0,
+                 min_eint=0.0, nthreads=0):
+    """Restatement: u (npg, 5, N, N, N) float64/float32 -> (out (npg,5,16,16,16), flux (npg,9,5,16,16))."""
+    u = np.ascontiguousarray(u)
+    npg = u.shape[0]
+    out, flux = _alloc_out(npg, u.dtype, store_flux)
+    fn = lib().oracle_fluid_solver_f64 if u.dtype == np.float64 else lib().oracle_fluid_solver_f32
+    rc = fn(_ptr(u), _ptr(out), _ptr(flux), npg, dt, dh, gamma, SCHEME[scheme], RSOLVER[rsolver],
+            int(store_flux), LIMITER[limiter], minmod_coeff, minmod_maxiter, min_dens, min_pres, min_eint, nthreads)
+    if rc != 0:
+        raise ValueError(f"oracle_fluid_solver returned {rc}")
+    return out, flux
+
+
+def ref_fluid_solver(u, dt, dh, gamma=5.0 / 3.0, scheme="MHM_RP", rsolver="HLLC", store_flux=True,
+                     limiter="VL_GMINMOD", minmod_coeff=1.5, minmod_maxiter=0, min_dens=0.0, min_pres=0.0,
+                     min_eint=0.0, nthreads=0):
+    """GAMER's own CPU_FluidSolver_MHM (compiled reference) on the same arguments."""
+    u = np.ascontiguousarray(u)
+    prec = "f64" if u.dtype == np.float64 else "f32"
+    L = ref_lib(scheme, rsolver, prec)
+    npg = u.shape[0]
+    out, flux = _alloc_out(npg, u.dtype, store_flux)
+    rc = L.gamer_ref_fluid_solver(_ptr(u), _ptr(out), _ptr(flux), npg, dt, dh, gamma, int(store_flux),
+                                  LIMITER[limiter], minmod_coeff, minmod_maxiter, min_dens, min_pres, min_eint,
+                                  nthreads)
+    if rc != 0:
+        raise ValueError(f"gamer_ref_fluid_solver returned {rc}")
+    return out, flux
+
+
+def riemann(rsolver, d, L, R, gamma=5.0 / 3.0, min_pres=0.0):
+    L = np.ascontiguousarray(L, dtype=np.float64)
+    R = np.ascontiguousarray(R, dtype=np.float64)
+    f = np.empty(5)
+    rc = lib().oracle_riemann_f64(RSOLVER[rsolver], d, _ptr(L), _ptr(R), gamma, min_pres, _ptr(f))
+    if rc != 0:
+        raise ValueError(rc)
+    return f
+
+
+def limit_slope(L, C, R, limiter, coeff=1.5):
+    L, C, R = (np.ascontiguousarray(a, dtype=np.float64) for a in (L, C, R))
+    s = np.empty(5)
+    rc = lib().oracle_limit_slope_f64(_ptr(L), _ptr(C), _ptr(R), LIMITER[limiter], coeff, _ptr(s))
+    if rc != 0:
+        raise ValueError(rc)
+    return s
+
+
+def dt_cfl(u_patch, dh, safety, gamma=5.0 / 3.0, min_pres=0.0, nthreads=0):
+    """Per-patch CFL dt; u_patch (npatch, 5, 8, 8, 8) float64."""
+    u = np.ascontiguousarray(u_patch, dtype=np.float64)
+    dt = np.empty(u.shape[0])
+    rc = lib().oracle_dt_cfl_f64(_ptr(u), _ptr(dt), u.shape[0], dh, safety, gamma, min_pres, nthreads)
+    if rc != 0:
+        raise ValueError(rc)
+    return dt

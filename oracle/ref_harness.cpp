@@ -1,0 +1,149 @@
+// Test-infrastructure harness: links GAMER's own CPU hydro solver sources (compiled in place from the
+// reference tree by oracle/Makefile) and exposes them through a flat C entry point, so the oracle
+// restatement and the GPU path can be checked against the reference on identical inputs.
+//
+// This file is NOT product code and contains no solver arithmetic. It only
+//   (1) defines the run-time parameters that GAMER's Main.cpp/Global.h would otherwise define
+//       (GAMMA, unit constants, EoS aux arrays and CPU function pointers; include/Global.h:70-351),
+//   (2) initialises the EOS_GAMMA plug-in exactly as EoS_Init does on a CPU build
+//       (src/EoS/EoS_Init.cpp:80-121 -> EoS_Init_Gamma, src/EoS/Gamma/CPU_EoS_Gamma.cpp),
+//   (3) allocates the per-PG host scratch arrays as Init_MemAllocate_Fluid does
+//       (src/Init/Init_MemAllocate_Fluid.cpp:91-95), and
+//   (4) calls CPU_FluidSolver_MHM with the same arguments CPU_FluidSolver passes
+//       (src/Fluid/CPU_FluidSolver.cpp:258-266), non-MHD / no gravity / no passive scalars.
+
+#include "GAMER.h"
+#include "CUFLU.h"
+#include <omp.h>
+#include <new>
+#include <cstdlib>
+#include <cstring>
+
+// ---- run-time parameters (declared extern in include/Global.h) ----
+double GAMMA            = 5.0/3.0;
+double MOLECULAR_WEIGHT = 0.6;
+double MU_NORM          = 1.0;
+double UNIT_L = 1.0, UNIT_M = 1.0, UNIT_T = 1.0, UNIT_V = 1.0, UNIT_D = 1.0, UNIT_E = 1.0, UNIT_P = 1.0;
+bool   OPT__UNIT        = false;
+
+double EoS_AuxArray_Flt[EOS_NAUX_MAX];
+int    EoS_AuxArray_Int[EOS_NAUX_MAX];
+EoS_GUESS_t   EoS_GuessHTilde_CPUPtr   = NULL;
+EoS_H2TEM_t   EoS_HTilde2Temp_CPUPtr   = NULL;
+EoS_TEM2H_t   EoS_Temp2HTilde_CPUPtr   = NULL;
+EoS_DE2P_t    EoS_DensEint2Pres_CPUPtr = NULL;
+EoS_DP2E_t    EoS_DensPres2Eint_CPUPtr = NULL;
+EoS_DP2C_t    EoS_DensPres2CSqr_CPUPtr = NULL;
+EoS_DE2T_t    EoS_DensEint2Temp_CPUPtr = NULL;
+EoS_DT2P_t    EoS_DensTemp2Pres_CPUPtr = NULL;
+EoS_DE2S_t    EoS_DensEint2Entr_CPUPtr = NULL;
+EoS_GENE_t    EoS_General_CPUPtr       = NULL;
+real         *h_EoS_Table[EOS_NTABLE_MAX];
+
+void EoS_Init_Gamma();
+
+void CPU_FluidSolver_MHM(
+   const real   g_Flu_Array_In [][NCOMP_TOTAL][ CUBE(FLU_NXT) ],
+         real   g_Flu_Array_Out[][NCOMP_TOTAL][ CUBE(PS2) ],
+   const real   g_Mag_Array_In [][NCOMP_MAG][ FLU_NXT_P1*SQR(FLU_NXT) ],
+         real   g_Mag_Array_Out[][NCOMP_MAG][ PS2P1*SQR(PS2) ],
+         char   g_DE_Array_Out [][ CUBE(PS2) ],
+         real   g_Flux_Array   [][9][NCOMP_TOTAL][ SQR(PS2) ],
+         real   g_Ele_Array    [][9][NCOMP_ELE][ PS2P1*PS2 ],
+   const double g_Corner_Array [][3],
+   const real   g_Pot_Array_USG[][ CUBE(USG_NXT_F) ],
+         real   g_PriVar       []   [NCOMP_LR            ][ CUBE(FLU_NXT) ],
+         real   g_Slope_PPM    [][3][NCOMP_LR            ][ CUBE(N_SLOPE_PPM) ],
+         real   g_FC_Var       [][6][NCOMP_TOTAL_PLUS_MAG][ CUBE(N_FC_VAR) ],
+         real   g_FC_Flux      [][3][NCOMP_TOTAL_PLUS_MAG][ CUBE(N_FC_FLUX) ],
+         real   g_FC_Mag_Half  [][NCOMP_MAG][ FLU_NXT_P1*SQR(FLU_NXT) ],
+         real   g_EC_Ele       [][NCOMP_MAG][ CUBE(N_EC_ELE) ],
+   const int NPatchGroup,
+   const real dt, const real dh,
+   const bool StoreFlux, const bool StoreElectric,
+   const LR_Limiter_t LR_Limiter, const real MinMod_Coeff, const int MinMod_MaxIter, const double Time,
+   const bool UsePot, const OptExtAcc_t ExtAcc, const ExtAcc_t ExtAcc_Func,
+   const double c_ExtAcc_AuxArray[],
+   const real MinDens, const real MinPres, const real MinEint,
+   const real DualEnergySwitch,
+   const long PassiveFloor,
+   const bool NormPassive, const int NNorm, const int c_NormIdx[],
+   const bool FracPassive, const int NFrac, const int c_FracIdx[],
+   const bool JeansMinPres, const real JeansMinPres_Coeff,
+   const EoS_t EoS, const MicroPhy_t MicroPhy );
+
+extern "C" {
+
+// Build-time identity of this reference object: sizeof(real), FLU_NXT, scheme and Riemann solver ids.
+int gamer_ref_info( int info[4] )
+{
+   info[0] = (int)sizeof(real);
+   info[1] = FLU_NXT;
+   info[2] = FLU_SCHEME;
+   info[3] = RSOLVER;
+   return 0;
+}
+
+// One call of GAMER's CPU_FluidSolver_MHM on NPG patch groups.
+//   in   : [npg][5][FLU_NXT^3]   out : [npg][5][16^3]   flux : [npg][9][5][16^2] (may be NULL if !store_flux)
+// Returns 0 on success, nonzero on a bad argument or allocation failure.
+int gamer_ref_fluid_solver( const void *in, void *out, void *flux, int npg,
+                            double dt, double dh, double gamma, int store_flux,
+                            int limiter, double minmod_coeff, int minmod_maxiter,
+                            double min_dens, double min_pres, double min_eint, int nthreads )
+{
+   if ( npg <= 0 || in == NULL || out == NULL || ( store_flux && flux == NULL ) )   return 1;
+
+   GAMMA = gamma;
+   EoS_Init_Gamma();
+
+   EoS_t EoS_obj;
+   memset( &EoS_obj, 0, sizeof(EoS_obj) );
+   EoS_obj.DensEint2Pres_FuncPtr = EoS_DensEint2Pres_CPUPtr;
+   EoS_obj.DensPres2Eint_FuncPtr = EoS_DensPres2Eint_CPUPtr;
+   EoS_obj.DensPres2CSqr_FuncPtr = EoS_DensPres2CSqr_CPUPtr;
+   EoS_obj.DensEint2Temp_FuncPtr = EoS_DensEint2Temp_CPUPtr;
+   EoS_obj.DensTemp2Pres_FuncPtr = EoS_DensTemp2Pres_CPUPtr;
+   EoS_obj.DensEint2Entr_FuncPtr = EoS_DensEint2Entr_CPUPtr;
+   EoS_obj.GuessHTilde_FuncPtr   = EoS_GuessHTilde_CPUPtr;
+   EoS_obj.HTilde2Temp_FuncPtr   = EoS_HTilde2Temp_CPUPtr;
+   EoS_obj.Temp2HTilde_FuncPtr   = EoS_Temp2HTilde_CPUPtr;
+   EoS_obj.General_FuncPtr       = EoS_General_CPUPtr;
+   EoS_obj.AuxArrayDevPtr_Flt    = EoS_AuxArray_Flt;
+   EoS_obj.AuxArrayDevPtr_Int    = EoS_AuxArray_Int;
+   EoS_obj.Table                 = h_EoS_Table;
+
+   MicroPhy_t MicroPhy_obj;
+   memset( &MicroPhy_obj, 0, sizeof(MicroPhy_obj) );
+
+   real (*h_PriVar)      [NCOMP_LR            ][ CUBE(FLU_NXT)     ] = new (std::nothrow) real [npg]   [NCOMP_LR            ][ CUBE(FLU_NXT)     ];
+   real (*h_Slope_PPM)[3][NCOMP_LR            ][ CUBE(N_SLOPE_PPM) ] = NULL;   // unused by PLM
+   real (*h_FC_Var)   [6][NCOMP_TOTAL_PLUS_MAG][ CUBE(N_FC_VAR)    ] = new (std::nothrow) real [npg][6][NCOMP_TOTAL_PLUS_MAG][ CUBE(N_FC_VAR)    ];
+   real (*h_FC_Flux)  [3][NCOMP_TOTAL_PLUS_MAG][ CUBE(N_FC_FLUX)   ] = new (std::nothrow) real [npg][3][NCOMP_TOTAL_PLUS_MAG][ CUBE(N_FC_FLUX)   ];
+   if ( h_PriVar == NULL || h_FC_Var == NULL || h_FC_Flux == NULL )
+   {
+      delete [] h_PriVar;  delete [] h_FC_Var;  delete [] h_FC_Flux;
+      return 2;
+   }
+
+   if ( nthreads > 0 )  omp_set_num_threads( nthreads );
+   omp_set_schedule( omp_sched_dynamic, 1 );     // GAMER's run-time OpenMP schedule (src/Init/Init_OpenMP.cpp:34)
+
+   CPU_FluidSolver_MHM( (const real (*)[NCOMP_TOTAL][ CUBE(FLU_NXT) ])in,
+                        (real (*)[NCOMP_TOTAL][ CUBE(PS2) ])out,
+                        NULL, NULL, NULL,
+                        (real (*)[9][NCOMP_TOTAL][ SQR(PS2) ])flux,
+                        NULL, NULL, NULL,
+                        h_PriVar, h_Slope_PPM, h_FC_Var, h_FC_Flux, NULL, NULL,
+                        npg, (real)dt, (real)dh, store_flux != 0, false,
+                        (LR_Limiter_t)limiter, (real)minmod_coeff, minmod_maxiter, 0.0,
+                        false, EXT_ACC_NONE, NULL, NULL,
+                        (real)min_dens, (real)min_pres, (real)min_eint,
+                        (real)0.0, 0L, false, 0, NULL, false, 0, NULL, false, (real)0.0,
+                        EoS_obj, MicroPhy_obj );
+
+   delete [] h_PriVar;  delete [] h_FC_Var;  delete [] h_FC_Flux;
+   return 0;
+}
+
+} // extern "C"
