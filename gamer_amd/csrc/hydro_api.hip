@@ -1,0 +1,236 @@
+// C-ABI layer of the gfx950 patch-group fluid solver: argument checking, solver constants, device/stream
+// and buffer management.  It replaces GAMER's src/GPU_API fluid path (CUAPI_SetDevice, CUAPI_MemAllocate_Fluid,
+// CUAPI_Asyn_FluidSolver, CUAPI_Synchronize, CUAPI_MemFree_Fluid) behind include/gamer_amd_hydro.h.
+
+#include "gamer_amd_hydro.h"
+#include "hydro_kernels.hpp"
+
+#include <cstdio>
+#include <cstring>
+#include <cstdarg>
+#include <vector>
+#include <mutex>
+
+using namespace gamer_amd;
+
+namespace {
+
+thread_local char t_err[512] = "";
+
+int fail( int code, const char *fmt, ... )
+{
+   va_list ap;
+   va_start( ap, fmt );
+   vsnprintf( t_err, sizeof(t_err), fmt, ap );
+   va_end( ap );
+   return code;
+}
+
+#define HIP_TRY( call )                                                                          \
+   do { hipError_t e_ = ( call );                                                                \
+        if ( e_ != hipSuccess ) return fail( HYDRO_E_DEVICE, "%s: %s", #call, hipGetErrorString( e_ ) ); } while (0)
+
+// GPU_API state: device buffers sized once for flu_npg patch groups, pinned host double buffers, streams
+// (the module globals of src/GPU_API/CUAPI_MemAllocate_Fluid.cu:211-334)
+struct FluidBuffers {
+   int    npg_max = 0, nstream = 0, scheme = HYDRO_MHM_RP, precision = HYDRO_FP64;
+   size_t elem = 8;
+   void  *d_in = nullptr, *d_out = nullptr, *d_flux = nullptr;
+   void  *h_in[2] = { nullptr, nullptr }, *h_out[2] = { nullptr, nullptr }, *h_flux[2] = { nullptr, nullptr };
+   std::vector<hipStream_t> streams;
+};
+FluidBuffers g_buf;
+std::mutex   g_mtx;
+
+int nx_of( int scheme ) { return scheme == HYDRO_MHM_RP ? 22 : 20; }
+
+int check_args( const HydroSolveArgs *a )
+{
+   if ( a == nullptr )                                       return fail( HYDRO_E_ARG, "args == NULL" );
+   if ( a->npg < 0 )                                         return fail( HYDRO_E_ARG, "npg = %d < 0", a->npg );
+   if ( a->npg > 0 && ( a->in == nullptr || a->out == nullptr ) ) return fail( HYDRO_E_ARG, "in/out == NULL" );
+   if ( a->store_flux && a->npg > 0 && a->flux == nullptr )  return fail( HYDRO_E_ARG, "store_flux with flux == NULL" );
+   if ( a->scheme != HYDRO_MHM_RP && a->scheme != HYDRO_MHM ) return fail( HYDRO_E_ARG, "unsupported scheme %d", a->scheme );
+   if ( a->scheme == HYDRO_MHM )                             return fail( HYDRO_E_ARG, "scheme MHM is not built in this library" );
+   if ( a->rsolver != HYDRO_HLLC && a->rsolver != HYDRO_HLLE && a->rsolver != HYDRO_ROE )
+                                                             return fail( HYDRO_E_ARG, "unsupported Riemann solver %d", a->rsolver );
+   const int l = a->limiter;
+   if ( l != 1 && l != 2 && l != 3 && l != 4 && l != 6 )      return fail( HYDRO_E_ARG, "unsupported limiter %d", l );
+   if ( a->minmod_maxiter < 0 )                              return fail( HYDRO_E_ARG, "minmod_maxiter < 0" );
+   if ( !( a->gamma > 1.0 ) )                                return fail( HYDRO_E_ARG, "gamma must be > 1" );
+   if ( !( a->dh > 0.0 ) || !( a->dt >= 0.0 ) )              return fail( HYDRO_E_ARG, "dt/dh invalid" );
+   if ( a->precision != HYDRO_FP64 && a->precision != HYDRO_FP32_BULK )
+                                                             return fail( HYDRO_E_ARG, "unsupported precision %d", a->precision );
+   if ( a->npg > ( 1 << 26 ) )                               return fail( HYDRO_E_ARG, "npg too large" );
+   return HYDRO_OK;
+}
+
+// solver constants; the EOS_GAMMA aux values are computed in double and narrowed like
+// EoS_SetAuxArray_Gamma (src/EoS/Gamma/CPU_EoS_Gamma.cpp:54-66)
+template <typename R>
+Prm<R> make_prm( const HydroSolveArgs *a )
+{
+   Prm<R> P;
+   P.gamma    = (R)a->gamma;
+   P.gm1      = (R)( a->gamma - 1.0 );
+   P.inv_gm1  = (R)( 1.0/( a->gamma - 1.0 ) );
+   P.min_dens = (R)a->min_dens;
+   P.min_pres = (R)a->min_pres;
+   P.min_eint = (R)a->min_eint;
+   P.coeff    = (R)a->minmod_coeff;
+   P.limiter  = a->limiter;
+   P.dt_dh2     = 0.5*a->dt/a->dh;        // (real)0.5*dt/dh, CPU_FluidSolver_MHM.cpp:845
+   P.dt_dh      = a->dt/a->dh;            // CPU_Shared_FullStepUpdate.cpp:75
+   P.gm1_d      = (double)P.gm1;
+   P.min_dens_d = (double)P.min_dens;
+   P.min_pres_d = (double)P.min_pres;
+   P.min_eint_d = (double)P.min_eint;
+   return P;
+}
+
+int enqueue( const HydroSolveArgs *a, const void *in, void *out, void *flux, int npg, hipStream_t s )
+{
+   int rc;
+   if ( a->precision == HYDRO_FP64 )
+      rc = launch_mhm_rp<double, double>( (const double*)in, (double*)out, (double*)flux, npg, a->rsolver,
+                                          a->store_flux, a->minmod_maxiter, a->minmod_coeff, make_prm<double>( a ), s );
+   else
+      rc = launch_mhm_rp<float, float>( (const float*)in, (float*)out, (float*)flux, npg, a->rsolver,
+                                        a->store_flux, a->minmod_maxiter, a->minmod_coeff, make_prm<float>( a ), s );
+   if ( rc != 0 ) return fail( HYDRO_E_DEVICE, "kernel launch failed (%d): %s", rc, hipGetErrorString( hipGetLastError() ) );
+   return HYDRO_OK;
+}
+
+void free_buffers()
+{
+   if ( g_buf.d_in )   (void)hipFree( g_buf.d_in );
+   if ( g_buf.d_out )  (void)hipFree( g_buf.d_out );
+   if ( g_buf.d_flux ) (void)hipFree( g_buf.d_flux );
+   for (int k=0; k<2; k++)
+   {
+      if ( g_buf.h_in[k] )   (void)hipHostFree( g_buf.h_in[k] );
+      if ( g_buf.h_out[k] )  (void)hipHostFree( g_buf.h_out[k] );
+      if ( g_buf.h_flux[k] ) (void)hipHostFree( g_buf.h_flux[k] );
+   }
+   for (hipStream_t s : g_buf.streams) (void)hipStreamDestroy( s );
+   g_buf = FluidBuffers();
+}
+
+} // namespace
+
+extern "C" {
+
+const char *hydro_last_error( void ) { return t_err; }
+
+int hydro_init( int device )
+{
+   int n = 0;
+   HIP_TRY( hipGetDeviceCount( &n ) );
+   if ( device < 0 || device >= n ) return fail( HYDRO_E_ARG, "device %d out of range (%d devices)", device, n );
+   HIP_TRY( hipSetDevice( device ) );
+   return HYDRO_OK;
+}
+
+int hydro_mhm_solve_async( const HydroSolveArgs *a, void *stream )
+{
+   const int rc = check_args( a );
+   if ( rc != HYDRO_OK ) return rc;
+   return enqueue( a, a->in, a->out, a->flux, a->npg, (hipStream_t)stream );
+}
+
+int hydro_synchronize( void )
+{
+   HIP_TRY( hipDeviceSynchronize() );
+   return HYDRO_OK;
+}
+
+int hydro_mem_allocate_fluid( int flu_npg, int nstream, int scheme, int precision )
+{
+   std::lock_guard<std::mutex> lk( g_mtx );
+   if ( flu_npg <= 0 || nstream <= 0 || nstream > 64 ) return fail( HYDRO_E_ARG, "flu_npg/nstream invalid" );
+   if ( scheme != HYDRO_MHM_RP && scheme != HYDRO_MHM ) return fail( HYDRO_E_ARG, "unsupported scheme %d", scheme );
+   if ( precision != HYDRO_FP64 && precision != HYDRO_FP32_BULK ) return fail( HYDRO_E_ARG, "unsupported precision" );
+   free_buffers();
+   const int nx = nx_of( scheme );
+   g_buf.elem = ( precision == HYDRO_FP64 ) ? 8 : 4;
+   const size_t in_b   = (size_t)flu_npg*5*nx*nx*nx*g_buf.elem;
+   const size_t out_b  = (size_t)flu_npg*5*16*16*16*g_buf.elem;
+   const size_t flux_b = (size_t)flu_npg*9*5*16*16*g_buf.elem;
+   bool ok = hipMalloc( &g_buf.d_in, in_b ) == hipSuccess &&
+             hipMalloc( &g_buf.d_out, out_b ) == hipSuccess &&
+             hipMalloc( &g_buf.d_flux, flux_b ) == hipSuccess;
+   for (int k=0; k<2 && ok; k++)
+      ok = hipHostMalloc( &g_buf.h_in[k], in_b, hipHostMallocDefault ) == hipSuccess &&
+           hipHostMalloc( &g_buf.h_out[k], out_b, hipHostMallocDefault ) == hipSuccess &&
+           hipHostMalloc( &g_buf.h_flux[k], flux_b, hipHostMallocDefault ) == hipSuccess;
+   if ( !ok )
+   {
+      (void)hipGetLastError();
+      free_buffers();
+      return fail( HYDRO_E_NOMEM, "allocation of %d patch groups x %d streams failed", flu_npg, nstream );
+   }
+   g_buf.streams.resize( nstream );
+   for (int s=0; s<nstream; s++)
+      if ( hipStreamCreateWithFlags( &g_buf.streams[s], hipStreamNonBlocking ) != hipSuccess )
+      {
+         g_buf.streams.resize( s );
+         free_buffers();
+         return fail( HYDRO_E_DEVICE, "stream creation failed" );
+      }
+   g_buf.npg_max = flu_npg;  g_buf.nstream = nstream;  g_buf.scheme = scheme;  g_buf.precision = precision;
+   return HYDRO_OK;
+}
+
+int hydro_mem_free_fluid( void )
+{
+   std::lock_guard<std::mutex> lk( g_mtx );
+   free_buffers();
+   return HYDRO_OK;
+}
+
+int hydro_host_buffers( int array_id, void **h_in, void **h_out, void **h_flux )
+{
+   if ( g_buf.npg_max == 0 ) return fail( HYDRO_E_STATE, "buffers not allocated" );
+   if ( array_id < 0 || array_id > 1 ) return fail( HYDRO_E_ARG, "array_id must be 0 or 1" );
+   if ( h_in )   *h_in   = g_buf.h_in[array_id];
+   if ( h_out )  *h_out  = g_buf.h_out[array_id];
+   if ( h_flux ) *h_flux = g_buf.h_flux[array_id];
+   return HYDRO_OK;
+}
+
+int hydro_asyn_fluid_solver( const void *h_in, void *h_out, void *h_flux, const HydroSolveArgs *args, int gpu_nstream )
+{
+   HydroSolveArgs a = *args;
+   a.in = h_in;  a.out = h_out;  a.flux = h_flux;
+   int rc = check_args( &a );
+   if ( rc != HYDRO_OK ) return rc;
+   if ( g_buf.npg_max == 0 )            return fail( HYDRO_E_STATE, "hydro_mem_allocate_fluid was not called" );
+   if ( a.npg > g_buf.npg_max )         return fail( HYDRO_E_ARG, "npg %d > allocated %d", a.npg, g_buf.npg_max );
+   if ( a.scheme != g_buf.scheme || a.precision != g_buf.precision )
+                                        return fail( HYDRO_E_ARG, "scheme/precision differ from the allocation" );
+   if ( gpu_nstream <= 0 || gpu_nstream > g_buf.nstream ) return fail( HYDRO_E_ARG, "gpu_nstream out of range" );
+   if ( a.npg == 0 ) return HYDRO_OK;
+
+   const int nx = nx_of( a.scheme );
+   const size_t in_pg = (size_t)5*nx*nx*nx*g_buf.elem, out_pg = (size_t)5*4096*g_buf.elem, fl_pg = (size_t)9*5*256*g_buf.elem;
+   // NPG/NStream per stream, remainder to the last stream (CUAPI_Asyn_FluidSolver.cu:433-445)
+   const int base = a.npg / gpu_nstream;
+   int off = 0;
+   for (int s=0; s<gpu_nstream; s++)
+   {
+      const int n = ( s == gpu_nstream - 1 ) ? a.npg - off : base;
+      if ( n <= 0 ) continue;
+      hipStream_t st = g_buf.streams[s];
+      char *din = (char*)g_buf.d_in + off*in_pg, *dout = (char*)g_buf.d_out + off*out_pg, *dfl = (char*)g_buf.d_flux + off*fl_pg;
+      HIP_TRY( hipMemcpyAsync( din, (const char*)h_in + off*in_pg, n*in_pg, hipMemcpyHostToDevice, st ) );
+      rc = enqueue( &a, din, dout, dfl, n, st );
+      if ( rc != HYDRO_OK ) return rc;
+      HIP_TRY( hipMemcpyAsync( (char*)h_out + off*out_pg, dout, n*out_pg, hipMemcpyDeviceToHost, st ) );
+      if ( a.store_flux )
+         HIP_TRY( hipMemcpyAsync( (char*)h_flux + off*fl_pg, dfl, n*fl_pg, hipMemcpyDeviceToHost, st ) );
+      off += n;
+   }
+   return HYDRO_OK;
+}
+
+} // extern "C"

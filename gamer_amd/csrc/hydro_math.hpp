@@ -1,0 +1,263 @@
+// Device-side building blocks of the MHM/MHM_RP patch-group solver: ideal-gas conversions, the HLLC / HLLE /
+// Roe interface solvers and the PLM slope limiters.  Written for gfx950 wave64 VALU; templated on the bulk
+// arithmetic type R (float or double).  Pressure recovery E - |m|^2/(2 rho) goes through type A (double).
+//
+// Operation order follows the reference where it matters for parity:
+//   Riemann solvers   src/Model_Hydro/CPU_Hydro/CPU_Shared_RiemannSolver_{HLLC:391-655, HLLE:301-698, Roe:87-711}.cpp
+//   limiter + clamp   src/Model_Hydro/CPU_Hydro/CPU_Shared_DataReconstruction.cpp:403-430, 1917-2032
+//   conversions       src/Model_Hydro/CPU_Hydro/CPU_Shared_FluUtility.cpp:209-551, 681-805
+// Interface states arrive already rotated to the face normal with their pressure, 1/rho and sound speed
+// evaluated once (per cell for the half step, per face for the full step) instead of inside every solve.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace gamer_amd {
+
+constexpr int NV = 5;
+
+// ---------------- scalar helpers ----------------
+__device__ __forceinline__ float  rcp( float x )  { return __builtin_amdgcn_rcpf( x ); }
+__device__ __forceinline__ double rcp( double x )
+{
+   // hardware estimate + two Newton steps: within 1 ulp of 1/x for normal x
+   double r = __builtin_amdgcn_rcp( x );
+   double e = __builtin_fma( -x, r, 1.0 );
+   r = __builtin_fma( r, e, r );
+   e = __builtin_fma( -x, r, 1.0 );
+   return __builtin_fma( r, e, r );
+}
+__device__ __forceinline__ float  vsqrt( float x )  { return __builtin_amdgcn_sqrtf( x ); }
+__device__ __forceinline__ double vsqrt( double x ) { return __builtin_sqrt( x ); }
+template <typename R> __device__ __forceinline__ R vmax( R a, R b ) { return __builtin_fmax( a, b ); }
+template <typename R> __device__ __forceinline__ R vmin( R a, R b ) { return __builtin_fmin( a, b ); }
+template <typename R> __device__ __forceinline__ R vabs( R a )      { return __builtin_fabs( a ); }
+
+// pressure floor that lets NaN through (FluUtility.cpp:681-690)
+template <typename R> __device__ __forceinline__ R floor_pres( R p, R pmin ) { return ( p == p ) ? vmax( p, pmin ) : p; }
+
+// ---------------- solver constants ----------------
+template <typename R>
+struct Prm {
+   R gamma, gm1, inv_gm1, min_dens, min_pres, min_eint, coeff;
+   double dt_dh2, dt_dh, gm1_d, min_dens_d, min_pres_d, min_eint_d;
+   int limiter;
+};
+
+// an interface state rotated to the face normal: conserved fields + derived quantities
+template <typename R>
+struct St { R d, mn, m1, m2, e, p, ir, cs, u; };
+
+// ---------------- Riemann solvers (Davis wave speeds), flux returned in the rotated frame ----------------
+template <typename R>
+__device__ __forceinline__ void xflux_minus( const St<R> &s, R sp, R f[NV] )
+{
+   // physical x-flux minus sp*U (HLLC.cpp:600-612, HLLE.cpp:640-651)
+   f[0] = s.mn - sp*s.d;
+   f[1] = ( s.u*s.mn + s.p ) - sp*s.mn;
+   f[2] = s.u*s.m1 - sp*s.m1;
+   f[3] = s.u*s.m2 - sp*s.m2;
+   f[4] = s.u*( s.e + s.p ) - sp*s.e;
+}
+
+template <typename R>
+__device__ __forceinline__ void xflux( const St<R> &s, R f[NV] )
+{
+   f[0] = s.mn;
+   f[1] = s.u*s.mn + s.p;
+   f[2] = s.u*s.m1;
+   f[3] = s.u*s.m2;
+   f[4] = s.u*( s.e + s.p );
+}
+
+template <typename R>
+__device__ __forceinline__ void rs_hlle( const St<R> &L, const St<R> &Q, R f[NV] )
+{
+   const R Z = (R)0;
+   R sl = vmin( L.u - L.cs, Q.u - Q.cs );
+   R sr = vmax( L.u + L.cs, Q.u + Q.cs );
+   sl = vmin( sl, Z );
+   sr = vmax( sr, Z );
+   R fl[NV], fr[NV];
+   xflux_minus( L, sl, fl );
+   xflux_minus( Q, sr, fr );
+   const bool both0 = ( sl == Z ) && ( sr == Z );
+   const R inv = rcp( sr - sl );
+#  pragma unroll
+   for (int v=0; v<NV; v++) f[v] = both0 ? fl[v] : inv*( sr*fl[v] - sl*fr[v] );
+}
+
+template <typename R>
+__device__ __forceinline__ void rs_hllc( const St<R> &L, const St<R> &Q, R pmin, R f[NV] )
+{
+   const R Z = (R)0, ONE = (R)1;
+   const R wl1 = L.u - L.cs, wl2 = Q.u - Q.cs, wr1 = L.u + L.cs, wr2 = Q.u + Q.cs;
+   const R wl = vmin( wl1, wl2 ), wr = vmax( wr1, wr2 );
+   const R du = L.u - Q.u;
+   const R tl =  L.d*( ( wl1 < wl2 ) ? L.cs : du + Q.cs );
+   const R tr = -Q.d*( ( wr2 > wr1 ) ? Q.cs : du + L.cs );
+   const R t2 = rcp( tl - tr );
+   const R vs = t2*( L.p - Q.p + tl*L.u - tr*Q.u );
+   R ps = t2*( tl*( Q.p + tr*Q.u ) - tr*( L.p + tl*L.u ) );
+   ps = floor_pres( ps, pmin );
+
+   const bool left = ( vs >= Z );
+   St<R> s;
+   s.d  = left ? L.d  : Q.d;   s.mn = left ? L.mn : Q.mn;  s.m1 = left ? L.m1 : Q.m1;
+   s.m2 = left ? L.m2 : Q.m2;  s.e  = left ? L.e  : Q.e;   s.p  = left ? L.p  : Q.p;
+   s.u  = left ? L.u  : Q.u;
+   const R sp = left ? vmin( wl, Z ) : vmax( wr, Z );
+   R F[NV];
+   xflux_minus( s, sp, F );
+   const bool deg = left && ( vs == Z ) && ( sp == Z );
+   const R t4 = rcp( vs - sp );
+   const R cf = deg ? ONE : t4*vs;
+   const R cs = deg ? Z   : -t4*sp*ps;
+   f[0] = cf*F[0];
+   f[1] = cf*F[1] + cs;
+   f[2] = cf*F[2];
+   f[3] = cf*F[3];
+   f[4] = cf*F[4] + cs*vs;
+}
+
+// Roe solver with the intermediate-state positivity check falling back to HLLE (Roe.cpp:87-711)
+template <typename R>
+__device__ __forceinline__ void rs_roe( const St<R> &L, const St<R> &Q, const Prm<R> &P, R f[NV] )
+{
+   const R Z = (R)0, ONE = (R)1, HALF = (R)0.5;
+   const R hl = L.ir*( L.e + L.p ), hr = Q.ir*( Q.e + Q.p );
+   const R sql = vsqrt( L.d ), sqr = vsqrt( Q.d );
+   const R rho = sql*sqr, irho = rcp( rho );
+   const R isql = rcp( sql ), isqr = rcp( sqr ), isum = rcp( sql + sqr );
+   const R u = isum*( isql*L.mn + isqr*Q.mn );
+   const R v = isum*( isql*L.m1 + isqr*Q.m1 );
+   const R w = isum*( isql*L.m2 + isqr*Q.m2 );
+   const R V2 = u*u + v*v + w*w;
+   const R H  = isum*( sql*hl + sqr*hr );
+   R gpr = P.gm1*( H - HALF*V2 );
+   gpr = P.gamma*irho*floor_pres( gpr*rho*rcp( P.gamma ), P.min_pres );
+   const R a2 = gpr, a = vsqrt( a2 );
+   const R e0 = u - a, e4 = u + a;
+
+   R fl[NV], fr[NV];
+   xflux( L, fl );
+   xflux( Q, fr );
+   if ( e0 >= Z ) { for (int k=0; k<NV; k++) f[k] = fl[k];  return; }
+   if ( e4 <= Z ) { for (int k=0; k<NV; k++) f[k] = fr[k];  return; }
+
+   const R j0 = Q.d - L.d, j1 = Q.mn - L.mn, j2 = Q.m1 - L.m1, j3 = Q.m2 - L.m2, j4 = Q.e - L.e;
+   const R a2_ = j2 - v*j0;
+   const R a3_ = j3 - w*j0;
+   const R a1_ = P.gm1*rcp( a2 )*( j0*( H - u*u ) + u*j1 - j4 + v*a2_ + w*a3_ );
+   const R a0_ = HALF*rcp( a )*( j0*( u + a ) - j1 - a*a1_ );
+   const R a4_ = j0 - a0_ - a1_;
+
+   // intermediate states after the 1st and 4th waves (the degenerate middle waves are skipped)
+   {
+      const R d1 = L.d + a0_, m1 = L.mn + a0_*e0, t1 = L.m1 + a0_*v, t2 = L.m2 + a0_*w, en1 = L.e + a0_*( H - u*a );
+      const R ie1 = en1 - HALF*( m1*m1 + t1*t1 + t2*t2 )*rcp( d1 );
+      bool bad = ( u > e0 ) && ( d1 <= Z || ie1*P.gm1 <= Z );
+      const R d4 = d1 + a1_, m4 = m1 + a1_*u, s4 = t1 + a1_*v + a2_, r4 = t2 + a1_*w + a3_,
+              en4 = en1 + a1_*( HALF*V2 ) + a2_*v + a3_*w;
+      const R ie4 = en4 - HALF*( m4*m4 + s4*s4 + r4*r4 )*rcp( d4 );
+      bad = bad || ( ( e4 > u ) && ( d4 <= Z || ie4*P.gm1 <= Z ) );
+      if ( bad ) { rs_hlle( L, Q, f );  return; }
+   }
+
+   const R A0 = a0_*vabs( e0 ), A1 = a1_*vabs( u ), A2 = a2_*vabs( u ), A3 = a3_*vabs( u ), A4 = a4_*vabs( e4 );
+   f[0] = HALF*( fl[0] + fr[0] - A0 - A1 - A4 );
+   f[1] = HALF*( fl[1] + fr[1] - A0*e0 - A1*u - A4*e4 );
+   f[2] = HALF*( fl[2] + fr[2] - A0*v - A1*v - A2 - A4*v );
+   f[3] = HALF*( fl[3] + fr[3] - A0*w - A1*w - A3 - A4*w );
+   f[4] = HALF*( fl[4] + fr[4] - A0*( H - u*a ) - A1*( HALF*V2 ) - A2*v - A3*w - A4*( H + u*a ) );
+}
+
+// ---------------- PLM reconstruction of one cell along one direction ----------------
+// Limited slope (DataReconstruction.cpp:1917-2032), faces C -/+ slope/2, then clamp each face between the
+// cell and its neighbour and mirror the other face about the cell value (:415-430).
+template <typename R>
+__device__ __forceinline__ void plm_face( R L, R C, R Rr, int lim, R coeff, R &fL, R &fR )
+{
+   const R Z = (R)0;
+   const R sl = C - L, sr = Rr - C, sc = (R)0.5*( sl + sr ), lr = sl*sr;
+   R s;
+   switch ( lim )
+   {
+      case 6:  s = sc;  break;                                           // central
+      case 1:  s = (R)2.0*lr*rcp( sl + sr );  break;                     // van Leer
+      case 2:  s = vmin( vabs( sc ), vmin( vabs( sl*coeff ), vabs( sr*coeff ) ) );
+               s = ( sc < Z ) ? -s : s;  break;                          // generalised MinMod
+      case 3:  s = lr*( sl + sr )*rcp( sl*sl + sr*sr );  break;          // van Albada
+      default: {                                                         // van Leer + generalised MinMod
+         const R sa = (R)2.0*sl*sr*rcp( sl + sr );
+         s = vmin( vabs( sa ), vmin( vabs( sc ), vmin( vabs( sl*coeff ), vabs( sr*coeff ) ) ) );
+         s = ( sc < Z ) ? -s : s;
+      } break;
+   }
+   s = ( lr > Z ) ? s : Z;
+   R fl = C - (R)0.5*s, fr = C + (R)0.5*s;
+   R lo = ( C < L ) ? C : L, hi = ( C > L ) ? C : L;
+   fl = ( fl > lo ) ? fl : lo;
+   fl = ( fl < hi ) ? fl : hi;
+   fr = (R)2.0*C - fl;
+   lo = ( C < Rr ) ? C : Rr;  hi = ( C > Rr ) ? C : Rr;
+   fr = ( fr > lo ) ? fr : lo;
+   fr = ( fr < hi ) ? fr : hi;
+   fL = (R)2.0*C - fr;
+   fR = fr;
+}
+
+// primitive face state (rho, vx, vy, vz, P) -> interface state rotated to direction D
+// (Pri2Con, FluUtility.cpp:343-420, with the solver's Con2Pres read directly from the face pressure)
+template <int D, typename R>
+__device__ __forceinline__ St<R> face_state( const R w[NV], const Prm<R> &P )
+{
+   St<R> s;
+   s.d = w[0];
+   const R mx = w[0]*w[1], my = w[0]*w[2], mz = w[0]*w[3];
+   s.ir = rcp( w[0] );
+   s.e  = (R)0.5*( mx*mx + my*my + mz*mz )*s.ir + w[4]*P.inv_gm1;
+   s.mn = ( D == 0 ) ? mx : ( D == 1 ) ? my : mz;
+   s.m1 = ( D == 0 ) ? my : ( D == 1 ) ? mz : mx;
+   s.m2 = ( D == 0 ) ? mz : ( D == 1 ) ? mx : my;
+   s.p  = floor_pres( w[4], P.min_pres );
+   s.u  = s.ir*s.mn;
+   s.cs = vsqrt( P.gamma*s.p*s.ir );
+   return s;
+}
+
+// conserved cell (with its pressure, 1/rho and sound speed) -> interface state rotated to direction D
+template <int D, typename R>
+__device__ __forceinline__ St<R> cell_state( const R c[NV], R p, R ir, R cs )
+{
+   St<R> s;
+   s.d  = c[0];
+   s.mn = c[1 + D];
+   s.m1 = c[1 + ( D + 1 ) % 3];
+   s.m2 = c[1 + ( D + 2 ) % 3];
+   s.e  = c[4];
+   s.p  = p;  s.ir = ir;  s.cs = cs;
+   s.u  = ir*s.mn;
+   return s;
+}
+
+// Riemann solve across a D-normal face; flux returned in the lab frame.  HLLC falls back to HLLE when a flux
+// component is NaN (RSOLVER_RESCUE, CPU_FluidSolver_MHM.cpp:737-786, ComputeFlux.cpp:313-362).
+template <int RS, int D, typename R>
+__device__ __forceinline__ void riemann( const St<R> &L, const St<R> &Q, const Prm<R> &P, R out[NV] )
+{
+   R f[NV];
+   if ( RS == 4 )
+   {
+      rs_hllc( L, Q, P.min_pres, f );
+      if ( f[0] != f[0] || f[1] != f[1] || f[2] != f[2] || f[3] != f[3] || f[4] != f[4] ) rs_hlle( L, Q, f );
+   }
+   else if ( RS == 3 ) rs_hlle( L, Q, f );
+   else                rs_roe( L, Q, P, f );
+   out[0] = f[0];
+   out[4] = f[4];
+   out[1 + D]             = f[1];
+   out[1 + ( D + 1 ) % 3] = f[2];
+   out[1 + ( D + 2 ) % 3] = f[3];
+}
+
+} // namespace gamer_amd

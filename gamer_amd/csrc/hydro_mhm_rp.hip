@@ -1,0 +1,423 @@
+// MHM_RP (MUSCL-Hancock with Riemann-solver prediction) patch-group kernel for gfx950.
+//
+// One 256-lane workgroup updates one patch group: 22^3 input cells (16^3 interior + 3 ghost zones) ->
+// 16^3 output cells + 9 coarse-fine flux planes.  The reference runs the same update as eight global-memory
+// phases over the whole 22^3 block (src/Model_Hydro/CPU_Hydro/CPU_FluidSolver_MHM.cpp:263-589, ~2.9 MB of
+// scratch per patch group in FP64).  Here the workgroup marches along z, one input plane per step, and keeps
+// every intermediate on chip:
+//
+//   * Lane t owns the output column (i,j) = (3 + t%16, 3 + t/16) and at most one "outer" ghost column
+//     (outer_column()).  Everything that couples planes along z — the previous input planes, the half-step
+//     z-flux, the last three half-step primitive planes, the lower z-face state and the last full-step
+//     z-flux — stays in that lane's registers.
+//   * Only x/y neighbours go through LDS, one plane at a time: the input plane with its pressure, 1/rho and
+//     sound speed, one direction of half-step fluxes, the half-step primitive plane, the upper x/y face
+//     states and one direction of full-step fluxes (77 KB in FP64, 39 KB in FP32).
+//
+// Plane schedule at step t (input plane t just loaded; all indices in input coordinates):
+//   half-step z-flux at t-1/2, half-step cell update of plane t-1, half-step x/y fluxes of plane t,
+//   PLM reconstruction of plane r = t-2, full-step fluxes of plane r (x/y) and at r-1/2 (z),
+//   full-step update and output of plane t-3.
+// All branches that contain a barrier depend only on t, so every barrier is reached by the whole workgroup.
+
+#include "hydro_math.hpp"
+#include "hydro_kernels.hpp"
+
+namespace gamer_amd {
+
+namespace {
+
+constexpr int NX  = 22;          // FLU_NXT for MHM_RP
+constexpr int NC  = NX*NX;       // columns per plane
+constexpr int PS  = 16;          // PS2
+constexpr int NT  = 256;         // lanes per workgroup
+
+template <typename R>
+struct SmemRP {
+   R a[10][NC];   // input plane (rho,mx,my,mz,E,P,1/rho,cs) | upper x faces (0-4) and upper y faces (5-9)
+   R b[NV][NC];   // half-step x or y fluxes | full-step x or y fluxes
+   R w[NV][NC];   // half-step primitive plane read by the reconstruction
+   int fail;
+};
+
+// ghost columns served by lane t besides its output column; -1 = none
+__device__ __forceinline__ void ring2( int k, int &i, int &j )
+{
+   // 76 columns with i or j in {1,20}, 1 <= i,j <= 20
+   if      ( k < 20 ) { i = 1 + k;        j = 1;  }
+   else if ( k < 40 ) { i = 1 + k - 20;   j = 20; }
+   else if ( k < 58 ) { i = 1;            j = 2 + k - 40; }
+   else               { i = 20;           j = 2 + k - 58; }
+}
+
+__device__ __forceinline__ bool outer_column( int t, int &i, int &j )
+{
+   if ( t <  16 ) { i = 2;          j = 3 + t;       return true; }     // wave 0: x-reconstruction ring
+   if ( t <  32 ) { i = 19;         j = 3 + t - 16;  return true; }
+   if ( t <  64 ) { ring2( t - 32, i, j );           return true; }
+   if ( t <  80 ) { i = 3 + t - 64; j = 2;           return true; }     // wave 1: y-reconstruction ring
+   if ( t <  96 ) { i = 3 + t - 80; j = 19;          return true; }
+   if ( t < 128 ) { ring2( t - 96 + 32, i, j );      return true; }
+   if ( t < 132 ) { const int c = t - 128; i = ( c & 1 ) ? 19 : 2; j = ( c & 2 ) ? 19 : 2; return true; }
+   if ( t < 144 ) { ring2( t - 132 + 64, i, j );     return true; }     // wave 2
+   if ( t < 164 ) { i = 21;         j = 1 + t - 144; return true; }
+   if ( t >= 192 && t < 212 ) { i = 1 + t - 192; j = 21; return true; }  // wave 3
+   i = j = -1;
+   return false;
+}
+
+// roles of a column (i,j) of the 22x22 footprint
+struct Col {
+   int  c;                   // i + 22 j
+   bool w;                   // half-step cell (1..20)^2, owns the half-step z-face below it
+   bool hx, hy;              // owns the half-step x (y) face on its low side
+   bool rx, ry;              // needs x (y) reconstruction
+   bool fx, fy;              // owns the full-step x (y) face on its low side
+   bool core;                // output column (3..18)^2
+   int  i, j;
+};
+
+__device__ __forceinline__ Col make_col( int i, int j )
+{
+   Col k;
+   k.i = i;  k.j = j;  k.c = i + NX*j;
+   const bool ii = ( i >= 1 && i <= 20 ), jj = ( j >= 1 && j <= 20 );
+   k.w  = ii && jj;
+   k.hx = ( i >= 1 && i <= 21 ) && jj;
+   k.hy = ( j >= 1 && j <= 21 ) && ii;
+   const bool ci = ( i >= 3 && i <= 18 ), cj = ( j >= 3 && j <= 18 );
+   k.rx = ( i >= 2 && i <= 19 ) && cj;
+   k.ry = ( j >= 2 && j <= 19 ) && ci;
+   k.fx = ( i >= 3 && i <= 19 ) && cj;
+   k.fy = ( j >= 3 && j <= 19 ) && ci;
+   k.core = ci && cj;
+   return k;
+}
+
+// cell quantities of one input cell: conserved + pressure (recovered in FP64), 1/rho, sound speed
+template <typename R, typename IO>
+__device__ __forceinline__ void derive_cell( const IO u[NV], R q[8], const Prm<R> &P )
+{
+   const double d = (double)u[0], mx = (double)u[1], my = (double)u[2], mz = (double)u[3], e = (double)u[4];
+   const double eint = e - 0.5*( mx*mx + my*my + mz*mz ) / d;
+   const R p = floor_pres( (R)( eint*P.gm1_d ), P.min_pres );
+   const R ir = rcp( (R)u[0] );
+   for (int v=0; v<NV; v++) q[v] = (R)u[v];
+   q[5] = p;
+   q[6] = ir;
+   q[7] = vsqrt( P.gamma*p*ir );
+}
+
+// half-step cell update, floors and conversion to primitive (CPU_FluidSolver_MHM.cpp:846-933), accumulated in FP64
+template <typename R>
+__device__ __forceinline__ void half_cell( const R u[NV], const R dfxy[NV], const R hzl[NV], const R hzu[NV],
+                                           const Prm<R> &P, R w[NV] )
+{
+   double c[NV];
+#  pragma unroll
+   for (int v=0; v<NV; v++)
+      c[v] = (double)u[v] - P.dt_dh2*( (double)dfxy[v] + ( (double)hzu[v] - (double)hzl[v] ) );
+   c[0] = __builtin_fmax( c[0], P.min_dens_d );
+   {
+      const double ein  = c[4] - 0.5*( c[1]*c[1] + c[2]*c[2] + c[3]*c[3] ) / c[0];
+      const double eout = ( ein == ein ) ? __builtin_fmax( ein, P.min_eint_d ) : ein;
+      c[4] = ( ein == eout ) ? c[4] : c[4] - ein + eout;
+   }
+   const double r = 1.0 / c[0];
+   w[0] = (R)c[0];
+   w[1] = (R)( c[1]*r );
+   w[2] = (R)( c[2]*r );
+   w[3] = (R)( c[3]*r );
+   const double p = ( c[4] - 0.5*( c[1]*c[1] + c[2]*c[2] + c[3]*c[3] ) / c[0] )*P.gm1_d;
+   w[4] = (R)( ( p == p ) ? __builtin_fmax( p, P.min_pres_d ) : p );
+}
+
+template <typename IO> struct IOLim;
+template <> struct IOLim<double> { static constexpr double huge = 1.7976931348623157e308; static constexpr double eps = 2.2204460492503131e-16; };
+template <> struct IOLim<float>  { static constexpr double huge = 3.40282346638528859812e+38; static constexpr double eps = 1.1920928955078125e-07; };
+
+// full-step update of one output cell in FP64 + the CONS-mode unphysical check (FullStepUpdate.cpp:106-235)
+template <typename R, typename IO>
+__device__ __forceinline__ bool full_cell( const R u[NV], const R dfxy[NV], const R fzl[NV], const R fzu[NV],
+                                           const Prm<R> &P, IO out[NV] )
+{
+   double c[NV];
+#  pragma unroll
+   for (int v=0; v<NV; v++)
+   {
+      c[v] = (double)u[v] - P.dt_dh*( (double)dfxy[v] + ( (double)fzu[v] - (double)fzl[v] ) );
+      out[v] = (IO)c[v];
+   }
+   double o[NV];
+   for (int v=0; v<NV; v++) o[v] = (double)out[v];
+   const double huge = IOLim<IO>::huge;
+   bool bad = false;
+   for (int v=0; v<NV; v++) bad |= ( o[v] != o[v] );
+   for (int v=1; v<=3; v++) bad |= ( o[v] < -huge || o[v] > huge );
+   bad |= ( o[0] <= 0.0 || o[0] > huge || o[4] <= 0.0 || o[4] > huge );
+   const double ei = o[4] - 0.5*( o[1]*o[1] + o[2]*o[2] + o[3]*o[3] ) / o[0];
+   bad |= ( ei < -3.0*o[4]*IOLim<IO>::eps || ei > huge || ei != ei );
+   return bad;
+}
+
+template <typename R>
+__device__ __forceinline__ void ld_cell( const R (*a)[NC], int c, R q[8] )
+{
+#  pragma unroll
+   for (int v=0; v<8; v++) q[v] = a[v][c];
+}
+
+template <int D, typename R>
+__device__ __forceinline__ St<R> st_of( const R q[8] ) { return cell_state<D>( q, q[5], q[6], q[7] ); }
+
+template <int D, typename R>
+__device__ __forceinline__ St<R> st_of( const R (*a)[NC], int c ) { R q[8]; ld_cell( a, c, q ); return st_of<D>( q ); }
+
+template <typename R, typename IO, int RS>
+__global__ void __launch_bounds__( NT )
+mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict__ g_flux,
+               int npg, int store_flux, int minmod_maxiter, R minmod_coeff, Prm<R> P )
+{
+   __shared__ SmemRP<R> S;
+
+   const int pg  = blockIdx.x;
+   const int tid = threadIdx.x;
+   if ( pg >= npg ) return;
+
+   const IO *in  = g_in  + (size_t)pg*NV*NC*NX;
+   IO       *out = g_out + (size_t)pg*NV*PS*PS*PS;
+   IO       *flx = store_flux ? g_flux + (size_t)pg*9*NV*PS*PS : nullptr;
+
+   const Col K = make_col( 3 + tid % PS, 3 + tid / PS );
+   int oi, oj;
+   const bool has_o = outer_column( tid, oi, oj );
+   const Col O = make_col( has_o ? oi : 0, has_o ? oj : 0 );
+
+   for (int iter = 0; ; iter++)
+   {
+      Prm<R> Q = P;
+      {
+         R cf = ( minmod_maxiter == 0 ) ? minmod_coeff : minmod_coeff - (R)iter*minmod_coeff/(R)minmod_maxiter;
+         Q.coeff = vmax( cf, (R)0 );
+      }
+      if ( tid == 0 ) S.fail = 0;
+
+      // lane state carried along z (core column: K, outer column: O)
+      R kU[8], kU1[8], kU2[NV], kU3[NV];         // input planes t, t-1, t-2, t-3
+      R kHz[NV], kDh[NV];                         // half-step z-flux at t-3/2 ; dFx+dFy of the half step at t-1
+      R kW1[NV], kW2[NV], kW3[NV];                // half-step primitives at t-1, t-2, t-3
+      R kFzR[NV], kFz[NV], kDf[NV];               // upper z face of plane r-1 ; full z-flux at r-3/2 ; full dFx+dFy at r-1
+      R oU[8], oU1[8], oHz[NV], oDh[NV], oW[NV];  // outer column
+
+      for (int t = 0; t < NX; t++)
+      {
+         // ---- A: input plane t -> LDS (coalesced), with pressure/1/rho/c_s evaluated once per cell
+         __syncthreads();
+         for (int c = tid; c < NC; c += NT)
+         {
+            IO u[NV];
+#           pragma unroll
+            for (int v=0; v<NV; v++) u[v] = in[ (size_t)v*NC*NX + (size_t)t*NC + c ];
+            R q[8];
+            derive_cell( u, q, Q );
+#           pragma unroll
+            for (int v=0; v<8; v++) S.a[v][c] = q[v];
+         }
+         __syncthreads();
+
+         // shift the per-column input history and pick up plane t
+         for (int v=0; v<NV; v++) { kU3[v] = kU2[v];  kU2[v] = kU1[v]; }
+         for (int v=0; v<8;  v++) { kU1[v] = kU[v];   oU1[v] = oU[v]; }
+         ld_cell( S.a, K.c, kU );
+         if ( has_o ) ld_cell( S.a, O.c, oU );
+
+         // ---- half-step z-flux at t-1/2 and half-step cell update of plane t-1
+         if ( t >= 1 )
+         {
+            R fz[NV], w[NV];
+            riemann<RS, 2>( st_of<2>( kU1 ), st_of<2>( kU ), Q, fz );
+            if ( t >= 2 )
+            {
+               half_cell( kU1, kDh, kHz, fz, Q, w );
+               for (int v=0; v<NV; v++) { kW3[v] = kW2[v];  kW2[v] = kW1[v];  kW1[v] = w[v]; }
+            }
+            for (int v=0; v<NV; v++) kHz[v] = fz[v];
+
+            if ( has_o && O.w )
+            {
+               riemann<RS, 2>( st_of<2>( oU1 ), st_of<2>( oU ), Q, fz );
+               if ( t >= 2 ) half_cell( oU1, oDh, oHz, fz, Q, oW );
+               for (int v=0; v<NV; v++) oHz[v] = fz[v];
+            }
+         }
+
+         // ---- half-step x and y fluxes of plane t (low face owned by each column)
+         if ( t >= 1 && t <= 20 )
+         {
+            R fk[NV], fo[NV];
+            riemann<RS, 0>( st_of<0>( S.a, K.c - 1 ), st_of<0>( kU ), Q, fk );
+            for (int v=0; v<NV; v++) S.b[v][K.c] = fk[v];
+            if ( has_o && O.hx )
+            {
+               riemann<RS, 0>( st_of<0>( S.a, O.c - 1 ), st_of<0>( oU ), Q, fo );
+               for (int v=0; v<NV; v++) S.b[v][O.c] = fo[v];
+            }
+            __syncthreads();
+            for (int v=0; v<NV; v++) kDh[v] = S.b[v][K.c + 1] - fk[v];
+            if ( has_o && O.w ) for (int v=0; v<NV; v++) oDh[v] = S.b[v][O.c + 1] - fo[v];
+            __syncthreads();
+
+            riemann<RS, 1>( st_of<1>( S.a, K.c - NX ), st_of<1>( kU ), Q, fk );
+            for (int v=0; v<NV; v++) S.b[v][K.c] = fk[v];
+            if ( has_o && O.hy )
+            {
+               riemann<RS, 1>( st_of<1>( S.a, O.c - NX ), st_of<1>( oU ), Q, fo );
+               for (int v=0; v<NV; v++) S.b[v][O.c] = fo[v];
+            }
+            __syncthreads();
+            for (int v=0; v<NV; v++) kDh[v] += S.b[v][K.c + NX] - fk[v];
+            if ( has_o && O.w ) for (int v=0; v<NV; v++) oDh[v] += S.b[v][O.c + NX] - fo[v];
+         }
+
+         // ---- PLM reconstruction of plane r = t-2 (faces of the x/y directions go to LDS, z stays in lanes)
+         const int r = t - 2;
+         R kFL[3][NV], kFRz[NV], oFL[NV];
+         if ( r >= 2 && r <= 19 )
+         {
+            for (int v=0; v<NV; v++) plm_face( kW3[v], kW2[v], kW1[v], Q.limiter, Q.coeff, kFL[2][v], kFRz[v] );
+            if ( r >= 3 && r <= 18 )
+            {
+               for (int v=0; v<NV; v++)
+               {
+                  R fr;
+                  plm_face( S.w[v][K.c - 1], kW2[v], S.w[v][K.c + 1], Q.limiter, Q.coeff, kFL[0][v], fr );
+                  S.a[v][K.c] = fr;
+                  plm_face( S.w[v][K.c - NX], kW2[v], S.w[v][K.c + NX], Q.limiter, Q.coeff, kFL[1][v], fr );
+                  S.a[NV + v][K.c] = fr;
+               }
+               if ( has_o && O.rx )
+                  for (int v=0; v<NV; v++)
+                  {
+                     R fr;
+                     plm_face( S.w[v][O.c - 1], S.w[v][O.c], S.w[v][O.c + 1], Q.limiter, Q.coeff, oFL[v], fr );
+                     S.a[v][O.c] = fr;
+                  }
+               if ( has_o && O.ry )
+                  for (int v=0; v<NV; v++)
+                  {
+                     R fr;
+                     plm_face( S.w[v][O.c - NX], S.w[v][O.c], S.w[v][O.c + NX], Q.limiter, Q.coeff, oFL[v], fr );
+                     S.a[NV + v][O.c] = fr;
+                  }
+            }
+         }
+         __syncthreads();
+         // the half-step plane t-1 replaces plane t-2 in LDS for the next reconstruction
+         if ( t >= 2 )
+         {
+            for (int v=0; v<NV; v++) S.w[v][K.c] = kW1[v];
+            if ( has_o && O.w ) for (int v=0; v<NV; v++) S.w[v][O.c] = oW[v];
+         }
+
+         // ---- full-step fluxes: z at r-1/2 (lane-local), x and y of plane r through LDS
+         R fzn[NV];
+         if ( r >= 3 && r <= 19 )
+         {
+            riemann<RS, 2>( face_state<2>( kFzR, Q ), face_state<2>( kFL[2], Q ), Q, fzn );
+            if ( flx && ( r - 3 ) % 8 == 0 )
+               for (int v=0; v<NV; v++)
+                  flx[ ( ( 6 + ( r - 3 )/8 )*NV + v )*PS*PS + ( K.i - 3 ) + PS*( K.j - 3 ) ] = (IO)fzn[v];
+         }
+         if ( r >= 2 ) for (int v=0; v<NV; v++) kFzR[v] = kFRz[v];
+
+         R dfn[NV];
+         if ( r >= 3 && r <= 18 )
+         {
+            R fk[NV], fo[NV], wl[NV];
+            for (int v=0; v<NV; v++) wl[v] = S.a[v][K.c - 1];
+            riemann<RS, 0>( face_state<0>( wl, Q ), face_state<0>( kFL[0], Q ), Q, fk );
+            for (int v=0; v<NV; v++) S.b[v][K.c] = fk[v];
+            if ( has_o && O.fx )
+            {
+               for (int v=0; v<NV; v++) wl[v] = S.a[v][O.c - 1];
+               riemann<RS, 0>( face_state<0>( wl, Q ), face_state<0>( oFL, Q ), Q, fo );
+               for (int v=0; v<NV; v++) S.b[v][O.c] = fo[v];
+            }
+            if ( flx )
+            {
+               if ( ( K.i - 3 ) % 8 == 0 )
+                  for (int v=0; v<NV; v++)
+                     flx[ ( ( ( K.i - 3 )/8 )*NV + v )*PS*PS + ( K.j - 3 ) + PS*( r - 3 ) ] = (IO)fk[v];
+               if ( has_o && O.fx )
+                  for (int v=0; v<NV; v++)
+                     flx[ ( 2*NV + v )*PS*PS + ( O.j - 3 ) + PS*( r - 3 ) ] = (IO)fo[v];
+            }
+            __syncthreads();
+            for (int v=0; v<NV; v++) dfn[v] = S.b[v][K.c + 1] - fk[v];
+            __syncthreads();
+
+            for (int v=0; v<NV; v++) wl[v] = S.a[NV + v][K.c - NX];
+            riemann<RS, 1>( face_state<1>( wl, Q ), face_state<1>( kFL[1], Q ), Q, fk );
+            for (int v=0; v<NV; v++) S.b[v][K.c] = fk[v];
+            if ( has_o && O.fy )
+            {
+               for (int v=0; v<NV; v++) wl[v] = S.a[NV + v][O.c - NX];
+               riemann<RS, 1>( face_state<1>( wl, Q ), face_state<1>( oFL, Q ), Q, fo );
+               for (int v=0; v<NV; v++) S.b[v][O.c] = fo[v];
+            }
+            if ( flx )
+            {
+               if ( ( K.j - 3 ) % 8 == 0 )
+                  for (int v=0; v<NV; v++)
+                     flx[ ( ( 3 + ( K.j - 3 )/8 )*NV + v )*PS*PS + ( K.i - 3 ) + PS*( r - 3 ) ] = (IO)fk[v];
+               if ( has_o && O.fy )
+                  for (int v=0; v<NV; v++)
+                     flx[ ( 5*NV + v )*PS*PS + ( O.i - 3 ) + PS*( r - 3 ) ] = (IO)fo[v];
+            }
+            __syncthreads();
+            for (int v=0; v<NV; v++) dfn[v] += S.b[v][K.c + NX] - fk[v];
+         }
+
+         // ---- full-step update and output of plane o = t-3
+         const int o = t - 3;
+         if ( o >= 3 && o <= 18 )
+         {
+            IO res[NV];
+            const bool bad = full_cell( kU3, kDf, kFz, fzn, Q, res );
+            const size_t co = (size_t)( K.i - 3 ) + PS*( K.j - 3 ) + (size_t)PS*PS*( o - 3 );
+            for (int v=0; v<NV; v++) out[ (size_t)v*PS*PS*PS + co ] = res[v];
+            if ( bad ) S.fail = 1;
+         }
+         if ( r >= 3 && r <= 19 ) for (int v=0; v<NV; v++) kFz[v] = fzn[v];
+         if ( r >= 3 && r <= 18 ) for (int v=0; v<NV; v++) kDf[v] = dfn[v];
+      } // for t
+
+      __syncthreads();
+      const bool redo = ( S.fail != 0 ) && ( iter + 1 <= minmod_maxiter );
+      __syncthreads();
+      if ( !redo ) break;
+   } // MinMod retry
+}
+
+} // namespace
+
+template <typename R, typename IO>
+int launch_mhm_rp( const IO *in, IO *out, IO *flux, int npg, int rsolver, int store_flux, int minmod_maxiter,
+                   double minmod_coeff, const Prm<R> &P, hipStream_t stream )
+{
+   if ( npg <= 0 ) return 0;
+   const dim3 grid( npg ), block( NT );
+   switch ( rsolver )
+   {
+      case 4: hipLaunchKernelGGL( ( mhm_rp_kernel<R, IO, 4> ), grid, block, 0, stream, in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P ); break;
+      case 3: hipLaunchKernelGGL( ( mhm_rp_kernel<R, IO, 3> ), grid, block, 0, stream, in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P ); break;
+      case 2: hipLaunchKernelGGL( ( mhm_rp_kernel<R, IO, 2> ), grid, block, 0, stream, in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P ); break;
+      default: return -1;
+   }
+   return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
+}
+
+template int launch_mhm_rp<double, double>( const double*, double*, double*, int, int, int, int, double, const Prm<double>&, hipStream_t );
+template int launch_mhm_rp<float,  float >( const float*,  float*,  float*,  int, int, int, int, double, const Prm<float>&,  hipStream_t );
+
+} // namespace gamer_amd

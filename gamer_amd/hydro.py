@@ -1,0 +1,155 @@
+"""Host-side mirror of GAMER's fluid-solver GPU API over the gfx950 C-ABI library.
+
+``fluid_solver`` is the device-array entry (one stream-ordered launch of ``hydro_mhm_solve_async`` on torch's
+current stream); ``GpuFluidApi`` mirrors the host-array path of ``CUAPI_MemAllocate_Fluid`` /
+``CUAPI_Asyn_FluidSolver`` / ``CUAPI_Synchronize`` (src/GPU_API/CUAPI_Asyn_FluidSolver.cu:266-749): pinned
+double buffers, N streams, asynchronous H2D + kernel + D2H.  Argument names and meanings follow
+``CUAPI_Asyn_FluidSolver`` (include/Prototype.h:646); failures raise ``HydroError`` with the library's message
+(the reference aborts through Aux_Error instead).
+
+The library must have been built (``python gamer_amd/build.py`` or ``__graft_entry__.build()``); there is no
+fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgamer_amd_hydro.so")
+
+HYDRO_OK = 0
+SCHEME = {"MHM": 3, "MHM_RP": 4}
+RSOLVER = {"ROE": 2, "HLLE": 3, "HLLC": 4}
+LIMITER = {"VANLEER": 1, "GMINMOD": 2, "ALBADA": 3, "VL_GMINMOD": 4, "CENTRAL": 6}
+PRECISION = {"fp64": 0, "fp32": 1}
+NX = {"MHM_RP": 22, "MHM": 20}
+
+EXPORTS = ("hydro_init", "hydro_mhm_solve_async", "hydro_synchronize", "hydro_last_error",
+           "hydro_mem_allocate_fluid", "hydro_mem_free_fluid", "hydro_host_buffers", "hydro_asyn_fluid_solver")
+
+
+class HydroError(RuntimeError):
+    pass
+
+
+class HydroSolveArgs(ctypes.Structure):
+    _fields_ = [
+        ("in_", ctypes.c_void_p), ("out", ctypes.c_void_p), ("flux", ctypes.c_void_p),
+        ("npg", ctypes.c_int), ("dt", ctypes.c_double), ("dh", ctypes.c_double), ("gamma", ctypes.c_double),
+        ("scheme", ctypes.c_int), ("rsolver", ctypes.c_int), ("limiter", ctypes.c_int),
+        ("minmod_coeff", ctypes.c_double), ("minmod_maxiter", ctypes.c_int),
+        ("min_dens", ctypes.c_double), ("min_pres", ctypes.c_double), ("min_eint", ctypes.c_double),
+        ("store_flux", ctypes.c_int), ("precision", ctypes.c_int),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load the gfx950 solver library; raises if it was never built (no CPU fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HydroError(f"{LIB_PATH} not built: run `python gamer_amd/build.py`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.hydro_last_error.restype = ctypes.c_char_p
+        L.hydro_init.argtypes = [ctypes.c_int]
+        L.hydro_mhm_solve_async.argtypes = [ctypes.POINTER(HydroSolveArgs), ctypes.c_void_p]
+        L.hydro_mem_allocate_fluid.argtypes = [ctypes.c_int] * 4
+        L.hydro_host_buffers.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_void_p)] * 3
+        L.hydro_asyn_fluid_solver.argtypes = [ctypes.c_void_p] * 3 + [ctypes.POINTER(HydroSolveArgs), ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != HYDRO_OK:
+        raise HydroError(f"hydro error {rc}: {lib().hydro_last_error().decode()}")
+
+
+def make_args(npg, dt, dh, gamma=5.0 / 3.0, scheme="MHM_RP", rsolver="HLLC", limiter="VL_GMINMOD",
+              minmod_coeff=1.5, minmod_maxiter=0, min_dens=0.0, min_pres=0.0, min_eint=0.0, store_flux=True,
+              precision="fp64", in_=None, out=None, flux=None) -> HydroSolveArgs:
+    return HydroSolveArgs(in_, out, flux, int(npg), float(dt), float(dh), float(gamma), SCHEME[scheme],
+                          RSOLVER[rsolver], LIMITER[limiter], float(minmod_coeff), int(minmod_maxiter),
+                          float(min_dens), float(min_pres), float(min_eint), int(bool(store_flux)),
+                          PRECISION[precision])
+
+
+def fluid_solver(u, dt, dh, gamma=5.0 / 3.0, scheme="MHM_RP", rsolver="HLLC", limiter="VL_GMINMOD",
+                 minmod_coeff=1.5, minmod_maxiter=0, min_dens=0.0, min_pres=0.0, min_eint=0.0, store_flux=True,
+                 out=None, flux=None, stream=None):
+    """One patch-group solver call on device tensors.
+
+    u: torch tensor on a ROCm device, float64 (FP64 path) or float32 (FP32-bulk path), shape (npg, 5, N, N, N).
+    Returns (out (npg,5,16,16,16), flux (npg,9,5,16,16) or None), enqueued on ``stream`` (default: torch's
+    current stream) — synchronise before reading on the host.
+    """
+    import torch
+    if not u.is_cuda:
+        raise HydroError("fluid_solver needs a device tensor")
+    if u.dtype not in (torch.float64, torch.float32):
+        raise HydroError(f"unsupported dtype {u.dtype}")
+    n = NX[scheme]
+    if u.dim() != 5 or tuple(u.shape[1:]) != (5, n, n, n):
+        raise HydroError(f"input shape {tuple(u.shape)} != (npg, 5, {n}, {n}, {n})")
+    u = u.contiguous()
+    npg = u.shape[0]
+    if out is None:
+        out = torch.empty((npg, 5, 16, 16, 16), dtype=u.dtype, device=u.device)
+    if store_flux and flux is None:
+        flux = torch.empty((npg, 9, 5, 16, 16), dtype=u.dtype, device=u.device)
+    prec = "fp64" if u.dtype == torch.float64 else "fp32"
+    args = make_args(npg, dt, dh, gamma, scheme, rsolver, limiter, minmod_coeff, minmod_maxiter, min_dens, min_pres,
+                     min_eint, store_flux, prec, u.data_ptr(), out.data_ptr(),
+                     flux.data_ptr() if store_flux else None)
+    s = stream if stream is not None else torch.cuda.current_stream(u.device)
+    _check(lib().hydro_mhm_solve_async(ctypes.byref(args), ctypes.c_void_p(s.cuda_stream)))
+    return out, (flux if store_flux else None)
+
+
+class GpuFluidApi:
+    """The GPU_API host-array path: pinned double buffers [2] + GPU_NStream streams (CUAPI_MemAllocate_Fluid)."""
+
+    def __init__(self, flu_npg: int, gpu_nstream: int = 4, scheme: str = "MHM_RP", precision: str = "fp64",
+                 device: int = 0):
+        L = lib()
+        _check(L.hydro_init(device))
+        _check(L.hydro_mem_allocate_fluid(flu_npg, gpu_nstream, SCHEME[scheme], PRECISION[precision]))
+        self.flu_npg, self.nstream, self.scheme, self.precision = flu_npg, gpu_nstream, scheme, precision
+        self.dtype = np.float64 if precision == "fp64" else np.float32
+        n = NX[scheme]
+        self._views = []
+        for aid in (0, 1):
+            pi, po, pf = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+            _check(L.hydro_host_buffers(aid, ctypes.byref(pi), ctypes.byref(po), ctypes.byref(pf)))
+            self._views.append((self._np(pi, (flu_npg, 5, n, n, n)), self._np(po, (flu_npg, 5, 16, 16, 16)),
+                                self._np(pf, (flu_npg, 9, 5, 16, 16)), pi, po, pf))
+
+    def _np(self, ptr, shape):
+        count = int(np.prod(shape))
+        buf = (ctypes.c_char * (count * np.dtype(self.dtype).itemsize)).from_address(ptr.value)
+        return np.frombuffer(buf, dtype=self.dtype).reshape(shape)
+
+    def host_arrays(self, array_id: int):
+        """(h_Flu_Array_In, h_Flu_Array_Out, h_Flux_Array) pinned numpy views of double buffer ``array_id``."""
+        return self._views[array_id][:3]
+
+    def asyn_fluid_solver(self, array_id: int, npg: int, dt, dh, gpu_nstream=None, **kw):
+        """CUAPI_Asyn_FluidSolver on double buffer ``array_id``; returns before the GPU finishes."""
+        v = self._views[array_id]
+        args = make_args(npg, dt, dh, scheme=self.scheme, precision=self.precision, **kw)
+        _check(lib().hydro_asyn_fluid_solver(v[3], v[4], v[5], ctypes.byref(args),
+                                             gpu_nstream if gpu_nstream is not None else self.nstream))
+
+    @staticmethod
+    def synchronize():
+        _check(lib().hydro_synchronize())
+
+    def free(self):
+        self._views = []
+        _check(lib().hydro_mem_free_fluid())

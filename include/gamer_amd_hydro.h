@@ -1,0 +1,96 @@
+/*
+ * gamer_amd_hydro.h — C ABI of the MI355X (gfx950) MUSCL-Hancock patch-group fluid solver.
+ *
+ * Drop-in boundary for GAMER's fluid-solver GPU API (src/GPU_API, include/Prototype.h:646-690).  Plain
+ * pointers, sizes and enums only.  Every function returns HYDRO_OK (0) or a negative HYDRO_E* code; the
+ * message of the last failure on the calling thread is available from hydro_last_error().
+ *
+ * Array layouts are those of the reference boundary (include/Macro.h:1197, x fastest):
+ *   in   [npg][5][N][N][N]      N = 22 (MHM_RP) or 20 (MHM); fields rho, mx, my, mz, E
+ *   out  [npg][5][16][16][16]
+ *   flux [npg][9][5][16][16]    x planes at 0/8/16, then y, then z (CPU_Shared_ComputeFlux.cpp:390-450)
+ * Element type is double for HYDRO_FP64 and float for HYDRO_FP32_BULK.
+ */
+#ifndef GAMER_AMD_HYDRO_H
+#define GAMER_AMD_HYDRO_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+   HYDRO_OK            =  0,
+   HYDRO_E_ARG         = -1,   /* invalid argument (null pointer, bad enum, npg out of range) */
+   HYDRO_E_STATE       = -2,   /* call order: not initialised / buffers not allocated */
+   HYDRO_E_DEVICE      = -3,   /* HIP runtime error */
+   HYDRO_E_NOMEM       = -4    /* device or pinned allocation failed (caller may retry with fewer streams) */
+};
+
+/* scheme ids = include/Macro.h:74-75; Riemann solvers = Macro.h:86-88; limiters = include/Typedef.h:188-193 */
+enum { HYDRO_MHM = 3, HYDRO_MHM_RP = 4 };
+enum { HYDRO_ROE = 2, HYDRO_HLLE = 3, HYDRO_HLLC = 4 };
+enum { HYDRO_LIM_VANLEER = 1, HYDRO_LIM_GMINMOD = 2, HYDRO_LIM_ALBADA = 3, HYDRO_LIM_VL_GMINMOD = 4,
+       HYDRO_LIM_CENTRAL = 6 };
+/* HYDRO_FP64: double storage and arithmetic.  HYDRO_FP32_BULK: float storage and bulk arithmetic, with
+   pressure recovery (E - |m|^2/2rho) and the half/full-step conserved accumulations in double. */
+enum { HYDRO_FP64 = 0, HYDRO_FP32_BULK = 1 };
+
+/* Arguments of one solver call (the scalar arguments of CUAPI_Asyn_FluidSolver, src/GPU_API/
+   CUAPI_Asyn_FluidSolver.cu:266-288, restricted to non-MHD hydro without passive scalars). */
+typedef struct HydroSolveArgs {
+   const void *in;         /* device pointer, layout above */
+   void       *out;        /* device pointer */
+   void       *flux;       /* device pointer, may be NULL when store_flux == 0 */
+   int    npg;             /* NPatchGroup */
+   double dt, dh;
+   double gamma;           /* EOS_GAMMA adiabatic index */
+   int    scheme;          /* HYDRO_MHM_RP or HYDRO_MHM */
+   int    rsolver;         /* HYDRO_HLLC / HYDRO_HLLE / HYDRO_ROE */
+   int    limiter;         /* LR_Limiter */
+   double minmod_coeff;    /* MinMod_Coeff */
+   int    minmod_maxiter;  /* MinMod_MaxIter (>= 0) */
+   double min_dens, min_pres, min_eint;
+   int    store_flux;      /* StoreFlux */
+   int    precision;       /* HYDRO_FP64 or HYDRO_FP32_BULK */
+} HydroSolveArgs;
+
+/* Select the HIP device (replaces CUAPI_SetDevice, src/GPU_API/CUAPI_SetDevice.cu). */
+int hydro_init( int device );
+
+/* Enqueue one solver call on `stream` (a hipStream_t, NULL = default stream) over device-resident arrays.
+   Returns after the launch; results are valid once the stream is synchronised.  Replaces the kernel launch
+   inside CUAPI_Asyn_FluidSolver (src/GPU_API/CUAPI_Asyn_FluidSolver.cu:557-576). */
+int hydro_mhm_solve_async( const HydroSolveArgs *args, void *stream );
+
+/* Wait for all work queued by this library (replaces CUAPI_Synchronize, src/GPU_API/CUAPI_Synchronize.cu:11-14). */
+int hydro_synchronize( void );
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char *hydro_last_error( void );
+
+/* ---- GAMER GPU_API-shaped entry points over host arrays (pinned double buffers + N streams) ---- */
+
+/* Allocate device in/out/flux buffers for flu_npg patch groups and nstream HIP streams, and pinned host
+   double buffers [2] (replaces CUAPI_MemAllocate_Fluid, src/GPU_API/CUAPI_MemAllocate_Fluid.cu:78-347).
+   Returns HYDRO_E_NOMEM on allocation failure so the caller can retry with fewer streams
+   (src/GPU_API/CUAPI_MemAllocate.cu:41-96). */
+int hydro_mem_allocate_fluid( int flu_npg, int nstream, int scheme, int precision );
+
+/* Free everything allocated by hydro_mem_allocate_fluid (replaces CUAPI_MemFree_Fluid). */
+int hydro_mem_free_fluid( void );
+
+/* Pinned host buffers [array_id] for the input, output and flux arrays (the h_Flu_Array_In/Out and
+   h_Flux_Array double buffers the caller fills and reads between calls). */
+int hydro_host_buffers( int array_id, void **h_in, void **h_out, void **h_flux );
+
+/* Asynchronous solve over host arrays: per stream H2D copy, kernel, D2H copy of out (and flux), the
+   NPG/NStream split with the remainder on the last stream (src/GPU_API/CUAPI_Asyn_FluidSolver.cu:433-722).
+   Host arrays should be the pinned buffers from hydro_host_buffers.  Returns before the GPU finishes. */
+int hydro_asyn_fluid_solver( const void *h_in, void *h_out, void *h_flux, const HydroSolveArgs *args,
+                             int gpu_nstream );
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GAMER_AMD_HYDRO_H */
