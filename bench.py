@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 MHM_RP patch-group solver on one MI355X.
+
+Workload (the reference's GPU configuration): one solver call on NPG = 4096 patch groups (16^3 PGs of a
+periodic 256^3 grid = 16.8M cells), MHM_RP + PLM (VL_GMINMOD, theta = 1.5) + HLLC, gamma = 5/3, CFL 0.5,
+StoreFlux on; seeded lognormal density/pressure, Gaussian velocities (0.3 c_s), a Sod plane through half of
+the patch groups and a x1e3 pressure spike in one cell of every 16th patch group.  Inputs are made
+resident in HBM before timing.  One "step" = one solver launch over all 4096 patch groups.
+
+The headline value is the FP32-bulk path (float storage and arithmetic, FP64 pressure recovery and
+conserved accumulations); the FP64 path is measured the same way and reported under "fp64".
+A bounded sample of the same workload is run through the reference's own CPU_FluidSolver (compiled from
+the reference sources into oracle/_ref) on the host cores as "cpu_baseline".
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0           # MI355X HBM3E
+CELLS_PER_PG = 16 ** 3
+
+
+def algorithmic_bytes_per_pg(elem: int, nx: int = 22, store_flux: bool = True) -> int:
+    """Each input value read once, each output (and flux) value written once."""
+    return elem * (5 * nx ** 3 + 5 * CELLS_PER_PG + (9 * 5 * 256 if store_flux else 0))
+
+
+def cut_on_device(cons, ghost, torch, dev, dtype):
+    """Periodic grid (5, Z, Y, X) -> (npg, 5, N, N, N) patch-group blocks on the GPU, PG index x-fastest."""
+    g = torch.from_numpy(cons).to(dev, dtype=dtype).unsqueeze(0)
+    g = torch.nn.functional.pad(g, (ghost,) * 6, mode="circular")[0]
+    n = 16 + 2 * ghost
+    blk = g.unfold(1, n, 16).unfold(2, n, 16).unfold(3, n, 16)        # (5, gz, gy, gx, n, n, n)
+    return blk.permute(1, 2, 3, 0, 4, 5, 6).reshape(-1, 5, n, n, n).contiguous()
+
+
+def time_path(torch, hydro, u, w, steps, warmup):
+    """Warm-up, then `steps` back-to-back launches bracketed by device syncs; kernel time from HIP events
+    recorded on the launch stream (torch's current stream)."""
+    out = torch.empty((u.shape[0], 5, 16, 16, 16), dtype=u.dtype, device=u.device)
+    flux = torch.empty((u.shape[0], 9, 5, 16, 16), dtype=u.dtype, device=u.device)
+    for _ in range(warmup):
+        hydro.fluid_solver(u, w["dt"], w["dh"], out=out, flux=flux)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(steps):
+        hydro.fluid_solver(u, w["dt"], w["dh"], out=out, flux=flux)
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = ev0.elapsed_time(ev1) / steps
+    return wall, kernel_ms, out, flux
+
+
+def cpu_baseline(u64_host, w, sample_pg, prec):
+    """GAMER's own CPU_FluidSolver_MHM (oracle/_ref) on the first `sample_pg` patch groups, all host threads."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    kind = "reference" if oracle.ref_available("MHM_RP", "HLLC", prec) else "port"
+    u = u64_host[:sample_pg]
+    u = u if prec == "f64" else u.astype(np.float32)
+    fn = oracle.ref_fluid_solver if kind == "reference" else oracle.fluid_solver
+    fn(u[:min(8, sample_pg)], w["dt"], w["dh"], nthreads=nthreads)            # warm caches / threads
+    t0 = time.perf_counter()
+    fn(u, w["dt"], w["dh"], nthreads=nthreads)
+    dt = time.perf_counter() - t0
+    return {"value": sample_pg * CELLS_PER_PG / dt, "unit": "cell-updates/s", "cores": nthreads, "kind": kind,
+            "seconds": round(dt, 3),
+            "sample": f"first {sample_pg} of the 4096 patch groups, one MHM_RP/HLLC step, "
+                      f"{'FP64' if prec == 'f64' else 'FP32'} build, OpenMP dynamic,1"}
+
+
+def load_traffic(name):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get(name)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--npg-axis", type=int, default=16, help="patch groups per axis (16 -> NPG 4096)")
+    ap.add_argument("--cpu-sample", type=int, default=1024, help="patch groups in the CPU baseline sample (0: skip)")
+    ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (3x), to scale rocprof byte counters")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); the solver's "
+              f"multi-GPU sharding is a design note, not a run.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from gamer_amd import hydro, ic
+    hydro.lib()                                      # fails loudly if the gfx950 library is missing
+    dev = torch.device("cuda:0")
+    if args.copy_calibration_mib > 0:
+        a = torch.ones(args.copy_calibration_mib * 2 ** 18, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+    n = args.npg_axis
+    print(f"[bench] generating workload {n}^3 patch groups", file=sys.stderr, flush=True)
+    cons, dt, dh = ic.random_sod_grid((n, n, n), seed=20260, sedov=True)
+    w = {"dt": dt, "dh": dh}
+    npg = n ** 3
+    cells = npg * CELLS_PER_PG
+
+    results = {}
+    for prec, tdt, elem in (("fp32", torch.float32, 4), ("fp64", torch.float64, 8)):
+        if prec == "fp64" and args.skip_fp64:
+            continue
+        u = cut_on_device(cons, 3, torch, dev, tdt)
+        torch.cuda.synchronize()
+        print(f"[bench] {prec}: {args.warmup} warm-up + {args.steps} timed steps", file=sys.stderr, flush=True)
+        wall, kms, out, flux = time_path(torch, hydro, u, w, args.steps, args.warmup)
+        bpl = algorithmic_bytes_per_pg(elem) * npg
+        achieved = bpl / (kms * 1e-3) / 1e9
+        results[prec] = {
+            "value": cells * args.steps / wall, "ms_per_step": wall / args.steps * 1e3, "kernel_ms": kms,
+            "kernel_cell_updates_per_s": cells / (kms * 1e-3),
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(prec),
+                         "algorithmic_bytes_per_launch": bpl},
+        }
+        if prec == "fp64":
+            u64_host = u.cpu().numpy()
+        del u, out, flux
+        torch.cuda.empty_cache()
+
+    cpu = None
+    if args.cpu_sample > 0:
+        if "fp64" not in results:
+            u64_host = cut_on_device(cons, 3, torch, dev, torch.float64).cpu().numpy()
+        print(f"[bench] CPU baseline on {args.cpu_sample} patch groups", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(u64_host, w, args.cpu_sample, "f32")
+        cpu64 = cpu_baseline(u64_host, w, args.cpu_sample, "f64")
+        cpu["value_fp64_build"] = cpu64["value"]
+
+    r32 = results["fp32"]
+    line = {
+        "metric": "hydro cell-updates/s @4096 PG: FP32 vs FP64 GPU, %HBM peak; CPU host baseline",
+        "value": r32["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded lognormal medium + Sod planes + x1e3 pressure spikes, periodic ghost zones)",
+        "config": {"workload": "MHM_RP+PLM(VL_GMINMOD)+HLLC, gamma 5/3, CFL 0.5, StoreFlux, one step of "
+                               f"NPG={npg} patch groups (16^3 cells + 3 ghosts each, {cells} cells)",
+                   "npg": npg, "precision": "FP32 bulk, FP64 pressure recovery and accumulations",
+                   "parallelism": "1 GPU"},
+        "roofline": r32["roofline"],
+        "kernel_cell_updates_per_s": r32["kernel_cell_updates_per_s"],
+    }
+    if "fp64" in results:
+        r64 = results["fp64"]
+        line["fp64"] = {"value": r64["value"], "ms_per_step": r64["ms_per_step"],
+                        "kernel_cell_updates_per_s": r64["kernel_cell_updates_per_s"], "roofline": r64["roofline"]}
+        line["fp32_over_fp64"] = r32["kernel_cell_updates_per_s"] / r64["kernel_cell_updates_per_s"]
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -173,35 +173,36 @@ __device__ __forceinline__ void rs_roe( const St<R> &L, const St<R> &Q, const Pr
 
 // ---------------- PLM reconstruction of one cell along one direction ----------------
 // Limited slope (DataReconstruction.cpp:1917-2032), faces C -/+ slope/2, then clamp each face between the
-// cell and its neighbour and mirror the other face about the cell value (:415-430).
-template <typename R>
-__device__ __forceinline__ void plm_face( R L, R C, R Rr, int lim, R coeff, R &fL, R &fR )
+// cell and its neighbour and mirror the other face about the cell value (:415-430).  LIM is the limiter id
+// (include/Typedef.h:188-193) when known at compile time, or 0 to dispatch on `lim` at run time.
+template <int LIM, typename R>
+__device__ __forceinline__ R plm_slope( R sl, R sr, R lim_rt, int lim, R coeff )
 {
    const R Z = (R)0;
-   const R sl = C - L, sr = Rr - C, sc = (R)0.5*( sl + sr ), lr = sl*sr;
+   const R sc = (R)0.5*( sl + sr ), lr = sl*sr;
+   const int L = ( LIM != 0 ) ? LIM : lim;
    R s;
-   switch ( lim )
+   if ( L == 6 )      s = sc;                                                   // central
+   else if ( L == 1 ) s = (R)2.0*lr*rcp( sl + sr );                             // van Leer
+   else if ( L == 3 ) s = lr*( sl + sr )*rcp( sl*sl + sr*sr );                  // van Albada
+   else
    {
-      case 6:  s = sc;  break;                                           // central
-      case 1:  s = (R)2.0*lr*rcp( sl + sr );  break;                     // van Leer
-      case 2:  s = vmin( vabs( sc ), vmin( vabs( sl*coeff ), vabs( sr*coeff ) ) );
-               s = ( sc < Z ) ? -s : s;  break;                          // generalised MinMod
-      case 3:  s = lr*( sl + sr )*rcp( sl*sl + sr*sr );  break;          // van Albada
-      default: {                                                         // van Leer + generalised MinMod
-         const R sa = (R)2.0*sl*sr*rcp( sl + sr );
-         s = vmin( vabs( sa ), vmin( vabs( sc ), vmin( vabs( sl*coeff ), vabs( sr*coeff ) ) ) );
-         s = ( sc < Z ) ? -s : s;
-      } break;
+      R m = vmin( vabs( sc ), vmin( vabs( sl*coeff ), vabs( sr*coeff ) ) );     // generalised MinMod
+      if ( L == 4 ) m = vmin( vabs( (R)2.0*sl*sr*rcp( sl + sr ) ), m );          // + van Leer
+      s = ( sc < Z ) ? -m : m;
    }
-   s = ( lr > Z ) ? s : Z;
-   R fl = C - (R)0.5*s, fr = C + (R)0.5*s;
-   R lo = ( C < L ) ? C : L, hi = ( C > L ) ? C : L;
-   fl = ( fl > lo ) ? fl : lo;
-   fl = ( fl < hi ) ? fl : hi;
-   fr = (R)2.0*C - fl;
-   lo = ( C < Rr ) ? C : Rr;  hi = ( C > Rr ) ? C : Rr;
-   fr = ( fr > lo ) ? fr : lo;
-   fr = ( fr < hi ) ? fr : hi;
+   (void)lim_rt;
+   return ( lr > Z ) ? s : Z;
+}
+
+template <int LIM, typename R>
+__device__ __forceinline__ void plm_face( R L, R C, R Rr, int lim, R coeff, R &fL, R &fR )
+{
+   const R s = plm_slope<LIM>( C - L, Rr - C, (R)0, lim, coeff );
+   R fl = C - (R)0.5*s;
+   fl = vmin( vmax( fl, vmin( C, L ) ), vmax( C, L ) );
+   R fr = (R)2.0*C - fl;
+   fr = vmin( vmax( fr, vmin( C, Rr ) ), vmax( C, Rr ) );
    fL = (R)2.0*C - fr;
    fR = fr;
 }

@@ -31,14 +31,13 @@ constexpr int NX  = 22;          // FLU_NXT for MHM_RP
 constexpr int NC  = NX*NX;       // columns per plane
 constexpr int PS  = 16;          // PS2
 constexpr int NT  = 256;         // lanes per workgroup
+#ifndef WPE32
+#define WPE32 2                  // target waves per SIMD (register budget) of the FP32-bulk kernels
+#endif
+#ifndef WPE64
+#define WPE64 1                  // ... and of the FP64 kernels
+#endif
 
-template <typename R>
-struct SmemRP {
-   R a[10][NC];   // input plane (rho,mx,my,mz,E,P,1/rho,cs) | upper x faces (0-4) and upper y faces (5-9)
-   R b[NV][NC];   // half-step x or y fluxes | full-step x or y fluxes
-   R w[NV][NC];   // half-step primitive plane read by the reconstruction
-   int fail;
-};
 
 // ghost columns served by lane t besides its output column; -1 = none
 __device__ __forceinline__ void ring2( int k, int &i, int &j )
@@ -99,9 +98,10 @@ template <typename R, typename IO>
 __device__ __forceinline__ void derive_cell( const IO u[NV], R q[8], const Prm<R> &P )
 {
    const double d = (double)u[0], mx = (double)u[1], my = (double)u[2], mz = (double)u[3], e = (double)u[4];
-   const double eint = e - 0.5*( mx*mx + my*my + mz*mz ) / d;
+   const double eint = e - 0.5*( mx*mx + my*my + mz*mz )*rcp( d );
    const R p = floor_pres( (R)( eint*P.gm1_d ), P.min_pres );
    const R ir = rcp( (R)u[0] );
+#  pragma unroll
    for (int v=0; v<NV; v++) q[v] = (R)u[v];
    q[5] = p;
    q[6] = ir;
@@ -118,45 +118,40 @@ __device__ __forceinline__ void half_cell( const R u[NV], const R dfxy[NV], cons
    for (int v=0; v<NV; v++)
       c[v] = (double)u[v] - P.dt_dh2*( (double)dfxy[v] + ( (double)hzu[v] - (double)hzl[v] ) );
    c[0] = __builtin_fmax( c[0], P.min_dens_d );
-   {
-      const double ein  = c[4] - 0.5*( c[1]*c[1] + c[2]*c[2] + c[3]*c[3] ) / c[0];
-      const double eout = ( ein == ein ) ? __builtin_fmax( ein, P.min_eint_d ) : ein;
-      c[4] = ( ein == eout ) ? c[4] : c[4] - ein + eout;
-   }
-   const double r = 1.0 / c[0];
+   const double r  = rcp( c[0] );
+   const double ke = 0.5*( c[1]*c[1] + c[2]*c[2] + c[3]*c[3] )*r;
+   double ein = c[4] - ke;
+   const double eout = ( ein == ein ) ? __builtin_fmax( ein, P.min_eint_d ) : ein;
+   if ( ein != eout ) { c[4] = c[4] - ein + eout;  ein = c[4] - ke; }
    w[0] = (R)c[0];
    w[1] = (R)( c[1]*r );
    w[2] = (R)( c[2]*r );
    w[3] = (R)( c[3]*r );
-   const double p = ( c[4] - 0.5*( c[1]*c[1] + c[2]*c[2] + c[3]*c[3] ) / c[0] )*P.gm1_d;
+   const double p = ein*P.gm1_d;
    w[4] = (R)( ( p == p ) ? __builtin_fmax( p, P.min_pres_d ) : p );
 }
 
 template <typename IO> struct IOLim;
 template <> struct IOLim<double> { static constexpr double huge = 1.7976931348623157e308; static constexpr double eps = 2.2204460492503131e-16; };
-template <> struct IOLim<float>  { static constexpr double huge = 3.40282346638528859812e+38; static constexpr double eps = 1.1920928955078125e-07; };
+template <> struct IOLim<float>  { static constexpr float  huge = 3.40282346638528859812e+38f; static constexpr float eps = 1.1920928955078125e-07f; };
 
-// full-step update of one output cell in FP64 + the CONS-mode unphysical check (FullStepUpdate.cpp:106-235)
+// full-step update of one output cell, accumulated in FP64, + the CONS-mode unphysical check on the stored
+// values (FullStepUpdate.cpp:106-235, FluUtility.cpp:843-940)
 template <typename R, typename IO>
-__device__ __forceinline__ bool full_cell( const R u[NV], const R dfxy[NV], const R fzl[NV], const R fzu[NV],
-                                           const Prm<R> &P, IO out[NV] )
+__device__ __forceinline__ bool full_cell( const IO u[NV], const R dfxy[NV], const R fzl[NV], const R fzu[NV],
+                                           const Prm<R> &P, IO o[NV] )
 {
-   double c[NV];
 #  pragma unroll
    for (int v=0; v<NV; v++)
-   {
-      c[v] = (double)u[v] - P.dt_dh*( (double)dfxy[v] + ( (double)fzu[v] - (double)fzl[v] ) );
-      out[v] = (IO)c[v];
-   }
-   double o[NV];
-   for (int v=0; v<NV; v++) o[v] = (double)out[v];
-   const double huge = IOLim<IO>::huge;
+      o[v] = (IO)( (double)u[v] - P.dt_dh*( (double)dfxy[v] + ( (double)fzu[v] - (double)fzl[v] ) ) );
+   const IO huge = IOLim<IO>::huge, Z = (IO)0;
    bool bad = false;
+#  pragma unroll
    for (int v=0; v<NV; v++) bad |= ( o[v] != o[v] );
-   for (int v=1; v<=3; v++) bad |= ( o[v] < -huge || o[v] > huge );
-   bad |= ( o[0] <= 0.0 || o[0] > huge || o[4] <= 0.0 || o[4] > huge );
-   const double ei = o[4] - 0.5*( o[1]*o[1] + o[2]*o[2] + o[3]*o[3] ) / o[0];
-   bad |= ( ei < -3.0*o[4]*IOLim<IO>::eps || ei > huge || ei != ei );
+   bad |= ( vabs( o[1] ) > huge || vabs( o[2] ) > huge || vabs( o[3] ) > huge );
+   bad |= ( o[0] <= Z || o[0] > huge || o[4] <= Z || o[4] > huge );
+   const IO ei = o[4] - (IO)0.5*( o[1]*o[1] + o[2]*o[2] + o[3]*o[3] )*rcp( o[0] );
+   bad |= ( ei < (IO)-3.0*o[4]*IOLim<IO>::eps || ei > huge || ei != ei );
    return bad;
 }
 
@@ -173,12 +168,31 @@ __device__ __forceinline__ St<R> st_of( const R q[8] ) { return cell_state<D>( q
 template <int D, typename R>
 __device__ __forceinline__ St<R> st_of( const R (*a)[NC], int c ) { R q[8]; ld_cell( a, c, q ); return st_of<D>( q ); }
 
-template <typename R, typename IO, int RS>
-__global__ void __launch_bounds__( NT )
+template <int D, typename R>
+__device__ __forceinline__ St<R> st_face( const R (*a)[NC], int v0, int c, const Prm<R> &P )
+{
+   R w[NV];
+#  pragma unroll
+   for (int v=0; v<NV; v++) w[v] = a[v0 + v][c];
+   return face_state<D>( w, P );
+}
+
+// LDS of one workgroup.  XY2: x and y fluxes have separate buffers (one barrier per flux stage instead of three).
+template <typename R, bool XY2>
+struct SmemRP {
+   R a[10][NC];                 // input plane (rho,mx,my,mz,E,P,1/rho,c_s) | upper x (0-4) and y (5-9) face states
+   R b[XY2 ? 2*NV : NV][NC];    // half-step or full-step x (0-4) [and y (5-9)] fluxes
+   R w[NV][NC];                 // half-step primitive plane read by the reconstruction
+   int fail;
+};
+
+template <typename R, typename IO, int RS, int LIM, bool XY2>
+__global__ void __launch_bounds__( NT ) __attribute__(( amdgpu_waves_per_eu( sizeof(R) == 4 ? WPE32 : WPE64 ) ))
 mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict__ g_flux,
                int npg, int store_flux, int minmod_maxiter, R minmod_coeff, Prm<R> P )
 {
-   __shared__ SmemRP<R> S;
+   __shared__ SmemRP<R, XY2> S;
+   constexpr int YB = XY2 ? NV : 0;         // offset of the y fluxes in S.b
 
    const int pg  = blockIdx.x;
    const int tid = threadIdx.x;
@@ -192,6 +206,7 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
    int oi, oj;
    const bool has_o = outer_column( tid, oi, oj );
    const Col O = make_col( has_o ? oi : 0, has_o ? oj : 0 );
+   const int  lim = P.limiter;
 
    for (int iter = 0; ; iter++)
    {
@@ -202,32 +217,54 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
       }
       if ( tid == 0 ) S.fail = 0;
 
-      // lane state carried along z (core column: K, outer column: O)
-      R kU[8], kU1[8], kU2[NV], kU3[NV];         // input planes t, t-1, t-2, t-3
-      R kHz[NV], kDh[NV];                         // half-step z-flux at t-3/2 ; dFx+dFy of the half step at t-1
-      R kW1[NV], kW2[NV], kW3[NV];                // half-step primitives at t-1, t-2, t-3
-      R kFzR[NV], kFz[NV], kDf[NV];               // upper z face of plane r-1 ; full z-flux at r-3/2 ; full dFx+dFy at r-1
+      // lane state carried along z (core column K, outer column O)
+      R kU[8], kU1[8];                            // input cell of planes t and t-1 (+ pressure, 1/rho, c_s)
+      R kHz[NV], kDh[NV];                         // half-step z-flux at t-3/2 ; half-step dFx+dFy of plane t-1
+      R kW1[NV], kW2[NV], kW3[NV];                // half-step primitives of planes t-1, t-2, t-3
+      R kFzR[NV], kFz[NV], kDf[NV];               // upper z face of plane r-1 ; full z-flux at r-3/2 ; full dFx+dFy of plane r-1
       R oU[8], oU1[8], oHz[NV], oDh[NV], oW[NV];  // outer column
+
+      // prefetch of the next input plane (two cells per lane, coalesced)
+      IO pf[2][NV];
+      auto issue = [&]( int t )
+      {
+#        pragma unroll
+         for (int k=0; k<2; k++)
+         {
+            const int c = tid + k*NT;
+            if ( c < NC )
+#              pragma unroll
+               for (int v=0; v<NV; v++) pf[k][v] = in[ (size_t)v*NC*NX + (size_t)t*NC + c ];
+         }
+      };
+      issue( 0 );
 
       for (int t = 0; t < NX; t++)
       {
-         // ---- A: input plane t -> LDS (coalesced), with pressure/1/rho/c_s evaluated once per cell
+         // ---- A: input plane t -> LDS, pressure/1/rho/c_s once per cell; then prefetch plane t+1
          __syncthreads();
-         for (int c = tid; c < NC; c += NT)
+#        pragma unroll
+         for (int k=0; k<2; k++)
          {
-            IO u[NV];
-#           pragma unroll
-            for (int v=0; v<NV; v++) u[v] = in[ (size_t)v*NC*NX + (size_t)t*NC + c ];
-            R q[8];
-            derive_cell( u, q, Q );
-#           pragma unroll
-            for (int v=0; v<8; v++) S.a[v][c] = q[v];
+            const int c = tid + k*NT;
+            if ( c < NC )
+            {
+               R q[8];
+               derive_cell( pf[k], q, Q );
+#              pragma unroll
+               for (int v=0; v<8; v++) S.a[v][c] = q[v];
+            }
          }
+         if ( t + 1 < NX ) issue( t + 1 );
+         const int o = t - 3;                    // output plane of this step
+         IO uo[NV];
+         if ( o >= 3 && o <= 18 )
+#           pragma unroll
+            for (int v=0; v<NV; v++) uo[v] = in[ (size_t)v*NC*NX + (size_t)o*NC + K.c ];
          __syncthreads();
 
-         // shift the per-column input history and pick up plane t
-         for (int v=0; v<NV; v++) { kU3[v] = kU2[v];  kU2[v] = kU1[v]; }
-         for (int v=0; v<8;  v++) { kU1[v] = kU[v];   oU1[v] = oU[v]; }
+#        pragma unroll
+         for (int v=0; v<8; v++) { kU1[v] = kU[v];  oU1[v] = oU[v]; }
          ld_cell( S.a, K.c, kU );
          if ( has_o ) ld_cell( S.a, O.c, oU );
 
@@ -239,14 +276,17 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
             if ( t >= 2 )
             {
                half_cell( kU1, kDh, kHz, fz, Q, w );
+#              pragma unroll
                for (int v=0; v<NV; v++) { kW3[v] = kW2[v];  kW2[v] = kW1[v];  kW1[v] = w[v]; }
             }
+#           pragma unroll
             for (int v=0; v<NV; v++) kHz[v] = fz[v];
 
             if ( has_o && O.w )
             {
                riemann<RS, 2>( st_of<2>( oU1 ), st_of<2>( oU ), Q, fz );
                if ( t >= 2 ) half_cell( oU1, oDh, oHz, fz, Q, oW );
+#              pragma unroll
                for (int v=0; v<NV; v++) oHz[v] = fz[v];
             }
          }
@@ -254,69 +294,94 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
          // ---- half-step x and y fluxes of plane t (low face owned by each column)
          if ( t >= 1 && t <= 20 )
          {
-            R fk[NV], fo[NV];
-            riemann<RS, 0>( st_of<0>( S.a, K.c - 1 ), st_of<0>( kU ), Q, fk );
-            for (int v=0; v<NV; v++) S.b[v][K.c] = fk[v];
-            if ( has_o && O.hx )
+            R fkx[NV], fox[NV], fky[NV], foy[NV];
+            riemann<RS, 0>( st_of<0>( S.a, K.c - 1 ), st_of<0>( kU ), Q, fkx );
+            if ( has_o && O.hx ) riemann<RS, 0>( st_of<0>( S.a, O.c - 1 ), st_of<0>( oU ), Q, fox );
+            if ( XY2 )
             {
-               riemann<RS, 0>( st_of<0>( S.a, O.c - 1 ), st_of<0>( oU ), Q, fo );
-               for (int v=0; v<NV; v++) S.b[v][O.c] = fo[v];
+               riemann<RS, 1>( st_of<1>( S.a, K.c - NX ), st_of<1>( kU ), Q, fky );
+               if ( has_o && O.hy ) riemann<RS, 1>( st_of<1>( S.a, O.c - NX ), st_of<1>( oU ), Q, foy );
+            }
+#           pragma unroll
+            for (int v=0; v<NV; v++)
+            {
+               S.b[v][K.c] = fkx[v];
+               if ( has_o && O.hx ) S.b[v][O.c] = fox[v];
+               if ( XY2 )
+               {
+                  S.b[YB + v][K.c] = fky[v];
+                  if ( has_o && O.hy ) S.b[YB + v][O.c] = foy[v];
+               }
             }
             __syncthreads();
-            for (int v=0; v<NV; v++) kDh[v] = S.b[v][K.c + 1] - fk[v];
-            if ( has_o && O.w ) for (int v=0; v<NV; v++) oDh[v] = S.b[v][O.c + 1] - fo[v];
-            __syncthreads();
-
-            riemann<RS, 1>( st_of<1>( S.a, K.c - NX ), st_of<1>( kU ), Q, fk );
-            for (int v=0; v<NV; v++) S.b[v][K.c] = fk[v];
-            if ( has_o && O.hy )
+#           pragma unroll
+            for (int v=0; v<NV; v++)
             {
-               riemann<RS, 1>( st_of<1>( S.a, O.c - NX ), st_of<1>( oU ), Q, fo );
-               for (int v=0; v<NV; v++) S.b[v][O.c] = fo[v];
+               kDh[v] = S.b[v][K.c + 1] - fkx[v];
+               if ( has_o && O.w ) oDh[v] = S.b[v][O.c + 1] - fox[v];
             }
-            __syncthreads();
-            for (int v=0; v<NV; v++) kDh[v] += S.b[v][K.c + NX] - fk[v];
-            if ( has_o && O.w ) for (int v=0; v<NV; v++) oDh[v] += S.b[v][O.c + NX] - fo[v];
+            if ( !XY2 )
+            {
+               __syncthreads();
+               riemann<RS, 1>( st_of<1>( S.a, K.c - NX ), st_of<1>( kU ), Q, fky );
+               if ( has_o && O.hy ) riemann<RS, 1>( st_of<1>( S.a, O.c - NX ), st_of<1>( oU ), Q, foy );
+#              pragma unroll
+               for (int v=0; v<NV; v++)
+               {
+                  S.b[v][K.c] = fky[v];
+                  if ( has_o && O.hy ) S.b[v][O.c] = foy[v];
+               }
+               __syncthreads();
+            }
+#           pragma unroll
+            for (int v=0; v<NV; v++)
+            {
+               kDh[v] += S.b[YB + v][K.c + NX] - fky[v];
+               if ( has_o && O.w ) oDh[v] += S.b[YB + v][O.c + NX] - foy[v];
+            }
          }
 
-         // ---- PLM reconstruction of plane r = t-2 (faces of the x/y directions go to LDS, z stays in lanes)
+         // ---- PLM reconstruction of plane r = t-2 (x/y upper faces to LDS, z faces stay in the lane)
          const int r = t - 2;
          R kFL[3][NV], kFRz[NV], oFL[NV];
          if ( r >= 2 && r <= 19 )
          {
-            for (int v=0; v<NV; v++) plm_face( kW3[v], kW2[v], kW1[v], Q.limiter, Q.coeff, kFL[2][v], kFRz[v] );
+#           pragma unroll
+            for (int v=0; v<NV; v++) plm_face<LIM>( kW3[v], kW2[v], kW1[v], lim, Q.coeff, kFL[2][v], kFRz[v] );
             if ( r >= 3 && r <= 18 )
             {
+#              pragma unroll
                for (int v=0; v<NV; v++)
                {
                   R fr;
-                  plm_face( S.w[v][K.c - 1], kW2[v], S.w[v][K.c + 1], Q.limiter, Q.coeff, kFL[0][v], fr );
+                  plm_face<LIM>( S.w[v][K.c - 1], kW2[v], S.w[v][K.c + 1], lim, Q.coeff, kFL[0][v], fr );
                   S.a[v][K.c] = fr;
-                  plm_face( S.w[v][K.c - NX], kW2[v], S.w[v][K.c + NX], Q.limiter, Q.coeff, kFL[1][v], fr );
+                  plm_face<LIM>( S.w[v][K.c - NX], kW2[v], S.w[v][K.c + NX], lim, Q.coeff, kFL[1][v], fr );
                   S.a[NV + v][K.c] = fr;
                }
-               if ( has_o && O.rx )
+               if ( has_o && ( O.rx || O.ry ) )
+               {
+                  const int st = O.rx ? 1 : NX, vo = O.rx ? 0 : NV;
+#                 pragma unroll
                   for (int v=0; v<NV; v++)
                   {
                      R fr;
-                     plm_face( S.w[v][O.c - 1], S.w[v][O.c], S.w[v][O.c + 1], Q.limiter, Q.coeff, oFL[v], fr );
-                     S.a[v][O.c] = fr;
+                     plm_face<LIM>( S.w[v][O.c - st], S.w[v][O.c], S.w[v][O.c + st], lim, Q.coeff, oFL[v], fr );
+                     S.a[vo + v][O.c] = fr;
                   }
-               if ( has_o && O.ry )
-                  for (int v=0; v<NV; v++)
-                  {
-                     R fr;
-                     plm_face( S.w[v][O.c - NX], S.w[v][O.c], S.w[v][O.c + NX], Q.limiter, Q.coeff, oFL[v], fr );
-                     S.a[NV + v][O.c] = fr;
-                  }
+               }
             }
          }
          __syncthreads();
          // the half-step plane t-1 replaces plane t-2 in LDS for the next reconstruction
          if ( t >= 2 )
          {
-            for (int v=0; v<NV; v++) S.w[v][K.c] = kW1[v];
-            if ( has_o && O.w ) for (int v=0; v<NV; v++) S.w[v][O.c] = oW[v];
+#           pragma unroll
+            for (int v=0; v<NV; v++)
+            {
+               S.w[v][K.c] = kW1[v];
+               if ( has_o && O.w ) S.w[v][O.c] = oW[v];
+            }
          }
 
          // ---- full-step fluxes: z at r-1/2 (lane-local), x and y of plane r through LDS
@@ -325,71 +390,93 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
          {
             riemann<RS, 2>( face_state<2>( kFzR, Q ), face_state<2>( kFL[2], Q ), Q, fzn );
             if ( flx && ( r - 3 ) % 8 == 0 )
+#              pragma unroll
                for (int v=0; v<NV; v++)
                   flx[ ( ( 6 + ( r - 3 )/8 )*NV + v )*PS*PS + ( K.i - 3 ) + PS*( K.j - 3 ) ] = (IO)fzn[v];
          }
-         if ( r >= 2 ) for (int v=0; v<NV; v++) kFzR[v] = kFRz[v];
+         if ( r >= 2 )
+#           pragma unroll
+            for (int v=0; v<NV; v++) kFzR[v] = kFRz[v];
 
          R dfn[NV];
          if ( r >= 3 && r <= 18 )
          {
-            R fk[NV], fo[NV], wl[NV];
-            for (int v=0; v<NV; v++) wl[v] = S.a[v][K.c - 1];
-            riemann<RS, 0>( face_state<0>( wl, Q ), face_state<0>( kFL[0], Q ), Q, fk );
-            for (int v=0; v<NV; v++) S.b[v][K.c] = fk[v];
-            if ( has_o && O.fx )
+            R fkx[NV], fky[NV], fo[NV];
+            riemann<RS, 0>( st_face<0>( S.a, 0, K.c - 1, Q ), face_state<0>( kFL[0], Q ), Q, fkx );
+            if ( XY2 ) riemann<RS, 1>( st_face<1>( S.a, NV, K.c - NX, Q ), face_state<1>( kFL[1], Q ), Q, fky );
+            if ( has_o && O.fx ) riemann<RS, 0>( st_face<0>( S.a, 0,  O.c - 1,  Q ), face_state<0>( oFL, Q ), Q, fo );
+            if ( XY2 && has_o && O.fy ) riemann<RS, 1>( st_face<1>( S.a, NV, O.c - NX, Q ), face_state<1>( oFL, Q ), Q, fo );
+#           pragma unroll
+            for (int v=0; v<NV; v++)
             {
-               for (int v=0; v<NV; v++) wl[v] = S.a[v][O.c - 1];
-               riemann<RS, 0>( face_state<0>( wl, Q ), face_state<0>( oFL, Q ), Q, fo );
-               for (int v=0; v<NV; v++) S.b[v][O.c] = fo[v];
+               S.b[v][K.c] = fkx[v];
+               if ( XY2 ) S.b[YB + v][K.c] = fky[v];
+               if ( has_o && O.fx ) S.b[v][O.c] = fo[v];
+               if ( XY2 && has_o && O.fy ) S.b[YB + v][O.c] = fo[v];
             }
             if ( flx )
             {
                if ( ( K.i - 3 ) % 8 == 0 )
+#                 pragma unroll
                   for (int v=0; v<NV; v++)
-                     flx[ ( ( ( K.i - 3 )/8 )*NV + v )*PS*PS + ( K.j - 3 ) + PS*( r - 3 ) ] = (IO)fk[v];
+                     flx[ ( ( ( K.i - 3 )/8 )*NV + v )*PS*PS + ( K.j - 3 ) + PS*( r - 3 ) ] = (IO)fkx[v];
+               if ( XY2 && ( K.j - 3 ) % 8 == 0 )
+#                 pragma unroll
+                  for (int v=0; v<NV; v++)
+                     flx[ ( ( 3 + ( K.j - 3 )/8 )*NV + v )*PS*PS + ( K.i - 3 ) + PS*( r - 3 ) ] = (IO)fky[v];
                if ( has_o && O.fx )
-                  for (int v=0; v<NV; v++)
-                     flx[ ( 2*NV + v )*PS*PS + ( O.j - 3 ) + PS*( r - 3 ) ] = (IO)fo[v];
+#                 pragma unroll
+                  for (int v=0; v<NV; v++) flx[ ( 2*NV + v )*PS*PS + ( O.j - 3 ) + PS*( r - 3 ) ] = (IO)fo[v];
+               if ( XY2 && has_o && O.fy )
+#                 pragma unroll
+                  for (int v=0; v<NV; v++) flx[ ( 5*NV + v )*PS*PS + ( O.i - 3 ) + PS*( r - 3 ) ] = (IO)fo[v];
             }
             __syncthreads();
-            for (int v=0; v<NV; v++) dfn[v] = S.b[v][K.c + 1] - fk[v];
-            __syncthreads();
-
-            for (int v=0; v<NV; v++) wl[v] = S.a[NV + v][K.c - NX];
-            riemann<RS, 1>( face_state<1>( wl, Q ), face_state<1>( kFL[1], Q ), Q, fk );
-            for (int v=0; v<NV; v++) S.b[v][K.c] = fk[v];
-            if ( has_o && O.fy )
+#           pragma unroll
+            for (int v=0; v<NV; v++) dfn[v] = S.b[v][K.c + 1] - fkx[v];
+            if ( !XY2 )
             {
-               for (int v=0; v<NV; v++) wl[v] = S.a[NV + v][O.c - NX];
-               riemann<RS, 1>( face_state<1>( wl, Q ), face_state<1>( oFL, Q ), Q, fo );
-               for (int v=0; v<NV; v++) S.b[v][O.c] = fo[v];
+               __syncthreads();
+               riemann<RS, 1>( st_face<1>( S.a, NV, K.c - NX, Q ), face_state<1>( kFL[1], Q ), Q, fky );
+               if ( has_o && O.fy ) riemann<RS, 1>( st_face<1>( S.a, NV, O.c - NX, Q ), face_state<1>( oFL, Q ), Q, fo );
+#              pragma unroll
+               for (int v=0; v<NV; v++)
+               {
+                  S.b[v][K.c] = fky[v];
+                  if ( has_o && O.fy ) S.b[v][O.c] = fo[v];
+               }
+               if ( flx )
+               {
+                  if ( ( K.j - 3 ) % 8 == 0 )
+#                    pragma unroll
+                     for (int v=0; v<NV; v++)
+                        flx[ ( ( 3 + ( K.j - 3 )/8 )*NV + v )*PS*PS + ( K.i - 3 ) + PS*( r - 3 ) ] = (IO)fky[v];
+                  if ( has_o && O.fy )
+#                    pragma unroll
+                     for (int v=0; v<NV; v++) flx[ ( 5*NV + v )*PS*PS + ( O.i - 3 ) + PS*( r - 3 ) ] = (IO)fo[v];
+               }
+               __syncthreads();
             }
-            if ( flx )
-            {
-               if ( ( K.j - 3 ) % 8 == 0 )
-                  for (int v=0; v<NV; v++)
-                     flx[ ( ( 3 + ( K.j - 3 )/8 )*NV + v )*PS*PS + ( K.i - 3 ) + PS*( r - 3 ) ] = (IO)fk[v];
-               if ( has_o && O.fy )
-                  for (int v=0; v<NV; v++)
-                     flx[ ( 5*NV + v )*PS*PS + ( O.i - 3 ) + PS*( r - 3 ) ] = (IO)fo[v];
-            }
-            __syncthreads();
-            for (int v=0; v<NV; v++) dfn[v] += S.b[v][K.c + NX] - fk[v];
+#           pragma unroll
+            for (int v=0; v<NV; v++) dfn[v] += S.b[YB + v][K.c + NX] - fky[v];
          }
 
          // ---- full-step update and output of plane o = t-3
-         const int o = t - 3;
          if ( o >= 3 && o <= 18 )
          {
             IO res[NV];
-            const bool bad = full_cell( kU3, kDf, kFz, fzn, Q, res );
+            const bool bad = full_cell( uo, kDf, kFz, fzn, Q, res );
             const size_t co = (size_t)( K.i - 3 ) + PS*( K.j - 3 ) + (size_t)PS*PS*( o - 3 );
+#           pragma unroll
             for (int v=0; v<NV; v++) out[ (size_t)v*PS*PS*PS + co ] = res[v];
             if ( bad ) S.fail = 1;
          }
-         if ( r >= 3 && r <= 19 ) for (int v=0; v<NV; v++) kFz[v] = fzn[v];
-         if ( r >= 3 && r <= 18 ) for (int v=0; v<NV; v++) kDf[v] = dfn[v];
+         if ( r >= 3 && r <= 19 )
+#           pragma unroll
+            for (int v=0; v<NV; v++) kFz[v] = fzn[v];
+         if ( r >= 3 && r <= 18 )
+#           pragma unroll
+            for (int v=0; v<NV; v++) kDf[v] = dfn[v];
       } // for t
 
       __syncthreads();
@@ -401,17 +488,28 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
 
 } // namespace
 
+template <typename R, typename IO, int RS, bool XY2>
+static void launch_rs( const IO *in, IO *out, IO *flux, int npg, int store_flux, int minmod_maxiter,
+                       double minmod_coeff, const Prm<R> &P, hipStream_t stream )
+{
+   const dim3 grid( npg ), block( NT );
+   if ( P.limiter == 4 )
+      hipLaunchKernelGGL( ( mhm_rp_kernel<R, IO, RS, 4, XY2> ), grid, block, 0, stream, in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P );
+   else
+      hipLaunchKernelGGL( ( mhm_rp_kernel<R, IO, RS, 0, XY2> ), grid, block, 0, stream, in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P );
+}
+
 template <typename R, typename IO>
 int launch_mhm_rp( const IO *in, IO *out, IO *flux, int npg, int rsolver, int store_flux, int minmod_maxiter,
                    double minmod_coeff, const Prm<R> &P, hipStream_t stream )
 {
    if ( npg <= 0 ) return 0;
-   const dim3 grid( npg ), block( NT );
+   constexpr bool XY2 = true;                  // separate x and y flux buffers (one barrier per flux stage)
    switch ( rsolver )
    {
-      case 4: hipLaunchKernelGGL( ( mhm_rp_kernel<R, IO, 4> ), grid, block, 0, stream, in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P ); break;
-      case 3: hipLaunchKernelGGL( ( mhm_rp_kernel<R, IO, 3> ), grid, block, 0, stream, in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P ); break;
-      case 2: hipLaunchKernelGGL( ( mhm_rp_kernel<R, IO, 2> ), grid, block, 0, stream, in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P ); break;
+      case 4: launch_rs<R, IO, 4, XY2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, stream ); break;
+      case 3: launch_rs<R, IO, 3, XY2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, stream ); break;
+      case 2: launch_rs<R, IO, 2, XY2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, stream ); break;
       default: return -1;
    }
    return ( hipGetLastError() == hipSuccess ) ? 0 : -2;

@@ -28,14 +28,17 @@ _TWO_PI_LO = 2.44929359829470635445e-16
 
 
 def det_exp(x: np.ndarray) -> np.ndarray:
-    """exp(x) from Cody-Waite reduction and an order-20 Taylor polynomial (basic IEEE ops only)."""
+    """exp(x) from Cody-Waite reduction (|r| <= ln2/2) and an order-16 Taylor polynomial, basic IEEE ops only."""
     x = np.asarray(x, dtype=np.float64)
     k = np.rint(x * _INV_LN2)
-    r = (x - k * _LN2_HI) - k * _LN2_LO
+    r = x - k * _LN2_HI
+    r -= k * _LN2_LO
     p = np.ones_like(r)
-    for n in range(20, 0, -1):
-        p = 1.0 + p * r / n
-    return np.ldexp(p, k.astype(np.int64))
+    for n in range(16, 0, -1):
+        p *= r
+        p /= n
+        p += 1.0
+    return np.ldexp(p, k.astype(np.int32))
 
 
 def det_sin(x: np.ndarray) -> np.ndarray:
@@ -67,8 +70,19 @@ class Workload:
 
 def prim_to_cons(rho, vx, vy, vz, p, gamma=GAMMA):
     """Conserved fields from primitive ones, E = P/(gamma-1) + rho |v|^2 / 2."""
-    e = p / (gamma - 1.0) + 0.5 * rho * (vx * vx + vy * vy + vz * vz)
-    return np.stack([rho, rho * vx, rho * vy, rho * vz, e])
+    out = np.empty((NCOMP,) + np.shape(rho))
+    out[0] = rho
+    np.multiply(rho, vx, out=out[1])
+    np.multiply(rho, vy, out=out[2])
+    np.multiply(rho, vz, out=out[3])
+    ke = vx * vx
+    ke += vy * vy
+    ke += vz * vz
+    ke *= rho
+    ke *= 0.5
+    np.divide(p, gamma - 1.0, out=out[4])
+    out[4] += ke
+    return out
 
 
 def cfl_dt(cons: np.ndarray, dh: float, cfl: float, gamma: float = GAMMA) -> float:
@@ -101,11 +115,18 @@ def cut_patch_groups(cons: np.ndarray, ghost: int) -> np.ndarray:
 
 def random_sod(npg_shape=(4, 4, 4), seed=1234, scheme_ghost=3, sedov=False, cfl=0.5, gamma=GAMMA,
                sigma_v=0.3) -> Workload:
-    """Random medium plus a Sod plane through half of the patch groups.
+    """``random_sod_grid`` cut into patch-group blocks."""
+    cons, dt, dh = random_sod_grid(npg_shape, seed, sedov, cfl, gamma, sigma_v)
+    return Workload(cut_patch_groups(cons, scheme_ghost), dt, dh, tuple(npg_shape), scheme_ghost, gamma)
+
+
+def random_sod_grid(npg_shape=(4, 4, 4), seed=1234, sedov=False, cfl=0.5, gamma=GAMMA, sigma_v=0.3):
+    """Random medium plus a Sod plane through half of the patch groups, as a periodic cell grid.
 
     rho, P ~ lognormal(0, 0.5); v_d ~ N(0, 1) * sigma_v * c_s.  In the lower half of the z range every
     patch group holds a Sod discontinuity at its x centre (rho 1 | 0.125, P 1 | 0.1, v = 0).  With
     ``sedov`` one random interior cell of every 16th patch group gets P x 1e3.
+    Returns (cons (5, Z, Y, X), dt, dh).
     """
     gz, gy, gx = npg_shape
     nz, ny, nx = gz * PS2, gy * PS2, gx * PS2
@@ -138,7 +159,7 @@ def random_sod(npg_shape=(4, 4, 4), seed=1234, scheme_ghost=3, sedov=False, cfl=
     cons = prim_to_cons(rho, vx, vy, vz, p, gamma)
     dh = 1.0 / nx
     dt = cfl_dt(cons, dh, cfl, gamma)
-    return Workload(cut_patch_groups(cons, scheme_ghost), dt, dh, tuple(npg_shape), scheme_ghost, gamma)
+    return cons, dt, dh
 
 
 def kelvin_helmholtz(npg_shape=(8, 16, 16), seed=123, scheme_ghost=3, cfl=0.5, gamma=GAMMA, amp=0.01) -> Workload:
