@@ -1,0 +1,65 @@
+"""The C-ABI library: it loads without a GPU, exports every function include/gamer_amd_hydro.h declares,
+and rejects bad arguments with HYDRO_E_* codes before touching the device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from gamer_amd import hydro
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "gamer_amd_hydro.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?(?:int|void|char)\s*\*?\s*(hydro_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_expected_api():
+    assert set(declared_functions()) == set(hydro.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = hydro.lib()
+    for name in declared_functions():
+        assert hasattr(L, name), name
+
+
+def _call(**kw):
+    a = hydro.make_args(**kw)
+    return hydro.lib().hydro_mhm_solve_async(ctypes.byref(a), None)
+
+
+@pytest.mark.parametrize("kw,frag", [
+    (dict(rsolver="HLLC"), "in/out"),
+    (dict(), "in/out"),
+])
+def test_null_pointers_rejected(kw, frag):
+    rc = _call(npg=4, dt=1e-3, dh=1e-2, **kw)
+    assert rc == -1 and frag in hydro.lib().hydro_last_error().decode()
+
+
+def test_bad_enums_rejected():
+    a = hydro.make_args(npg=0, dt=1e-3, dh=1e-2)
+    for field, bad in (("rsolver", 7), ("limiter", 5), ("precision", 3), ("scheme", 9), ("minmod_maxiter", -1)):
+        b = hydro.HydroSolveArgs.from_buffer_copy(a)
+        setattr(b, field, bad)
+        assert hydro.lib().hydro_mhm_solve_async(ctypes.byref(b), None) == -1, field
+    b = hydro.HydroSolveArgs.from_buffer_copy(a)
+    b.gamma = 1.0
+    assert hydro.lib().hydro_mhm_solve_async(ctypes.byref(b), None) == -1
+
+
+def test_empty_batch_is_a_no_op():
+    assert _call(npg=0, dt=1e-3, dh=1e-2) == 0
+
+
+def test_host_path_requires_allocation():
+    a = hydro.make_args(npg=1, dt=1e-3, dh=1e-2)
+    buf = ctypes.create_string_buffer(8)
+    rc = hydro.lib().hydro_asyn_fluid_solver(ctypes.cast(buf, ctypes.c_void_p), ctypes.cast(buf, ctypes.c_void_p),
+                                             ctypes.cast(buf, ctypes.c_void_p), ctypes.byref(a), 1)
+    assert rc == -2 and "hydro_mem_allocate_fluid" in hydro.lib().hydro_last_error().decode()

@@ -1,0 +1,169 @@
+"""Pin the oracle restatement by what the mathematics fixes, independently of any reference build:
+Riemann-solver consistency and upwinding, exact contact resolution (HLLC/Roe), limiter special values and
+symmetries, the uniform-state fixed point, per-patch-group flux telescoping, periodic-domain conservation and
+x<->y mirror symmetry."""
+import numpy as np
+import pytest
+
+from gamer_amd import ic
+
+oracle = pytest.importorskip("oracle")
+G = 5.0 / 3.0
+
+
+def cons(rho, u, v, w, p):
+    return np.array([rho, rho * u, rho * v, rho * w, p / (G - 1) + 0.5 * rho * (u * u + v * v + w * w)])
+
+
+def euler_flux(U, d):
+    """Euler flux along direction d: (rho v_d, rho v v_d + P e_d, v_d (E + P))."""
+    rho, m, E = U[0], U[1:4], U[4]
+    vel = m / rho
+    p = (G - 1) * (E - 0.5 * rho * vel @ vel)
+    f = np.empty(5)
+    f[0] = m[d]
+    f[1:4] = m * vel[d]
+    f[1 + d] += p
+    f[4] = vel[d] * (E + p)
+    return f
+
+
+def rand_state(rng, mach=0.5):
+    rho, p = np.exp(rng.normal(0, 0.5, 2))
+    cs = np.sqrt(G * p / rho)
+    u, v, w = rng.normal(0, mach * cs, 3)
+    return cons(rho, u, v, w, p)
+
+
+@pytest.mark.parametrize("rs", ["HLLC", "HLLE", "ROE"])
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_riemann_consistency(rs, d):
+    rng = np.random.default_rng(1 + d)
+    for _ in range(50):
+        U = rand_state(rng)
+        np.testing.assert_allclose(oracle.riemann(rs, d, U, U), euler_flux(U, d), rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("rs", ["HLLC", "HLLE", "ROE"])
+def test_supersonic_upwinding(rs):
+    rng = np.random.default_rng(7)
+    for _ in range(50):
+        rho, p = np.exp(rng.normal(0, 0.3, 2)), np.exp(rng.normal(0, 0.3, 2))
+        cs = np.sqrt(G * p / rho)
+        s = 3.0 * cs.max()
+        L, R = cons(rho[0], s, 0.1, -0.2, p[0]), cons(rho[1], s, 0.3, 0.1, p[1])
+        np.testing.assert_allclose(oracle.riemann(rs, 0, L, R), euler_flux(L, 0), rtol=1e-14)
+        L, R = cons(rho[0], -s, 0.1, -0.2, p[0]), cons(rho[1], -s, 0.3, 0.1, p[1])
+        np.testing.assert_allclose(oracle.riemann(rs, 0, L, R), euler_flux(R, 0), rtol=1e-14)
+
+
+@pytest.mark.parametrize("rs", ["HLLC", "ROE"])
+@pytest.mark.parametrize("u", [0.0, 0.3, -0.4])
+def test_contact_resolved_exactly(rs, u):
+    """An isolated contact (equal u and P, different rho) gives the flux of the upwind state."""
+    L, R = cons(1.0, u, 0.2, -0.1, 1.0), cons(0.125, u, -0.3, 0.25, 1.0)
+    up = L if u >= 0 else R
+    np.testing.assert_allclose(oracle.riemann(rs, 0, L, R), euler_flux(up, 0), rtol=1e-13, atol=1e-14)
+
+
+def test_hlle_diffuses_contact():
+    L, R = cons(1.0, 0.0, 0.0, 0.0, 1.0), cons(0.125, 0.0, 0.0, 0.0, 1.0)
+    f = oracle.riemann("HLLE", 0, L, R)
+    assert f[0] > 1e-3          # HLL has no contact wave: a mass flux appears at a stationary contact
+
+
+LIMS = ["VANLEER", "GMINMOD", "ALBADA", "VL_GMINMOD", "CENTRAL"]
+
+
+@pytest.mark.parametrize("lim", LIMS)
+def test_limiter_extremum_and_linear(lim):
+    C = np.array([1.0, 2.0, -3.0, 0.5, 4.0])
+    # extremum or flat side -> zero slope
+    for dl, dr in ((1.0, -1.0), (-2.0, 0.5), (0.0, 1.0), (1.0, 0.0)):
+        s = oracle.limit_slope(C - dl, C, C + dr, lim, 1.5)
+        assert np.all(s == 0.0)
+    # linear data -> exact gradient (theta >= 1)
+    g = np.array([0.25, -0.5, 1.0, -2.0, 0.125])
+    np.testing.assert_array_equal(oracle.limit_slope(C - g, C, C + g, lim, 1.5), g)
+
+
+def test_limiter_closed_forms():
+    C = np.zeros(5)
+    a, b = np.array([1.0, 1.0, 2.0, 1.0, 0.5]), np.array([10.0, 3.0, 2.0, 4.0, 0.25])
+    L, R = C - a, C + b
+    np.testing.assert_allclose(oracle.limit_slope(L, C, R, "VANLEER"), 2 * a * b / (a + b), rtol=1e-15)
+    np.testing.assert_allclose(oracle.limit_slope(L, C, R, "ALBADA"), a * b * (a + b) / (a * a + b * b), rtol=1e-15)
+    np.testing.assert_allclose(oracle.limit_slope(L, C, R, "CENTRAL"), 0.5 * (a + b), rtol=1e-15)
+    for th in (1.0, 1.5, 2.5):
+        mm = np.minimum(np.minimum(th * a, th * b), 0.5 * (a + b))
+        np.testing.assert_allclose(oracle.limit_slope(L, C, R, "GMINMOD", th), mm, rtol=1e-15)
+        np.testing.assert_allclose(oracle.limit_slope(L, C, R, "VL_GMINMOD", th), np.minimum(mm, 2 * a * b / (a + b)),
+                                   rtol=1e-15)
+
+
+@pytest.mark.parametrize("lim", LIMS)
+def test_limiter_odd_and_homogeneous(lim):
+    rng = np.random.default_rng(3)
+    for _ in range(100):
+        L, C, R = rng.normal(size=(3, 5))
+        s = oracle.limit_slope(L, C, R, lim)
+        np.testing.assert_array_equal(oracle.limit_slope(-L, -C, -R, lim), -s)
+        np.testing.assert_array_equal(oracle.limit_slope(4 * L, 4 * C, 4 * R, lim), 4 * s)
+        np.testing.assert_array_equal(oracle.limit_slope(R, C, L, lim), -s)
+
+
+@pytest.mark.parametrize("scheme,ghost", [("MHM_RP", 3), ("MHM", 2)])
+@pytest.mark.parametrize("rs", ["HLLC", "HLLE", "ROE"])
+def test_uniform_state_fixed_point(scheme, ghost, rs):
+    n = 16 + 2 * ghost
+    u = np.empty((2, 5, n, n, n))
+    for v, x in enumerate(cons(1.3, 0.2, -0.1, 0.05, 0.9)):
+        u[:, v] = x
+    out, _ = oracle.fluid_solver(u, 1e-3, 1 / 32, scheme=scheme, rsolver=rs)
+    np.testing.assert_array_equal(out, ic.pg_interior(u, ghost))
+
+
+@pytest.mark.parametrize("scheme,ghost", [("MHM_RP", 3), ("MHM", 2)])
+@pytest.mark.parametrize("rs", ["HLLC", "HLLE", "ROE"])
+def test_flux_telescoping_per_patch_group(scheme, ghost, rs):
+    """sum_cells(U_out - U_in) = -dt/dh * (net flux through the 6 boundary planes of g_Flux_Array)."""
+    w = ic.random_sod((1, 2, 2), seed=12, scheme_ghost=ghost, sedov=True)
+    out, flux = oracle.fluid_solver(w.u, w.dt, w.dh, scheme=scheme, rsolver=rs)
+    dU = (out - ic.pg_interior(w.u, ghost)).sum(axis=(2, 3, 4))
+    net = (flux[:, 2] - flux[:, 0] + flux[:, 5] - flux[:, 3] + flux[:, 8] - flux[:, 6]).sum(axis=(2, 3))
+    scale = np.abs(ic.pg_interior(w.u, ghost)).sum(axis=(2, 3, 4))
+    assert np.all(np.abs(dU + w.dt / w.dh * net) <= 1e-13 * scale)
+
+
+def test_periodic_domain_conservation():
+    w = ic.random_sod((2, 2, 2), seed=4, sedov=True)
+    out, _ = oracle.fluid_solver(w.u, w.dt, w.dh)
+    s0 = ic.pg_interior(w.u, 3).sum(axis=(0, 2, 3, 4))
+    s1 = out.sum(axis=(0, 2, 3, 4))
+    tot = np.abs(ic.pg_interior(w.u, 3)).sum(axis=(0, 2, 3, 4))
+    assert np.all(np.abs(s1 - s0) <= 1e-14 * tot)
+
+
+@pytest.mark.parametrize("rs", ["HLLC", "HLLE", "ROE"])
+def test_xy_mirror_symmetry(rs):
+    """Swapping the x and y axes (and m_x <-> m_y) commutes with the solver up to round-off."""
+    w = ic.random_sod((1, 1, 2), seed=8, sedov=True)
+    u = w.u
+    ut = u.transpose(0, 1, 2, 4, 3)[:, [0, 2, 1, 3, 4]].copy()
+    o, _ = oracle.fluid_solver(u, w.dt, w.dh, rsolver=rs)
+    ot, _ = oracle.fluid_solver(ut, w.dt, w.dh, rsolver=rs)
+    back = ot.transpose(0, 1, 2, 4, 3)[:, [0, 2, 1, 3, 4]]
+    scale = np.abs(o).max(axis=(0, 2, 3, 4))[None, :, None, None, None]
+    assert np.all(np.abs(back - o) <= 1e-13 * scale)
+
+
+def test_cfl_dt_matches_definition():
+    w = ic.random_sod((1, 1, 1), seed=3)
+    grid = ic.pg_interior(w.u, 3)[0]
+    patch = grid[:, :8, :8, :8][None]
+    dt = oracle.dt_cfl(patch, w.dh, 0.5)[0]
+    rho = patch[0, 0]
+    vel = patch[0, 1:4] / rho
+    p = (G - 1) * (patch[0, 4] - 0.5 * rho * (vel ** 2).sum(axis=0))
+    c = np.sqrt(G * p / rho)
+    assert dt == pytest.approx(0.5 * w.dh / (np.abs(vel).max(axis=0) + c).max(), rel=1e-14)
