@@ -51,7 +51,6 @@ int check_args( const HydroSolveArgs *a )
    if ( a->npg > 0 && ( a->in == nullptr || a->out == nullptr ) ) return fail( HYDRO_E_ARG, "in/out == NULL" );
    if ( a->store_flux && a->npg > 0 && a->flux == nullptr )  return fail( HYDRO_E_ARG, "store_flux with flux == NULL" );
    if ( a->scheme != HYDRO_MHM_RP && a->scheme != HYDRO_MHM ) return fail( HYDRO_E_ARG, "unsupported scheme %d", a->scheme );
-   if ( a->scheme == HYDRO_MHM )                             return fail( HYDRO_E_ARG, "scheme MHM is not built in this library" );
    if ( a->rsolver != HYDRO_HLLC && a->rsolver != HYDRO_HLLE && a->rsolver != HYDRO_ROE )
                                                              return fail( HYDRO_E_ARG, "unsupported Riemann solver %d", a->rsolver );
    const int l = a->limiter;
@@ -91,12 +90,15 @@ Prm<R> make_prm( const HydroSolveArgs *a )
 int enqueue( const HydroSolveArgs *a, const void *in, void *out, void *flux, int npg, hipStream_t s )
 {
    int rc;
+   const bool rp = ( a->scheme == HYDRO_MHM_RP );
    if ( a->precision == HYDRO_FP64 )
-      rc = launch_mhm_rp<double, double>( (const double*)in, (double*)out, (double*)flux, npg, a->rsolver,
-                                          a->store_flux, a->minmod_maxiter, a->minmod_coeff, make_prm<double>( a ), s );
+      rc = ( rp ? launch_mhm_rp<double, double> : launch_mhm<double, double> )
+           ( (const double*)in, (double*)out, (double*)flux, npg, a->rsolver, a->store_flux, a->minmod_maxiter,
+             a->minmod_coeff, make_prm<double>( a ), s );
    else
-      rc = launch_mhm_rp<float, float>( (const float*)in, (float*)out, (float*)flux, npg, a->rsolver,
-                                        a->store_flux, a->minmod_maxiter, a->minmod_coeff, make_prm<float>( a ), s );
+      rc = ( rp ? launch_mhm_rp<float, float> : launch_mhm<float, float> )
+           ( (const float*)in, (float*)out, (float*)flux, npg, a->rsolver, a->store_flux, a->minmod_maxiter,
+             a->minmod_coeff, make_prm<float>( a ), s );
    if ( rc != 0 ) return fail( HYDRO_E_DEVICE, "kernel launch failed (%d): %s", rc, hipGetErrorString( hipGetLastError() ) );
    return HYDRO_OK;
 }

@@ -195,33 +195,50 @@ __device__ __forceinline__ R plm_slope( R sl, R sr, R lim_rt, int lim, R coeff )
    return ( lr > Z ) ? s : Z;
 }
 
+// clamp x into [min(a,b), max(a,b)] = median of (x, a, b)
+__device__ __forceinline__ float  clamp_between( float x, float a, float b )   { return __builtin_amdgcn_fmed3f( x, a, b ); }
+__device__ __forceinline__ double clamp_between( double x, double a, double b ) { return vmin( vmax( x, vmin( a, b ) ), vmax( a, b ) ); }
+
+// van Leer + generalised MinMod slope without branches:
+//   sign(sc) * min( theta|sl|, theta|sr|, |sc|, |2 sl sr/(sl+sr)| )  if sl*sr > 0, else 0.
+// h = 2 sl sr / |sl+sr| equals the harmonic-mean magnitude when sl*sr > 0 and is <= 0 (or NaN) otherwise,
+// so max(min(...), 0) realises the sign test; |sc| >= |harmonic mean| is implied and dropped.
+template <typename R>
+__device__ __forceinline__ R slope_vl_gminmod( R sl, R sr, R coeff )
+{
+   const R t = sl + sr;
+   const R h = (R)2.0*( sl*sr )*rcp( vabs( t ) );
+   R m = vmin( vmin( vabs( sl )*coeff, vabs( sr )*coeff ), h );
+   m = vmax( m, (R)0 );
+   return __builtin_copysign( m, t );
+}
+
 template <int LIM, typename R>
 __device__ __forceinline__ void plm_face( R L, R C, R Rr, int lim, R coeff, R &fL, R &fR )
 {
-   const R s = plm_slope<LIM>( C - L, Rr - C, (R)0, lim, coeff );
-   R fl = C - (R)0.5*s;
-   fl = vmin( vmax( fl, vmin( C, L ) ), vmax( C, L ) );
-   R fr = (R)2.0*C - fl;
-   fr = vmin( vmax( fr, vmin( C, Rr ) ), vmax( C, Rr ) );
+   const R s = ( LIM == 4 ) ? slope_vl_gminmod( C - L, Rr - C, coeff ) : plm_slope<LIM>( C - L, Rr - C, (R)0, lim, coeff );
+   R fl = clamp_between( C - (R)0.5*s, C, L );
+   R fr = clamp_between( (R)2.0*C - fl, C, Rr );
    fL = (R)2.0*C - fr;
    fR = fr;
 }
 
 // primitive face state (rho, vx, vy, vz, P) -> interface state rotated to direction D
-// (Pri2Con, FluUtility.cpp:343-420, with the solver's Con2Pres read directly from the face pressure)
+// (Pri2Con, FluUtility.cpp:343-420; the solver takes the face pressure and velocity directly instead of
+// recovering them from the conserved face state)
 template <int D, typename R>
 __device__ __forceinline__ St<R> face_state( const R w[NV], const Prm<R> &P )
 {
    St<R> s;
-   s.d = w[0];
-   const R mx = w[0]*w[1], my = w[0]*w[2], mz = w[0]*w[3];
+   const R vn = w[1 + D], v1 = w[1 + ( D + 1 ) % 3], v2 = w[1 + ( D + 2 ) % 3];
+   s.d  = w[0];
+   s.mn = w[0]*vn;
+   s.m1 = w[0]*v1;
+   s.m2 = w[0]*v2;
+   s.u  = vn;
    s.ir = rcp( w[0] );
-   s.e  = (R)0.5*( mx*mx + my*my + mz*mz )*s.ir + w[4]*P.inv_gm1;
-   s.mn = ( D == 0 ) ? mx : ( D == 1 ) ? my : mz;
-   s.m1 = ( D == 0 ) ? my : ( D == 1 ) ? mz : mx;
-   s.m2 = ( D == 0 ) ? mz : ( D == 1 ) ? mx : my;
    s.p  = floor_pres( w[4], P.min_pres );
-   s.u  = s.ir*s.mn;
+   s.e  = (R)0.5*w[0]*( w[1]*w[1] + w[2]*w[2] + w[3]*w[3] ) + w[4]*P.inv_gm1;
    s.cs = vsqrt( P.gamma*s.p*s.ir );
    return s;
 }
@@ -259,6 +276,122 @@ __device__ __forceinline__ void riemann( const St<R> &L, const St<R> &Q, const P
    out[1 + D]             = f[1];
    out[1 + ( D + 1 ) % 3] = f[2];
    out[1 + ( D + 2 ) % 3] = f[3];
+}
+
+template <typename IO> struct IOLim;
+template <> struct IOLim<double> { static constexpr double huge = 1.7976931348623157e308; static constexpr double eps = 2.2204460492503131e-16; };
+template <> struct IOLim<float>  { static constexpr float  huge = 3.40282346638528859812e+38f; static constexpr float eps = 1.1920928955078125e-07f; };
+
+// full-step update of one output cell, accumulated in FP64, + the CONS-mode unphysical check on the stored
+// values (FullStepUpdate.cpp:106-235, FluUtility.cpp:843-940)
+template <typename R, typename IO>
+__device__ __forceinline__ bool full_cell( const IO u[NV], const R dfxy[NV], const R fzl[NV], const R fzu[NV],
+                                           const Prm<R> &P, IO o[NV] )
+{
+#  pragma unroll
+   for (int v=0; v<NV; v++)
+      o[v] = (IO)( (double)u[v] - P.dt_dh*( (double)dfxy[v] + ( (double)fzu[v] - (double)fzl[v] ) ) );
+   const IO huge = IOLim<IO>::huge, Z = (IO)0;
+   bool bad = false;
+#  pragma unroll
+   for (int v=0; v<NV; v++) bad |= ( o[v] != o[v] );
+   bad |= ( vabs( o[1] ) > huge || vabs( o[2] ) > huge || vabs( o[3] ) > huge );
+   bad |= ( o[0] <= Z || o[0] > huge || o[4] <= Z || o[4] > huge );
+   const IO ei = o[4] - (IO)0.5*( o[1]*o[1] + o[2]*o[2] + o[3]*o[3] )*rcp( o[0] );
+   bad |= ( ei < (IO)-3.0*o[4]*IOLim<IO>::eps || ei > huge || ei != ei );
+   return bad;
+}
+
+// Hancock half-step advance of the six face states of one cell (MHM only;
+// DataReconstruction.cpp:2063-2160): fluxes of the conserved face states (pressure recovered in FP64),
+// U_f -= dt/(2dh) * sum_d (F_{d,R} - F_{d,L}), then either a reset of all six faces to the cell value if
+// any face density/energy or input face pressure is invalid, or density and internal-energy floors.
+template <typename R, typename IO>
+__device__ __forceinline__ void hancock( R fc[6][NV], const R fcp[6][NV], const R ucell[NV], const Prm<R> &P )
+{
+   R F[6][NV];
+#  pragma unroll
+   for (int f=0; f<6; f++)
+   {
+      const int d = f/2;
+      const R mn = fc[f][1 + d];
+      const double eint = (double)fc[f][4] - 0.5*( (double)fc[f][1]*fc[f][1] + (double)fc[f][2]*fc[f][2]
+                                                    + (double)fc[f][3]*fc[f][3] )*rcp( (double)fc[f][0] );
+      const R p  = floor_pres( (R)( eint*P.gm1_d ), P.min_pres );
+      const R vn = rcp( fc[f][0] )*mn;
+      F[f][0] = mn;
+      F[f][1] = vn*fc[f][1];
+      F[f][2] = vn*fc[f][2];
+      F[f][3] = vn*fc[f][3];
+      F[f][1 + d] += p;
+      F[f][4] = vn*( fc[f][4] + p );
+   }
+   const R dt_dh2 = (R)P.dt_dh2;
+#  pragma unroll
+   for (int v=0; v<NV; v++)
+   {
+      const R dF = dt_dh2*( F[1][v] - F[0][v] + F[3][v] - F[2][v] + F[5][v] - F[4][v] );
+#     pragma unroll
+      for (int f=0; f<6; f++) fc[f][v] -= dF;
+   }
+   const R tiny = ( sizeof(R) == 8 ) ? (R)2.2250738585072014e-308 : (R)1.17549435e-38f;
+   const R huge = (R)IOLim<IO>::huge;
+   bool reset = false;
+#  pragma unroll
+   for (int f=0; f<6; f++)
+   {
+      const R d = fc[f][0], e = fc[f][4], pr = fcp[f][4];
+      reset |= ( d < tiny || d > huge || d != d ) || ( e < tiny || e > huge || e != e ) ||
+               ( pr < (R)0 || pr > huge || pr != pr );
+   }
+   if ( reset )
+   {
+#     pragma unroll
+      for (int f=0; f<6; f++)
+#        pragma unroll
+         for (int v=0; v<NV; v++) fc[f][v] = ucell[v];
+   }
+   else
+   {
+#     pragma unroll
+      for (int f=0; f<6; f++)
+      {
+         fc[f][0] = vmax( fc[f][0], P.min_dens );
+         const double ke  = 0.5*( (double)fc[f][1]*fc[f][1] + (double)fc[f][2]*fc[f][2] + (double)fc[f][3]*fc[f][3] )
+                            *rcp( (double)fc[f][0] );
+         const double ein = (double)fc[f][4] - ke;
+         const double eo  = ( ein == ein ) ? __builtin_fmax( ein, P.min_eint_d ) : ein;
+         if ( ein != eo ) fc[f][4] = (R)( (double)fc[f][4] - ein + eo );
+      }
+   }
+}
+
+// primitive -> conserved (Pri2Con, FluUtility.cpp:343-420)
+template <typename R>
+__device__ __forceinline__ void pri2con( const R w[NV], R c[NV], const Prm<R> &P )
+{
+   c[0] = w[0];
+   c[1] = w[0]*w[1];  c[2] = w[0]*w[2];  c[3] = w[0]*w[3];
+   c[4] = (R)0.5*( c[1]*c[1] + c[2]*c[2] + c[3]*c[3] )*rcp( w[0] ) + w[4]*P.inv_gm1;
+}
+
+// conserved face state -> interface state rotated to direction D, pressure recovered in FP64
+template <int D, typename R>
+__device__ __forceinline__ St<R> cons_state( const R c[NV], const Prm<R> &P )
+{
+   St<R> s;
+   const double eint = (double)c[4] - 0.5*( (double)c[1]*c[1] + (double)c[2]*c[2] + (double)c[3]*c[3] )
+                                      *rcp( (double)c[0] );
+   s.d  = c[0];
+   s.mn = c[1 + D];
+   s.m1 = c[1 + ( D + 1 ) % 3];
+   s.m2 = c[1 + ( D + 2 ) % 3];
+   s.e  = c[4];
+   s.p  = floor_pres( (R)( eint*P.gm1_d ), P.min_pres );
+   s.ir = rcp( c[0] );
+   s.u  = s.ir*s.mn;
+   s.cs = vsqrt( P.gamma*s.p*s.ir );
+   return s;
 }
 
 } // namespace gamer_amd

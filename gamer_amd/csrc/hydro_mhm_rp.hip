@@ -131,30 +131,6 @@ __device__ __forceinline__ void half_cell( const R u[NV], const R dfxy[NV], cons
    w[4] = (R)( ( p == p ) ? __builtin_fmax( p, P.min_pres_d ) : p );
 }
 
-template <typename IO> struct IOLim;
-template <> struct IOLim<double> { static constexpr double huge = 1.7976931348623157e308; static constexpr double eps = 2.2204460492503131e-16; };
-template <> struct IOLim<float>  { static constexpr float  huge = 3.40282346638528859812e+38f; static constexpr float eps = 1.1920928955078125e-07f; };
-
-// full-step update of one output cell, accumulated in FP64, + the CONS-mode unphysical check on the stored
-// values (FullStepUpdate.cpp:106-235, FluUtility.cpp:843-940)
-template <typename R, typename IO>
-__device__ __forceinline__ bool full_cell( const IO u[NV], const R dfxy[NV], const R fzl[NV], const R fzu[NV],
-                                           const Prm<R> &P, IO o[NV] )
-{
-#  pragma unroll
-   for (int v=0; v<NV; v++)
-      o[v] = (IO)( (double)u[v] - P.dt_dh*( (double)dfxy[v] + ( (double)fzu[v] - (double)fzl[v] ) ) );
-   const IO huge = IOLim<IO>::huge, Z = (IO)0;
-   bool bad = false;
-#  pragma unroll
-   for (int v=0; v<NV; v++) bad |= ( o[v] != o[v] );
-   bad |= ( vabs( o[1] ) > huge || vabs( o[2] ) > huge || vabs( o[3] ) > huge );
-   bad |= ( o[0] <= Z || o[0] > huge || o[4] <= Z || o[4] > huge );
-   const IO ei = o[4] - (IO)0.5*( o[1]*o[1] + o[2]*o[2] + o[3]*o[3] )*rcp( o[0] );
-   bad |= ( ei < (IO)-3.0*o[4]*IOLim<IO>::eps || ei > huge || ei != ei );
-   return bad;
-}
-
 template <typename R>
 __device__ __forceinline__ void ld_cell( const R (*a)[NC], int c, R q[8] )
 {

@@ -31,16 +31,23 @@ def _oracle(u, w, **kw):
     return oracle.fluid_solver(u, w.dt, w.dh, **kw)
 
 
-@pytest.fixture(scope="module")
-def sod():
-    return ic.random_sod((2, 2, 4), seed=2024, sedov=True)
+GHOST = {"MHM_RP": 3, "MHM": 2}
+_SOD = {}
 
 
+def sod_for(scheme):
+    if scheme not in _SOD:
+        _SOD[scheme] = ic.random_sod((2, 2, 4), seed=2024, sedov=True, scheme_ghost=GHOST[scheme])
+    return _SOD[scheme]
+
+
+@pytest.mark.parametrize("scheme", ["MHM_RP", "MHM"])
 @pytest.mark.parametrize("rsolver", ["HLLC", "HLLE", "ROE"])
-def test_fp64_matches_oracle(sod, rsolver):
-    print(f"[gpu] fp64 parity rsolver={rsolver} npg={sod.npg}", flush=True)
-    o, f = _gpu(sod.u, sod, rsolver=rsolver)
-    ro, rf = _oracle(sod.u, sod, rsolver=rsolver)
+def test_fp64_matches_oracle(scheme, rsolver):
+    sod = sod_for(scheme)
+    print(f"[gpu] fp64 parity scheme={scheme} rsolver={rsolver} npg={sod.npg}", flush=True)
+    o, f = _gpu(sod.u, sod, scheme=scheme, rsolver=rsolver)
+    ro, rf = _oracle(sod.u, sod, scheme=scheme, rsolver=rsolver)
     err = rel_err_cells(o, ro)
     ferr = flux_rel_err(f, rf, ro)
     print(f"  per-field max rel err {err}, flux {ferr:.3e}")
@@ -48,36 +55,54 @@ def test_fp64_matches_oracle(sod, rsolver):
     assert ferr <= FP64_TOL
 
 
+@pytest.mark.parametrize("scheme", ["MHM_RP", "MHM"])
 @pytest.mark.parametrize("limiter", ["VANLEER", "GMINMOD", "ALBADA", "VL_GMINMOD", "CENTRAL"])
-def test_fp64_limiters(limiter):
-    w = ic.random_sod((2, 2, 2), seed=77, sedov=False)
-    print(f"[gpu] fp64 limiter={limiter}", flush=True)
-    o, f = _gpu(w.u, w, limiter=limiter)
-    ro, rf = _oracle(w.u, w, limiter=limiter)
+def test_fp64_limiters(scheme, limiter):
+    w = ic.random_sod((2, 2, 2), seed=77, sedov=False, scheme_ghost=GHOST[scheme])
+    print(f"[gpu] fp64 scheme={scheme} limiter={limiter}", flush=True)
+    o, f = _gpu(w.u, w, scheme=scheme, limiter=limiter)
+    ro, rf = _oracle(w.u, w, scheme=scheme, limiter=limiter)
     err = rel_err_cells(o, ro)
     assert np.all(err <= FP64_TOL), err
     assert flux_rel_err(f, rf, ro) <= FP64_TOL
 
 
+@pytest.mark.parametrize("scheme", ["MHM_RP", "MHM"])
 @pytest.mark.parametrize("rsolver", ["HLLC", "HLLE", "ROE"])
-def test_fp32_bulk_vs_fp64_oracle(sod, rsolver):
-    print(f"[gpu] fp32-bulk rsolver={rsolver}", flush=True)
-    o, f = _gpu(sod.u.astype(np.float32), sod, rsolver=rsolver)
-    ro, rf = _oracle(sod.u, sod, rsolver=rsolver)
+def test_fp32_bulk_vs_fp64_oracle(scheme, rsolver):
+    sod = sod_for(scheme)
+    print(f"[gpu] fp32-bulk scheme={scheme} rsolver={rsolver}", flush=True)
+    o, f = _gpu(sod.u.astype(np.float32), sod, scheme=scheme, rsolver=rsolver)
+    ro, rf = _oracle(sod.u, sod, scheme=scheme, rsolver=rsolver)
     l1 = rel_l1(o, ro)
     print(f"  per-field rel L1 {l1}")
     assert np.all(l1 <= FP32_L1_TOL), l1
 
 
-def test_uniform_state_is_fixed_point():
+@pytest.mark.parametrize("scheme", ["MHM_RP", "MHM"])
+def test_uniform_state_is_fixed_point(scheme):
     """All slopes and flux differences vanish: output == input interior bit for bit."""
-    n = 22
+    g = GHOST[scheme]
+    n = 16 + 2 * g
     u = np.empty((3, 5, n, n, n))
     u[:, 0], u[:, 1], u[:, 2], u[:, 3], u[:, 4] = 1.3, 0.2, -0.1, 0.05, 2.0
-    w = ic.Workload(u, 1e-3, 1.0 / 64, (1, 1, 3), 3)
+    w = ic.Workload(u, 1e-3, 1.0 / 64, (1, 1, 3), g)
     for dt in (np.float64, np.float32):
-        o, f = _gpu(u.astype(dt), w)
-        assert np.array_equal(o, ic.pg_interior(u.astype(dt), 3)), dt
+        o, f = _gpu(u.astype(dt), w, scheme=scheme)
+        assert np.array_equal(o, ic.pg_interior(u.astype(dt), g)), dt
+
+
+@pytest.mark.parametrize("scheme", ["MHM_RP", "MHM"])
+@pytest.mark.parametrize("case", ["retry", "floors"])
+def test_fp64_retry_and_floors(scheme, case):
+    """Unphysical full-step results (CFL 1.2) trigger the MinMod-coefficient retry loop; floors active."""
+    w = ic.random_sod((1, 1, 2), seed=31, sedov=True, cfl=1.2, sigma_v=1.5, scheme_ghost=GHOST[scheme])
+    kw = dict(minmod_maxiter=4) if case == "retry" else dict(min_dens=1e-2, min_pres=1e-3, min_eint=1e-3)
+    print(f"[gpu] fp64 {case} scheme={scheme}", flush=True)
+    o, f = _gpu(w.u, w, scheme=scheme, **kw)
+    ro, rf = _oracle(w.u, w, scheme=scheme, **kw)
+    err = rel_err_cells(o, ro)
+    assert np.all(err <= FP64_TOL), err
 
 
 def test_conservation_and_determinism():
