@@ -61,6 +61,7 @@ int check_args( const HydroSolveArgs *a )
    if ( a->precision != HYDRO_FP64 && a->precision != HYDRO_FP32_BULK )
                                                              return fail( HYDRO_E_ARG, "unsupported precision %d", a->precision );
    if ( a->npg > ( 1 << 26 ) )                               return fail( HYDRO_E_ARG, "npg too large" );
+   if ( a->dt_patch != nullptr && !( a->safety > 0.0 ) )     return fail( HYDRO_E_ARG, "fused dt needs safety > 0" );
    return HYDRO_OK;
 }
 
@@ -87,18 +88,18 @@ Prm<R> make_prm( const HydroSolveArgs *a )
    return P;
 }
 
-int enqueue( const HydroSolveArgs *a, const void *in, void *out, void *flux, int npg, hipStream_t s )
+int enqueue( const HydroSolveArgs *a, const void *in, void *out, void *flux, void *dt, int npg, hipStream_t s )
 {
    int rc;
    const bool rp = ( a->scheme == HYDRO_MHM_RP );
    if ( a->precision == HYDRO_FP64 )
       rc = ( rp ? launch_mhm_rp<double, double> : launch_mhm<double, double> )
            ( (const double*)in, (double*)out, (double*)flux, npg, a->rsolver, a->store_flux, a->minmod_maxiter,
-             a->minmod_coeff, make_prm<double>( a ), s );
+             a->minmod_coeff, make_prm<double>( a ), (double*)dt, a->safety*a->dh, s );
    else
       rc = ( rp ? launch_mhm_rp<float, float> : launch_mhm<float, float> )
            ( (const float*)in, (float*)out, (float*)flux, npg, a->rsolver, a->store_flux, a->minmod_maxiter,
-             a->minmod_coeff, make_prm<float>( a ), s );
+             a->minmod_coeff, make_prm<float>( a ), (float*)dt, a->safety*a->dh, s );
    if ( rc != 0 ) return fail( HYDRO_E_DEVICE, "kernel launch failed (%d): %s", rc, hipGetErrorString( hipGetLastError() ) );
    return HYDRO_OK;
 }
@@ -137,7 +138,48 @@ int hydro_mhm_solve_async( const HydroSolveArgs *a, void *stream )
 {
    const int rc = check_args( a );
    if ( rc != HYDRO_OK ) return rc;
-   return enqueue( a, a->in, a->out, a->flux, a->npg, (hipStream_t)stream );
+   return enqueue( a, a->in, a->out, a->flux, a->dt_patch, a->npg, (hipStream_t)stream );
+}
+
+int hydro_dt_cfl_async( const void *in, void *dt, int npatch, double dh, double safety, double gamma,
+                        double min_pres, int precision, void *stream )
+{
+   if ( npatch < 0 || ( npatch > 0 && ( in == nullptr || dt == nullptr ) ) ) return fail( HYDRO_E_ARG, "dt_cfl: bad pointers/size" );
+   if ( !( dh > 0.0 ) || !( safety > 0.0 ) || !( gamma > 1.0 ) )           return fail( HYDRO_E_ARG, "dt_cfl: dh/safety/gamma invalid" );
+   int rc;
+   if      ( precision == HYDRO_FP64 )      rc = launch_dt_cfl<double>( (const double*)in, (double*)dt, npatch, dh, safety, gamma, min_pres, (hipStream_t)stream );
+   else if ( precision == HYDRO_FP32_BULK ) rc = launch_dt_cfl<float> ( (const float*)in,  (float*)dt,  npatch, dh, safety, gamma, min_pres, (hipStream_t)stream );
+   else return fail( HYDRO_E_ARG, "unsupported precision %d", precision );
+   return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "dt_cfl launch failed" );
+}
+
+static int check_level( int nx, int ny, int nz )
+{
+   if ( nx <= 0 || ny <= 0 || nz <= 0 || nx % 16 || ny % 16 || nz % 16 ) return fail( HYDRO_E_ARG, "level extents must be positive multiples of 16" );
+   return HYDRO_OK;
+}
+
+int hydro_prepare_uniform_async( const void *level, void *pg_in, int nx, int ny, int nz, int ghost, int precision, void *stream )
+{
+   if ( check_level( nx, ny, nz ) ) return HYDRO_E_ARG;
+   if ( level == nullptr || pg_in == nullptr ) return fail( HYDRO_E_ARG, "prepare: NULL pointer" );
+   if ( ghost < 0 || ghost > 8 ) return fail( HYDRO_E_ARG, "prepare: ghost %d out of range", ghost );
+   int rc;
+   if      ( precision == HYDRO_FP64 )      rc = launch_gather<double>( (const double*)level, (double*)pg_in, nx, ny, nz, ghost, (hipStream_t)stream );
+   else if ( precision == HYDRO_FP32_BULK ) rc = launch_gather<float> ( (const float*)level,  (float*)pg_in,  nx, ny, nz, ghost, (hipStream_t)stream );
+   else return fail( HYDRO_E_ARG, "unsupported precision %d", precision );
+   return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "prepare launch failed" );
+}
+
+int hydro_close_uniform_async( const void *pg_out, void *level, int nx, int ny, int nz, int precision, void *stream )
+{
+   if ( check_level( nx, ny, nz ) ) return HYDRO_E_ARG;
+   if ( level == nullptr || pg_out == nullptr ) return fail( HYDRO_E_ARG, "close: NULL pointer" );
+   int rc;
+   if      ( precision == HYDRO_FP64 )      rc = launch_scatter<double>( (const double*)pg_out, (double*)level, nx, ny, nz, (hipStream_t)stream );
+   else if ( precision == HYDRO_FP32_BULK ) rc = launch_scatter<float> ( (const float*)pg_out,  (float*)level,  nx, ny, nz, (hipStream_t)stream );
+   else return fail( HYDRO_E_ARG, "unsupported precision %d", precision );
+   return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "close launch failed" );
 }
 
 int hydro_synchronize( void )
@@ -225,7 +267,7 @@ int hydro_asyn_fluid_solver( const void *h_in, void *h_out, void *h_flux, const 
       hipStream_t st = g_buf.streams[s];
       char *din = (char*)g_buf.d_in + off*in_pg, *dout = (char*)g_buf.d_out + off*out_pg, *dfl = (char*)g_buf.d_flux + off*fl_pg;
       HIP_TRY( hipMemcpyAsync( din, (const char*)h_in + off*in_pg, n*in_pg, hipMemcpyHostToDevice, st ) );
-      rc = enqueue( &a, din, dout, dfl, n, st );
+      rc = enqueue( &a, din, dout, dfl, nullptr, n, st );
       if ( rc != HYDRO_OK ) return rc;
       HIP_TRY( hipMemcpyAsync( (char*)h_out + off*out_pg, dout, n*out_pg, hipMemcpyDeviceToHost, st ) );
       if ( a.store_flux )

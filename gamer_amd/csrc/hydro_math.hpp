@@ -394,4 +394,42 @@ __device__ __forceinline__ St<R> cons_state( const R c[NV], const Prm<R> &P )
    return s;
 }
 
+// max_d(|v_d| + c_s) of one cell, pressure recovered in FP64 (CPU_dtSolver_HydroCFL.cpp:104-190)
+template <typename IO>
+__device__ __forceinline__ IO cell_cfl( const IO u[NV], double gamma, double gm1, double min_pres )
+{
+   const double d = (double)u[0], ir = 1.0/d;
+   double p = ( (double)u[4] - 0.5*( (double)u[1]*u[1] + (double)u[2]*u[2] + (double)u[3]*u[3] )*ir )*gm1;
+   p = ( p == p ) ? __builtin_fmax( p, min_pres ) : p;
+   const IO cs = (IO)__builtin_sqrt( gamma*p*ir );
+   const IO ri = (IO)ir;
+   const IO vx = vabs( u[1] )*ri, vy = vabs( u[2] )*ri, vz = vabs( u[3] )*ri;
+   return vmax( vmax( cs + vx, cs + vy ), cs + vz );
+}
+
+// Fused CFL of one patch group: each lane holds the maxima of its output column in the two z-halves; the 8
+// patches (px + 2 py + 4 pz, x fastest) are reduced with wave64 butterflies and LDS across the 4 waves.
+// Called by all 256 lanes; writes dt[8] = dh_safety / max.
+template <typename IO>
+__device__ __forceinline__ void fused_dt( const IO cfl[2], int pxy, float (*red)[4], IO *dt, double dh_safety )
+{
+   const int tid = threadIdx.x, w = tid >> 6;
+#  pragma unroll
+   for (int pz=0; pz<2; pz++)
+#     pragma unroll
+      for (int q=0; q<4; q++)
+      {
+         float m = ( pxy == q ) ? (float)cfl[pz] : 0.0f;
+#        pragma unroll
+         for (int o = 32; o > 0; o >>= 1) m = vmax( m, __shfl_xor( m, o, 64 ) );
+         if ( ( tid & 63 ) == 0 ) red[q + 4*pz][w] = m;
+      }
+   __syncthreads();
+   if ( tid < 8 )
+   {
+      const float m = vmax( vmax( red[tid][0], red[tid][1] ), vmax( red[tid][2], red[tid][3] ) );
+      dt[tid] = (IO)( dh_safety / (double)m );
+   }
+}
+
 } // namespace gamer_amd

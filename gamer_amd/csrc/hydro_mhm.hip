@@ -29,6 +29,7 @@ struct SmemM {
    R f[2*NV][NC];     // upper x (0-4) and y (5-9) face states (conserved, after the Hancock prediction)
    R b[2*NV][NC];     // full-step x (0-4) and y (5-9) fluxes
    int fail;
+   float cfl[8][4];   // fused CFL: per-patch maxima of the 4 waves
 };
 
 // reconstruction-ring column of lane t (i or j in {1,18}, the corners need nothing); false = none
@@ -75,7 +76,8 @@ __device__ __forceinline__ void recon_cell( const R (*wl)[NC], int c, const R wm
 template <typename R, typename IO, int RS, int LIM>
 __global__ void __launch_bounds__( NT ) __attribute__(( amdgpu_waves_per_eu( sizeof(R) == 4 ? 2 : 1 ) ))
 mhm_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict__ g_flux,
-            int npg, int store_flux, int minmod_maxiter, R minmod_coeff, Prm<R> P )
+            int npg, int store_flux, int minmod_maxiter, R minmod_coeff, Prm<R> P,
+               IO *__restrict__ g_dt, double dh_safety )
 {
    __shared__ SmemM<R> S;
 
@@ -103,6 +105,7 @@ mhm_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict_
          Q.coeff = vmax( cf, (R)0 );
       }
       if ( tid == 0 ) S.fail = 0;
+      IO cfl[2] = { (IO)0, (IO)0 };            // fused CFL: max(|v_d|+c_s) of this column in patches z=0,1
 
       R kWm[NV], kW0[NV], kWp[NV], kU0[NV], kUp[NV];        // core column: primitive planes t-2, t-1, t; input t-1, t
       R oWm[NV], oW0[NV], oWp[NV], oU0[NV], oUp[NV];        // ring column
@@ -251,6 +254,7 @@ mhm_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict_
 #           pragma unroll
             for (int v=0; v<NV; v++) out[ (size_t)v*PS*PS*PS + co ] = res[v];
             if ( bad ) S.fail = 1;
+            if ( g_dt ) cfl[ ( o - 2 )/8 ] = vmax( cfl[ ( o - 2 )/8 ], cell_cfl( res, (double)Q.gamma, Q.gm1_d, Q.min_pres_d ) );
          }
          if ( r >= 2 && r <= 18 )
 #           pragma unroll
@@ -261,6 +265,7 @@ mhm_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict_
          (void)orx;
       } // for t
 
+      if ( g_dt ) fused_dt( cfl, ( ki - 2 )/8 + 2*( ( kj - 2 )/8 ), S.cfl, g_dt + (size_t)pg*8, dh_safety );
       __syncthreads();
       const bool redo = ( S.fail != 0 ) && ( iter + 1 <= minmod_maxiter );
       __syncthreads();
@@ -270,33 +275,33 @@ mhm_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict_
 
 template <typename R, typename IO, int RS>
 void launch_rs( const IO *in, IO *out, IO *flux, int npg, int store_flux, int minmod_maxiter, double minmod_coeff,
-                const Prm<R> &P, hipStream_t stream )
+                const Prm<R> &P, IO *dt, double dh_safety, hipStream_t stream )
 {
    const dim3 grid( npg ), block( NT );
    if ( P.limiter == 4 )
-      hipLaunchKernelGGL( ( mhm_kernel<R, IO, RS, 4> ), grid, block, 0, stream, in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P );
+      hipLaunchKernelGGL( ( mhm_kernel<R, IO, RS, 4> ), grid, block, 0, stream, in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P, dt, dh_safety );
    else
-      hipLaunchKernelGGL( ( mhm_kernel<R, IO, RS, 0> ), grid, block, 0, stream, in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P );
+      hipLaunchKernelGGL( ( mhm_kernel<R, IO, RS, 0> ), grid, block, 0, stream, in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P, dt, dh_safety );
 }
 
 } // namespace
 
 template <typename R, typename IO>
 int launch_mhm( const IO *in, IO *out, IO *flux, int npg, int rsolver, int store_flux, int minmod_maxiter,
-                double minmod_coeff, const Prm<R> &P, hipStream_t stream )
+                double minmod_coeff, const Prm<R> &P, IO *dt, double dh_safety, hipStream_t stream )
 {
    if ( npg <= 0 ) return 0;
    switch ( rsolver )
    {
-      case 4: launch_rs<R, IO, 4>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, stream ); break;
-      case 3: launch_rs<R, IO, 3>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, stream ); break;
-      case 2: launch_rs<R, IO, 2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, stream ); break;
+      case 4: launch_rs<R, IO, 4>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, stream ); break;
+      case 3: launch_rs<R, IO, 3>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, stream ); break;
+      case 2: launch_rs<R, IO, 2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, stream ); break;
       default: return -1;
    }
    return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
 }
 
-template int launch_mhm<double, double>( const double*, double*, double*, int, int, int, int, double, const Prm<double>&, hipStream_t );
-template int launch_mhm<float,  float >( const float*,  float*,  float*,  int, int, int, int, double, const Prm<float>&,  hipStream_t );
+template int launch_mhm<double, double>( const double*, double*, double*, int, int, int, int, double, const Prm<double>&, double*, double, hipStream_t );
+template int launch_mhm<float,  float >( const float*,  float*,  float*,  int, int, int, int, double, const Prm<float>&,  float*, double, hipStream_t );
 
 } // namespace gamer_amd

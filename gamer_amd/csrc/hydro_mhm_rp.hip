@@ -160,12 +160,14 @@ struct SmemRP {
    R b[XY2 ? 2*NV : NV][NC];    // half-step or full-step x (0-4) [and y (5-9)] fluxes
    R w[NV][NC];                 // half-step primitive plane read by the reconstruction
    int fail;
+   float cfl[8][4];   // fused CFL: per-patch maxima of the 4 waves
 };
 
 template <typename R, typename IO, int RS, int LIM, bool XY2>
 __global__ void __launch_bounds__( NT ) __attribute__(( amdgpu_waves_per_eu( sizeof(R) == 4 ? WPE32 : WPE64 ) ))
 mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict__ g_flux,
-               int npg, int store_flux, int minmod_maxiter, R minmod_coeff, Prm<R> P )
+               int npg, int store_flux, int minmod_maxiter, R minmod_coeff, Prm<R> P,
+               IO *__restrict__ g_dt, double dh_safety )
 {
    __shared__ SmemRP<R, XY2> S;
    constexpr int YB = XY2 ? NV : 0;         // offset of the y fluxes in S.b
@@ -192,6 +194,7 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
          Q.coeff = vmax( cf, (R)0 );
       }
       if ( tid == 0 ) S.fail = 0;
+      IO cfl[2] = { (IO)0, (IO)0 };            // fused CFL: max(|v_d|+c_s) of this column in patches z=0,1
 
       // lane state carried along z (core column K, outer column O)
       R kU[8], kU1[8];                            // input cell of planes t and t-1 (+ pressure, 1/rho, c_s)
@@ -446,6 +449,7 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
 #           pragma unroll
             for (int v=0; v<NV; v++) out[ (size_t)v*PS*PS*PS + co ] = res[v];
             if ( bad ) S.fail = 1;
+            if ( g_dt ) cfl[ ( o - 3 )/8 ] = vmax( cfl[ ( o - 3 )/8 ], cell_cfl( res, (double)Q.gamma, Q.gm1_d, Q.min_pres_d ) );
          }
          if ( r >= 3 && r <= 19 )
 #           pragma unroll
@@ -455,6 +459,7 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
             for (int v=0; v<NV; v++) kDf[v] = dfn[v];
       } // for t
 
+      if ( g_dt ) fused_dt( cfl, ( K.i - 3 )/8 + 2*( ( K.j - 3 )/8 ), S.cfl, g_dt + (size_t)pg*8, dh_safety );
       __syncthreads();
       const bool redo = ( S.fail != 0 ) && ( iter + 1 <= minmod_maxiter );
       __syncthreads();
@@ -466,32 +471,32 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
 
 template <typename R, typename IO, int RS, bool XY2>
 static void launch_rs( const IO *in, IO *out, IO *flux, int npg, int store_flux, int minmod_maxiter,
-                       double minmod_coeff, const Prm<R> &P, hipStream_t stream )
+                       double minmod_coeff, const Prm<R> &P, IO *dt, double dh_safety, hipStream_t stream )
 {
    const dim3 grid( npg ), block( NT );
    if ( P.limiter == 4 )
-      hipLaunchKernelGGL( ( mhm_rp_kernel<R, IO, RS, 4, XY2> ), grid, block, 0, stream, in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P );
+      hipLaunchKernelGGL( ( mhm_rp_kernel<R, IO, RS, 4, XY2> ), grid, block, 0, stream, in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P, dt, dh_safety );
    else
-      hipLaunchKernelGGL( ( mhm_rp_kernel<R, IO, RS, 0, XY2> ), grid, block, 0, stream, in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P );
+      hipLaunchKernelGGL( ( mhm_rp_kernel<R, IO, RS, 0, XY2> ), grid, block, 0, stream, in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P, dt, dh_safety );
 }
 
 template <typename R, typename IO>
 int launch_mhm_rp( const IO *in, IO *out, IO *flux, int npg, int rsolver, int store_flux, int minmod_maxiter,
-                   double minmod_coeff, const Prm<R> &P, hipStream_t stream )
+                   double minmod_coeff, const Prm<R> &P, IO *dt, double dh_safety, hipStream_t stream )
 {
    if ( npg <= 0 ) return 0;
    constexpr bool XY2 = true;                  // separate x and y flux buffers (one barrier per flux stage)
    switch ( rsolver )
    {
-      case 4: launch_rs<R, IO, 4, XY2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, stream ); break;
-      case 3: launch_rs<R, IO, 3, XY2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, stream ); break;
-      case 2: launch_rs<R, IO, 2, XY2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, stream ); break;
+      case 4: launch_rs<R, IO, 4, XY2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, stream ); break;
+      case 3: launch_rs<R, IO, 3, XY2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, stream ); break;
+      case 2: launch_rs<R, IO, 2, XY2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, stream ); break;
       default: return -1;
    }
    return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
 }
 
-template int launch_mhm_rp<double, double>( const double*, double*, double*, int, int, int, int, double, const Prm<double>&, hipStream_t );
-template int launch_mhm_rp<float,  float >( const float*,  float*,  float*,  int, int, int, int, double, const Prm<float>&,  hipStream_t );
+template int launch_mhm_rp<double, double>( const double*, double*, double*, int, int, int, int, double, const Prm<double>&, double*, double, hipStream_t );
+template int launch_mhm_rp<float,  float >( const float*,  float*,  float*,  int, int, int, int, double, const Prm<float>&,  float*, double, hipStream_t );
 
 } // namespace gamer_amd

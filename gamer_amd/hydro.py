@@ -28,7 +28,8 @@ PRECISION = {"fp64": 0, "fp32": 1}
 NX = {"MHM_RP": 22, "MHM": 20}
 
 EXPORTS = ("hydro_init", "hydro_mhm_solve_async", "hydro_synchronize", "hydro_last_error",
-           "hydro_mem_allocate_fluid", "hydro_mem_free_fluid", "hydro_host_buffers", "hydro_asyn_fluid_solver")
+           "hydro_mem_allocate_fluid", "hydro_mem_free_fluid", "hydro_host_buffers", "hydro_asyn_fluid_solver",
+           "hydro_dt_cfl_async", "hydro_prepare_uniform_async", "hydro_close_uniform_async")
 
 
 class HydroError(RuntimeError):
@@ -43,6 +44,7 @@ class HydroSolveArgs(ctypes.Structure):
         ("minmod_coeff", ctypes.c_double), ("minmod_maxiter", ctypes.c_int),
         ("min_dens", ctypes.c_double), ("min_pres", ctypes.c_double), ("min_eint", ctypes.c_double),
         ("store_flux", ctypes.c_int), ("precision", ctypes.c_int),
+        ("dt_patch", ctypes.c_void_p), ("safety", ctypes.c_double),
     ]
 
 
@@ -62,6 +64,11 @@ def lib():
         L.hydro_mem_allocate_fluid.argtypes = [ctypes.c_int] * 4
         L.hydro_host_buffers.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_void_p)] * 3
         L.hydro_asyn_fluid_solver.argtypes = [ctypes.c_void_p] * 3 + [ctypes.POINTER(HydroSolveArgs), ctypes.c_int]
+        L.hydro_dt_cfl_async.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_void_p]
+        L.hydro_prepare_uniform_async.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 5 + [ctypes.c_void_p]
+        L.hydro_close_uniform_async.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 4 + [ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -73,21 +80,37 @@ def _check(rc: int):
 
 def make_args(npg, dt, dh, gamma=5.0 / 3.0, scheme="MHM_RP", rsolver="HLLC", limiter="VL_GMINMOD",
               minmod_coeff=1.5, minmod_maxiter=0, min_dens=0.0, min_pres=0.0, min_eint=0.0, store_flux=True,
-              precision="fp64", in_=None, out=None, flux=None) -> HydroSolveArgs:
+              precision="fp64", in_=None, out=None, flux=None, dt_patch=None, safety=0.0) -> HydroSolveArgs:
     return HydroSolveArgs(in_, out, flux, int(npg), float(dt), float(dh), float(gamma), SCHEME[scheme],
                           RSOLVER[rsolver], LIMITER[limiter], float(minmod_coeff), int(minmod_maxiter),
                           float(min_dens), float(min_pres), float(min_eint), int(bool(store_flux)),
-                          PRECISION[precision])
+                          PRECISION[precision], dt_patch, float(safety))
+
+
+def _stream(dev, stream):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream(dev)
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _prec(t):
+    import torch
+    if t.dtype == torch.float64:
+        return PRECISION["fp64"]
+    if t.dtype == torch.float32:
+        return PRECISION["fp32"]
+    raise HydroError(f"unsupported dtype {t.dtype}")
 
 
 def fluid_solver(u, dt, dh, gamma=5.0 / 3.0, scheme="MHM_RP", rsolver="HLLC", limiter="VL_GMINMOD",
                  minmod_coeff=1.5, minmod_maxiter=0, min_dens=0.0, min_pres=0.0, min_eint=0.0, store_flux=True,
-                 out=None, flux=None, stream=None):
+                 out=None, flux=None, stream=None, dt_patch=None, safety=0.5):
     """One patch-group solver call on device tensors.
 
     u: torch tensor on a ROCm device, float64 (FP64 path) or float32 (FP32-bulk path), shape (npg, 5, N, N, N).
     Returns (out (npg,5,16,16,16), flux (npg,9,5,16,16) or None), enqueued on ``stream`` (default: torch's
-    current stream) — synchronise before reading on the host.
+    current stream) — synchronise before reading on the host.  With ``dt_patch`` (a (npg, 8) tensor) the CFL dt
+    of the 8 updated patches of every patch group is computed in the same kernel (index px + 2 py + 4 pz).
     """
     import torch
     if not u.is_cuda:
@@ -104,12 +127,76 @@ def fluid_solver(u, dt, dh, gamma=5.0 / 3.0, scheme="MHM_RP", rsolver="HLLC", li
     if store_flux and flux is None:
         flux = torch.empty((npg, 9, 5, 16, 16), dtype=u.dtype, device=u.device)
     prec = "fp64" if u.dtype == torch.float64 else "fp32"
+    if dt_patch is not None and (tuple(dt_patch.shape) != (npg, 8) or dt_patch.dtype != u.dtype):
+        raise HydroError("dt_patch must be a (npg, 8) tensor of the input dtype")
     args = make_args(npg, dt, dh, gamma, scheme, rsolver, limiter, minmod_coeff, minmod_maxiter, min_dens, min_pres,
                      min_eint, store_flux, prec, u.data_ptr(), out.data_ptr(),
-                     flux.data_ptr() if store_flux else None)
-    s = stream if stream is not None else torch.cuda.current_stream(u.device)
-    _check(lib().hydro_mhm_solve_async(ctypes.byref(args), ctypes.c_void_p(s.cuda_stream)))
+                     flux.data_ptr() if store_flux else None,
+                     dt_patch.data_ptr() if dt_patch is not None else None, safety)
+    _check(lib().hydro_mhm_solve_async(ctypes.byref(args), _stream(u.device, stream)))
     return out, (flux if store_flux else None)
+
+
+def dt_cfl(u_patch, dh, safety, gamma=5.0 / 3.0, min_pres=0.0, out=None, stream=None):
+    """Per-patch CFL dt of (npatch, 5, 8, 8, 8) device data (CUAPI_Asyn_dtSolver for the hydro CFL)."""
+    import torch
+    u_patch = u_patch.contiguous()
+    if u_patch.dim() != 5 or tuple(u_patch.shape[1:]) != (5, 8, 8, 8):
+        raise HydroError(f"patch shape {tuple(u_patch.shape)} != (npatch, 5, 8, 8, 8)")
+    if out is None:
+        out = torch.empty(u_patch.shape[0], dtype=u_patch.dtype, device=u_patch.device)
+    _check(lib().hydro_dt_cfl_async(u_patch.data_ptr(), out.data_ptr(), u_patch.shape[0], dh, safety, gamma, min_pres,
+                                    _prec(u_patch), _stream(u_patch.device, stream)))
+    return out
+
+
+def prepare_uniform(level, ghost, out=None, stream=None):
+    """Patch-group inputs (npg, 5, 16+2g, ...) with periodic ghost zones from a device level (5, Z, Y, X)."""
+    import torch
+    level = level.contiguous()
+    _, nz, ny, nx = level.shape
+    n = 16 + 2 * ghost
+    npg = (nx // 16) * (ny // 16) * (nz // 16)
+    if out is None:
+        out = torch.empty((npg, 5, n, n, n), dtype=level.dtype, device=level.device)
+    _check(lib().hydro_prepare_uniform_async(level.data_ptr(), out.data_ptr(), nx, ny, nz, ghost, _prec(level),
+                                             _stream(level.device, stream)))
+    return out
+
+
+def close_uniform(pg_out, level, stream=None):
+    """Scatter (npg, 5, 16, 16, 16) patch-group outputs back into the device level (5, Z, Y, X) in place."""
+    _, nz, ny, nx = level.shape
+    _check(lib().hydro_close_uniform_async(pg_out.contiguous().data_ptr(), level.data_ptr(), nx, ny, nz,
+                                           _prec(level), _stream(level.device, stream)))
+    return level
+
+
+class UniformLevel:
+    """A periodic uniform level advanced entirely on the device: ghost-zone preparation -> patch-group solver
+    (with the fused CFL dt) -> scatter back, one pass of the Flu_AdvanceDt pipeline (src/Fluid/Flu_AdvanceDt.cpp:38-100)
+    for a single level without refinement."""
+
+    def __init__(self, level, dh, scheme="MHM_RP", safety=0.5, **solver_kw):
+        import torch
+        self.level = level.contiguous()
+        self.dh, self.scheme, self.safety, self.kw = dh, scheme, safety, solver_kw
+        self.ghost = 3 if scheme == "MHM_RP" else 2
+        _, nz, ny, nx = level.shape
+        self.npg = (nx // 16) * (ny // 16) * (nz // 16)
+        n = 16 + 2 * self.ghost
+        self.pg_in = torch.empty((self.npg, 5, n, n, n), dtype=level.dtype, device=level.device)
+        self.pg_out = torch.empty((self.npg, 5, 16, 16, 16), dtype=level.dtype, device=level.device)
+        self.flux = torch.empty((self.npg, 9, 5, 16, 16), dtype=level.dtype, device=level.device)
+        self.dt_patch = torch.empty((self.npg, 8), dtype=level.dtype, device=level.device)
+
+    def step(self, dt):
+        """Advance by dt; returns the CFL-limited dt of the new state (device scalar tensor)."""
+        prepare_uniform(self.level, self.ghost, out=self.pg_in)
+        fluid_solver(self.pg_in, dt, self.dh, scheme=self.scheme, out=self.pg_out, flux=self.flux,
+                     dt_patch=self.dt_patch, safety=self.safety, **self.kw)
+        close_uniform(self.pg_out, self.level)
+        return self.dt_patch.min()
 
 
 class GpuFluidApi:

@@ -52,6 +52,9 @@ typedef struct HydroSolveArgs {
    double min_dens, min_pres, min_eint;
    int    store_flux;      /* StoreFlux */
    int    precision;       /* HYDRO_FP64 or HYDRO_FP32_BULK */
+   void  *dt_patch;        /* optional device pointer [npg][8]: fused CFL dt of the updated patches
+                              (index px + 2 py + 4 pz within the PG); NULL = not computed */
+   double safety;          /* CFL safety factor of the fused dt (DT__FLUID) */
 } HydroSolveArgs;
 
 /* Select the HIP device (replaces CUAPI_SetDevice, src/GPU_API/CUAPI_SetDevice.cu). */
@@ -61,6 +64,22 @@ int hydro_init( int device );
    Returns after the launch; results are valid once the stream is synchronised.  Replaces the kernel launch
    inside CUAPI_Asyn_FluidSolver (src/GPU_API/CUAPI_Asyn_FluidSolver.cu:557-576). */
 int hydro_mhm_solve_async( const HydroSolveArgs *args, void *stream );
+
+/* Per-patch hydro CFL time step on device arrays: in[npatch][5][8][8][8] -> dt[npatch],
+   dt = safety*dh / max_cells max_d(|v_d| + c_s) (replaces CUAPI_Asyn_dtSolver for DT_FLU_SOLVER,
+   src/GPU_API/CUAPI_Asyn_dtSolver.cu:88, kernel src/Model_Hydro/CPU_Hydro/CPU_dtSolver_HydroCFL.cpp:57-209). */
+int hydro_dt_cfl_async( const void *in, void *dt, int npatch, double dh, double safety, double gamma,
+                        double min_pres, int precision, void *stream );
+
+/* Ghost-zone preparation for one periodic uniform level held on the device as level[5][nz][ny][nx]
+   (nx, ny, nz multiples of 16): writes pg_in[npg][5][N^3], N = 16 + 2*ghost, patch groups numbered x fastest
+   (the same-level periodic case of Prepare_PatchData for the fluid solver, src/Fluid/Flu_Prepare.cpp:91-94). */
+int hydro_prepare_uniform_async( const void *level, void *pg_in, int nx, int ny, int nz, int ghost, int precision,
+                                 void *stream );
+
+/* Scatter of patch-group outputs pg_out[npg][5][16^3] back into level[5][nz][ny][nx]
+   (the copy-back step of Flu_Close, src/Fluid/Flu_Close.cpp:133-217). */
+int hydro_close_uniform_async( const void *pg_out, void *level, int nx, int ny, int nz, int precision, void *stream );
 
 /* Wait for all work queued by this library (replaces CUAPI_Synchronize, src/GPU_API/CUAPI_Synchronize.cu:11-14). */
 int hydro_synchronize( void );
