@@ -84,6 +84,67 @@ def cpu_baseline(u64_host, w, sample_pg, prec):
                       f"{'FP64' if prec == 'f64' else 'FP32'} build, OpenMP dynamic,1"}
 
 
+def riemann_sweep(args):
+    """Roe vs HLLE vs HLLC on a Kelvin-Helmholtz shear layer (rho 1/2, v_x -/+0.5, P 2.5, seeded 1% sinusoidal
+    v_y at wavenumbers 1-4), NPG = 8192: FP32-bulk and FP64 GPU results against the FP64 oracle on every patch
+    group, per-field relative L1/Linf, conserved-sum drift, and kernel throughput per solver."""
+    import torch
+    from gamer_amd import hydro, ic
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle
+    from parity import rel_l1, rel_err_cells
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    shape = (32, 16, 16)
+    print(f"[sweep] generating KH workload {shape}", file=sys.stderr, flush=True)
+    w = ic.kelvin_helmholtz(shape, seed=123)
+    dev = torch.device("cuda:0")
+    npg = w.npg
+    cells = npg * CELLS_PER_PG
+    tot_in = ic.pg_interior(w.u, 3).sum(axis=(0, 2, 3, 4))
+    abs_in = np.abs(ic.pg_interior(w.u, 3)).sum(axis=(0, 2, 3, 4))
+    res = {"metric": "Riemann-solver sweep: FP32-bulk and FP64 GPU vs FP64 oracle (KH shear layer)",
+           "config": {"workload": f"KH shear layer, NPG={npg}, MHM_RP+PLM(VL_GMINMOD)+{{HLLC,HLLE,ROE}}, one step",
+                      "npg": npg, "dt": w.dt, "dh": w.dh}, "solvers": {}}
+    for rs in ("HLLC", "HLLE", "ROE"):
+        print(f"[sweep] {rs}: oracle on {npg} patch groups ({nthreads} threads)", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        ro, _ = oracle.fluid_solver(w.u, w.dt, w.dh, rsolver=rs, nthreads=nthreads)
+        t_cpu = time.perf_counter() - t0
+        entry = {"cpu_oracle_seconds": round(t_cpu, 2)}
+        for prec, tdt in (("fp32", torch.float32), ("fp64", torch.float64)):
+            u = torch.from_numpy(w.u).to(dev, dtype=tdt)
+            out, flux = hydro.fluid_solver(u, w.dt, w.dh, rsolver=rs)
+            torch.cuda.synchronize()
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ev0.record()
+            for _ in range(args.steps):
+                hydro.fluid_solver(u, w.dt, w.dh, rsolver=rs, out=out, flux=flux)
+            ev1.record()
+            torch.cuda.synchronize()
+            kms = ev0.elapsed_time(ev1) / args.steps
+            o = out.cpu().numpy().astype(np.float64)
+            d = np.abs(o - ro)
+            linf = d.max(axis=(0, 2, 3, 4)) / _field_scale(np.abs(ro).max(axis=(0, 2, 3, 4)), np.max)
+            drift = (o.sum(axis=(0, 2, 3, 4)) - tot_in) / _field_scale(abs_in, np.max)
+            entry[prec] = {"rel_l1": [float(x) for x in rel_l1(o, ro)], "rel_linf": [float(x) for x in linf],
+                           "max_cell_rel_err": [float(x) for x in rel_err_cells(o, ro)],
+                           "conserved_sum_drift": [float(x) for x in drift],
+                           "kernel_ms": kms, "cell_updates_per_s": cells / (kms * 1e-3)}
+            del u, out, flux
+            torch.cuda.empty_cache()
+        res["solvers"][rs] = entry
+    print(json.dumps(res))
+
+
+def _field_scale(per_field, agg):
+    """Per-field normalisers where the three momentum components share one scale (a component that is zero
+    everywhere, e.g. m_z of a 2D shear layer, is measured against the momentum magnitude, not against 0)."""
+    sc = np.array(per_field, dtype=np.float64)
+    sc[1:4] = agg(sc[1:4])
+    return sc
+
+
 def load_traffic(name):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -100,6 +161,8 @@ def main():
     ap.add_argument("--npg-axis", type=int, default=16, help="patch groups per axis (16 -> NPG 4096)")
     ap.add_argument("--cpu-sample", type=int, default=1024, help="patch groups in the CPU baseline sample (0: skip)")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--riemann-sweep", action="store_true",
+                    help="run the Roe/HLLE/HLLC accuracy sweep on the KH workload instead of the throughput bench")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (3x), to scale rocprof byte counters")
     args = ap.parse_args()
@@ -108,6 +171,10 @@ def main():
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); the solver's "
               f"multi-GPU sharding is a design note, not a run.", file=sys.stderr)
         sys.exit(2)
+
+    if args.riemann_sweep:
+        riemann_sweep(args)
+        return
 
     import torch
     from gamer_amd import hydro, ic

@@ -88,18 +88,22 @@ Prm<R> make_prm( const HydroSolveArgs *a )
    return P;
 }
 
-int enqueue( const HydroSolveArgs *a, const void *in, void *out, void *flux, void *dt, int npg, hipStream_t s )
+int enqueue( const HydroSolveArgs *a, const void *in, void *out, void *flux, void *dt, int npg, hipStream_t s,
+             const Batch *batch = nullptr, int grid_blocks = 0 )
 {
+   Batch nb;
+   nb.pg_dt_dh = nullptr;  nb.counter = nullptr;  nb.safety = 0.0;
+   const Batch &B = batch ? *batch : nb;
    int rc;
    const bool rp = ( a->scheme == HYDRO_MHM_RP );
    if ( a->precision == HYDRO_FP64 )
       rc = ( rp ? launch_mhm_rp<double, double> : launch_mhm<double, double> )
            ( (const double*)in, (double*)out, (double*)flux, npg, a->rsolver, a->store_flux, a->minmod_maxiter,
-             a->minmod_coeff, make_prm<double>( a ), (double*)dt, a->safety*a->dh, s );
+             a->minmod_coeff, make_prm<double>( a ), (double*)dt, a->safety*a->dh, B, grid_blocks, s );
    else
       rc = ( rp ? launch_mhm_rp<float, float> : launch_mhm<float, float> )
            ( (const float*)in, (float*)out, (float*)flux, npg, a->rsolver, a->store_flux, a->minmod_maxiter,
-             a->minmod_coeff, make_prm<float>( a ), (float*)dt, a->safety*a->dh, s );
+             a->minmod_coeff, make_prm<float>( a ), (float*)dt, a->safety*a->dh, B, grid_blocks, s );
    if ( rc != 0 ) return fail( HYDRO_E_DEVICE, "kernel launch failed (%d): %s", rc, hipGetErrorString( hipGetLastError() ) );
    return HYDRO_OK;
 }
@@ -180,6 +184,30 @@ int hydro_close_uniform_async( const void *pg_out, void *level, int nx, int ny, 
    else if ( precision == HYDRO_FP32_BULK ) rc = launch_scatter<float> ( (const float*)pg_out,  (float*)level,  nx, ny, nz, (hipStream_t)stream );
    else return fail( HYDRO_E_ARG, "unsupported precision %d", precision );
    return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "close launch failed" );
+}
+
+int hydro_mhm_solve_batch_async( const HydroSolveArgs *a, const double *pg_dt_dh, unsigned *queue_counter,
+                                 int groups_per_cu, void *stream )
+{
+   int rc = check_args( a );
+   if ( rc != HYDRO_OK ) return rc;
+   Batch B;
+   B.pg_dt_dh = pg_dt_dh;
+   B.counter  = queue_counter;
+   B.safety   = a->safety;
+   int grid = 0;
+   if ( queue_counter != nullptr )
+   {
+      if ( groups_per_cu < 1 || groups_per_cu > 16 ) return fail( HYDRO_E_ARG, "batch: groups_per_cu must be 1..16" );
+      int dev = 0, ncu = 0;
+      HIP_TRY( hipGetDevice( &dev ) );
+      HIP_TRY( hipDeviceGetAttribute( &ncu, hipDeviceAttributeMultiprocessorCount, dev ) );
+      grid = groups_per_cu*ncu;
+      if ( grid > a->npg ) grid = a->npg;
+      if ( a->npg > 0 && launch_reset_counter( queue_counter, (hipStream_t)stream ) != 0 )
+         return fail( HYDRO_E_DEVICE, "queue reset launch failed" );
+   }
+   return enqueue( a, a->in, a->out, a->flux, a->dt_patch, a->npg, (hipStream_t)stream, &B, grid );
 }
 
 int hydro_synchronize( void )

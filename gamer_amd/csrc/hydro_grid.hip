@@ -97,6 +97,8 @@ __global__ void scatter_pg_kernel( const IO *__restrict__ pg_out, IO *__restrict
    }
 }
 
+__global__ void reset_counter_kernel( unsigned *c ) { if ( threadIdx.x == 0 ) *c = 0u; }
+
 int grid_blocks( long total )
 {
    long b = ( total + 255 )/256;
@@ -130,6 +132,12 @@ int launch_scatter( const IO *pg_out, IO *grid, int nx, int ny, int nz, hipStrea
    const long total = (long)( nx/PS )*( ny/PS )*( nz/PS )*NV*4096;
    if ( total <= 0 ) return 0;
    hipLaunchKernelGGL( scatter_pg_kernel<IO>, dim3( grid_blocks( total ) ), dim3( 256 ), 0, stream, pg_out, grid, nx, ny, nz, total );
+   return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
+}
+
+int launch_reset_counter( unsigned *counter, hipStream_t stream )
+{
+   hipLaunchKernelGGL( reset_counter_kernel, dim3( 1 ), dim3( 64 ), 0, stream, counter );
    return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
 }
 

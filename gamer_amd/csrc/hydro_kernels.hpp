@@ -8,12 +8,14 @@ namespace gamer_amd {
 // MHM_RP patch-group update: in[npg][5][22^3] -> out[npg][5][16^3] (+ flux[npg][9][5][16^2] if store_flux)
 template <typename R, typename IO>
 int launch_mhm_rp( const IO *in, IO *out, IO *flux, int npg, int rsolver, int store_flux, int minmod_maxiter,
-                   double minmod_coeff, const Prm<R> &P, IO *dt, double dh_safety, hipStream_t stream );
+                   double minmod_coeff, const Prm<R> &P, IO *dt, double dh_safety, const Batch &B, int grid_blocks,
+                   hipStream_t stream );
 
 // MHM patch-group update: in[npg][5][20^3] -> out[npg][5][16^3] (+ flux[npg][9][5][16^2] if store_flux)
 template <typename R, typename IO>
 int launch_mhm( const IO *in, IO *out, IO *flux, int npg, int rsolver, int store_flux, int minmod_maxiter,
-                double minmod_coeff, const Prm<R> &P, IO *dt, double dh_safety, hipStream_t stream );
+                double minmod_coeff, const Prm<R> &P, IO *dt, double dh_safety, const Batch &B, int grid_blocks,
+                   hipStream_t stream );
 
 // per-patch CFL dt: in[npatch][5][8^3] -> dt[npatch]
 template <typename IO>
@@ -25,5 +27,8 @@ int launch_gather( const IO *grid, IO *pg_in, int nx, int ny, int nz, int ghost,
 // patch-group outputs [npg][5][16^3] -> level [5][nz][ny][nx]
 template <typename IO>
 int launch_scatter( const IO *pg_out, IO *grid, int nx, int ny, int nz, hipStream_t stream );
+
+// set the work-queue counter to 0 (a kernel, so the reset can live inside a captured graph)
+int launch_reset_counter( unsigned *counter, hipStream_t stream );
 
 } // namespace gamer_amd

@@ -28,7 +28,7 @@ struct SmemM {
    R w[2][NV][NC];    // primitive planes t-1 and t (ping-pong)
    R f[2*NV][NC];     // upper x (0-4) and y (5-9) face states (conserved, after the Hancock prediction)
    R b[2*NV][NC];     // full-step x (0-4) and y (5-9) fluxes
-   int fail;
+   int fail, next;
    float cfl[8][4];   // fused CFL: per-patch maxima of the 4 waves
 };
 
@@ -73,21 +73,15 @@ __device__ __forceinline__ void recon_cell( const R (*wl)[NC], int c, const R wm
    hancock<R, IO>( fc, fp, u0, P );
 }
 
-template <typename R, typename IO, int RS, int LIM>
+template <typename R, typename IO, int RS, int LIM, bool PERSIST>
 __global__ void __launch_bounds__( NT ) __attribute__(( amdgpu_waves_per_eu( sizeof(R) == 4 ? 2 : 1 ) ))
 mhm_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict__ g_flux,
-            int npg, int store_flux, int minmod_maxiter, R minmod_coeff, Prm<R> P,
-               IO *__restrict__ g_dt, double dh_safety )
+            int npg, int store_flux, int minmod_maxiter, R minmod_coeff, Prm<R> P0,
+               IO *__restrict__ g_dt, double dh_safety0, Batch B )
 {
    __shared__ SmemM<R> S;
 
-   const int pg  = blockIdx.x;
    const int tid = threadIdx.x;
-   if ( pg >= npg ) return;
-
-   const IO *in  = g_in  + (size_t)pg*NV*NC*NX;
-   IO       *out = g_out + (size_t)pg*NV*PS*PS*PS;
-   IO       *flx = store_flux ? g_flux + (size_t)pg*9*NV*PS*PS : nullptr;
 
    const int ki = 2 + tid % PS, kj = 2 + tid / PS, kc = ki + NX*kj;
    int oi, oj;
@@ -96,6 +90,35 @@ mhm_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict_
    const bool orx = has_o && ( oi == 1 || oi == 18 );      // ring column on an x side: its x faces are needed
    const bool ofx = has_o && oi == 18;                      // owns the full-step x face (17,18)
    const bool ofy = has_o && oj == 18;
+
+   for (int item = 0; ; item++)
+   {
+   // next patch group: blockIdx.x, or the next ticket of the persistent work queue
+   int pg;
+   if ( PERSIST )
+   {
+      __syncthreads();
+      if ( tid == 0 ) S.next = (int)atomicAdd( B.counter, 1u );
+      __syncthreads();
+      pg = __builtin_amdgcn_readfirstlane( S.next );      // uniform: keep pointers and dt in SGPRs
+   }
+   else
+   {
+      if ( item > 0 ) break;
+      pg = blockIdx.x;
+   }
+   if ( pg >= npg ) break;
+
+   const IO *in  = g_in  + (size_t)pg*NV*NC*NX;
+   IO       *out = g_out + (size_t)pg*NV*PS*PS*PS;
+   IO       *flx = store_flux ? g_flux + (size_t)pg*9*NV*PS*PS : nullptr;
+   Prm<R> P = P0;
+   double dh_safety = dh_safety0;
+   if ( B.pg_dt_dh )
+   {
+      const double dt = B.pg_dt_dh[2*pg], dh = B.pg_dt_dh[2*pg + 1];
+      P.dt_dh2 = 0.5*dt/dh;  P.dt_dh = dt/dh;  dh_safety = B.safety*dh;
+   }
 
    for (int iter = 0; ; iter++)
    {
@@ -270,38 +293,41 @@ mhm_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict_
       const bool redo = ( S.fail != 0 ) && ( iter + 1 <= minmod_maxiter );
       __syncthreads();
       if ( !redo ) break;
-   }
+   } // MinMod retry
+   } // work items
 }
 
 template <typename R, typename IO, int RS>
 void launch_rs( const IO *in, IO *out, IO *flux, int npg, int store_flux, int minmod_maxiter, double minmod_coeff,
-                const Prm<R> &P, IO *dt, double dh_safety, hipStream_t stream )
+                const Prm<R> &P, IO *dt, double dh_safety, const Batch &B, int grid_blocks, hipStream_t stream )
 {
-   const dim3 grid( npg ), block( NT );
-   if ( P.limiter == 4 )
-      hipLaunchKernelGGL( ( mhm_kernel<R, IO, RS, 4> ), grid, block, 0, stream, in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P, dt, dh_safety );
-   else
-      hipLaunchKernelGGL( ( mhm_kernel<R, IO, RS, 0> ), grid, block, 0, stream, in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P, dt, dh_safety );
+   const dim3 grid( grid_blocks > 0 ? grid_blocks : npg ), block( NT );
+   const bool pers = ( B.counter != nullptr );
+#  define GA_LAUNCH( LIM_, PERS_ ) hipLaunchKernelGGL( ( mhm_kernel<R, IO, RS, LIM_, PERS_> ), grid, block, 0, stream, \
+                                                      in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P, dt, dh_safety, B )
+   if ( P.limiter == 4 ) { if ( pers ) GA_LAUNCH( 4, true ); else GA_LAUNCH( 4, false ); }
+   else                  { if ( pers ) GA_LAUNCH( 0, true ); else GA_LAUNCH( 0, false ); }
+#  undef GA_LAUNCH
 }
 
 } // namespace
 
 template <typename R, typename IO>
 int launch_mhm( const IO *in, IO *out, IO *flux, int npg, int rsolver, int store_flux, int minmod_maxiter,
-                double minmod_coeff, const Prm<R> &P, IO *dt, double dh_safety, hipStream_t stream )
+                double minmod_coeff, const Prm<R> &P, IO *dt, double dh_safety, const Batch &B, int grid_blocks, hipStream_t stream )
 {
    if ( npg <= 0 ) return 0;
    switch ( rsolver )
    {
-      case 4: launch_rs<R, IO, 4>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, stream ); break;
-      case 3: launch_rs<R, IO, 3>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, stream ); break;
-      case 2: launch_rs<R, IO, 2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, stream ); break;
+      case 4: launch_rs<R, IO, 4>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, B, grid_blocks, stream ); break;
+      case 3: launch_rs<R, IO, 3>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, B, grid_blocks, stream ); break;
+      case 2: launch_rs<R, IO, 2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, B, grid_blocks, stream ); break;
       default: return -1;
    }
    return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
 }
 
-template int launch_mhm<double, double>( const double*, double*, double*, int, int, int, int, double, const Prm<double>&, double*, double, hipStream_t );
-template int launch_mhm<float,  float >( const float*,  float*,  float*,  int, int, int, int, double, const Prm<float>&,  float*, double, hipStream_t );
+template int launch_mhm<double, double>( const double*, double*, double*, int, int, int, int, double, const Prm<double>&, double*, double, const Batch&, int, hipStream_t );
+template int launch_mhm<float,  float >( const float*,  float*,  float*,  int, int, int, int, double, const Prm<float>&,  float*, double, const Batch&, int, hipStream_t );
 
 } // namespace gamer_amd

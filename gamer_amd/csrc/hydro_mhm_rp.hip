@@ -159,32 +159,55 @@ struct SmemRP {
    R a[10][NC];                 // input plane (rho,mx,my,mz,E,P,1/rho,c_s) | upper x (0-4) and y (5-9) face states
    R b[XY2 ? 2*NV : NV][NC];    // half-step or full-step x (0-4) [and y (5-9)] fluxes
    R w[NV][NC];                 // half-step primitive plane read by the reconstruction
-   int fail;
+   int fail, next;
    float cfl[8][4];   // fused CFL: per-patch maxima of the 4 waves
 };
 
-template <typename R, typename IO, int RS, int LIM, bool XY2>
+template <typename R, typename IO, int RS, int LIM, bool XY2, bool PERSIST>
 __global__ void __launch_bounds__( NT ) __attribute__(( amdgpu_waves_per_eu( sizeof(R) == 4 ? WPE32 : WPE64 ) ))
 mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict__ g_flux,
-               int npg, int store_flux, int minmod_maxiter, R minmod_coeff, Prm<R> P,
-               IO *__restrict__ g_dt, double dh_safety )
+               int npg, int store_flux, int minmod_maxiter, R minmod_coeff, Prm<R> P0,
+               IO *__restrict__ g_dt, double dh_safety0, Batch B )
 {
    __shared__ SmemRP<R, XY2> S;
    constexpr int YB = XY2 ? NV : 0;         // offset of the y fluxes in S.b
 
-   const int pg  = blockIdx.x;
    const int tid = threadIdx.x;
-   if ( pg >= npg ) return;
-
-   const IO *in  = g_in  + (size_t)pg*NV*NC*NX;
-   IO       *out = g_out + (size_t)pg*NV*PS*PS*PS;
-   IO       *flx = store_flux ? g_flux + (size_t)pg*9*NV*PS*PS : nullptr;
 
    const Col K = make_col( 3 + tid % PS, 3 + tid / PS );
    int oi, oj;
    const bool has_o = outer_column( tid, oi, oj );
    const Col O = make_col( has_o ? oi : 0, has_o ? oj : 0 );
-   const int  lim = P.limiter;
+   const int  lim = P0.limiter;
+
+   for (int item = 0; ; item++)
+   {
+   // next patch group: blockIdx.x, or the next ticket of the persistent work queue
+   int pg;
+   if ( PERSIST )
+   {
+      __syncthreads();
+      if ( tid == 0 ) S.next = (int)atomicAdd( B.counter, 1u );
+      __syncthreads();
+      pg = __builtin_amdgcn_readfirstlane( S.next );      // uniform: keep pointers and dt in SGPRs
+   }
+   else
+   {
+      if ( item > 0 ) break;
+      pg = blockIdx.x;
+   }
+   if ( pg >= npg ) break;
+
+   const IO *in  = g_in  + (size_t)pg*NV*NC*NX;
+   IO       *out = g_out + (size_t)pg*NV*PS*PS*PS;
+   IO       *flx = store_flux ? g_flux + (size_t)pg*9*NV*PS*PS : nullptr;
+   Prm<R> P = P0;
+   double dh_safety = dh_safety0;
+   if ( B.pg_dt_dh )
+   {
+      const double dt = B.pg_dt_dh[2*pg], dh = B.pg_dt_dh[2*pg + 1];
+      P.dt_dh2 = 0.5*dt/dh;  P.dt_dh = dt/dh;  dh_safety = B.safety*dh;
+   }
 
    for (int iter = 0; ; iter++)
    {
@@ -465,38 +488,41 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
       __syncthreads();
       if ( !redo ) break;
    } // MinMod retry
+   } // work items
 }
 
 } // namespace
 
 template <typename R, typename IO, int RS, bool XY2>
 static void launch_rs( const IO *in, IO *out, IO *flux, int npg, int store_flux, int minmod_maxiter,
-                       double minmod_coeff, const Prm<R> &P, IO *dt, double dh_safety, hipStream_t stream )
+                       double minmod_coeff, const Prm<R> &P, IO *dt, double dh_safety, const Batch &B, int grid_blocks, hipStream_t stream )
 {
-   const dim3 grid( npg ), block( NT );
-   if ( P.limiter == 4 )
-      hipLaunchKernelGGL( ( mhm_rp_kernel<R, IO, RS, 4, XY2> ), grid, block, 0, stream, in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P, dt, dh_safety );
-   else
-      hipLaunchKernelGGL( ( mhm_rp_kernel<R, IO, RS, 0, XY2> ), grid, block, 0, stream, in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P, dt, dh_safety );
+   const dim3 grid( grid_blocks > 0 ? grid_blocks : npg ), block( NT );
+   const bool pers = ( B.counter != nullptr );
+#  define GA_LAUNCH( LIM_, PERS_ ) hipLaunchKernelGGL( ( mhm_rp_kernel<R, IO, RS, LIM_, XY2, PERS_> ), grid, block, 0, stream, \
+                                                      in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P, dt, dh_safety, B )
+   if ( P.limiter == 4 ) { if ( pers ) GA_LAUNCH( 4, true ); else GA_LAUNCH( 4, false ); }
+   else                  { if ( pers ) GA_LAUNCH( 0, true ); else GA_LAUNCH( 0, false ); }
+#  undef GA_LAUNCH
 }
 
 template <typename R, typename IO>
 int launch_mhm_rp( const IO *in, IO *out, IO *flux, int npg, int rsolver, int store_flux, int minmod_maxiter,
-                   double minmod_coeff, const Prm<R> &P, IO *dt, double dh_safety, hipStream_t stream )
+                   double minmod_coeff, const Prm<R> &P, IO *dt, double dh_safety, const Batch &B, int grid_blocks, hipStream_t stream )
 {
    if ( npg <= 0 ) return 0;
    constexpr bool XY2 = true;                  // separate x and y flux buffers (one barrier per flux stage)
    switch ( rsolver )
    {
-      case 4: launch_rs<R, IO, 4, XY2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, stream ); break;
-      case 3: launch_rs<R, IO, 3, XY2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, stream ); break;
-      case 2: launch_rs<R, IO, 2, XY2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, stream ); break;
+      case 4: launch_rs<R, IO, 4, XY2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, B, grid_blocks, stream ); break;
+      case 3: launch_rs<R, IO, 3, XY2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, B, grid_blocks, stream ); break;
+      case 2: launch_rs<R, IO, 2, XY2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, B, grid_blocks, stream ); break;
       default: return -1;
    }
    return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
 }
 
-template int launch_mhm_rp<double, double>( const double*, double*, double*, int, int, int, int, double, const Prm<double>&, double*, double, hipStream_t );
-template int launch_mhm_rp<float,  float >( const float*,  float*,  float*,  int, int, int, int, double, const Prm<float>&,  float*, double, hipStream_t );
+template int launch_mhm_rp<double, double>( const double*, double*, double*, int, int, int, int, double, const Prm<double>&, double*, double, const Batch&, int, hipStream_t );
+template int launch_mhm_rp<float,  float >( const float*,  float*,  float*,  int, int, int, int, double, const Prm<float>&,  float*, double, const Batch&, int, hipStream_t );
 
 } // namespace gamer_amd

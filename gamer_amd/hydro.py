@@ -29,7 +29,8 @@ NX = {"MHM_RP": 22, "MHM": 20}
 
 EXPORTS = ("hydro_init", "hydro_mhm_solve_async", "hydro_synchronize", "hydro_last_error",
            "hydro_mem_allocate_fluid", "hydro_mem_free_fluid", "hydro_host_buffers", "hydro_asyn_fluid_solver",
-           "hydro_dt_cfl_async", "hydro_prepare_uniform_async", "hydro_close_uniform_async")
+           "hydro_dt_cfl_async", "hydro_prepare_uniform_async", "hydro_close_uniform_async",
+           "hydro_mhm_solve_batch_async")
 
 
 class HydroError(RuntimeError):
@@ -69,6 +70,8 @@ def lib():
                                          ctypes.c_void_p]
         L.hydro_prepare_uniform_async.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 5 + [ctypes.c_void_p]
         L.hydro_close_uniform_async.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 4 + [ctypes.c_void_p]
+        L.hydro_mhm_solve_batch_async.argtypes = [ctypes.POINTER(HydroSolveArgs), ctypes.c_void_p, ctypes.c_void_p,
+                                                  ctypes.c_int, ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -134,6 +137,47 @@ def fluid_solver(u, dt, dh, gamma=5.0 / 3.0, scheme="MHM_RP", rsolver="HLLC", li
                      flux.data_ptr() if store_flux else None,
                      dt_patch.data_ptr() if dt_patch is not None else None, safety)
     _check(lib().hydro_mhm_solve_async(ctypes.byref(args), _stream(u.device, stream)))
+    return out, (flux if store_flux else None)
+
+
+def solve_batch(u, pg_dt_dh=None, out=None, flux=None, queue_counter=None, groups_per_cu=2, stream=None,
+                dt_patch=None, safety=0.5, dt=0.0, dh=1.0, **kw):
+    """One launch over patch groups with per-PG time steps (e.g. several AMR levels at once).
+
+    pg_dt_dh: float64 device tensor (npg, 2) of (dt, dh) per patch group (None: scalar dt, dh for all).
+    queue_counter: a 1-element int32 device tensor enables the persistent work-queue mode with ``groups_per_cu``
+    workgroups per compute unit; None = one workgroup per PG.
+    """
+    import torch
+    scheme = kw.pop("scheme", "MHM_RP")
+    n = NX[scheme]
+    if not u.is_cuda or u.dtype not in (torch.float64, torch.float32):
+        raise HydroError("solve_batch needs a float64 or float32 device tensor")
+    u = u.contiguous()
+    if u.dim() != 5 or tuple(u.shape[1:]) != (5, n, n, n):
+        raise HydroError(f"input shape {tuple(u.shape)} != (npg, 5, {n}, {n}, {n})")
+    npg = u.shape[0]
+    if pg_dt_dh is not None and (pg_dt_dh.dtype != torch.float64 or tuple(pg_dt_dh.shape) != (npg, 2)
+                                 or not pg_dt_dh.is_cuda or not pg_dt_dh.is_contiguous()):
+        raise HydroError("pg_dt_dh must be a contiguous float64 device tensor of shape (npg, 2)")
+    if queue_counter is not None and (queue_counter.dtype != torch.int32 or queue_counter.numel() < 1
+                                      or not queue_counter.is_cuda):
+        raise HydroError("queue_counter must be an int32 device tensor with at least one element")
+    if dt_patch is not None and (tuple(dt_patch.shape) != (npg, 8) or dt_patch.dtype != u.dtype):
+        raise HydroError("dt_patch must be a (npg, 8) tensor of the input dtype")
+    if out is None:
+        out = torch.empty((npg, 5, 16, 16, 16), dtype=u.dtype, device=u.device)
+    store_flux = kw.pop("store_flux", True)
+    if store_flux and flux is None:
+        flux = torch.empty((npg, 9, 5, 16, 16), dtype=u.dtype, device=u.device)
+    args = make_args(npg, dt, dh, scheme=scheme, store_flux=store_flux,
+                     precision="fp64" if u.dtype == torch.float64 else "fp32",
+                     in_=u.data_ptr(), out=out.data_ptr(), flux=flux.data_ptr() if store_flux else None,
+                     dt_patch=dt_patch.data_ptr() if dt_patch is not None else None, safety=safety, **kw)
+    _check(lib().hydro_mhm_solve_batch_async(ctypes.byref(args),
+                                             pg_dt_dh.data_ptr() if pg_dt_dh is not None else None,
+                                             queue_counter.data_ptr() if queue_counter is not None else None,
+                                             int(groups_per_cu), _stream(u.device, stream)))
     return out, (flux if store_flux else None)
 
 
