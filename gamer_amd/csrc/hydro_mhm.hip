@@ -7,7 +7,9 @@
 // Same organisation as the MHM_RP kernel: one 256-lane workgroup per patch group marching along z; lane t owns
 // output column (2 + t%16, 2 + t/16) and, on waves 0-1, one column of the reconstruction ring (i or j in {1,18}).
 // Every lane that reconstructs a cell keeps that column's primitive planes r-1, r, r+1 and the conserved cell
-// value of plane r in registers; x/y neighbours come from a double-buffered primitive plane in LDS.
+// value of plane r in registers; x/y neighbours come from a double-buffered primitive plane in LDS, the conserved
+// input plane is staged in LDS too, and a lane-private LDS slot keeps the own output column's input until its update
+// two planes later (every input value is read from HBM once).
 // Plane schedule at step t (input plane t just loaded): primitive conversion of plane t, PLM + Hancock
 // prediction of plane r = t-1, full-step fluxes of plane r (x/y) and at r-1/2 (z), full-step update of t-2.
 
@@ -28,6 +30,8 @@ struct SmemM {
    R w[2][NV][NC];    // primitive planes t-1 and t (ping-pong)
    R f[2*NV][NC];     // upper x (0-4) and y (5-9) face states (conserved, after the Hancock prediction)
    R b[2*NV][NC];     // full-step x (0-4) and y (5-9) fluxes
+   R u[NV][NC];       // conserved input plane t
+   R u2[NV][NT];      // lane-private: input cells of the own output column, plane t-2
    int fail, next;
    float cfl[8][4];   // fused CFL: per-patch maxima of the 4 waves
 };
@@ -79,6 +83,7 @@ mhm_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict_
             int npg, int store_flux, int minmod_maxiter, R minmod_coeff, Prm<R> P0,
                IO *__restrict__ g_dt, double dh_safety0, Batch B )
 {
+   static_assert( sizeof( R ) == sizeof( IO ), "the input planes keep IO values in R storage" );
    __shared__ SmemM<R> S;
 
    const int tid = threadIdx.x;
@@ -162,25 +167,19 @@ mhm_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict_
                R w[NV];
                prim_of( pf[k], w, Q );
 #              pragma unroll
-               for (int v=0; v<NV; v++) S.w[slot][v][c] = w[v];
+               for (int v=0; v<NV; v++) { S.w[slot][v][c] = w[v];  S.u[v][c] = (R)pf[k][v]; }
             }
          }
          if ( t + 1 < NX ) issue( t + 1 );
          const int o = t - 2;
-         IO uo[NV];
-         if ( o >= 2 && o <= 17 )
-#           pragma unroll
-            for (int v=0; v<NV; v++) uo[v] = in[ (size_t)v*NC*NX + (size_t)o*NC + kc ];
 #        pragma unroll
-         for (int v=0; v<NV; v++)
-         {
-            kU0[v] = kUp[v];  kUp[v] = (R)in[ (size_t)v*NC*NX + (size_t)t*NC + kc ];
-            if ( has_o ) { oU0[v] = oUp[v];  oUp[v] = (R)in[ (size_t)v*NC*NX + (size_t)t*NC + oc ]; }
-         }
+         for (int v=0; v<NV; v++) S.u2[v][tid] = kU0[v];          // input plane t-2 of the own column
          __syncthreads();
 #        pragma unroll
          for (int v=0; v<NV; v++)
          {
+            kU0[v] = kUp[v];  kUp[v] = S.u[v][kc];
+            if ( has_o ) { oU0[v] = oUp[v];  oUp[v] = S.u[v][oc]; }
             kWm[v] = kW0[v];  kW0[v] = kWp[v];  kWp[v] = S.w[slot][v][kc];
             if ( has_o ) { oWm[v] = oW0[v];  oW0[v] = oWp[v];  oWp[v] = S.w[slot][v][oc]; }
          }
@@ -271,7 +270,9 @@ mhm_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict_
          // ---- full-step update and output of plane o = t-2
          if ( o >= 2 && o <= 17 )
          {
-            IO res[NV];
+            IO res[NV], uo[NV];
+#           pragma unroll
+            for (int v=0; v<NV; v++) uo[v] = S.u2[v][tid];
             const bool bad = full_cell( uo, kDf, kFz, fzn, Q, res );
             const size_t co = (size_t)( ki - 2 ) + PS*( kj - 2 ) + (size_t)PS*PS*( o - 2 );
 #           pragma unroll

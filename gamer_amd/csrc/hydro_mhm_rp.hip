@@ -12,7 +12,9 @@
 //     z-flux — stays in that lane's registers.
 //   * Only x/y neighbours go through LDS, one plane at a time: the input plane with its pressure, 1/rho and
 //     sound speed, one direction of half-step fluxes, the half-step primitive plane, the upper x/y face
-//     states and one direction of full-step fluxes (77 KB in FP64, 39 KB in FP32).
+//     states and x/y full-step fluxes.  A lane-private LDS ring keeps the input cells of the own output column
+//     for the three planes between load and full-step update, so every input value is read from HBM once
+//     (125 KB of LDS per workgroup in FP64, 63 KB in FP32).
 //
 // Plane schedule at step t (input plane t just loaded; all indices in input coordinates):
 //   half-step z-flux at t-1/2, half-step cell update of plane t-1, half-step x/y fluxes of plane t,
@@ -159,6 +161,7 @@ struct SmemRP {
    R a[10][NC];                 // input plane (rho,mx,my,mz,E,P,1/rho,c_s) | upper x (0-4) and y (5-9) face states
    R b[XY2 ? 2*NV : NV][NC];    // half-step or full-step x (0-4) [and y (5-9)] fluxes
    R w[NV][NC];                 // half-step primitive plane read by the reconstruction
+   R u3[3][NV][NT];             // lane-private ring: input cells of the own output column, planes t-3..t-1
    int fail, next;
    float cfl[8][4];   // fused CFL: per-patch maxima of the 4 waves
 };
@@ -169,6 +172,7 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
                int npg, int store_flux, int minmod_maxiter, R minmod_coeff, Prm<R> P0,
                IO *__restrict__ g_dt, double dh_safety0, Batch B )
 {
+   static_assert( sizeof( R ) == sizeof( IO ), "the input ring holds IO values in R storage" );
    __shared__ SmemRP<R, XY2> S;
    constexpr int YB = XY2 ? NV : 0;         // offset of the y fluxes in S.b
 
@@ -259,10 +263,7 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
          }
          if ( t + 1 < NX ) issue( t + 1 );
          const int o = t - 3;                    // output plane of this step
-         IO uo[NV];
-         if ( o >= 3 && o <= 18 )
-#           pragma unroll
-            for (int v=0; v<NV; v++) uo[v] = in[ (size_t)v*NC*NX + (size_t)o*NC + K.c ];
+         const int ring = t % 3;                 // u3 slot holding input plane t-3 until the end of this step
          __syncthreads();
 
 #        pragma unroll
@@ -466,7 +467,9 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
          // ---- full-step update and output of plane o = t-3
          if ( o >= 3 && o <= 18 )
          {
-            IO res[NV];
+            IO res[NV], uo[NV];
+#           pragma unroll
+            for (int v=0; v<NV; v++) uo[v] = S.u3[ring][v][tid];
             const bool bad = full_cell( uo, kDf, kFz, fzn, Q, res );
             const size_t co = (size_t)( K.i - 3 ) + PS*( K.j - 3 ) + (size_t)PS*PS*( o - 3 );
 #           pragma unroll
@@ -480,6 +483,8 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
          if ( r >= 3 && r <= 18 )
 #           pragma unroll
             for (int v=0; v<NV; v++) kDf[v] = dfn[v];
+#        pragma unroll
+         for (int v=0; v<NV; v++) S.u3[ring][v][tid] = kU[v];      // input plane t, read back at step t+3
       } // for t
 
       if ( g_dt ) fused_dt( cfl, ( K.i - 3 )/8 + 2*( ( K.j - 3 )/8 ), S.cfl, g_dt + (size_t)pg*8, dh_safety );
