@@ -1,0 +1,568 @@
+// Adjoint (vector-Jacobian product) and tangent-linear (Jacobian-vector product) of one MHM_RP step, FP64.
+//
+// The step is the map U_in (22^3 x 5 per patch group) -> U_out (16^3 x 5) of the forward kernel
+// (src/Model_Hydro/CPU_Hydro/CPU_FluidSolver_MHM.cpp:263-589 with the half-step prediction of :615-949).
+// One 256-lane workgroup per patch group runs explicit phases over a per-PG global workspace:
+//
+//   forward   F1 half-step fluxes Fh (Hydro_RiemannPredict_Flux)   -> ws
+//             F2 half-step update, floors, primitive W (Hydro_RiemannPredict)   -> ws
+//             F3 PLM faces FC for every cell of the 18^3 reconstruction block (Hydro_DataReconstruction)   -> ws
+//             F4 (tangent / plain forward only) full-step fluxes and update of the 16^3 output cells
+//   reverse   R4 lambda_F of each full-step face from lambda_out (transpose of Hydro_FullStepUpdate),
+//                lambda_FC = J_RS^T lambda_F (transpose of Hydro_ComputeFlux)
+//             R3 lambda_W = sum over the three stencil cells of J_PLM^T lambda_FC (transpose of the limiter,
+//                face clamp and mirror)
+//             R2 lambda_U_half = J_Con2Pri^T lambda_W (transpose of the floors and primitive conversion)
+//             R1 lambda of every half-step face, then J_RS^T of the cell-centred half-step solve per side
+//             R0 lambda_in = lambda_out (core) + lambda_U_half + the half-step face contributions
+//
+// The reverse sweep is written out by hand.  The local Jacobians of the nonlinear pieces (Riemann solver,
+// limiter + clamp, conversions with floors) come from forward-mode dual numbers (hydro_dual.hpp) pushed
+// through the same templated device functions the production kernels use, one input direction at a time:
+// they take exactly the branches of the forward evaluation.  The reverse phases are gathers: every workspace
+// entry has one writer, so there are no atomics and the result is deterministic.
+//
+// Scope: MHM_RP, FP64, any Riemann solver and limiter of the forward path.  The MinMod retry is not
+// differentiated (the first-iteration coefficient is used); dt and dh are constants.
+
+#include "hydro_dual.hpp"
+#include "hydro_kernels.hpp"
+
+namespace gamer_amd {
+
+namespace {
+
+constexpr int NX = 22, N2 = NX*NX, N3 = NX*N2;
+constexpr int PS = 16;
+constexpr int NT = 256;
+constexpr int STRIDE[3] = { 1, NX, N2 };
+
+// workspace layout per patch group, in units of N3 doubles
+constexpr int WS_W   = 0;            // [5]       half-step primitives (cells 1..20)
+constexpr int WS_FC  = 5;            // [3][2][5] PLM faces L (0) / R (1) (cells 2..19)
+constexpr int WS_LW  = 35;           // [5]       lambda_W
+constexpr int WS_LU  = 40;           // [5]       lambda of the half-step conserved state
+constexpr int WS_LFC = 45;           // [3][2][5] lambda of the faces
+constexpr int WS_LS  = 75;           // [3][2][5] half-step face contributions to lambda_in (low / high cell);
+                                     //           the first 15 arrays hold Fh [3][5] during the forward phases
+constexpr int WS_ADJ = 105;
+
+__device__ __forceinline__ void coords( int c, int &i, int &j, int &k ) { i = c % NX;  j = ( c / NX ) % NX;  k = c / N2; }
+__device__ __forceinline__ bool in_rng( int x, int lo, int hi ) { return x >= lo && x <= hi; }
+
+// face c + 1/2 e_d with normal coordinate n, tangential a, b
+__device__ __forceinline__ void split( int d, int i, int j, int k, int &n, int &a, int &b )
+{
+   n = ( d == 0 ) ? i : ( d == 1 ) ? j : k;
+   a = ( d == 0 ) ? j : i;
+   b = ( d == 2 ) ? j : k;
+}
+__device__ __forceinline__ bool half_face( int n, int a, int b ) { return in_rng( n, 0, 20 ) && in_rng( a, 1, 20 ) && in_rng( b, 1, 20 ); }
+__device__ __forceinline__ bool full_face( int n, int a, int b ) { return in_rng( n, 2, 18 ) && in_rng( a, 3, 18 ) && in_rng( b, 3, 18 ); }
+__device__ __forceinline__ bool core3( int i, int j, int k ) { return in_rng( i, 3, 18 ) && in_rng( j, 3, 18 ) && in_rng( k, 3, 18 ); }
+__device__ __forceinline__ int  out_index( int i, int j, int k ) { return ( i - 3 ) + PS*( ( j - 3 ) + PS*( k - 3 ) ); }
+
+// ---------------- pieces of the step, generic in the arithmetic type ----------------
+// cell-centred interface state of the half step (pressure recovery + floor, 1/rho, sound speed)
+template <int D, typename T>
+__device__ __forceinline__ St<T> cell_st( const T u[NV], const Prm<T> &P )
+{
+   const T ir = rcp( u[0] );
+   const T eint = u[4] - (T)0.5*( u[1]*u[1] + u[2]*u[2] + u[3]*u[3] )*ir;
+   const T p = floor_pres( eint*P.gm1, P.min_pres );
+   return cell_state<D>( u, p, ir, vsqrt( P.gamma*p*ir ) );
+}
+
+template <int RS, typename T>
+__device__ __forceinline__ void half_flux( int d, const T uL[NV], const T uR[NV], const Prm<T> &P, T f[NV] )
+{
+   if      ( d == 0 ) riemann<RS, 0>( cell_st<0>( uL, P ), cell_st<0>( uR, P ), P, f );
+   else if ( d == 1 ) riemann<RS, 1>( cell_st<1>( uL, P ), cell_st<1>( uR, P ), P, f );
+   else               riemann<RS, 2>( cell_st<2>( uL, P ), cell_st<2>( uR, P ), P, f );
+}
+
+template <int RS, typename T>
+__device__ __forceinline__ void full_flux( int d, const T wL[NV], const T wR[NV], const Prm<T> &P, T f[NV] )
+{
+   if      ( d == 0 ) riemann<RS, 0>( face_state<0>( wL, P ), face_state<0>( wR, P ), P, f );
+   else if ( d == 1 ) riemann<RS, 1>( face_state<1>( wL, P ), face_state<1>( wR, P ), P, f );
+   else               riemann<RS, 2>( face_state<2>( wL, P ), face_state<2>( wR, P ), P, f );
+}
+
+// half-step conserved state -> density floor, internal-energy floor, primitive variables with pressure floor
+// (CPU_FluidSolver_MHM.cpp:900-933)
+template <typename T>
+__device__ __forceinline__ void con2pri_floor( const T u[NV], const Prm<T> &P, T w[NV] )
+{
+   T c[NV];
+#  pragma unroll
+   for (int v=0; v<NV; v++) c[v] = u[v];
+   c[0] = vmax( c[0], P.min_dens );
+   const T r  = rcp( c[0] );
+   const T ke = (T)0.5*( c[1]*c[1] + c[2]*c[2] + c[3]*c[3] )*r;
+   T ein = c[4] - ke;
+   const T eout = ( ein == ein ) ? vmax( ein, P.min_eint ) : ein;
+   if ( ein != eout ) { c[4] = c[4] - ein + eout;  ein = c[4] - ke; }
+   w[0] = c[0];
+   w[1] = c[1]*r;
+   w[2] = c[2]*r;
+   w[3] = c[3]*r;
+   const T p = ein*P.gm1;
+   w[4] = ( p == p ) ? vmax( p, P.min_pres ) : p;
+}
+
+template <int LIM, typename T>
+__device__ __forceinline__ void plm( const T &L, const T &C, const T &R, const Prm<T> &P, T &fL, T &fR )
+{
+   plm_face<LIM>( L, C, R, P.limiter, P.coeff, fL, fR );
+}
+
+// ---------------- workspace views ----------------
+// The adjoint keeps Fh in the lambda-side slots (read until R2, overwritten in R1); the tangent kernel stores
+// dual numbers (two doubles each) and packs Fh right after the faces: 50 duals = 100 doubles per cell.
+constexpr int WS_FH_TAN = 35;
+static_assert( 2*( WS_FH_TAN + 15 ) <= WS_ADJ, "tangent workspace must fit the per-PG slot" );
+
+template <typename T>
+struct Fwd {
+   T *W, *FC, *Fh;
+   __device__ Fwd( T *base, int fh_off ) : W( base + WS_W*N3 ), FC( base + WS_FC*N3 ), Fh( base + (size_t)fh_off*N3 ) {}
+   __device__ T &w( int v, int c )                { return W[ v*N3 + c ]; }
+   __device__ T &fc( int d, int s, int v, int c ) { return FC[ ( ( d*2 + s )*NV + v )*N3 + c ]; }
+   __device__ T &fh( int d, int v, int c )        { return Fh[ ( d*NV + v )*N3 + c ]; }
+};
+
+// load cell c of the input (value, and tangent for Dual<1>)
+template <typename T> struct Load;
+template <> struct Load<double>
+{
+   __device__ static void cell( const double *in, const double *, int c, double u[NV] )
+   {
+#     pragma unroll
+      for (int v=0; v<NV; v++) u[v] = in[ v*N3 + c ];
+   }
+};
+template <> struct Load< Dual<1> >
+{
+   __device__ static void cell( const double *in, const double *tin, int c, Dual<1> u[NV] )
+   {
+#     pragma unroll
+      for (int v=0; v<NV; v++) { u[v] = Dual<1>( in[ v*N3 + c ] );  u[v].d[0] = tin[ v*N3 + c ]; }
+   }
+};
+
+// F1-F3: forward phases into the workspace
+template <int RS, int LIM, typename T>
+__device__ void forward_phases( const double *in, const double *tin, Fwd<T> &F, const Prm<T> &P )
+{
+   const int tid = threadIdx.x;
+   // F1 half-step fluxes at the 25,200 faces (stored at the low cell)
+   for (int e = tid; e < 3*N3; e += NT)
+   {
+      const int d = e / N3, c = e % N3;
+      int i, j, k, n, a, b;
+      coords( c, i, j, k );  split( d, i, j, k, n, a, b );
+      if ( !half_face( n, a, b ) ) continue;
+      T uL[NV], uR[NV], f[NV];
+      Load<T>::cell( in, tin, c, uL );
+      Load<T>::cell( in, tin, c + STRIDE[d], uR );
+      half_flux<RS>( d, uL, uR, P, f );
+#     pragma unroll
+      for (int v=0; v<NV; v++) F.fh( d, v, c ) = f[v];
+   }
+   __syncthreads();
+   // F2 half-step update of the 20^3 cells (x and y flux differences, then z, as the production kernel)
+   for (int c = tid; c < N3; c += NT)
+   {
+      int i, j, k;
+      coords( c, i, j, k );
+      if ( !( in_rng( i, 1, 20 ) && in_rng( j, 1, 20 ) && in_rng( k, 1, 20 ) ) ) continue;
+      T u[NV], h[NV], w[NV];
+      Load<T>::cell( in, tin, c, u );
+#     pragma unroll
+      for (int v=0; v<NV; v++)
+      {
+         const T dxy = ( F.fh( 0, v, c ) - F.fh( 0, v, c - 1 ) ) + ( F.fh( 1, v, c ) - F.fh( 1, v, c - NX ) );
+         h[v] = u[v] - P.dt_dh2*( dxy + ( F.fh( 2, v, c ) - F.fh( 2, v, c - N2 ) ) );
+      }
+      con2pri_floor( h, P, w );
+#     pragma unroll
+      for (int v=0; v<NV; v++) F.w( v, c ) = w[v];
+   }
+   __syncthreads();
+   // F3 PLM faces of the 18^3 reconstruction block, all three directions
+   for (int c = tid; c < N3; c += NT)
+   {
+      int i, j, k;
+      coords( c, i, j, k );
+      if ( !( in_rng( i, 2, 19 ) && in_rng( j, 2, 19 ) && in_rng( k, 2, 19 ) ) ) continue;
+      for (int d=0; d<3; d++)
+      {
+         const int s = STRIDE[d];
+#        pragma unroll
+         for (int v=0; v<NV; v++)
+         {
+            T fL, fR;
+            plm<LIM>( F.w( v, c - s ), F.w( v, c ), F.w( v, c + s ), P, fL, fR );
+            F.fc( d, 0, v, c ) = fL;
+            F.fc( d, 1, v, c ) = fR;
+         }
+      }
+   }
+   __syncthreads();
+}
+
+template <typename T> struct Store;
+template <> struct Store<double>
+{
+   __device__ static void put( double *out, double *, size_t o, const double &x ) { out[o] = x; }
+};
+template <> struct Store< Dual<1> >
+{
+   __device__ static void put( double *out, double *tout, size_t o, const Dual<1> &x ) { if ( out ) out[o] = x.v;  tout[o] = x.d[0]; }
+};
+
+// F4: full-step update of the 16^3 output cells, each of its six face fluxes evaluated from the stored faces
+template <int RS, typename T>
+__device__ void full_update( const double *in, const double *tin, Fwd<T> &F, const Prm<T> &P, double *out, double *tout )
+{
+   for (int c = threadIdx.x; c < N3; c += NT)
+   {
+      int i, j, k;
+      coords( c, i, j, k );
+      if ( !core3( i, j, k ) ) continue;
+      T u[NV], dF[3][NV];
+      Load<T>::cell( in, tin, c, u );
+      for (int d=0; d<3; d++)
+      {
+         const int s = STRIDE[d];
+         T wl[NV], wr[NV], fm[NV], fp[NV];
+#        pragma unroll
+         for (int v=0; v<NV; v++) { wl[v] = F.fc( d, 1, v, c - s );  wr[v] = F.fc( d, 0, v, c ); }
+         full_flux<RS>( d, wl, wr, P, fm );
+#        pragma unroll
+         for (int v=0; v<NV; v++) { wl[v] = F.fc( d, 1, v, c );  wr[v] = F.fc( d, 0, v, c + s ); }
+         full_flux<RS>( d, wl, wr, P, fp );
+#        pragma unroll
+         for (int v=0; v<NV; v++) dF[d][v] = fp[v] - fm[v];
+      }
+      const size_t o = out_index( i, j, k );
+#     pragma unroll
+      for (int v=0; v<NV; v++)
+         Store<T>::put( out, tout, (size_t)v*PS*PS*PS + o, u[v] - P.dt_dh*( ( dF[0][v] + dF[1][v] ) + dF[2][v] ) );
+   }
+}
+
+// Tangent-linear step (T = Dual<1>: out = U_out, tout = J v) or plain phase-wise forward (T = double).
+template <int RS, int LIM, typename T>
+__global__ void __launch_bounds__( NT )
+tangent_kernel( const double *__restrict__ g_in, const double *__restrict__ g_tin, double *__restrict__ g_out,
+                double *__restrict__ g_tout, double *__restrict__ g_ws, int npg, Prm<double> P0, double minmod_coeff )
+{
+   const int pg = blockIdx.x;
+   if ( pg >= npg ) return;
+   Prm<T> P = prm_as<T>( P0 );
+   P.coeff = vmax( (T)minmod_coeff, (T)0.0 );
+   const size_t nin = (size_t)NV*N3, nout = (size_t)NV*PS*PS*PS;
+   const double *in  = g_in + pg*nin;
+   const double *tin = g_tin ? g_tin + pg*nin : nullptr;
+   Fwd<T> F( reinterpret_cast<T*>( g_ws + (size_t)pg*WS_ADJ*N3 ), WS_FH_TAN );
+   forward_phases<RS, LIM, T>( in, tin, F, P );
+   full_update<RS, T>( in, tin, F, P, g_out ? g_out + pg*nout : nullptr, g_tout ? g_tout + pg*nout : nullptr );
+}
+
+// ---------------- reverse phases ----------------
+struct Adj {
+   double *LW, *LU, *LFC, *LS;
+   __device__ Adj( double *base ) : LW( base + WS_LW*N3 ), LU( base + WS_LU*N3 ), LFC( base + WS_LFC*N3 ), LS( base + WS_LS*N3 ) {}
+   __device__ double &lw( int v, int c )                { return LW[ v*N3 + c ]; }
+   __device__ double &lu( int v, int c )                { return LU[ v*N3 + c ]; }
+   __device__ double &lfc( int d, int s, int v, int c ) { return LFC[ ( ( d*2 + s )*NV + v )*N3 + c ]; }
+   __device__ double &ls( int d, int s, int v, int c )  { return LS[ ( ( d*2 + s )*NV + v )*N3 + c ]; }
+};
+
+typedef Dual<1> D1;
+
+template <int RS, int LIM>
+__global__ void __launch_bounds__( NT )
+adjoint_kernel( const double *__restrict__ g_in, const double *__restrict__ g_lout, double *__restrict__ g_lin,
+                double *__restrict__ g_ws, int npg, Prm<double> P0, double minmod_coeff )
+{
+   const int pg = blockIdx.x, tid = threadIdx.x;
+   if ( pg >= npg ) return;
+   Prm<double> P = P0;
+   P.coeff = vmax( minmod_coeff, 0.0 );
+   Prm<D1> PD = prm_as<D1>( P );
+   const size_t nin = (size_t)NV*N3, nout = (size_t)NV*PS*PS*PS;
+   const double *in   = g_in + pg*nin;
+   const double *lout = g_lout + pg*nout;
+   double *lin = g_lin + pg*nin;
+   double *ws = g_ws + (size_t)pg*WS_ADJ*N3;
+   Fwd<double> F( ws, WS_LS );
+   Adj A( ws );
+
+   forward_phases<RS, LIM, double>( in, nullptr, F, P );
+
+   // R4 (a) zero lambda_FC: only the faces that feed a full-step flux receive a value
+   for (int e = tid; e < 30*N3; e += NT) A.LFC[e] = 0.0;
+   __syncthreads();
+   // R4 (b) full-step faces: lambda_F = dt/dh (lambda_out(c+e_d) - lambda_out(c)), then J_RS^T per side
+   for (int e = tid; e < 3*N3; e += NT)
+   {
+      const int d = e / N3, c = e % N3;
+      int i, j, k, n, a, b;
+      coords( c, i, j, k );  split( d, i, j, k, n, a, b );
+      if ( !full_face( n, a, b ) ) continue;
+      const int s = STRIDE[d];
+      int i2 = i, j2 = j, k2 = k;
+      if ( d == 0 ) i2++; else if ( d == 1 ) j2++; else k2++;
+      double lf[NV];
+      const bool lo = core3( i, j, k ), hi = core3( i2, j2, k2 );
+#     pragma unroll
+      for (int v=0; v<NV; v++)
+      {
+         const double ll = lo ? lout[ (size_t)v*PS*PS*PS + out_index( i, j, k ) ] : 0.0;
+         const double lh = hi ? lout[ (size_t)v*PS*PS*PS + out_index( i2, j2, k2 ) ] : 0.0;
+         lf[v] = P.dt_dh*( lh - ll );
+      }
+      for (int q=0; q<2*NV; q++)                  // input direction: R face of c (q < 5), L face of c + e_d
+      {
+         D1 wl[NV], wr[NV], f[NV];
+#        pragma unroll
+         for (int v=0; v<NV; v++)
+         {
+            wl[v] = D1( F.fc( d, 1, v, c ) );      wl[v].d[0] = ( q == v )      ? 1.0 : 0.0;
+            wr[v] = D1( F.fc( d, 0, v, c + s ) );  wr[v].d[0] = ( q == v + NV ) ? 1.0 : 0.0;
+         }
+         full_flux<RS>( d, wl, wr, PD, f );
+         double g = 0.0;
+#        pragma unroll
+         for (int v=0; v<NV; v++) g += lf[v]*f[v].d[0];
+         if ( q < NV ) A.lfc( d, 1, q, c ) = g;
+         else          A.lfc( d, 0, q - NV, c + s ) = g;
+      }
+   }
+   __syncthreads();
+
+   // R3 lambda_W of the 20^3 half-step cells: gather over the reconstruction stencils that contain the cell
+   for (int c = tid; c < N3; c += NT)
+   {
+      int i, j, k;
+      coords( c, i, j, k );
+      if ( !( in_rng( i, 1, 20 ) && in_rng( j, 1, 20 ) && in_rng( k, 1, 20 ) ) ) continue;
+      double acc[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+      for (int d=0; d<3; d++)
+      {
+         const int s = STRIDE[d];
+         for (int off = -1; off <= 1; off++)        // stencil centre c' = c - off e_d; c sits at position off
+         {
+            const int cc = c - off*s;
+            int ci, cj, ck;
+            coords( cc, ci, cj, ck );
+            if ( !( in_rng( ci, 2, 19 ) && in_rng( cj, 2, 19 ) && in_rng( ck, 2, 19 ) ) ) continue;
+#           pragma unroll
+            for (int v=0; v<NV; v++)
+            {
+               const double gl = A.lfc( d, 0, v, cc ), gr = A.lfc( d, 1, v, cc );
+               if ( gl == 0.0 && gr == 0.0 ) continue;
+               D1 L( F.w( v, cc - s ) ), C( F.w( v, cc ) ), R( F.w( v, cc + s ) ), fL, fR;
+               L.d[0] = ( off == -1 ) ? 1.0 : 0.0;
+               C.d[0] = ( off ==  0 ) ? 1.0 : 0.0;
+               R.d[0] = ( off ==  1 ) ? 1.0 : 0.0;
+               plm<LIM>( L, C, R, PD, fL, fR );
+               acc[v] += gl*fL.d[0] + gr*fR.d[0];
+            }
+         }
+      }
+#     pragma unroll
+      for (int v=0; v<NV; v++) A.lw( v, c ) = acc[v];
+   }
+   __syncthreads();
+
+   // R2 lambda of the half-step conserved state: J_Con2Pri^T lambda_W (floors included)
+   for (int c = tid; c < N3; c += NT)
+   {
+      int i, j, k;
+      coords( c, i, j, k );
+      if ( !( in_rng( i, 1, 20 ) && in_rng( j, 1, 20 ) && in_rng( k, 1, 20 ) ) ) continue;
+      double h[NV], lw[NV];
+#     pragma unroll
+      for (int v=0; v<NV; v++)
+      {
+         const double dxy = ( F.fh( 0, v, c ) - F.fh( 0, v, c - 1 ) ) + ( F.fh( 1, v, c ) - F.fh( 1, v, c - NX ) );
+         h[v]  = in[ v*N3 + c ] - P.dt_dh2*( dxy + ( F.fh( 2, v, c ) - F.fh( 2, v, c - N2 ) ) );
+         lw[v] = A.lw( v, c );
+      }
+      for (int q=0; q<NV; q++)
+      {
+         D1 hd[NV], w[NV];
+#        pragma unroll
+         for (int v=0; v<NV; v++) { hd[v] = D1( h[v] );  hd[v].d[0] = ( q == v ) ? 1.0 : 0.0; }
+         con2pri_floor( hd, PD, w );
+         double g = 0.0;
+#        pragma unroll
+         for (int v=0; v<NV; v++) g += lw[v]*w[v].d[0];
+         A.lu( q, c ) = g;
+      }
+   }
+   __syncthreads();
+
+   // R1 half-step faces: lambda_Fh = dt/(2dh) (lambda_U(c+e_d) - lambda_U(c)), then J_RS^T per side.
+   // (The Fh arrays alias the low-side slots written here; every reader of Fh has passed the barrier above.)
+   for (int e = tid; e < 3*N3; e += NT)
+   {
+      const int d = e / N3, c = e % N3;
+      int i, j, k, n, a, b;
+      coords( c, i, j, k );  split( d, i, j, k, n, a, b );
+      if ( !half_face( n, a, b ) ) continue;
+      const int s = STRIDE[d];
+      const bool lo = in_rng( n, 1, 20 ), hi = in_rng( n + 1, 1, 20 );   // tangential range is 1..20 already
+      double lf[NV];
+#     pragma unroll
+      for (int v=0; v<NV; v++) lf[v] = P.dt_dh2*( ( hi ? A.lu( v, c + s ) : 0.0 ) - ( lo ? A.lu( v, c ) : 0.0 ) );
+      double uL[NV], uR[NV];
+#     pragma unroll
+      for (int v=0; v<NV; v++) { uL[v] = in[ v*N3 + c ];  uR[v] = in[ v*N3 + c + s ]; }
+      double gl[NV], gr[NV];
+      for (int q=0; q<2*NV; q++)
+      {
+         D1 dl[NV], dr[NV], f[NV];
+#        pragma unroll
+         for (int v=0; v<NV; v++)
+         {
+            dl[v] = D1( uL[v] );  dl[v].d[0] = ( q == v )      ? 1.0 : 0.0;
+            dr[v] = D1( uR[v] );  dr[v].d[0] = ( q == v + NV ) ? 1.0 : 0.0;
+         }
+         half_flux<RS>( d, dl, dr, PD, f );
+         double g = 0.0;
+#        pragma unroll
+         for (int v=0; v<NV; v++) g += lf[v]*f[v].d[0];
+         if ( q < NV ) gl[q] = g; else gr[q - NV] = g;
+      }
+#     pragma unroll
+      for (int v=0; v<NV; v++) { A.ls( d, 0, v, c ) = gl[v];  A.ls( d, 1, v, c + s ) = gr[v]; }
+   }
+   __syncthreads();
+
+   // R0 lambda_in: direct dependence of the output and of the half-step state on U_in, plus the half-step faces
+   for (int c = tid; c < N3; c += NT)
+   {
+      int i, j, k;
+      coords( c, i, j, k );
+      const bool core = core3( i, j, k ), half = in_rng( i, 1, 20 ) && in_rng( j, 1, 20 ) && in_rng( k, 1, 20 );
+#     pragma unroll
+      for (int v=0; v<NV; v++)
+      {
+         double acc = core ? lout[ (size_t)v*PS*PS*PS + out_index( i, j, k ) ] : 0.0;
+         if ( half ) acc += A.lu( v, c );
+         for (int d=0; d<3; d++)
+         {
+            int n, a, b;
+            split( d, i, j, k, n, a, b );
+            if ( half_face( n, a, b ) )     acc += A.ls( d, 0, v, c );      // c is the low cell of face c+1/2
+            if ( half_face( n - 1, a, b ) ) acc += A.ls( d, 1, v, c );      // c is the high cell of face c-1/2
+         }
+         lin[ v*N3 + c ] = acc;
+      }
+   }
+}
+
+// ---------------- transpose of the periodic ghost-zone preparation ----------------
+// level_lam[v][z][y][x] = sum over every patch-group input element that copies level cell (x,y,z)
+// (gather form: each level cell collects its copies, so there is one writer per element)
+__device__ __forceinline__ int copies( int x, int nx, int ghost, int n, int pl[6], int il[6] )
+{
+   const int G = nx/PS, q = x/PS;
+   int cand[3] = { ( q - 1 + G ) % G, q, ( q + 1 ) % G }, m = 0;
+   for (int t=0; t<3; t++)
+   {
+      bool dup = false;
+      for (int u=0; u<t; u++) dup |= ( cand[u] == cand[t] );
+      if ( dup ) continue;
+      const int p = cand[t];
+      for (int i = ( ( x - p*PS + ghost ) % nx + nx ) % nx; i < n; i += nx) { pl[m] = p;  il[m] = i;  m++; }
+   }
+   return m;
+}
+
+template <typename IO>
+__global__ void prepare_adjoint_kernel( const IO *__restrict__ pg_lam, IO *__restrict__ lam, int nx, int ny, int nz,
+                                        int ghost, long total )
+{
+   const int n = PS + 2*ghost;
+   const long n3 = (long)n*n*n;
+   const int gx = nx/PS, gy = ny/PS;
+   for (long e = blockIdx.x*(long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x*blockDim.x)
+   {
+      const int x = (int)( e % nx ), y = (int)( ( e / nx ) % ny ), z = (int)( ( e / ( (long)nx*ny ) ) % nz );
+      const int v = (int)( e / ( (long)nx*ny*nz ) );
+      int px[6], ix[6], py[6], iy[6], pz[6], iz[6];
+      const int mx = copies( x, nx, ghost, n, px, ix ), my = copies( y, ny, ghost, n, py, iy ), mz = copies( z, nz, ghost, n, pz, iz );
+      double acc = 0.0;
+      for (int c=0; c<mz; c++)
+      for (int b=0; b<my; b++)
+      for (int a=0; a<mx; a++)
+      {
+         const long pg = px[a] + (long)gx*( py[b] + (long)gy*pz[c] );
+         acc += (double)pg_lam[ ( pg*NV + v )*n3 + ix[a] + (long)n*( iy[b] + (long)n*iz[c] ) ];
+      }
+      lam[e] = (IO)acc;
+   }
+}
+
+} // namespace
+
+size_t adjoint_workspace_bytes( int npg ) { return (size_t)( npg > 0 ? npg : 0 )*WS_ADJ*N3*sizeof( double ); }
+
+int launch_mhm_rp_adjoint( const double *in, const double *lam_out, double *lam_in, double *ws, int npg, int rsolver,
+                           double minmod_coeff, const Prm<double> &P, hipStream_t stream )
+{
+   if ( npg <= 0 ) return 0;
+   const dim3 grid( npg ), block( NT );
+#  define GA_ADJ( RS_ ) do { if ( P.limiter == 4 ) hipLaunchKernelGGL( ( adjoint_kernel<RS_, 4> ), grid, block, 0, stream, in, lam_out, lam_in, ws, npg, P, minmod_coeff ); \
+                             else                  hipLaunchKernelGGL( ( adjoint_kernel<RS_, 0> ), grid, block, 0, stream, in, lam_out, lam_in, ws, npg, P, minmod_coeff ); } while ( 0 )
+   switch ( rsolver )
+   {
+      case 4: GA_ADJ( 4 ); break;
+      case 3: GA_ADJ( 3 ); break;
+      case 2: GA_ADJ( 2 ); break;
+      default: return -1;
+   }
+#  undef GA_ADJ
+   return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
+}
+
+int launch_mhm_rp_tangent( const double *in, const double *tin, double *out, double *tout, double *ws, int npg,
+                           int rsolver, double minmod_coeff, const Prm<double> &P, hipStream_t stream )
+{
+   if ( npg <= 0 ) return 0;
+   const dim3 grid( npg ), block( NT );
+#  define GA_TAN( RS_, T_ ) do { if ( P.limiter == 4 ) hipLaunchKernelGGL( ( tangent_kernel<RS_, 4, T_> ), grid, block, 0, stream, in, tin, out, tout, ws, npg, P, minmod_coeff ); \
+                                 else                  hipLaunchKernelGGL( ( tangent_kernel<RS_, 0, T_> ), grid, block, 0, stream, in, tin, out, tout, ws, npg, P, minmod_coeff ); } while ( 0 )
+   const bool tl = ( tin != nullptr );
+   switch ( rsolver )
+   {
+      case 4: if ( tl ) GA_TAN( 4, D1 ); else GA_TAN( 4, double ); break;
+      case 3: if ( tl ) GA_TAN( 3, D1 ); else GA_TAN( 3, double ); break;
+      case 2: if ( tl ) GA_TAN( 2, D1 ); else GA_TAN( 2, double ); break;
+      default: return -1;
+   }
+#  undef GA_TAN
+   return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
+}
+
+template <typename IO>
+int launch_prepare_adjoint( const IO *pg_lam, IO *lam, int nx, int ny, int nz, int ghost, hipStream_t stream )
+{
+   const long total = (long)NV*nx*ny*nz;
+   if ( total <= 0 ) return 0;
+   long b = ( total + 255 )/256;
+   if ( b > 65536 ) b = 65536;
+   hipLaunchKernelGGL( prepare_adjoint_kernel<IO>, dim3( (int)b ), dim3( 256 ), 0, stream, pg_lam, lam, nx, ny, nz, ghost, total );
+   return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
+}
+
+template int launch_prepare_adjoint<double>( const double*, double*, int, int, int, int, hipStream_t );
+template int launch_prepare_adjoint<float> ( const float*,  float*,  int, int, int, int, hipStream_t );
+
+} // namespace gamer_amd

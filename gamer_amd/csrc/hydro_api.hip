@@ -186,6 +186,57 @@ int hydro_close_uniform_async( const void *pg_out, void *level, int nx, int ny, 
    return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "close launch failed" );
 }
 
+static int check_adjoint_args( const HydroSolveArgs *a, const void *ws )
+{
+   if ( a == nullptr ) return fail( HYDRO_E_ARG, "args == NULL" );
+   HydroSolveArgs b = *a;                 // out and the flux arrays are optional here
+   if ( b.out == nullptr ) b.out = (void*)b.in;
+   b.store_flux = 0;
+   const int rc = check_args( &b );
+   if ( rc != HYDRO_OK ) return rc;
+   if ( a->scheme != HYDRO_MHM_RP )  return fail( HYDRO_E_ARG, "adjoint/tangent: only the MHM_RP scheme" );
+   if ( a->precision != HYDRO_FP64 ) return fail( HYDRO_E_ARG, "adjoint/tangent: only HYDRO_FP64" );
+   if ( a->npg > 0 && ws == nullptr ) return fail( HYDRO_E_ARG, "adjoint/tangent: workspace == NULL" );
+   return HYDRO_OK;
+}
+
+size_t hydro_mhm_adjoint_workspace_bytes( int npg ) { return adjoint_workspace_bytes( npg ); }
+
+int hydro_mhm_adjoint_async( const HydroSolveArgs *a, const void *lam_out, void *lam_in, void *workspace, void *stream )
+{
+   int rc = check_adjoint_args( a, workspace );
+   if ( rc != HYDRO_OK ) return rc;
+   if ( a->npg > 0 && ( lam_out == nullptr || lam_in == nullptr ) ) return fail( HYDRO_E_ARG, "adjoint: lam_out/lam_in == NULL" );
+   rc = launch_mhm_rp_adjoint( (const double*)a->in, (const double*)lam_out, (double*)lam_in, (double*)workspace, a->npg,
+                               a->rsolver, a->minmod_coeff, make_prm<double>( a ), (hipStream_t)stream );
+   return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "adjoint launch failed: %s", hipGetErrorString( hipGetLastError() ) );
+}
+
+int hydro_mhm_tangent_async( const HydroSolveArgs *a, const void *v_in, void *jv_out, void *workspace, void *stream )
+{
+   int rc = check_adjoint_args( a, workspace );
+   if ( rc != HYDRO_OK ) return rc;
+   if ( a->npg > 0 && v_in != nullptr && jv_out == nullptr ) return fail( HYDRO_E_ARG, "tangent: jv_out == NULL" );
+   if ( a->npg > 0 && v_in == nullptr && a->out == nullptr )  return fail( HYDRO_E_ARG, "tangent: v_in and out both NULL" );
+   rc = launch_mhm_rp_tangent( (const double*)a->in, (const double*)v_in, (double*)a->out, (double*)jv_out,
+                               (double*)workspace, a->npg, a->rsolver, a->minmod_coeff, make_prm<double>( a ),
+                               (hipStream_t)stream );
+   return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "tangent launch failed: %s", hipGetErrorString( hipGetLastError() ) );
+}
+
+int hydro_prepare_uniform_adjoint_async( const void *pg_lam, void *level_lam, int nx, int ny, int nz, int ghost,
+                                         int precision, void *stream )
+{
+   if ( check_level( nx, ny, nz ) ) return HYDRO_E_ARG;
+   if ( pg_lam == nullptr || level_lam == nullptr ) return fail( HYDRO_E_ARG, "prepare adjoint: NULL pointer" );
+   if ( ghost < 0 || ghost > 8 ) return fail( HYDRO_E_ARG, "prepare adjoint: ghost %d out of range", ghost );
+   int rc;
+   if      ( precision == HYDRO_FP64 )      rc = launch_prepare_adjoint<double>( (const double*)pg_lam, (double*)level_lam, nx, ny, nz, ghost, (hipStream_t)stream );
+   else if ( precision == HYDRO_FP32_BULK ) rc = launch_prepare_adjoint<float> ( (const float*)pg_lam,  (float*)level_lam,  nx, ny, nz, ghost, (hipStream_t)stream );
+   else return fail( HYDRO_E_ARG, "unsupported precision %d", precision );
+   return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "prepare adjoint launch failed" );
+}
+
 int hydro_mhm_solve_batch_async( const HydroSolveArgs *a, const double *pg_dt_dh, unsigned *queue_counter,
                                  int groups_per_cu, void *stream )
 {

@@ -31,4 +31,15 @@ int launch_scatter( const IO *pg_out, IO *grid, int nx, int ny, int nz, hipStrea
 // set the work-queue counter to 0 (a kernel, so the reset can live inside a captured graph)
 int launch_reset_counter( unsigned *counter, hipStream_t stream );
 
+// MHM_RP step, FP64: adjoint lam_in = J^T lam_out, and tangent-linear tout = J tin (+ out = U_out; tin == NULL:
+// plain phase-wise forward).  ws: adjoint_workspace_bytes(npg) of device memory.
+size_t adjoint_workspace_bytes( int npg );
+int launch_mhm_rp_adjoint( const double *in, const double *lam_out, double *lam_in, double *ws, int npg, int rsolver,
+                           double minmod_coeff, const Prm<double> &P, hipStream_t stream );
+int launch_mhm_rp_tangent( const double *in, const double *tin, double *out, double *tout, double *ws, int npg,
+                           int rsolver, double minmod_coeff, const Prm<double> &P, hipStream_t stream );
+// transpose of launch_gather: level_lam[5][nz][ny][nx] = sum of the patch-group input copies of each cell
+template <typename IO>
+int launch_prepare_adjoint( const IO *pg_lam, IO *lam, int nx, int ny, int nz, int ghost, hipStream_t stream );
+
 } // namespace gamer_amd

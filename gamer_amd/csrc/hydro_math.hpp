@@ -31,6 +31,7 @@ __device__ __forceinline__ double vsqrt( double x ) { return __builtin_sqrt( x )
 template <typename R> __device__ __forceinline__ R vmax( R a, R b ) { return __builtin_fmax( a, b ); }
 template <typename R> __device__ __forceinline__ R vmin( R a, R b ) { return __builtin_fmin( a, b ); }
 template <typename R> __device__ __forceinline__ R vabs( R a )      { return __builtin_fabs( a ); }
+template <typename R> __device__ __forceinline__ R vcopysign( R m, R s ) { return __builtin_copysign( m, s ); }
 
 // pressure floor that lets NaN through (FluUtility.cpp:681-690)
 template <typename R> __device__ __forceinline__ R floor_pres( R p, R pmin ) { return ( p == p ) ? vmax( p, pmin ) : p; }
@@ -220,7 +221,7 @@ __device__ __forceinline__ R slope_vl_gminmod( R sl, R sr, R coeff )
    const R h = (R)2.0*( sl*sr )*rcp( vabs( t ) );
    R m = vmin( vmin( vabs( sl )*coeff, vabs( sr )*coeff ), h );
    m = vmax( m, (R)0 );
-   return __builtin_copysign( m, t );
+   return vcopysign( m, t );
 }
 
 template <int LIM, typename R>

@@ -21,6 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgamer_amd_hydro.so")
 
 HYDRO_OK = 0
+HYDRO_E_ARG, HYDRO_E_STATE, HYDRO_E_DEVICE, HYDRO_E_NOMEM = -1, -2, -3, -4
 SCHEME = {"MHM": 3, "MHM_RP": 4}
 RSOLVER = {"ROE": 2, "HLLE": 3, "HLLC": 4}
 LIMITER = {"VANLEER": 1, "GMINMOD": 2, "ALBADA": 3, "VL_GMINMOD": 4, "CENTRAL": 6}
@@ -30,7 +31,8 @@ NX = {"MHM_RP": 22, "MHM": 20}
 EXPORTS = ("hydro_init", "hydro_mhm_solve_async", "hydro_synchronize", "hydro_last_error",
            "hydro_mem_allocate_fluid", "hydro_mem_free_fluid", "hydro_host_buffers", "hydro_asyn_fluid_solver",
            "hydro_dt_cfl_async", "hydro_prepare_uniform_async", "hydro_close_uniform_async",
-           "hydro_mhm_solve_batch_async")
+           "hydro_mhm_solve_batch_async", "hydro_mhm_adjoint_workspace_bytes", "hydro_mhm_adjoint_async",
+           "hydro_mhm_tangent_async", "hydro_prepare_uniform_adjoint_async")
 
 
 class HydroError(RuntimeError):
@@ -72,6 +74,11 @@ def lib():
         L.hydro_close_uniform_async.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 4 + [ctypes.c_void_p]
         L.hydro_mhm_solve_batch_async.argtypes = [ctypes.POINTER(HydroSolveArgs), ctypes.c_void_p, ctypes.c_void_p,
                                                   ctypes.c_int, ctypes.c_void_p]
+        L.hydro_mhm_adjoint_workspace_bytes.argtypes = [ctypes.c_int]
+        L.hydro_mhm_adjoint_workspace_bytes.restype = ctypes.c_size_t
+        L.hydro_mhm_adjoint_async.argtypes = [ctypes.POINTER(HydroSolveArgs)] + [ctypes.c_void_p] * 4
+        L.hydro_mhm_tangent_async.argtypes = [ctypes.POINTER(HydroSolveArgs)] + [ctypes.c_void_p] * 4
+        L.hydro_prepare_uniform_adjoint_async.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 5 + [ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -214,6 +221,72 @@ def close_uniform(pg_out, level, stream=None):
     _check(lib().hydro_close_uniform_async(pg_out.contiguous().data_ptr(), level.data_ptr(), nx, ny, nz,
                                            _prec(level), _stream(level.device, stream)))
     return level
+
+
+def adjoint_workspace(npg, device="cuda:0"):
+    """Device workspace for the adjoint / tangent entry points (one call at a time)."""
+    import torch
+    nbytes = int(lib().hydro_mhm_adjoint_workspace_bytes(int(npg)))
+    return torch.empty((max(nbytes, 8) + 7) // 8, dtype=torch.float64, device=device)
+
+
+def _check_fp64_pg(u, name):
+    import torch
+    if not u.is_cuda or u.dtype != torch.float64:
+        raise HydroError(f"{name} must be a float64 device tensor")
+    if u.dim() != 5 or tuple(u.shape[1:]) != (5, 22, 22, 22):
+        raise HydroError(f"{name} shape {tuple(u.shape)} != (npg, 5, 22, 22, 22)")
+
+
+def mhm_adjoint(u, lam_out, dt, dh, workspace=None, stream=None, **kw):
+    """Vector-Jacobian product of one MHM_RP FP64 step at input ``u`` (npg, 5, 22^3):
+    returns lam_in = (d out / d u)^T lam_out, lam_out of shape (npg, 5, 16, 16, 16)."""
+    import torch
+    _check_fp64_pg(u, "u")
+    u = u.contiguous()
+    npg = u.shape[0]
+    lam_out = lam_out.contiguous()
+    if tuple(lam_out.shape) != (npg, 5, 16, 16, 16) or lam_out.dtype != torch.float64:
+        raise HydroError("lam_out must be float64 of shape (npg, 5, 16, 16, 16)")
+    ws = workspace if workspace is not None else adjoint_workspace(npg, u.device)
+    lam_in = torch.empty_like(u)
+    args = make_args(npg, dt, dh, store_flux=False, precision="fp64", in_=u.data_ptr(), **kw)
+    _check(lib().hydro_mhm_adjoint_async(ctypes.byref(args), lam_out.data_ptr(), lam_in.data_ptr(), ws.data_ptr(),
+                                         _stream(u.device, stream)))
+    return lam_in
+
+
+def mhm_tangent(u, v, dt, dh, workspace=None, stream=None, **kw):
+    """Jacobian-vector product of one MHM_RP FP64 step: returns (out, J v); v=None gives (out, None) from the
+    phase-wise evaluation used by the derivative kernels."""
+    import torch
+    _check_fp64_pg(u, "u")
+    u = u.contiguous()
+    npg = u.shape[0]
+    if v is not None:
+        _check_fp64_pg(v, "v")
+        if v.shape != u.shape:
+            raise HydroError("v must have the shape of u")
+        v = v.contiguous()
+    ws = workspace if workspace is not None else adjoint_workspace(npg, u.device)
+    out = torch.empty((npg, 5, 16, 16, 16), dtype=torch.float64, device=u.device)
+    jv = torch.empty_like(out) if v is not None else None
+    args = make_args(npg, dt, dh, store_flux=False, precision="fp64", in_=u.data_ptr(), out=out.data_ptr(), **kw)
+    _check(lib().hydro_mhm_tangent_async(ctypes.byref(args), v.data_ptr() if v is not None else None,
+                                         jv.data_ptr() if jv is not None else None, ws.data_ptr(),
+                                         _stream(u.device, stream)))
+    return out, jv
+
+
+def prepare_uniform_adjoint(pg_lam, level_shape, ghost, stream=None):
+    """Transpose of prepare_uniform: sum the (npg, 5, N^3) patch-group copies back onto a (5, Z, Y, X) level."""
+    import torch
+    pg_lam = pg_lam.contiguous()
+    _, nz, ny, nx = level_shape
+    lam = torch.empty(tuple(level_shape), dtype=pg_lam.dtype, device=pg_lam.device)
+    _check(lib().hydro_prepare_uniform_adjoint_async(pg_lam.data_ptr(), lam.data_ptr(), nx, ny, nz, ghost,
+                                                     _prec(pg_lam), _stream(pg_lam.device, stream)))
+    return lam
 
 
 class UniformLevel:

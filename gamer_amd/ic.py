@@ -101,16 +101,10 @@ def cut_patch_groups(cons: np.ndarray, ghost: int) -> np.ndarray:
     _, nz, ny, nx = cons.shape
     assert nz % PS2 == 0 and ny % PS2 == 0 and nx % PS2 == 0
     pad = np.pad(cons, ((0, 0), (ghost, ghost), (ghost, ghost), (ghost, ghost)), mode="wrap")
-    gz, gy, gx = nz // PS2, ny // PS2, nx // PS2
     n = PS2 + 2 * ghost
-    out = np.empty((gz * gy * gx, NCOMP, n, n, n), dtype=cons.dtype)
-    p = 0
-    for kz in range(gz):
-        for ky in range(gy):
-            for kx in range(gx):
-                out[p] = pad[:, kz * PS2:kz * PS2 + n, ky * PS2:ky * PS2 + n, kx * PS2:kx * PS2 + n]
-                p += 1
-    return out
+    win = np.lib.stride_tricks.sliding_window_view(pad, (n, n, n), axis=(1, 2, 3))[:, ::PS2, ::PS2, ::PS2]
+    gz, gy, gx = nz // PS2, ny // PS2, nx // PS2
+    return np.ascontiguousarray(win.transpose(1, 2, 3, 0, 4, 5, 6)).reshape(gz * gy * gx, NCOMP, n, n, n)
 
 
 def random_sod(npg_shape=(4, 4, 4), seed=1234, scheme_ghost=3, sedov=False, cfl=0.5, gamma=GAMMA,

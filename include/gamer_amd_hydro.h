@@ -14,6 +14,8 @@
 #ifndef GAMER_AMD_HYDRO_H
 #define GAMER_AMD_HYDRO_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -90,6 +92,29 @@ int hydro_prepare_uniform_async( const void *level, void *pg_in, int nx, int ny,
 /* Scatter of patch-group outputs pg_out[npg][5][16^3] back into level[5][nz][ny][nx]
    (the copy-back step of Flu_Close, src/Fluid/Flu_Close.cpp:133-217). */
 int hydro_close_uniform_async( const void *pg_out, void *level, int nx, int ny, int nz, int precision, void *stream );
+
+/* ---- Differentiation of one MHM_RP step (HYDRO_FP64 only) ----
+   The step is the map in[npg][5][22^3] -> out[npg][5][16^3] of hydro_mhm_solve_async with the same arguments
+   (dt, dh constant; the MinMod retry is not differentiated).  No counterpart in the reference, which has no
+   derivative of CPU_FluidSolver_MHM (src/Model_Hydro/CPU_Hydro/CPU_FluidSolver_MHM.cpp:263-589); these are the
+   reverse and forward derivatives of that function.  `workspace` is device memory of
+   hydro_mhm_adjoint_workspace_bytes(npg) bytes, used by one call at a time. */
+size_t hydro_mhm_adjoint_workspace_bytes( int npg );
+
+/* Vector-Jacobian product: lam_in[npg][5][22^3] = (d out / d in)^T lam_out[npg][5][16^3], evaluated at args->in
+   (args->out and args->flux are not used). */
+int hydro_mhm_adjoint_async( const HydroSolveArgs *args, const void *lam_out, void *lam_in, void *workspace,
+                             void *stream );
+
+/* Jacobian-vector product: jv_out[npg][5][16^3] = (d out / d in) v_in[npg][5][22^3], and args->out (if not NULL)
+   receives the step's output.  v_in == NULL runs the same phase-wise evaluation without tangents (out only). */
+int hydro_mhm_tangent_async( const HydroSolveArgs *args, const void *v_in, void *jv_out, void *workspace,
+                             void *stream );
+
+/* Transpose of hydro_prepare_uniform_async: level_lam[5][nz][ny][nx] = sum over all patch-group input elements
+   pg_lam[npg][5][N^3] that copy the same level cell (ghost copies included). */
+int hydro_prepare_uniform_adjoint_async( const void *pg_lam, void *level_lam, int nx, int ny, int nz, int ghost,
+                                         int precision, void *stream );
 
 /* Wait for all work queued by this library (replaces CUAPI_Synchronize, src/GPU_API/CUAPI_Synchronize.cu:11-14). */
 int hydro_synchronize( void );

@@ -15,7 +15,7 @@ HEADER = os.path.join(ROOT, "include", "gamer_amd_hydro.h")
 def declared_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?(?:int|void|char)\s*\*?\s*(hydro_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?(?:int|void|char|size_t)\s*\*?\s*(hydro_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_header_declares_expected_api():
@@ -26,6 +26,24 @@ def test_library_exports_every_declared_symbol():
     L = hydro.lib()
     for name in declared_functions():
         assert hasattr(L, name), name
+
+
+def test_adjoint_entry_points_reject_bad_arguments():
+    """Only MHM_RP in FP64 is differentiated; a workspace is required; checks run before any device work."""
+    L = hydro.lib()
+    assert L.hydro_mhm_adjoint_workspace_bytes(0) == 0
+    assert L.hydro_mhm_adjoint_workspace_bytes(2) == 2 * L.hydro_mhm_adjoint_workspace_bytes(1) > 0
+    ok = dict(npg=4, dt=1e-3, dh=1e-2, in_=16, store_flux=False)
+    for kw, frag in [(dict(scheme="MHM"), "MHM_RP"), (dict(precision="fp32"), "FP64")]:
+        a = hydro.make_args(**{**ok, **kw})
+        assert L.hydro_mhm_adjoint_async(ctypes.byref(a), 16, 16, 16, None) == hydro.HYDRO_E_ARG
+        assert frag in L.hydro_last_error().decode()
+    a = hydro.make_args(**ok)
+    assert L.hydro_mhm_adjoint_async(ctypes.byref(a), 16, 16, None, None) == hydro.HYDRO_E_ARG
+    assert "workspace" in L.hydro_last_error().decode()
+    assert L.hydro_mhm_adjoint_async(ctypes.byref(a), None, 16, 16, None) == hydro.HYDRO_E_ARG
+    assert L.hydro_mhm_tangent_async(ctypes.byref(a), None, None, 16, None) == hydro.HYDRO_E_ARG
+    assert L.hydro_prepare_uniform_adjoint_async(16, 16, 20, 16, 16, 3, 0, None) == hydro.HYDRO_E_ARG
 
 
 def _call(**kw):
