@@ -9,7 +9,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgamer_amd_hydro.so")
-SOURCES = ["hydro_api.hip", "hydro_mhm_rp.hip", "hydro_mhm.hip", "hydro_grid.hip", "hydro_adjoint.hip"]
+SOURCES = ["hydro_api.hip", "hydro_mhm_rp.hip", "hydro_mhm.hip", "hydro_grid.hip", "hydro_adjoint.hip",
+           "hydro_sync.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -237,6 +237,87 @@ int hydro_prepare_uniform_adjoint_async( const void *pg_lam, void *level_lam, in
    return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "prepare adjoint launch failed" );
 }
 
+int hydro_correct_unphysical_async( const HydroCorrectArgs *a, void *stream )
+{
+   if ( a == nullptr ) return fail( HYDRO_E_ARG, "args == NULL" );
+   if ( a->npg < 0 || a->npg > ( 1 << 26 ) ) return fail( HYDRO_E_ARG, "correct: npg = %d out of range", a->npg );
+   if ( a->npg > 0 && ( a->in == nullptr || a->out == nullptr || a->counters == nullptr ) )
+      return fail( HYDRO_E_ARG, "correct: in/out/counters == NULL" );
+   if ( a->scheme != HYDRO_MHM_RP && a->scheme != HYDRO_MHM ) return fail( HYDRO_E_ARG, "correct: unsupported scheme %d", a->scheme );
+   if ( a->mode < 0 || a->mode > 2 ) return fail( HYDRO_E_ARG, "correct: mode %d (0 none, 1 3D, 2 3D1D)", a->mode );
+   if ( a->rsolver != HYDRO_HLLC && a->rsolver != HYDRO_HLLE && a->rsolver != HYDRO_ROE )
+      return fail( HYDRO_E_ARG, "correct: unsupported Riemann solver %d", a->rsolver );
+   if ( !( a->gamma > 1.0 ) || !( a->dh > 0.0 ) || !( a->dt >= 0.0 ) ) return fail( HYDRO_E_ARG, "correct: gamma/dt/dh invalid" );
+   CorrPrm c;
+   c.mode = a->mode;  c.rsolver = a->rsolver;  c.last_resort_floor = a->last_resort_floor;
+   c.auto_reduce = a->auto_reduce_dt;  c.fixup_flux = a->fixup_flux;  c.check_pres = a->check_pres;
+   c.NX = ( a->scheme == HYDRO_MHM_RP ) ? 22 : 20;  c.ghost = ( c.NX - 16 )/2;
+   c.min_dens = a->min_dens;  c.min_pres = a->min_pres;  c.min_eint = a->min_eint;  c.gm1 = a->gamma - 1.0;
+   HydroSolveArgs s = {};                     // solver constants for the first-order interface solves
+   s.npg = a->npg;  s.dt = a->dt;  s.dh = a->dh;  s.gamma = a->gamma;  s.min_dens = a->min_dens;
+   s.min_pres = a->min_pres;  s.min_eint = a->min_eint;  s.limiter = 4;  s.minmod_coeff = 1.5;
+   int rc;
+   if ( a->precision == HYDRO_FP64 )
+      rc = launch_correct_unphysical<double>( (const double*)a->in, (double*)a->out, (double*)a->flux, a->npg, c,
+                                              make_prm<double>( &s ), a->dt/a->dh, a->counters, (hipStream_t)stream );
+   else if ( a->precision == HYDRO_FP32_BULK )
+      rc = launch_correct_unphysical<float>( (const float*)a->in, (float*)a->out, (float*)a->flux, a->npg, c,
+                                             make_prm<float>( &s ), (float)a->dt/(float)a->dh, a->counters, (hipStream_t)stream );
+   else return fail( HYDRO_E_ARG, "unsupported precision %d", a->precision );
+   return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "correct launch failed" );
+}
+
+#define GA_PREC_DISPATCH( CALL_D, CALL_F )                                                                           \
+   int rc;                                                                                                           \
+   if      ( precision == HYDRO_FP64 )      rc = CALL_D;                                                             \
+   else if ( precision == HYDRO_FP32_BULK ) rc = CALL_F;                                                             \
+   else return fail( HYDRO_E_ARG, "unsupported precision %d", precision );
+
+int hydro_store_flux_async( const void *flux, const int *slot, int npg, double dt, double *reg, int precision, void *stream )
+{
+   if ( npg < 0 || ( npg > 0 && ( flux == nullptr || slot == nullptr || reg == nullptr ) ) ) return fail( HYDRO_E_ARG, "store_flux: bad pointers/size" );
+   GA_PREC_DISPATCH( launch_store_flux<double>( (const double*)flux, slot, npg, dt, reg, (hipStream_t)stream ),
+                     launch_store_flux<float> ( (const float*)flux,  slot, npg, dt, reg, (hipStream_t)stream ) )
+   return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "store_flux launch failed" );
+}
+
+int hydro_correct_flux_async( const void *flux, const int *slot, int npg, double dt, double *reg, int precision, void *stream )
+{
+   if ( npg < 0 || ( npg > 0 && ( flux == nullptr || slot == nullptr || reg == nullptr ) ) ) return fail( HYDRO_E_ARG, "correct_flux: bad pointers/size" );
+   GA_PREC_DISPATCH( launch_correct_flux<double>( (const double*)flux, slot, npg, dt, reg, (hipStream_t)stream ),
+                     launch_correct_flux<float> ( (const float*)flux,  slot, npg, dt, reg, (hipStream_t)stream ) )
+   return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "correct_flux launch failed" );
+}
+
+int hydro_fixup_flux_async( void *patch, const int *slot, const double *reg, int npatch, double dh, double gamma,
+                            double min_dens, double min_pres, double min_eint, int precision, void *stream )
+{
+   if ( npatch < 0 || ( npatch > 0 && ( patch == nullptr || slot == nullptr || reg == nullptr ) ) ) return fail( HYDRO_E_ARG, "fixup_flux: bad pointers/size" );
+   if ( !( dh > 0.0 ) || !( gamma > 1.0 ) ) return fail( HYDRO_E_ARG, "fixup_flux: dh/gamma invalid" );
+   GA_PREC_DISPATCH( launch_fixup_flux<double>( (double*)patch, slot, reg, npatch, dh, gamma - 1.0, min_dens, min_pres, min_eint, (hipStream_t)stream ),
+                     launch_fixup_flux<float> ( (float*)patch,  slot, reg, npatch, dh, gamma - 1.0, min_dens, min_pres, min_eint, (hipStream_t)stream ) )
+   return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "fixup_flux launch failed" );
+}
+
+int hydro_fixup_restrict_async( const void *son, void *fa, int ngroup, const int *son0, const int *fa_idx, double min_eint,
+                                int precision, void *stream )
+{
+   if ( ngroup < 0 || ( ngroup > 0 && ( son == nullptr || fa == nullptr || son0 == nullptr || fa_idx == nullptr ) ) )
+      return fail( HYDRO_E_ARG, "fixup_restrict: bad pointers/size" );
+   GA_PREC_DISPATCH( launch_fixup_restrict<double>( (const double*)son, (double*)fa, ngroup, son0, fa_idx, min_eint, (hipStream_t)stream ),
+                     launch_fixup_restrict<float> ( (const float*)son,  (float*)fa,  ngroup, son0, fa_idx, min_eint, (hipStream_t)stream ) )
+   return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "fixup_restrict launch failed" );
+}
+
+int hydro_close_patches_async( const void *pg_out, void *patch, int npg, const int *pid0, int precision, void *stream )
+{
+   if ( npg < 0 || ( npg > 0 && ( pg_out == nullptr || patch == nullptr || pid0 == nullptr ) ) ) return fail( HYDRO_E_ARG, "close_patches: bad pointers/size" );
+   GA_PREC_DISPATCH( launch_close_patches<double>( (const double*)pg_out, (double*)patch, npg, pid0, (hipStream_t)stream ),
+                     launch_close_patches<float> ( (const float*)pg_out,  (float*)patch,  npg, pid0, (hipStream_t)stream ) )
+   return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "close_patches launch failed" );
+}
+#undef GA_PREC_DISPATCH
+
 int hydro_mhm_solve_batch_async( const HydroSolveArgs *a, const double *pg_dt_dh, unsigned *queue_counter,
                                  int groups_per_cu, void *stream )
 {

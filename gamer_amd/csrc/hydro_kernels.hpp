@@ -31,6 +31,31 @@ int launch_scatter( const IO *pg_out, IO *grid, int nx, int ny, int nz, hipStrea
 // set the work-queue counter to 0 (a kernel, so the reset can live inside a captured graph)
 int launch_reset_counter( unsigned *counter, hipStream_t stream );
 
+// options of the device CorrectUnphysical (src/Fluid/Flu_Close.cpp:542-1275)
+struct CorrPrm {
+   int    mode;                // OPT__1ST_FLUX_CORR: 0 none, 1 3D, 2 3D1D
+   int    rsolver;             // first-order solver: HYDRO_ROE / HYDRO_HLLE / HYDRO_HLLC
+   int    last_resort_floor, auto_reduce, fixup_flux, check_pres;
+   int    NX, ghost;           // input patch-group width and ghost zones of the solver call
+   double min_dens, min_pres, min_eint, gm1;
+};
+template <typename R>
+int launch_correct_unphysical( const R *in, R *out, R *flux, int npg, const CorrPrm &c, const Prm<R> &P, R dt_dh,
+                               int *counters, hipStream_t stream );
+// flux registers reg[nreg][5][8][8] (FP64); slot tables: store [npg][8][6], correct [npg][6], fix-up [npatch][6]
+template <typename IO>
+int launch_store_flux( const IO *flux, const int *slot, int npg, double dt, double *reg, hipStream_t stream );
+template <typename IO>
+int launch_correct_flux( const IO *flux, const int *slot, int npg, double dt, double *reg, hipStream_t stream );
+template <typename IO>
+int launch_fixup_flux( IO *patch, const int *slot, const double *reg, int npatch, double dh, double gm1,
+                       double min_dens, double min_pres, double min_eint, hipStream_t stream );
+template <typename IO>
+int launch_fixup_restrict( const IO *son, IO *fa, int ngroup, const int *son0, const int *fa_idx, double min_eint,
+                           hipStream_t stream );
+template <typename IO>
+int launch_close_patches( const IO *pg_out, IO *patch, int npg, const int *pid0, hipStream_t stream );
+
 // MHM_RP step, FP64: adjoint lam_in = J^T lam_out, and tangent-linear tout = J tin (+ out = U_out; tin == NULL:
 // plain phase-wise forward).  ws: adjoint_workspace_bytes(npg) of device memory.
 size_t adjoint_workspace_bytes( int npg );

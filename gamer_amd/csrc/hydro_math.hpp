@@ -33,8 +33,9 @@ template <typename R> __device__ __forceinline__ R vmin( R a, R b ) { return __b
 template <typename R> __device__ __forceinline__ R vabs( R a )      { return __builtin_fabs( a ); }
 template <typename R> __device__ __forceinline__ R vcopysign( R m, R s ) { return __builtin_copysign( m, s ); }
 
-// pressure floor that lets NaN through (FluUtility.cpp:681-690)
-template <typename R> __device__ __forceinline__ R floor_pres( R p, R pmin ) { return ( p == p ) ? vmax( p, pmin ) : p; }
+// pressure floor that lets NaN through (FluUtility.cpp:681-690): a NaN compares false and is returned as is.
+// One compare + select; max() would first have to quiet operands read from memory (IEEE mode).
+template <typename R> __device__ __forceinline__ R floor_pres( R p, R pmin ) { return ( p < pmin ) ? pmin : p; }
 
 // ---------------- solver constants ----------------
 template <typename R>
@@ -420,11 +421,12 @@ __device__ __forceinline__ IO cell_cfl( const IO u[NV], double gamma, double gm1
 
 // Fused CFL of one patch group: each lane holds the maxima of its output column in the two z-halves; the 8
 // patches (px + 2 py + 4 pz, x fastest) are reduced with wave64 butterflies and LDS across the 4 waves.
-// Called by all 256 lanes; writes dt[8] = dh_safety / max.
+// Called by every lane of the workgroup (waves 0-3 hold the output columns); writes dt[8] = dh_safety / max.
 template <typename IO>
 __device__ __forceinline__ void fused_dt( const IO cfl[2], int pxy, float (*red)[4], IO *dt, double dh_safety )
 {
    const int tid = threadIdx.x, w = tid >> 6;
+   if ( w < 4 )                      // the output columns live on waves 0-3; further waves hold ghost columns
 #  pragma unroll
    for (int pz=0; pz<2; pz++)
 #     pragma unroll

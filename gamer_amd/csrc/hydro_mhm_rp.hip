@@ -110,25 +110,29 @@ __device__ __forceinline__ void derive_cell( const IO u[NV], R q[8], const Prm<R
    q[7] = vsqrt( P.gamma*p*ir );
 }
 
-// half-step cell update, floors and conversion to primitive (CPU_FluidSolver_MHM.cpp:846-933), accumulated in FP64
+// half-step cell update, floors and conversion to primitive (CPU_FluidSolver_MHM.cpp:846-933).  The update
+// U - dt/(2dh) sum dF runs in the bulk type; the pressure recovery E - |m|^2/(2 rho) and the internal-energy
+// floor run in FP64.
 template <typename R>
 __device__ __forceinline__ void half_cell( const R u[NV], const R dfxy[NV], const R hzl[NV], const R hzu[NV],
                                            const Prm<R> &P, R w[NV] )
 {
-   double c[NV];
+   R c[NV];
+   const R k = (R)P.dt_dh2;
 #  pragma unroll
-   for (int v=0; v<NV; v++)
-      c[v] = (double)u[v] - P.dt_dh2*( (double)dfxy[v] + ( (double)hzu[v] - (double)hzl[v] ) );
-   c[0] = __builtin_fmax( c[0], P.min_dens_d );
-   const double r  = rcp( c[0] );
-   const double ke = 0.5*( c[1]*c[1] + c[2]*c[2] + c[3]*c[3] )*r;
-   double ein = c[4] - ke;
+   for (int v=0; v<NV; v++) c[v] = u[v] - k*( dfxy[v] + ( hzu[v] - hzl[v] ) );
+   c[0] = vmax( c[0], P.min_dens );
+   const double d = (double)c[0], m1 = (double)c[1], m2 = (double)c[2], m3 = (double)c[3];
+   const double ke = 0.5*( m1*m1 + m2*m2 + m3*m3 )*rcp( d );
+   double e = (double)c[4];
+   double ein = e - ke;
    const double eout = ( ein == ein ) ? __builtin_fmax( ein, P.min_eint_d ) : ein;
-   if ( ein != eout ) { c[4] = c[4] - ein + eout;  ein = c[4] - ke; }
-   w[0] = (R)c[0];
-   w[1] = (R)( c[1]*r );
-   w[2] = (R)( c[2]*r );
-   w[3] = (R)( c[3]*r );
+   if ( ein != eout ) { e = e - ein + eout;  ein = e - ke; }
+   const R r = rcp( c[0] );
+   w[0] = c[0];
+   w[1] = c[1]*r;
+   w[2] = c[2]*r;
+   w[3] = c[3]*r;
    const double p = ein*P.gm1_d;
    w[4] = (R)( ( p == p ) ? __builtin_fmax( p, P.min_pres_d ) : p );
 }

@@ -32,7 +32,9 @@ EXPORTS = ("hydro_init", "hydro_mhm_solve_async", "hydro_synchronize", "hydro_la
            "hydro_mem_allocate_fluid", "hydro_mem_free_fluid", "hydro_host_buffers", "hydro_asyn_fluid_solver",
            "hydro_dt_cfl_async", "hydro_prepare_uniform_async", "hydro_close_uniform_async",
            "hydro_mhm_solve_batch_async", "hydro_mhm_adjoint_workspace_bytes", "hydro_mhm_adjoint_async",
-           "hydro_mhm_tangent_async", "hydro_prepare_uniform_adjoint_async")
+           "hydro_mhm_tangent_async", "hydro_prepare_uniform_adjoint_async", "hydro_correct_unphysical_async",
+           "hydro_store_flux_async", "hydro_correct_flux_async", "hydro_fixup_flux_async",
+           "hydro_fixup_restrict_async", "hydro_close_patches_async")
 
 
 class HydroError(RuntimeError):
@@ -50,6 +52,19 @@ class HydroSolveArgs(ctypes.Structure):
         ("dt_patch", ctypes.c_void_p), ("safety", ctypes.c_double),
     ]
 
+
+class HydroCorrectArgs(ctypes.Structure):
+    _fields_ = [
+        ("in_", ctypes.c_void_p), ("out", ctypes.c_void_p), ("flux", ctypes.c_void_p),
+        ("npg", ctypes.c_int), ("dt", ctypes.c_double), ("dh", ctypes.c_double), ("gamma", ctypes.c_double),
+        ("scheme", ctypes.c_int), ("mode", ctypes.c_int), ("rsolver", ctypes.c_int),
+        ("last_resort_floor", ctypes.c_int), ("auto_reduce_dt", ctypes.c_int), ("fixup_flux", ctypes.c_int),
+        ("check_pres", ctypes.c_int), ("min_dens", ctypes.c_double), ("min_pres", ctypes.c_double),
+        ("min_eint", ctypes.c_double), ("precision", ctypes.c_int), ("counters", ctypes.c_void_p),
+    ]
+
+
+FIRST_FLUX_CORR = {"NONE": 0, "3D": 1, "3D1D": 2}
 
 _lib = None
 
@@ -79,6 +94,16 @@ def lib():
         L.hydro_mhm_adjoint_async.argtypes = [ctypes.POINTER(HydroSolveArgs)] + [ctypes.c_void_p] * 4
         L.hydro_mhm_tangent_async.argtypes = [ctypes.POINTER(HydroSolveArgs)] + [ctypes.c_void_p] * 4
         L.hydro_prepare_uniform_adjoint_async.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 5 + [ctypes.c_void_p]
+        L.hydro_correct_unphysical_async.argtypes = [ctypes.POINTER(HydroCorrectArgs), ctypes.c_void_p]
+        for f in (L.hydro_store_flux_async, L.hydro_correct_flux_async):
+            f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
+                          ctypes.c_int, ctypes.c_void_p]
+        L.hydro_fixup_flux_async.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int] + \
+            [ctypes.c_double] * 5 + [ctypes.c_int, ctypes.c_void_p]
+        L.hydro_fixup_restrict_async.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                 ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+        L.hydro_close_patches_async.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                ctypes.c_int, ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -287,6 +312,77 @@ def prepare_uniform_adjoint(pg_lam, level_shape, ghost, stream=None):
     _check(lib().hydro_prepare_uniform_adjoint_async(pg_lam.data_ptr(), lam.data_ptr(), nx, ny, nz, ghost,
                                                      _prec(pg_lam), _stream(pg_lam.device, stream)))
     return lam
+
+
+# ---------------- Flu_Close and level synchronisation (device tensors) ----------------
+def _int_table(t, shape, name):
+    import torch
+    if not t.is_cuda or t.dtype != torch.int32 or tuple(t.shape) != tuple(shape) or not t.is_contiguous():
+        raise HydroError(f"{name} must be a contiguous int32 device tensor of shape {tuple(shape)}")
+
+
+def correct_unphysical(u_in, out, flux, dt, dh, mode="3D1D", rsolver="HLLE", scheme="MHM_RP", gamma=5.0 / 3.0,
+                       last_resort_floor=True, auto_reduce_dt=False, fixup_flux=True, check_pres=False,
+                       min_dens=0.0, min_pres=0.0, min_eint=0.0, stream=None):
+    """CorrectUnphysical on device: corrects ``out`` (and the coarse-fine planes of ``flux``) in place.
+    Returns a 2-element int32 device tensor: [corrected cells, cells still unphysical]."""
+    import torch
+    n = NX[scheme]
+    if tuple(u_in.shape[1:]) != (5, n, n, n) or tuple(out.shape) != (u_in.shape[0], 5, 16, 16, 16):
+        raise HydroError("correct_unphysical: shapes must be (npg, 5, N^3) and (npg, 5, 16^3)")
+    if u_in.dtype != out.dtype or (flux is not None and flux.dtype != out.dtype):
+        raise HydroError("correct_unphysical: dtypes differ")
+    cnt = torch.zeros(2, dtype=torch.int32, device=out.device)
+    a = HydroCorrectArgs(u_in.contiguous().data_ptr(), out.data_ptr(), flux.data_ptr() if flux is not None else None,
+                         u_in.shape[0], float(dt), float(dh), float(gamma), SCHEME[scheme], FIRST_FLUX_CORR[mode],
+                         RSOLVER[rsolver], int(last_resort_floor), int(auto_reduce_dt), int(fixup_flux),
+                         int(check_pres), float(min_dens), float(min_pres), float(min_eint), _prec(out),
+                         cnt.data_ptr())
+    _check(lib().hydro_correct_unphysical_async(ctypes.byref(a), _stream(out.device, stream)))
+    return cnt
+
+
+def store_flux(flux, slot, dt, reg, stream=None):
+    """StoreFlux: reg[slot[pg, LocalID, s]] = dt * patch-face fluxes (reg: float64 (nreg, 5, 8, 8))."""
+    _int_table(slot, (flux.shape[0], 8, 6), "slot")
+    _check(lib().hydro_store_flux_async(flux.contiguous().data_ptr(), slot.data_ptr(), flux.shape[0], float(dt),
+                                        reg.data_ptr(), _prec(flux), _stream(flux.device, stream)))
+    return reg
+
+
+def correct_flux(flux, slot, dt, reg, stream=None):
+    """CorrectFlux: reg[slot[pg, s]] -= dt/4 * the fine fluxes on patch-group face s."""
+    _int_table(slot, (flux.shape[0], 6), "slot")
+    _check(lib().hydro_correct_flux_async(flux.contiguous().data_ptr(), slot.data_ptr(), flux.shape[0], float(dt),
+                                          reg.data_ptr(), _prec(flux), _stream(flux.device, stream)))
+    return reg
+
+
+def fixup_flux(patch, slot, reg, dh, gamma=5.0 / 3.0, min_dens=0.0, min_pres=0.0, min_eint=0.0, stream=None):
+    """Flu_FixUp_Flux on patches (npatch, 5, 8, 8, 8) in place, registers reg[slot[patch, s]]."""
+    _int_table(slot, (patch.shape[0], 6), "slot")
+    _check(lib().hydro_fixup_flux_async(patch.data_ptr(), slot.data_ptr(), reg.data_ptr(), patch.shape[0],
+                                        float(dh), float(gamma), float(min_dens), float(min_pres), float(min_eint),
+                                        _prec(patch), _stream(patch.device, stream)))
+    return patch
+
+
+def fixup_restrict(son, fa, son0, fa_idx, min_eint=0.0, stream=None):
+    """Flu_FixUp_Restrict: fa[fa_idx[q]] = mean of son[son0[q] + LocalID], then the Eint floor (in place)."""
+    _int_table(son0, (son0.shape[0],), "son0")
+    _int_table(fa_idx, (son0.shape[0],), "fa_idx")
+    _check(lib().hydro_fixup_restrict_async(son.contiguous().data_ptr(), fa.data_ptr(), son0.shape[0],
+                                            son0.data_ptr(), fa_idx.data_ptr(), float(min_eint), _prec(fa),
+                                            _stream(fa.device, stream)))
+    return fa
+
+
+def close_patches(pg_out, patch, pid0, stream=None):
+    """Copy (npg, 5, 16^3) patch-group outputs into patches pid0[pg] + LocalID of (npatch, 5, 8, 8, 8)."""
+    _int_table(pid0, (pg_out.shape[0],), "pid0")
+    _check(lib().hydro_close_patches_async(pg_out.contiguous().data_ptr(), patch.data_ptr(), pg_out.shape[0],
+                                           pid0.data_ptr(), _prec(patch), _stream(patch.device, stream)))
+    return patch
 
 
 class UniformLevel:

@@ -93,6 +93,59 @@ int hydro_prepare_uniform_async( const void *level, void *pg_in, int nx, int ny,
    (the copy-back step of Flu_Close, src/Fluid/Flu_Close.cpp:133-217). */
 int hydro_close_uniform_async( const void *pg_out, void *level, int nx, int ny, int nz, int precision, void *stream );
 
+/* ---- Flu_Close and the AMR level synchronisation on device arrays ----
+   Patch data use GAMER's per-patch layout patch[npatch][5][8][8][8]; the 8 patches of a patch group are
+   PID0 + LocalID with the TABLE_02 offsets (src/Tables/Table_02.cpp:22-70): LocalID 0..7 =
+   (0,0,0) (1,0,0) (0,1,0) (0,0,1) (1,1,0) (0,1,1) (1,0,1) (1,1,1).  Flux registers (the patch->flux[s] buffers of
+   coarse patches on coarse-fine faces) are double reg[nreg][5][8][8] in both precisions. */
+
+/* Options of CorrectUnphysical (src/Fluid/Flu_Close.cpp:542-1275, run-time parameters of the same names). */
+typedef struct HydroCorrectArgs {
+   const void *in;            /* the solver input of the call, [npg][5][N^3] */
+   void       *out;           /* the solver output [npg][5][16^3], corrected in place */
+   void       *flux;          /* [npg][9][5][16^2] or NULL; coarse-fine planes get the first-order fluxes */
+   int    npg;
+   double dt, dh, gamma;
+   int    scheme;             /* HYDRO_MHM_RP or HYDRO_MHM (ghost width of `in`) */
+   int    mode;               /* OPT__1ST_FLUX_CORR: 0 none, 1 3D, 2 3D1D */
+   int    rsolver;            /* OPT__1ST_FLUX_CORR_SCHEME as HYDRO_ROE / HYDRO_HLLE / HYDRO_HLLC */
+   int    last_resort_floor;  /* OPT__LAST_RESORT_FLOOR */
+   int    auto_reduce_dt;     /* AutoReduceDt_Continue */
+   int    fixup_flux;         /* OPT__FIXUP_FLUX */
+   int    check_pres;         /* OPT__CHECK_PRES_AFTER_FLU */
+   double min_dens, min_pres, min_eint;
+   int    precision;
+   int   *counters;           /* device int[2]: [0] += corrected cells, [1] += cells still unphysical (failure) */
+} HydroCorrectArgs;
+
+/* Replaces CorrectUnphysical (src/Fluid/Flu_Close.cpp:542-1275) and its Unphysical test (:412-535). */
+int hydro_correct_unphysical_async( const HydroCorrectArgs *args, void *stream );
+
+/* StoreFlux (src/Fluid/Flu_Close.cpp:234-307): reg[slot[pg][LocalID][s]] = dt * flux of patch face s
+   (slot [npg][8][6], -1 = no register). */
+int hydro_store_flux_async( const void *flux, const int *slot, int npg, double dt, double *reg, int precision,
+                            void *stream );
+
+/* CorrectFlux (src/Fluid/Flu_Close.cpp:321-395): reg[slot[pg][s]] -= dt/4 * (the 4 fine fluxes of each coarse face
+   cell on patch-group face s), slot [npg][6] = the coarse sibling's register, -1 = none. */
+int hydro_correct_flux_async( const void *flux, const int *slot, int npg, double dt, double *reg, int precision,
+                              void *stream );
+
+/* Flu_FixUp_Flux (src/Fluid/Flu_FixUp_Flux.cpp): cells on registered faces (slot [npatch][6]) get
+   U += reg * (-1/dh, +1/dh per face side) unless the result fails the density / Eint / pressure test. */
+int hydro_fixup_flux_async( void *patch, const int *slot, const double *reg, int npatch, double dh, double gamma,
+                            double min_dens, double min_pres, double min_eint, int precision, void *stream );
+
+/* Flu_FixUp_Restrict (src/Fluid/Flu_FixUp_Restrict.cpp): fa[fa_idx[q]] = mean of the 8 son patches
+   son[son0[q] + LocalID], then the internal-energy floor (Hydro_CheckMinEintInEngy). */
+int hydro_fixup_restrict_async( const void *son, void *fa, int ngroup, const int *son0, const int *fa_idx,
+                                double min_eint, int precision, void *stream );
+
+/* The copy-back of Flu_Close (src/Fluid/Flu_Close.cpp:133-160): patch-group outputs pg_out[npg][5][16^3] into
+   the patches pid0[pg] + LocalID. */
+int hydro_close_patches_async( const void *pg_out, void *patch, int npg, const int *pid0, int precision,
+                               void *stream );
+
 /* ---- Differentiation of one MHM_RP step (HYDRO_FP64 only) ----
    The step is the map in[npg][5][22^3] -> out[npg][5][16^3] of hydro_mhm_solve_async with the same arguments
    (dt, dh constant; the MinMod retry is not differentiated).  No counterpart in the reference, which has no

@@ -616,6 +616,270 @@ int run( const T *in, T *out, T *flux, int npg, double dt, double dh, double gam
    return 0;
 }
 
+// ---------------- Flu_Close and the AMR level synchronisation ----------------
+// TABLE_02 offsets of the 8 patches in a patch group (src/Tables/Table_02.cpp:22-70)
+const int LOCAL_XYZ[8][3] = { {0,0,0}, {1,0,0}, {0,1,0}, {0,0,1}, {1,1,0}, {0,1,1}, {1,0,1}, {1,1,1} };
+
+struct CorrOpt {
+   int    mode, rsolver, last_resort_floor, auto_reduce, fixup_flux, check_pres;
+   double min_dens, min_pres, min_eint, gm1;      // gm1: EoS aux constant gamma-1, a double (CPU_EoS_Gamma.cpp:54-66)
+};
+
+// Unphysical() for HYDRO without dual energy, MHD or unit checks (src/Fluid/Flu_Close.cpp:412-535)
+template <typename T> bool close_unphysical( const Gas<T> &g, const T f[NV], bool eint_mode, const CorrOpt &c )
+{
+   for (int v=0; v<NV; v++) if ( !std::isfinite( f[v] ) ) return true;
+   if ( f[0] < (T)c.min_dens ) return true;
+   if ( !eint_mode && ( f[4] < (T)c.min_eint || f[4] != f[4] ) ) return true;
+   if ( eint_mode )
+   {
+      const T e = eint_of( f[0], f[1], f[2], f[3], f[4] );
+      if ( e < (T)c.min_eint || !std::isfinite( e ) ) return true;
+   }
+   if ( c.check_pres )
+   {
+      // EoS_DensEint2Pres of EOS_GAMMA multiplies by the double aux constant and narrows the result
+      const T p = (T)( (double)eint_of( f[0], f[1], f[2], f[3], f[4] )*c.gm1 );
+      if ( p < (T)c.min_pres || !std::isfinite( p ) ) return true;
+   }
+   return false;
+}
+
+// the first-order interface solve of CorrectUnphysical: the solver itself, without the NaN rescue
+// (src/Fluid/Flu_Close.cpp:647-745)
+template <typename T> void riemann_1st( const Gas<T> &g, int rsolver, int d, const T Lin[NV], const T Rin[NV], T pmin, T out[NV] )
+{
+   T L[NV], R[NV];
+   for (int v=0; v<NV; v++) { L[v] = Lin[v];  R[v] = Rin[v]; }
+   rotate( L, d, true );
+   rotate( R, d, true );
+   switch ( rsolver )
+   {
+      case ORC_HLLC: rs_hllc_x( g, L, R, pmin, out ); break;
+      case ORC_HLLE: rs_hlle_x( g, L, R, pmin, out ); break;
+      default:       rs_roe_x ( g, L, R, pmin, out ); break;
+   }
+   rotate( out, d, false );
+}
+
+// CorrectUnphysical() for one patch group (src/Fluid/Flu_Close.cpp:542-1275): cells that fail the
+// Eint-mode check are recomputed from the input with first-order fluxes (unsplit 3D, then the x->y->z split
+// 3D1D sweep over the 3^3 neighbourhood if still unphysical), floored, re-checked, stored, and their first-order
+// fluxes replace the coarse-fine flux planes they touch.  cnt[0] counts corrected cells, cnt[1] failures.
+template <typename T>
+void correct_unphysical_pg( const Gas<T> &g, const CorrOpt &c, const T *in, T *out, T *flux, int NX, int ghost,
+                            T dt_dh, long cnt[2] )
+{
+   const int N3 = NX*NX*NX;
+   const int didx[3] = { 1, NX, NX*NX };
+   const T pmin = (T)c.min_pres;
+   for (int k=0; k<PS; k++)
+   for (int j=0; j<PS; j++)
+   for (int i=0; i<PS; i++)
+   {
+      const int ijk[3] = { i, j, k };
+      const int io = ( k*PS + j )*PS + i;
+      T Out[NV], Update[NV], FL[3][NV], FR[3][NV];
+      for (int v=0; v<NV; v++) Out[v] = out[ v*PS*PS*PS + io ];
+      if ( !close_unphysical( g, Out, true, c ) ) continue;
+
+      const int ii = ( ( k + ghost )*NX + j + ghost )*NX + i + ghost;
+      if ( c.mode != 0 )
+      {
+         T C[NV], L[3][NV], R[3][NV], dF[3][NV];
+         for (int v=0; v<NV; v++)
+         {
+            C[v] = in[ v*N3 + ii ];
+            for (int d=0; d<3; d++) { L[d][v] = in[ v*N3 + ii - didx[d] ];  R[d][v] = in[ v*N3 + ii + didx[d] ]; }
+         }
+         for (int d=0; d<3; d++)
+         {
+            riemann_1st( g, c.rsolver, d, L[d], C, pmin, FL[d] );
+            riemann_1st( g, c.rsolver, d, C, R[d], pmin, FR[d] );
+         }
+         for (int d=0; d<3; d++) for (int v=0; v<NV; v++) dF[d][v] = FR[d][v] - FL[d][v];
+         for (int v=0; v<NV; v++) Update[v] = C[v] - dt_dh*( dF[0][v] + dF[1][v] + dF[2][v] );
+      }
+      if ( c.mode == 2 && close_unphysical( g, Update, true, c ) )
+      {
+         T blk[3][3][3][NV];
+         for (int z=0; z<3; z++) for (int y=0; y<3; y++) for (int x=0; x<3; x++)
+            for (int v=0; v<NV; v++) blk[z][y][x][v] = in[ v*N3 + ii + ( x - 1 ) + ( y - 1 )*NX + ( z - 1 )*NX*NX ];
+         const int lo[3][3] = { { 1, 0, 0 }, { 1, 1, 0 }, { 1, 1, 1 } };     // [d][x,y,z]
+         const int hi[3][3] = { { 1, 2, 2 }, { 1, 1, 2 }, { 1, 1, 1 } };
+         for (int d=0; d<3; d++)
+            for (int z=lo[d][2]; z<=hi[d][2]; z++)
+            for (int y=lo[d][1]; y<=hi[d][1]; y++)
+            for (int x=lo[d][0]; x<=hi[d][0]; x++)
+            {
+               const int ox = ( d == 0 ), oy = ( d == 1 ), oz = ( d == 2 );
+               T fl[NV], fr[NV];
+               riemann_1st( g, c.rsolver, d, blk[z-oz][y-oy][x-ox], blk[z][y][x], pmin, fl );
+               riemann_1st( g, c.rsolver, d, blk[z][y][x], blk[z+oz][y+oy][x+ox], pmin, fr );
+               for (int v=0; v<NV; v++) blk[z][y][x][v] -= dt_dh*( fr[v] - fl[v] );
+               if ( x == 1 && y == 1 && z == 1 )
+                  for (int v=0; v<NV; v++) { FL[d][v] = fl[v];  FR[d][v] = fr[v]; }
+            }
+         for (int v=0; v<NV; v++) Update[v] = blk[1][1][1][v];
+      }
+      if ( c.mode == 0 ) for (int v=0; v<NV; v++) Update[v] = Out[v];
+
+      if ( !c.auto_reduce && c.last_resort_floor )
+      {
+         Update[0] = vmax( Update[0], (T)c.min_dens );
+         Update[4] = engy_with_eint_floor( Update[0], Update[1], Update[2], Update[3], Update[4], (T)c.min_eint );
+      }
+      if ( close_unphysical( g, Update, c.auto_reduce != 0, c ) ) { cnt[1]++;  continue; }
+
+      for (int v=0; v<NV; v++) out[ v*PS*PS*PS + io ] = Update[v];
+      if ( c.fixup_flux && c.mode != 0 && flux != nullptr )
+      {
+         const int dim_map[3][2] = { { 1, 2 }, { 0, 2 }, { 0, 1 } };
+         for (int d=0; d<3; d++)
+         {
+            const T *F = nullptr;
+            int face = -1;
+            if      ( ijk[d] == 0    || ijk[d] == PS/2 )   { F = FL[d];  face = 3*d + ijk[d]/( PS/2 ); }
+            else if ( ijk[d] == PS/2 - 1 || ijk[d] == PS - 1 ) { F = FR[d];  face = 3*d + ( ijk[d] + 1 )/( PS/2 ); }
+            if ( F == nullptr ) continue;
+            const int st = ijk[ dim_map[d][1] ]*PS + ijk[ dim_map[d][0] ];
+            for (int v=0; v<NV; v++) flux[ ( face*NV + v )*PS*PS + st ] = F[v];
+         }
+      }
+      cnt[0]++;
+   }
+}
+
+// StoreFlux() (src/Fluid/Flu_Close.cpp:234-307) into FP64 registers: reg[slot[pg][LocalID][s]][5][8][8] = dt*F
+template <typename T>
+void store_flux( const T *flux, int npg, const int *slot, double dt, double *reg )
+{
+   const int H = PS/2;
+   for (int p=0; p<npg; p++)
+   for (int lid=0; lid<8; lid++)
+   for (int s=0; s<6; s++)
+   {
+      const int r = slot[ ( p*8 + lid )*6 + s ];
+      if ( r < 0 ) continue;
+      const int *o = LOCAL_XYZ[lid];
+      int face, dm, dn;
+      if      ( s < 2 ) { face = o[0] + s;      dm = o[2]*H;  dn = o[1]*H; }
+      else if ( s < 4 ) { face = o[1] + 1 + s;  dm = o[2]*H;  dn = o[0]*H; }
+      else              { face = o[2] + 2 + s;  dm = o[1]*H;  dn = o[0]*H; }
+      for (int v=0; v<NV; v++)
+      for (int m=0; m<H; m++)
+      for (int n=0; n<H; n++)
+         reg[ ( ( (long)r*NV + v )*H + m )*H + n ] = dt*(double)flux[ ( ( (long)p*9 + face )*NV + v )*PS*PS + ( m + dm )*PS + n + dn ];
+   }
+}
+
+// CorrectFlux() (src/Fluid/Flu_Close.cpp:321-395): reg[slot[pg][s]] -= dt/4 * (fine fluxes on PG face s)
+template <typename T>
+void correct_flux( const T *flux, int npg, const int *slot, double dt, double *reg )
+{
+   const int H = PS/2;
+   const int map[6] = { 0, 2, 3, 5, 6, 8 };
+   const double dt4 = 0.25*dt;
+   for (int p=0; p<npg; p++)
+   for (int s=0; s<6; s++)
+   {
+      const int r = slot[ p*6 + s ];
+      if ( r < 0 ) continue;
+      for (int v=0; v<NV; v++)
+      for (int m=0; m<PS; m++)
+      for (int n=0; n<PS; n++)
+         reg[ ( ( (long)r*NV + v )*H + m/2 )*H + n/2 ] -= dt4*(double)flux[ ( ( (long)p*9 + map[s] )*NV + v )*PS*PS + m*PS + n ];
+   }
+}
+
+// Flu_FixUp_Flux() (src/Fluid/Flu_FixUp_Flux.cpp) on patches [npatch][5][8^3] with FP64 registers
+template <typename T>
+void fixup_flux( double gm1, T *patch, int npatch, const int *slot, const double *reg, double dh,
+                 double min_dens, double min_pres, double min_eint )
+{
+   const int H = PS/2;
+   const double cst[6] = { -1.0/dh, +1.0/dh, -1.0/dh, +1.0/dh, -1.0/dh, +1.0/dh };
+   for (int p=0; p<npatch; p++)
+   for (int s=0; s<6; s++)
+   {
+      const int r = slot[ p*6 + s ];
+      if ( r < 0 ) continue;
+      for (int m=0; m<H; m++)
+      for (int n=0; n<H; n++)
+      {
+         int i, j, k;
+         switch ( s )
+         {
+            case 0:  i = 0;      j = n;      k = m;      break;
+            case 1:  i = H-1;    j = n;      k = m;      break;
+            case 2:  i = n;      j = 0;      k = m;      break;
+            case 3:  i = n;      j = H-1;    k = m;      break;
+            case 4:  i = n;      j = m;      k = 0;      break;
+            default: i = n;      j = m;      k = H-1;    break;
+         }
+         T *u = patch + (long)p*NV*H*H*H + ( k*H + j )*H + i;
+         T cv[NV];
+         for (int v=0; v<NV; v++) cv[v] = (T)( (double)u[ v*H*H*H ] + reg[ ( ( (long)r*NV + v )*H + m )*H + n ]*cst[s] );
+         const T e = eint_of( cv[0], cv[1], cv[2], cv[3], cv[4] );
+         const T pr = (T)( (double)e*gm1 );
+         // MIN_DENS / MIN_EINT / MIN_PRES are doubles in the reference: the comparisons promote
+         if ( (double)cv[0] <= min_dens || (double)e <= min_eint || !std::isfinite( e ) ||
+              (double)pr <= min_pres || !std::isfinite( pr ) ) continue;
+         cv[4] = etot_of( cv[0], cv[1], cv[2], cv[3], e );
+         for (int v=0; v<NV; v++) u[ v*H*H*H ] = cv[v];
+      }
+   }
+}
+
+// Flu_FixUp_Restrict() (src/Fluid/Flu_FixUp_Restrict.cpp): father patch fa[fa_idx[q]] = average of the 8 son
+// patches son[son0[q] + LocalID], then the internal-energy floor on every father cell
+template <typename T>
+void fixup_restrict( const T *son, T *fa, int ngroup, const int *son0, const int *fa_idx, double min_eint )
+{
+   const int H = PS/2, H2 = H/2, P3 = H*H*H;
+   for (int q=0; q<ngroup; q++)
+   {
+      T *F = fa + (long)fa_idx[q]*NV*P3;
+      for (int lid=0; lid<8; lid++)
+      {
+         const T *S = son + (long)( son0[q] + lid )*NV*P3;
+         const int *o = LOCAL_XYZ[lid];
+         for (int v=0; v<NV; v++)
+         for (int k=0; k<H2; k++)
+         for (int j=0; j<H2; j++)
+         for (int i=0; i<H2; i++)
+         {
+            const T *s0 = S + v*P3;
+            const int I = 2*i, J = 2*j, K = 2*k;
+            auto at = [&]( int kk, int jj, int ii ) { return s0[ ( kk*H + jj )*H + ii ]; };
+            F[ v*P3 + ( ( k + o[2]*H2 )*H + j + o[1]*H2 )*H + i + o[0]*H2 ] =
+               (T)( 0.125*( at( K, J, I ) + at( K, J, I+1 ) + at( K, J+1, I ) + at( K+1, J, I ) +
+                            at( K, J+1, I+1 ) + at( K+1, J+1, I ) + at( K+1, J, I+1 ) + at( K+1, J+1, I+1 ) ) );
+         }
+      }
+      for (int c=0; c<P3; c++)
+         F[ 4*P3 + c ] = engy_with_eint_floor( F[c], F[P3 + c], F[2*P3 + c], F[3*P3 + c], F[4*P3 + c], (T)min_eint );
+   }
+}
+
+// copy of patch-group outputs into their 8 patches (src/Fluid/Flu_Close.cpp:133-160)
+template <typename T>
+void close_patches( const T *pg_out, T *patch, int npg, const int *pid0 )
+{
+   const int H = PS/2, P3 = H*H*H;
+   for (int p=0; p<npg; p++)
+   for (int lid=0; lid<8; lid++)
+   {
+      T *D = patch + (long)( pid0[p] + lid )*NV*P3;
+      const int *o = LOCAL_XYZ[lid];
+      for (int v=0; v<NV; v++)
+      for (int k=0; k<H; k++)
+      for (int j=0; j<H; j++)
+      for (int i=0; i<H; i++)
+         D[ v*P3 + ( k*H + j )*H + i ] = pg_out[ ( (long)p*NV + v )*PS*PS*PS + ( ( o[2]*H + k )*PS + o[1]*H + j )*PS + o[0]*H + i ];
+   }
+}
+
 } // namespace
 
 extern "C" {
@@ -686,5 +950,39 @@ int oracle_dt_cfl_f64( const double *in, double *dt, int npatch, double dh, doub
    }
    return 0;
 }
+
+#define ORC_SYNC_API( SUF, T )                                                                                       \
+int oracle_correct_unphysical_##SUF( const T *in, T *out, T *flux, int npg, int scheme, double dt, double dh,          \
+                                     double gamma, int mode, int rsolver, int last_resort_floor, int auto_reduce,      \
+                                     int fixup_flux, int check_pres, double min_dens, double min_pres,               \
+                                     double min_eint, long counters[2] )                                               \
+{                                                                                                                      \
+   if ( scheme != ORC_MHM && scheme != ORC_MHM_RP ) return 2;                                                          \
+   if ( mode < 0 || mode > 2 ) return 3;                                                                               \
+   const int NX = ( scheme == ORC_MHM_RP ) ? 22 : 20, ghost = ( NX - PS )/2;                                           \
+   Gas<T> g( gamma );                                                                                                  \
+   CorrOpt c = { mode, rsolver, last_resort_floor, auto_reduce, fixup_flux, check_pres, min_dens, min_pres, min_eint, \
+                 gamma - 1.0 };                                                                                        \
+   const T dt_dh = (T)dt/(T)dh;                                                                                        \
+   counters[0] = counters[1] = 0;                                                                                      \
+   for (int p=0; p<npg; p++)                                                                                           \
+      correct_unphysical_pg( g, c, in + (long)p*NV*NX*NX*NX, out + (long)p*NV*PS*PS*PS,                              \
+                             flux ? flux + (long)p*9*NV*PS*PS : nullptr, NX, ghost, dt_dh, counters );               \
+   return 0;                                                                                                           \
+}                                                                                                                      \
+int oracle_store_flux_##SUF( const T *flux, int npg, const int *slot, double dt, double *reg )                        \
+{ store_flux( flux, npg, slot, dt, reg );  return 0; }                                                                 \
+int oracle_correct_flux_##SUF( const T *flux, int npg, const int *slot, double dt, double *reg )                      \
+{ correct_flux( flux, npg, slot, dt, reg );  return 0; }                                                               \
+int oracle_fixup_flux_##SUF( T *patch, int npatch, const int *slot, const double *reg, double dh, double gamma,      \
+                             double min_dens, double min_pres, double min_eint )                                      \
+{ fixup_flux( gamma - 1.0, patch, npatch, slot, reg, dh, min_dens, min_pres, min_eint );  return 0; }                \
+int oracle_fixup_restrict_##SUF( const T *son, T *fa, int ngroup, const int *son0, const int *fa_idx, double min_eint )\
+{ fixup_restrict( son, fa, ngroup, son0, fa_idx, min_eint );  return 0; }                                              \
+int oracle_close_patches_##SUF( const T *pg_out, T *patch, int npg, const int *pid0 )                                 \
+{ close_patches( pg_out, patch, npg, pid0 );  return 0; }
+
+ORC_SYNC_API( f64, double )
+ORC_SYNC_API( f32, float )
 
 } // extern "C"

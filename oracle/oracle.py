@@ -42,6 +42,14 @@ def lib():
         L.oracle_riemann_f64.argtypes = [_I, _I, _P, _P, _D, _D, _P]
         L.oracle_limit_slope_f64.argtypes = [_P, _P, _P, _I, _D, _P]
         L.oracle_dt_cfl_f64.argtypes = [_P, _P, _I, _D, _D, _D, _D, _I]
+        for suf in ("f64", "f32"):
+            getattr(L, "oracle_correct_unphysical_" + suf).argtypes = [_P, _P, _P, _I, _I, _D, _D, _D, _I, _I, _I, _I,
+                                                                         _I, _I, _D, _D, _D, _P]
+            getattr(L, "oracle_store_flux_" + suf).argtypes = [_P, _I, _P, _D, _P]
+            getattr(L, "oracle_correct_flux_" + suf).argtypes = [_P, _I, _P, _D, _P]
+            getattr(L, "oracle_fixup_flux_" + suf).argtypes = [_P, _I, _P, _P, _D, _D, _D, _D, _D]
+            getattr(L, "oracle_fixup_restrict_" + suf).argtypes = [_P, _P, _I, _P, _P, _D]
+            getattr(L, "oracle_close_patches_" + suf).argtypes = [_P, _P, _I, _P]
         _lib = L
     return _lib
 
@@ -136,3 +144,70 @@ def dt_cfl(u_patch, dh, safety, gamma=5.0 / 3.0, min_pres=0.0, nthreads=0):
     if rc != 0:
         raise ValueError(rc)
     return dt
+
+
+# ---------------- Flu_Close and the AMR level synchronisation ----------------
+_FFC = {"NONE": 0, "3D": 1, "3D1D": 2}
+
+
+def _suffix(a):
+    return "f64" if a.dtype == np.float64 else "f32"
+
+
+def correct_unphysical(u_in, out, flux, dt, dh, mode="3D1D", rsolver="HLLE", scheme="MHM_RP", gamma=5.0 / 3.0,
+                       last_resort_floor=True, auto_reduce_dt=False, fixup_flux=True, check_pres=False,
+                       min_dens=0.0, min_pres=0.0, min_eint=0.0):
+    """Restatement of CorrectUnphysical; returns (out, flux, counters) as new arrays."""
+    u_in = np.ascontiguousarray(u_in)
+    out = np.array(out, copy=True)
+    flux = None if flux is None else np.array(flux, copy=True)
+    cnt = np.zeros(2, dtype=np.int64)
+    fn = getattr(lib(), "oracle_correct_unphysical_" + _suffix(out))
+    rc = fn(_ptr(u_in), _ptr(out), _ptr(flux) if flux is not None else None, out.shape[0], SCHEME[scheme],
+            ctypes.c_double(dt), ctypes.c_double(dh), ctypes.c_double(gamma), _FFC[mode], RSOLVER[rsolver],
+            int(last_resort_floor), int(auto_reduce_dt), int(fixup_flux), int(check_pres),
+            ctypes.c_double(min_dens), ctypes.c_double(min_pres), ctypes.c_double(min_eint), _ptr(cnt))
+    if rc != 0:
+        raise ValueError(f"oracle_correct_unphysical returned {rc}")
+    return out, flux, cnt
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def store_flux(flux, slot, dt, reg):
+    flux, slot, reg = np.ascontiguousarray(flux), _i32(slot), np.array(reg, dtype=np.float64, copy=True)
+    getattr(lib(), "oracle_store_flux_" + _suffix(flux))(_ptr(flux), flux.shape[0], _ptr(slot), ctypes.c_double(dt),
+                                                         _ptr(reg))
+    return reg
+
+
+def correct_flux(flux, slot, dt, reg):
+    flux, slot, reg = np.ascontiguousarray(flux), _i32(slot), np.array(reg, dtype=np.float64, copy=True)
+    getattr(lib(), "oracle_correct_flux_" + _suffix(flux))(_ptr(flux), flux.shape[0], _ptr(slot),
+                                                           ctypes.c_double(dt), _ptr(reg))
+    return reg
+
+
+def fixup_flux(patch, slot, reg, dh, gamma=5.0 / 3.0, min_dens=0.0, min_pres=0.0, min_eint=0.0):
+    patch, slot, reg = np.array(patch, copy=True), _i32(slot), np.ascontiguousarray(reg, dtype=np.float64)
+    getattr(lib(), "oracle_fixup_flux_" + _suffix(patch))(_ptr(patch), patch.shape[0], _ptr(slot), _ptr(reg),
+                                                          ctypes.c_double(dh), ctypes.c_double(gamma),
+                                                          ctypes.c_double(min_dens), ctypes.c_double(min_pres),
+                                                          ctypes.c_double(min_eint))
+    return patch
+
+
+def fixup_restrict(son, fa, son0, fa_idx, min_eint=0.0):
+    son, fa = np.ascontiguousarray(son), np.array(fa, copy=True)
+    son0, fa_idx = _i32(son0), _i32(fa_idx)
+    getattr(lib(), "oracle_fixup_restrict_" + _suffix(fa))(_ptr(son), _ptr(fa), son0.shape[0], _ptr(son0),
+                                                           _ptr(fa_idx), ctypes.c_double(min_eint))
+    return fa
+
+
+def close_patches(pg_out, patch, pid0):
+    pg_out, patch, pid0 = np.ascontiguousarray(pg_out), np.array(patch, copy=True), _i32(pid0)
+    getattr(lib(), "oracle_close_patches_" + _suffix(patch))(_ptr(pg_out), _ptr(patch), pg_out.shape[0], _ptr(pid0))
+    return patch
