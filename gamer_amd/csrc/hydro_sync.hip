@@ -190,6 +190,7 @@ template <typename IO>
 __global__ void store_flux_kernel( const IO *__restrict__ flux, const int *__restrict__ slot, int npg, double dt,
                                    double *__restrict__ reg )
 {
+   #pragma clang fp contract(off)      // separate multiply and add, as the reference accumulates
    const long total = (long)npg*8*6*NV*H*H;
    for (long e = blockIdx.x*(long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x*blockDim.x)
    {
@@ -213,6 +214,7 @@ template <typename IO>
 __global__ void correct_flux_kernel( const IO *__restrict__ flux, const int *__restrict__ slot, int npg, double dt,
                                      double *__restrict__ reg )
 {
+   #pragma clang fp contract(off)      // separate multiply and add, as the reference accumulates
    const int map[6] = { 0, 2, 3, 5, 6, 8 };
    const double dt4 = 0.25*dt;
    const long total = (long)npg*6*NV*H*H;
@@ -238,6 +240,7 @@ template <typename IO>
 __global__ void fixup_flux_kernel( IO *__restrict__ patch, const int *__restrict__ slot, const double *__restrict__ reg,
                                    int npatch, double dh, double gm1, double min_dens, double min_pres, double min_eint )
 {
+   #pragma clang fp contract(off)      // separate multiply and add, as the reference accumulates
    const long total = (long)npatch*P3;
    for (long e = blockIdx.x*(long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x*blockDim.x)
    {
@@ -273,6 +276,7 @@ template <typename IO>
 __global__ void fixup_restrict_kernel( const IO *__restrict__ son, IO *__restrict__ fa, int ngroup,
                                        const int *__restrict__ son0, const int *__restrict__ fa_idx, double min_eint )
 {
+   #pragma clang fp contract(off)      // separate multiply and add, as the reference accumulates
    const int H2 = H/2;
    const long total = (long)ngroup*P3;
    for (long e = blockIdx.x*(long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x*blockDim.x)

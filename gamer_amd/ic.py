@@ -104,7 +104,8 @@ def cut_patch_groups(cons: np.ndarray, ghost: int) -> np.ndarray:
     n = PS2 + 2 * ghost
     win = np.lib.stride_tricks.sliding_window_view(pad, (n, n, n), axis=(1, 2, 3))[:, ::PS2, ::PS2, ::PS2]
     gz, gy, gx = nz // PS2, ny // PS2, nx // PS2
-    return np.ascontiguousarray(win.transpose(1, 2, 3, 0, 4, 5, 6)).reshape(gz * gy * gx, NCOMP, n, n, n)
+    # always a fresh writable array (for a single patch group the strided window view is already contiguous)
+    return win.transpose(1, 2, 3, 0, 4, 5, 6).copy(order="C").reshape(gz * gy * gx, NCOMP, n, n, n)
 
 
 def random_sod(npg_shape=(4, 4, 4), seed=1234, scheme_ghost=3, sedov=False, cfl=0.5, gamma=GAMMA,

@@ -880,6 +880,203 @@ void close_patches( const T *pg_out, T *patch, int npg, const int *pid0 )
    }
 }
 
+// ---------------- coarse -> fine interpolation (src/Interpolation) ----------------
+// Int_CQuadratic slope of one split pass (Int_CQuadratic.cpp:53-78): (R-L)/8, optionally limited by the one-sided
+// slopes (C-L)/4, (R-C)/4 with the smaller one scaled by MonoCoeff; zero across a sign change of L,R if requested
+template <typename T> T cquad_slope( T L, T C, T R, bool mono, T coeff, bool opp0 )
+{
+   T s = (T)0.125*( R - L );
+   if ( mono )
+   {
+      T l = (T)0.25*( C - L ), r = (T)0.25*( R - C );
+      if ( l*r > (T)0.0 )
+      {
+         const T sg = sgn( l );
+         s *= sg;  l *= sg;  r *= sg;
+         if ( l < r ) l *= coeff; else r *= coeff;
+         s = vmin( l, s );
+         s = vmin( r, s );
+         s *= sg;
+      }
+      else s = (T)0.0;
+   }
+   if ( opp0 && L*R < (T)0.0 ) s = (T)0.0;
+   return s;
+}
+
+struct Box { int cs[3], c0[3], cr[3], fs[3], f0[3]; };
+
+// Int_CQuadratic (src/Interpolation/Int_CQuadratic.cpp): x pass over the coarse rows (with one ghost row in y and
+// z), then y, then z, each producing two fine values C -/+ slope
+template <typename T>
+void int_cquad( const T *C, T *F, const Box &b, int mono_mask, T coeff, bool opp0 )
+{
+   const int cdy = b.cs[0], cdz = b.cs[0]*b.cs[1], C3 = cdz*b.cs[2], F3 = b.fs[0]*b.fs[1]*b.fs[2];
+   const int nx = 2*b.cr[0], ny = b.cr[1] + 2, nz = b.cr[2] + 2;
+   std::vector<T> X( (size_t)nx*ny*nz ), Y( (size_t)nx*2*b.cr[1]*nz );
+   for (int v=0; v<NV; v++)
+   {
+      const T *c = C + (size_t)v*C3;
+      T *f = F + (size_t)v*F3;
+      const bool mono = ( mono_mask >> v ) & 1;
+      for (int z=0; z<nz; z++) for (int y=0; y<ny; y++) for (int x=0; x<b.cr[0]; x++)
+      {
+         const int i = ( b.c0[2] - 1 + z )*cdz + ( b.c0[1] - 1 + y )*cdy + b.c0[0] + x;
+         const T sl = cquad_slope( c[i-1], c[i], c[i+1], mono, coeff, opp0 );
+         X[ ( (size_t)z*ny + y )*nx + 2*x     ] = c[i] - sl;
+         X[ ( (size_t)z*ny + y )*nx + 2*x + 1 ] = c[i] + sl;
+      }
+      for (int z=0; z<nz; z++) for (int y=0; y<b.cr[1]; y++) for (int x=0; x<nx; x++)
+      {
+         const T L = X[ ( (size_t)z*ny + y     )*nx + x ], Cc = X[ ( (size_t)z*ny + y + 1 )*nx + x ],
+                 R = X[ ( (size_t)z*ny + y + 2 )*nx + x ];
+         const T sl = cquad_slope( L, Cc, R, mono, coeff, opp0 );
+         Y[ ( (size_t)z*2*b.cr[1] + 2*y     )*nx + x ] = Cc - sl;
+         Y[ ( (size_t)z*2*b.cr[1] + 2*y + 1 )*nx + x ] = Cc + sl;
+      }
+      for (int z=0; z<b.cr[2]; z++) for (int y=0; y<2*b.cr[1]; y++) for (int x=0; x<nx; x++)
+      {
+         const T L = Y[ ( (size_t)z*2*b.cr[1] + y )*nx + x ], Cc = Y[ ( (size_t)( z + 1 )*2*b.cr[1] + y )*nx + x ],
+                 R = Y[ ( (size_t)( z + 2 )*2*b.cr[1] + y )*nx + x ];
+         const T sl = cquad_slope( L, Cc, R, mono, coeff, opp0 );
+         const int o = ( ( b.f0[2] + 2*z )*b.fs[1] + b.f0[1] + y )*b.fs[0] + b.f0[0] + x;
+         f[o]                       = Cc - sl;
+         f[o + b.fs[0]*b.fs[1]]     = Cc + sl;
+      }
+   }
+}
+
+// Int_MinMod1D (src/Interpolation/Int_MinMod1D.cpp): per-direction minmod slope / 4, eight fine values
+template <typename T>
+void int_minmod1d( const T *C, T *F, const Box &b, bool opp0 )
+{
+   const int cd[3] = { 1, b.cs[0], b.cs[0]*b.cs[1] }, C3 = cd[2]*b.cs[2];
+   const int fdy = b.fs[0], fdz = b.fs[0]*b.fs[1], F3 = fdz*b.fs[2];
+   for (int v=0; v<NV; v++)
+   for (int z=0; z<b.cr[2]; z++) for (int y=0; y<b.cr[1]; y++) for (int x=0; x<b.cr[0]; x++)
+   {
+      const T *c = C + (size_t)v*C3;
+      const int i = ( b.c0[2] + z )*cd[2] + ( b.c0[1] + y )*cd[1] + b.c0[0] + x;
+      T sl[3];
+      for (int d=0; d<3; d++)
+      {
+         const T l = c[i] - c[i - cd[d]], r = c[i + cd[d]] - c[i];
+         sl[d] = ( r*l <= (T)0.0 ) ? (T)0.0 : (T)0.25*( std::fabs( r ) < std::fabs( l ) ? r : l );
+         if ( opp0 && c[i - cd[d]]*c[i + cd[d]] < (T)0.0 ) sl[d] = (T)0.0;
+      }
+      T *f = F + (size_t)v*F3 + ( b.f0[2] + 2*z )*fdz + ( b.f0[1] + 2*y )*fdy + b.f0[0] + 2*x;
+      for (int k=0; k<2; k++) for (int j=0; j<2; j++) for (int ii=0; ii<2; ii++)
+         f[ k*fdz + j*fdy + ii ] = c[i] + ( k ? sl[2] : -sl[2] ) + ( j ? sl[1] : -sl[1] ) + ( ii ? sl[0] : -sl[0] );
+   }
+}
+
+// the PRIM-mode check of Interpolate_Iterate: Hydro_IsUnphysical(PRIM) then the internal energy from the EoS
+// (src/Model_Hydro/CPU_Hydro/CPU_Shared_FluUtility.cpp:969-1005, src/Interpolation/Interpolate.cpp:234-262)
+template <typename T> bool prim_unphysical( const T w[NV], double inv_gm1 )
+{
+   const T huge = Lim<T>::huge();
+   for (int v=0; v<NV; v++)
+   {
+      if ( w[v] != w[v] ) return true;
+      if ( v >= 1 && v <= 3 ) { if ( w[v] < -huge || w[v] > huge ) return true; }
+      else                    { if ( w[v] <= (T)0.0 || w[v] > huge ) return true; }
+   }
+   const T e = (T)( (double)w[4]*inv_gm1 );
+   return ( e < (T)0.0 || e > huge || e != e );
+}
+
+// Interpolate() with Interpolate_Iterate (src/Interpolation/Interpolate.cpp:90-330), INTERP_MASK semantics:
+// iteration 0 in conserved variables with INT_MONO_COEFF; iteration 1 (OPT__INT_PRIM) in primitive variables (the
+// coarse box is converted in place); later iterations lower the monotonic coefficient by INT_MONO_COEFF/MONO_MAX_ITER.
+// Fine cells accepted in an iteration keep their values unless their 2x2x2 block still holds a failed cell.
+// Returns 0 on success, 1 if cells are still unphysical after the last iteration.
+template <typename T>
+int interpolate( T *C, T *F, const Box &b, int scheme, int mono_mask, double mono_coeff, bool opp0, bool int_prim,
+                 bool reduce, int mono_max_iter, double gamma, double min_pres )
+{
+   const Gas<T> g( gamma );
+   const double inv_gm1 = 1.0/( gamma - 1.0 );
+   auto run = [&]( T *out, T coeff ) {
+      if ( scheme == 4 ) int_cquad( C, out, b, mono_mask, coeff, opp0 );
+      else               int_minmod1d( C, out, b, opp0 );
+   };
+   if ( !reduce && !int_prim ) { run( F, (T)mono_coeff );  return 0; }
+
+   const int C3 = b.cs[0]*b.cs[1]*b.cs[2], F3 = b.fs[0]*b.fs[1]*b.fs[2];
+   const int mono_max = reduce ? mono_max_iter : 0, max_iter = int_prim ? mono_max + 1 : mono_max;
+   std::vector<T> Ft( (size_t)NV*F3 );
+   std::vector<char> mask( F3, 0 );                 // 1 = accepted
+   T coeff = (T)mono_coeff;
+   bool prim = false;
+   for (int it=0; ; it++)
+   {
+      if ( it == 1 && int_prim )
+      {
+         for (int i=0; i<C3; i++)
+         {
+            T c[NV], w[NV];
+            for (int v=0; v<NV; v++) c[v] = C[ (size_t)v*C3 + i ];
+            con2pri( g, c, w, (T)min_pres );
+            for (int v=0; v<NV; v++) C[ (size_t)v*C3 + i ] = w[v];
+         }
+         prim = true;
+      }
+      else if ( it > 0 )
+      {
+         if ( scheme != 4 ) return 1;               // MINMOD1D has no monotonic coefficient to lower
+         coeff -= (T)mono_coeff/(T)mono_max;
+         coeff  = vmax( coeff, (T)0.0 );
+      }
+      run( Ft.data(), coeff );
+      bool any = false;
+      for (int i=0; i<F3; i++)
+      {
+         if ( mask[i] ) continue;
+         T t[NV];
+         for (int v=0; v<NV; v++) t[v] = Ft[ (size_t)v*F3 + i ];
+         const bool bad = prim ? prim_unphysical( t, inv_gm1 ) : unphysical( t );
+         if ( bad ) { any = true;  if ( it == max_iter ) return 1;  continue; }
+         if ( prim )
+         {
+            const T w4 = (T)( (double)t[4]*inv_gm1 );          // EoS_DensPres2Eint
+            T c[NV];
+            c[0] = t[0];  c[1] = t[0]*t[1];  c[2] = t[0]*t[2];  c[3] = t[0]*t[3];
+            c[4] = etot_of( c[0], c[1], c[2], c[3], w4 );
+            for (int v=0; v<NV; v++) t[v] = c[v];
+         }
+         for (int v=0; v<NV; v++) F[ (size_t)v*F3 + i ] = t[v];
+         mask[i] = 1;
+      }
+      if ( !( any && it < max_iter ) ) return 0;
+      for (int k=0; k<b.fs[2]; k+=2) for (int j=0; j<b.fs[1]; j+=2) for (int i=0; i<b.fs[0]; i+=2)
+      {
+         bool open = false;
+         for (int dk=0; dk<2; dk++) for (int dj=0; dj<2; dj++) for (int di=0; di<2; di++)
+            open |= !mask[ ( ( k + dk )*b.fs[1] + j + dj )*b.fs[0] + i + di ];
+         if ( open )
+            for (int dk=0; dk<2; dk++) for (int dj=0; dj<2; dj++) for (int di=0; di<2; di++)
+               mask[ ( ( k + dk )*b.fs[1] + j + dj )*b.fs[0] + i + di ] = 0;
+      }
+   }
+}
+
+// same-level part of the patch-group input preparation from per-patch data: window cell (x,y,z) of PG p copies
+// patch slot[p][sz][sy][sx] (-1 = not at this level: the cell keeps what the interpolation step wrote)
+template <typename T>
+void prepare_patches( const T *patch, T *pg_in, int npg, const int *slot, int ghost )
+{
+   const int H = PS/2, P3 = H*H*H, n = PS + 2*ghost, n3 = n*n*n;
+   for (int p=0; p<npg; p++)
+   for (int v=0; v<NV; v++)
+   for (int z=0; z<n; z++) for (int y=0; y<n; y++) for (int x=0; x<n; x++)
+   {
+      const int X = x - ghost + H, Y = y - ghost + H, Z = z - ghost + H;
+      const int s = slot[ ( ( p*4 + Z/H )*4 + Y/H )*4 + X/H ];
+      if ( s < 0 ) continue;
+      pg_in[ ( (size_t)p*NV + v )*n3 + ( z*n + y )*n + x ] = patch[ ( (size_t)s*NV + v )*P3 + ( ( Z % H )*H + Y % H )*H + X % H ];
+   }
+}
+
 } // namespace
 
 extern "C" {
@@ -981,6 +1178,27 @@ int oracle_fixup_restrict_##SUF( const T *son, T *fa, int ngroup, const int *son
 { fixup_restrict( son, fa, ngroup, son0, fa_idx, min_eint );  return 0; }                                              \
 int oracle_close_patches_##SUF( const T *pg_out, T *patch, int npg, const int *pid0 )                                 \
 { close_patches( pg_out, patch, npg, pid0 );  return 0; }
+
+#define ORC_INTERP_API( SUF, T )                                                                                     \
+int oracle_interpolate_##SUF( T *cdata, T *fdata, int nbox, const int csize[3], const int cstart[3],                 \
+                              const int crange[3], const int fsize[3], const int fstart[3], int scheme, int mono_mask,\
+                              double mono_coeff, int opp0, int int_prim, int reduce, int mono_max_iter, double gamma,  \
+                              double min_pres, int *nfail )                                                            \
+{                                                                                                                      \
+   Box b;                                                                                                              \
+   for (int d=0; d<3; d++) { b.cs[d] = csize[d]; b.c0[d] = cstart[d]; b.cr[d] = crange[d]; b.fs[d] = fsize[d]; b.f0[d] = fstart[d]; } \
+   const long C3 = (long)csize[0]*csize[1]*csize[2], F3 = (long)fsize[0]*fsize[1]*fsize[2];                          \
+   *nfail = 0;                                                                                                         \
+   for (int q=0; q<nbox; q++)                                                                                          \
+      *nfail += interpolate( cdata + q*NV*C3, fdata + q*NV*F3, b, scheme, mono_mask, mono_coeff, opp0 != 0,            \
+                             int_prim != 0, reduce != 0, mono_max_iter, gamma, min_pres );                            \
+   return 0;                                                                                                           \
+}                                                                                                                      \
+int oracle_prepare_patches_##SUF( const T *patch, T *pg_in, int npg, const int *slot, int ghost )                    \
+{ prepare_patches( patch, pg_in, npg, slot, ghost );  return 0; }
+
+ORC_INTERP_API( f64, double )
+ORC_INTERP_API( f32, float )
 
 ORC_SYNC_API( f64, double )
 ORC_SYNC_API( f32, float )
