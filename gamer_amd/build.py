@@ -10,7 +10,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgamer_amd_hydro.so")
 SOURCES = ["hydro_api.hip", "hydro_mhm_rp.hip", "hydro_mhm.hip", "hydro_grid.hip", "hydro_adjoint.hip",
-           "hydro_sync.hip"]
+           "hydro_sync.hip", "hydro_interp.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

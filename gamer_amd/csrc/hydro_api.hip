@@ -318,6 +318,51 @@ int hydro_close_patches_async( const void *pg_out, void *patch, int npg, const i
 }
 #undef GA_PREC_DISPATCH
 
+int hydro_interpolate_async( const HydroInterpArgs *a, void *stream )
+{
+   if ( a == nullptr ) return fail( HYDRO_E_ARG, "args == NULL" );
+   if ( a->nbox < 0 || ( a->nbox > 0 && ( a->cdata == nullptr || a->fdata == nullptr || a->nfail == nullptr ) ) )
+      return fail( HYDRO_E_ARG, "interpolate: bad pointers/size" );
+   if ( a->scheme != 2 && a->scheme != 4 ) return fail( HYDRO_E_ARG, "interpolate: scheme %d (INT_MINMOD1D = 2, INT_CQUAD = 4)", a->scheme );
+   if ( a->scheme == 2 && a->reduce_mono ) return fail( HYDRO_E_ARG, "interpolate: INT_MINMOD1D has no monotonic coefficient to reduce" );
+   if ( a->reduce_mono && a->mono_max_iter <= 0 ) return fail( HYDRO_E_ARG, "interpolate: mono_max_iter must be > 0" );
+   if ( !( a->gamma > 1.0 ) ) return fail( HYDRO_E_ARG, "interpolate: gamma must be > 1" );
+   long ncell = 1;
+   for (int d=0; d<3; d++)
+   {
+      if ( a->crange[d] <= 0 || a->cstart[d] < 1 || a->cstart[d] + a->crange[d] > a->csize[d] - 1 )
+         return fail( HYDRO_E_ARG, "interpolate: coarse range out of the box (1 ghost cell needed) in dim %d", d );
+      if ( a->fstart[d] < 0 || a->fstart[d] + 2*a->crange[d] > a->fsize[d] )
+         return fail( HYDRO_E_ARG, "interpolate: fine range out of the box in dim %d", d );
+      if ( ( a->int_prim || a->reduce_mono ) && a->fstart[d] != 0 )
+         return fail( HYDRO_E_ARG, "interpolate: the iterative interpolation needs fstart = 0" );
+      ncell *= a->crange[d];
+   }
+   if ( ncell > INTERP_MAX_CELLS ) return fail( HYDRO_E_ARG, "interpolate: %ld coarse cells > %d per box", ncell, INTERP_MAX_CELLS );
+   InterpPrm p;
+   for (int d=0; d<3; d++) { p.cs[d] = a->csize[d];  p.c0[d] = a->cstart[d];  p.cr[d] = a->crange[d];  p.fs[d] = a->fsize[d];  p.f0[d] = a->fstart[d]; }
+   p.scheme = a->scheme;  p.mono = a->monotonic;  p.opp0 = a->opp_sign_0th;  p.int_prim = a->int_prim;
+   p.reduce = a->reduce_mono;  p.mono_max_iter = a->mono_max_iter;  p.mono_coeff = a->mono_coeff;
+   p.gm1 = a->gamma - 1.0;  p.inv_gm1 = 1.0/( a->gamma - 1.0 );  p.min_pres = a->min_pres;
+   int rc;
+   if      ( a->precision == HYDRO_FP64 )      rc = launch_interpolate<double>( (double*)a->cdata, (double*)a->fdata, a->nbox, p, a->nfail, (hipStream_t)stream );
+   else if ( a->precision == HYDRO_FP32_BULK ) rc = launch_interpolate<float> ( (float*)a->cdata,  (float*)a->fdata,  a->nbox, p, a->nfail, (hipStream_t)stream );
+   else return fail( HYDRO_E_ARG, "unsupported precision %d", a->precision );
+   return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "interpolate launch failed" );
+}
+
+int hydro_prepare_patches_async( const void *patch, void *pg_in, int npg, const int *slot, int ghost, int precision,
+                                 void *stream )
+{
+   if ( npg < 0 || ( npg > 0 && ( patch == nullptr || pg_in == nullptr || slot == nullptr ) ) ) return fail( HYDRO_E_ARG, "prepare_patches: bad pointers/size" );
+   if ( ghost < 0 || ghost > 8 ) return fail( HYDRO_E_ARG, "prepare_patches: ghost %d out of range 0..8", ghost );
+   int rc;
+   if      ( precision == HYDRO_FP64 )      rc = launch_prepare_patches<double>( (const double*)patch, (double*)pg_in, npg, slot, ghost, (hipStream_t)stream );
+   else if ( precision == HYDRO_FP32_BULK ) rc = launch_prepare_patches<float> ( (const float*)patch,  (float*)pg_in,  npg, slot, ghost, (hipStream_t)stream );
+   else return fail( HYDRO_E_ARG, "unsupported precision %d", precision );
+   return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "prepare_patches launch failed" );
+}
+
 int hydro_mhm_solve_batch_async( const HydroSolveArgs *a, const double *pg_dt_dh, unsigned *queue_counter,
                                  int groups_per_cu, void *stream )
 {

@@ -1,0 +1,295 @@
+// Coarse -> fine interpolation of ghost-zone data, and the same-level gather of patch-group inputs from per-patch
+// data: the two halves of Prepare_PatchData for the fluid solver at a refinement boundary
+// (src/Main/Prepare_PatchData.cpp:166, src/Main/InterpolateGhostZone.cpp, src/Interpolation/Interpolate.cpp:90).
+//
+//   interpolate_kernel   Interpolate() with Interpolate_Iterate (Interpolate.cpp:90-330) for INT_CQUAD
+//                        (Int_CQuadratic.cpp) and INT_MINMOD1D (Int_MinMod1D.cpp): one workgroup per coarse box,
+//                        one lane per coarse cell producing its 2x2x2 fine cells.  For INT_CQUAD the lane evaluates
+//                        the x pass on the 3x3 coarse (y,z) neighbourhood, then the y pass, then the z pass: the same
+//                        operations in the same order as the reference's split temporaries.  Iterations (conserved
+//                        variables, then primitive variables, then lower monotonic coefficients) redo only coarse
+//                        cells whose fine cells failed the physicality test: the INTERP_MASK rule (include/Macro.h:947)
+//                        at the granularity of 2x2x2 fine blocks.
+//   prepare_patches_kernel  the same-level copies: window cell (x,y,z) of a patch group from the patch in its
+//                        4x4x4 slot table (-1: no patch at this level, the cell keeps the interpolated value)
+
+#include "hydro_math.hpp"
+#include "hydro_kernels.hpp"
+
+namespace gamer_amd {
+
+namespace {
+
+constexpr int PS = 16, H = 8, P3 = H*H*H;
+constexpr int NT = 256;
+
+template <typename T> __device__ __forceinline__ T sgn( T a ) { return ( a < (T)0 ) ? (T)-1 : (T)+1; }
+
+template <typename T>
+__device__ __forceinline__ T cquad_slope( T L, T C, T R, bool mono, T coeff, bool opp0 )
+{
+   #pragma clang fp contract(off)
+   T s = (T)0.125*( R - L );
+   if ( mono )
+   {
+      T l = (T)0.25*( C - L ), r = (T)0.25*( R - C );
+      if ( l*r > (T)0 )
+      {
+         const T sg = sgn( l );
+         s *= sg;  l *= sg;  r *= sg;
+         if ( l < r ) l *= coeff; else r *= coeff;
+         s = vmin( l, s );
+         s = vmin( r, s );
+         s *= sg;
+      }
+      else s = (T)0;
+   }
+   if ( opp0 && L*R < (T)0 ) s = (T)0;
+   return s;
+}
+
+// the 2x2x2 fine values of coarse cell i of one field: f[(k*2 + j)*2 + i]
+template <typename T, int SCHEME>
+__device__ __forceinline__ void interp_cell( const T *__restrict__ c, int i, int dy, int dz, bool mono, T coeff,
+                                             bool opp0, T f[8] )
+{
+   #pragma clang fp contract(off)
+   if ( SCHEME == 4 )
+   {
+      T X[3][3][2], Y[3][2][2];
+#     pragma unroll
+      for (int a=0; a<3; a++)
+#        pragma unroll
+         for (int b=0; b<3; b++)
+         {
+            const int q = i + ( a - 1 )*dz + ( b - 1 )*dy;
+            const T s = cquad_slope( c[q-1], c[q], c[q+1], mono, coeff, opp0 );
+            X[a][b][0] = c[q] - s;
+            X[a][b][1] = c[q] + s;
+         }
+#     pragma unroll
+      for (int a=0; a<3; a++)
+#        pragma unroll
+         for (int x=0; x<2; x++)
+         {
+            const T s = cquad_slope( X[a][0][x], X[a][1][x], X[a][2][x], mono, coeff, opp0 );
+            Y[a][0][x] = X[a][1][x] - s;
+            Y[a][1][x] = X[a][1][x] + s;
+         }
+#     pragma unroll
+      for (int y=0; y<2; y++)
+#        pragma unroll
+         for (int x=0; x<2; x++)
+         {
+            const T s = cquad_slope( Y[0][y][x], Y[1][y][x], Y[2][y][x], mono, coeff, opp0 );
+            f[( 0*2 + y )*2 + x] = Y[1][y][x] - s;
+            f[( 1*2 + y )*2 + x] = Y[1][y][x] + s;
+         }
+   }
+   else
+   {
+      const int cd[3] = { 1, dy, dz };
+      T sl[3];
+#     pragma unroll
+      for (int d=0; d<3; d++)
+      {
+         const T l = c[i] - c[i - cd[d]], r = c[i + cd[d]] - c[i];
+         sl[d] = ( r*l <= (T)0 ) ? (T)0 : (T)0.25*( vabs( r ) < vabs( l ) ? r : l );
+         if ( opp0 && c[i - cd[d]]*c[i + cd[d]] < (T)0 ) sl[d] = (T)0;
+      }
+#     pragma unroll
+      for (int k=0; k<2; k++)
+#        pragma unroll
+         for (int j=0; j<2; j++)
+#           pragma unroll
+            for (int x=0; x<2; x++)
+               f[( k*2 + j )*2 + x] = c[i] + ( k ? sl[2] : -sl[2] ) + ( j ? sl[1] : -sl[1] ) + ( x ? sl[0] : -sl[0] );
+   }
+}
+
+template <typename T> struct Huge;
+template <> struct Huge<double> { static constexpr double v = 1.7976931348623157e308; static constexpr double eps = 2.2204460492503131e-16; };
+template <> struct Huge<float>  { static constexpr float  v = 3.40282346638528859812e+38f; static constexpr float eps = 1.1920928955078125e-07f; };
+
+// Hydro_IsUnphysical in CONS mode (FluUtility.cpp:843-940)
+template <typename T>
+__device__ __forceinline__ bool cons_bad( const T u[NV] )
+{
+   #pragma clang fp contract(off)
+   const T huge = Huge<T>::v;
+   bool bad = false;
+#  pragma unroll
+   for (int v=0; v<NV; v++)
+   {
+      bad |= ( u[v] != u[v] );
+      if ( v >= 1 && v <= 3 ) bad |= ( u[v] < -huge || u[v] > huge );
+      else                    bad |= ( u[v] <= (T)0 || u[v] > huge );
+   }
+   const T e = u[4] - (T)0.5*( u[1]*u[1] + u[2]*u[2] + u[3]*u[3] )/u[0];
+   bad |= ( e < (T)-3.0*u[4]*Huge<T>::eps || e > huge || e != e );
+   return bad;
+}
+
+// Hydro_IsUnphysical in PRIM mode plus the interpolated internal energy from the EoS (Interpolate.cpp:234-262)
+template <typename T>
+__device__ __forceinline__ bool prim_bad( const T w[NV], double inv_gm1 )
+{
+   const T huge = Huge<T>::v;
+   bool bad = false;
+#  pragma unroll
+   for (int v=0; v<NV; v++)
+   {
+      bad |= ( w[v] != w[v] );
+      if ( v >= 1 && v <= 3 ) bad |= ( w[v] < -huge || w[v] > huge );
+      else                    bad |= ( w[v] <= (T)0 || w[v] > huge );
+   }
+   const T e = (T)( (double)w[4]*inv_gm1 );
+   return bad || e < (T)0 || e > huge || e != e;
+}
+
+template <typename T, int SCHEME>
+__global__ void __launch_bounds__( NT )
+interpolate_kernel( T *__restrict__ g_c, T *__restrict__ g_f, int nbox, InterpPrm p, int *__restrict__ nfail )
+{
+   #pragma clang fp contract(off)
+   __shared__ int s_fail;
+   __shared__ unsigned char s_done[INTERP_MAX_CELLS];
+   const int q = blockIdx.x, tid = threadIdx.x;
+   if ( q >= nbox ) return;
+   const int dy = p.cs[0], dz = p.cs[0]*p.cs[1], C3 = dz*p.cs[2];
+   const int fdy = p.fs[0], fdz = p.fs[0]*p.fs[1], F3 = fdz*p.fs[2];
+   const int ncell = p.cr[0]*p.cr[1]*p.cr[2];
+   T *C = g_c + (size_t)q*NV*C3;
+   T *F = g_f + (size_t)q*NV*F3;
+   const bool iterate = p.int_prim || p.reduce;
+   const int mono_max = p.reduce ? p.mono_max_iter : 0, max_iter = p.int_prim ? mono_max + 1 : mono_max;
+
+   for (int c = tid; c < ncell; c += NT) s_done[c] = 0;
+   T coeff = (T)p.mono_coeff;
+   bool prim = false;
+   for (int it = 0; ; it++)
+   {
+      if ( it == 1 && p.int_prim )
+      {
+         // Hydro_Con2Pri of the whole coarse box, in place (Interpolate.cpp:148-170)
+         for (int i = tid; i < C3; i += NT)
+         {
+            T u[NV];
+#           pragma unroll
+            for (int v=0; v<NV; v++) u[v] = C[ (size_t)v*C3 + i ];
+            const T r = (T)1/u[0];
+            const T e = u[4] - (T)0.5*( u[1]*u[1] + u[2]*u[2] + u[3]*u[3] )/u[0];
+            T pr = (T)( (double)e*p.gm1 );
+            pr = ( pr == pr ) ? vmax( pr, (T)p.min_pres ) : pr;
+            C[ (size_t)1*C3 + i ] = u[1]*r;
+            C[ (size_t)2*C3 + i ] = u[2]*r;
+            C[ (size_t)3*C3 + i ] = u[3]*r;
+            C[ (size_t)4*C3 + i ] = pr;
+         }
+         prim = true;
+         __syncthreads();
+      }
+      else if ( it > 0 )
+      {
+         coeff -= (T)p.mono_coeff/(T)mono_max;
+         coeff  = vmax( coeff, (T)0 );
+      }
+      if ( tid == 0 ) s_fail = 0;
+      __syncthreads();
+
+      for (int c = tid; c < ncell; c += NT)
+      {
+         if ( s_done[c] ) continue;
+         const int x = c % p.cr[0], y = ( c/p.cr[0] ) % p.cr[1], z = c/( p.cr[0]*p.cr[1] );
+         const int ic = ( p.c0[2] + z )*dz + ( p.c0[1] + y )*dy + p.c0[0] + x;
+         T f[NV][8];
+#        pragma unroll
+         for (int v=0; v<NV; v++) interp_cell<T, SCHEME>( C + (size_t)v*C3, ic, dy, dz, ( p.mono >> v ) & 1, coeff, p.opp0, f[v] );
+         bool bad = false;
+         if ( iterate )
+#           pragma unroll
+            for (int b=0; b<8; b++)
+            {
+               T t[NV];
+#              pragma unroll
+               for (int v=0; v<NV; v++) t[v] = f[v][b];
+               bad |= prim ? prim_bad( t, p.inv_gm1 ) : cons_bad( t );
+            }
+         if ( bad ) { s_fail = 1;  continue; }
+         const int fo = ( p.f0[2] + 2*z )*fdz + ( p.f0[1] + 2*y )*fdy + p.f0[0] + 2*x;
+#        pragma unroll
+         for (int b=0; b<8; b++)
+         {
+            T t[NV];
+#           pragma unroll
+            for (int v=0; v<NV; v++) t[v] = f[v][b];
+            if ( prim )
+            {
+               // Hydro_Pri2Con with EoS_DensPres2Eint
+               const T ei = (T)( (double)t[4]*p.inv_gm1 );
+               t[1] = t[0]*t[1];  t[2] = t[0]*t[2];  t[3] = t[0]*t[3];
+               t[4] = (T)0.5*( t[1]*t[1] + t[2]*t[2] + t[3]*t[3] )/t[0] + ei;
+            }
+            const int o = fo + ( b >> 2 )*fdz + ( ( b >> 1 ) & 1 )*fdy + ( b & 1 );
+#           pragma unroll
+            for (int v=0; v<NV; v++) F[ (size_t)v*F3 + o ] = t[v];
+         }
+         s_done[c] = 1;
+      }
+      __syncthreads();
+      const bool any = ( s_fail != 0 );
+      if ( !iterate || !any ) break;
+      if ( it == max_iter )
+      {
+         if ( tid == 0 ) atomicAdd( nfail, 1 );
+         break;
+      }
+      __syncthreads();
+   }
+}
+
+template <typename IO>
+__global__ void prepare_patches_kernel( const IO *__restrict__ patch, IO *__restrict__ pg_in, int npg,
+                                        const int *__restrict__ slot, int ghost )
+{
+   const int n = PS + 2*ghost, n3 = n*n*n;
+   const long total = (long)npg*NV*n3;
+   for (long e = blockIdx.x*(long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x*blockDim.x)
+   {
+      const int c = (int)( e % n3 ), v = (int)( ( e/n3 ) % NV );
+      const long p = e/( (long)NV*n3 );
+      const int X = c % n - ghost + H, Y = ( c/n ) % n - ghost + H, Z = c/( n*n ) - ghost + H;
+      const int s = slot[ ( ( p*4 + Z/H )*4 + Y/H )*4 + X/H ];
+      if ( s < 0 ) continue;
+      pg_in[e] = patch[ ( (long)s*NV + v )*P3 + ( ( Z % H )*H + Y % H )*H + X % H ];
+   }
+}
+
+} // namespace
+
+template <typename T>
+int launch_interpolate( T *cdata, T *fdata, int nbox, const InterpPrm &p, int *nfail, hipStream_t stream )
+{
+   if ( nbox <= 0 ) return 0;
+   if ( p.scheme == 4 ) hipLaunchKernelGGL( ( interpolate_kernel<T, 4> ), dim3( nbox ), dim3( NT ), 0, stream, cdata, fdata, nbox, p, nfail );
+   else                 hipLaunchKernelGGL( ( interpolate_kernel<T, 2> ), dim3( nbox ), dim3( NT ), 0, stream, cdata, fdata, nbox, p, nfail );
+   return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
+}
+
+template <typename IO>
+int launch_prepare_patches( const IO *patch, IO *pg_in, int npg, const int *slot, int ghost, hipStream_t stream )
+{
+   if ( npg <= 0 ) return 0;
+   const long total = (long)npg*NV*( PS + 2*ghost )*( PS + 2*ghost )*( PS + 2*ghost );
+   long b = ( total + 255 )/256;
+   if ( b > 65536 ) b = 65536;
+   hipLaunchKernelGGL( prepare_patches_kernel<IO>, dim3( (int)b ), dim3( 256 ), 0, stream, patch, pg_in, npg, slot, ghost );
+   return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
+}
+
+template int launch_interpolate<double>( double*, double*, int, const InterpPrm&, int*, hipStream_t );
+template int launch_interpolate<float> ( float*,  float*,  int, const InterpPrm&, int*, hipStream_t );
+template int launch_prepare_patches<double>( const double*, double*, int, const int*, int, hipStream_t );
+template int launch_prepare_patches<float> ( const float*,  float*,  int, const int*, int, hipStream_t );
+
+} // namespace gamer_amd

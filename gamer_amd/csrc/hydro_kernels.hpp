@@ -56,6 +56,21 @@ int launch_fixup_restrict( const IO *son, IO *fa, int ngroup, const int *son0, c
 template <typename IO>
 int launch_close_patches( const IO *pg_out, IO *patch, int npg, const int *pid0, hipStream_t stream );
 
+// coarse -> fine interpolation of one batch of equally shaped boxes (src/Interpolation/Interpolate.cpp:90-330)
+constexpr int INTERP_MAX_CELLS = 4096;      // coarse cells per box (CRange product)
+struct InterpPrm {
+   int    cs[3], c0[3], cr[3], fs[3], f0[3];   // CSize, CStart, CRange, FSize, FStart
+   int    scheme;                             // 2 = INT_MINMOD1D, 4 = INT_CQUAD
+   int    mono;                               // bit v: Monotonic[v]
+   int    opp0, int_prim, reduce, mono_max_iter;
+   double mono_coeff, gm1, inv_gm1, min_pres;
+};
+template <typename T>
+int launch_interpolate( T *cdata, T *fdata, int nbox, const InterpPrm &p, int *nfail, hipStream_t stream );
+// same-level part of the patch-group input preparation: slot[npg][4][4][4] of patch indices (-1 = keep)
+template <typename IO>
+int launch_prepare_patches( const IO *patch, IO *pg_in, int npg, const int *slot, int ghost, hipStream_t stream );
+
 // MHM_RP step, FP64: adjoint lam_in = J^T lam_out, and tangent-linear tout = J tin (+ out = U_out; tin == NULL:
 // plain phase-wise forward).  ws: adjoint_workspace_bytes(npg) of device memory.
 size_t adjoint_workspace_bytes( int npg );

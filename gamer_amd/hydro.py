@@ -34,7 +34,8 @@ EXPORTS = ("hydro_init", "hydro_mhm_solve_async", "hydro_synchronize", "hydro_la
            "hydro_mhm_solve_batch_async", "hydro_mhm_adjoint_workspace_bytes", "hydro_mhm_adjoint_async",
            "hydro_mhm_tangent_async", "hydro_prepare_uniform_adjoint_async", "hydro_correct_unphysical_async",
            "hydro_store_flux_async", "hydro_correct_flux_async", "hydro_fixup_flux_async",
-           "hydro_fixup_restrict_async", "hydro_close_patches_async")
+           "hydro_fixup_restrict_async", "hydro_close_patches_async", "hydro_interpolate_async",
+           "hydro_prepare_patches_async")
 
 
 class HydroError(RuntimeError):
@@ -65,6 +66,19 @@ class HydroCorrectArgs(ctypes.Structure):
 
 
 FIRST_FLUX_CORR = {"NONE": 0, "3D": 1, "3D1D": 2}
+INT_SCHEME = {"MINMOD1D": 2, "CQUAD": 4}
+
+
+class HydroInterpArgs(ctypes.Structure):
+    _fields_ = [
+        ("cdata", ctypes.c_void_p), ("fdata", ctypes.c_void_p), ("nbox", ctypes.c_int),
+        ("csize", ctypes.c_int * 3), ("cstart", ctypes.c_int * 3), ("crange", ctypes.c_int * 3),
+        ("fsize", ctypes.c_int * 3), ("fstart", ctypes.c_int * 3), ("scheme", ctypes.c_int),
+        ("monotonic", ctypes.c_int), ("mono_coeff", ctypes.c_double), ("opp_sign_0th", ctypes.c_int),
+        ("int_prim", ctypes.c_int), ("reduce_mono", ctypes.c_int), ("mono_max_iter", ctypes.c_int),
+        ("gamma", ctypes.c_double), ("min_pres", ctypes.c_double), ("precision", ctypes.c_int),
+        ("nfail", ctypes.c_void_p),
+    ]
 
 _lib = None
 
@@ -104,6 +118,9 @@ def lib():
                                                  ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
         L.hydro_close_patches_async.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                                 ctypes.c_int, ctypes.c_void_p]
+        L.hydro_interpolate_async.argtypes = [ctypes.POINTER(HydroInterpArgs), ctypes.c_void_p]
+        L.hydro_prepare_patches_async.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                  ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -383,6 +400,36 @@ def close_patches(pg_out, patch, pid0, stream=None):
     _check(lib().hydro_close_patches_async(pg_out.contiguous().data_ptr(), patch.data_ptr(), pg_out.shape[0],
                                            pid0.data_ptr(), _prec(patch), _stream(patch.device, stream)))
     return patch
+
+
+def interpolate(cdata, fdata, cstart, crange, fstart=(0, 0, 0), scheme="CQUAD", monotonic=0x1F, mono_coeff=2.0,
+                opp_sign_0th=False, int_prim=True, reduce_mono=True, mono_max_iter=10, gamma=5.0 / 3.0, min_pres=0.0,
+                stream=None):
+    """Coarse -> fine interpolation of boxes cdata (nbox, 5, cz, cy, cx) into fdata (nbox, 5, fz, fy, fx), in place.
+    Index tuples are (x, y, z).  Returns an int32 device tensor [number of boxes still unphysical]."""
+    import torch
+    nbox = cdata.shape[0]
+    if cdata.dim() != 5 or fdata.dim() != 5 or fdata.shape[0] != nbox or cdata.shape[1] != 5 or fdata.shape[1] != 5:
+        raise HydroError("interpolate: cdata/fdata must be (nbox, 5, z, y, x)")
+    if cdata.dtype != fdata.dtype or not cdata.is_contiguous() or not fdata.is_contiguous():
+        raise HydroError("interpolate: contiguous tensors of one dtype required")
+    nf = torch.zeros(1, dtype=torch.int32, device=fdata.device)
+    I3 = ctypes.c_int * 3
+    a = HydroInterpArgs(cdata.data_ptr(), fdata.data_ptr(), nbox, I3(*reversed(cdata.shape[2:])), I3(*cstart),
+                        I3(*crange), I3(*reversed(fdata.shape[2:])), I3(*fstart), INT_SCHEME[scheme], int(monotonic),
+                        float(mono_coeff), int(opp_sign_0th), int(int_prim), int(reduce_mono), int(mono_max_iter),
+                        float(gamma), float(min_pres), _prec(fdata), nf.data_ptr())
+    _check(lib().hydro_interpolate_async(ctypes.byref(a), _stream(fdata.device, stream)))
+    return nf
+
+
+def prepare_patches(patch, pg_in, slot, ghost, stream=None):
+    """Same-level patch-group inputs from patches (npatch, 5, 8, 8, 8); slot (npg, 4, 4, 4) int32, -1 = keep."""
+    _int_table(slot, (pg_in.shape[0], 4, 4, 4), "slot")
+    _check(lib().hydro_prepare_patches_async(patch.contiguous().data_ptr(), pg_in.data_ptr(), pg_in.shape[0],
+                                             slot.data_ptr(), int(ghost), _prec(pg_in),
+                                             _stream(pg_in.device, stream)))
+    return pg_in
 
 
 class UniformLevel:

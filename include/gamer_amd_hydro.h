@@ -146,6 +146,39 @@ int hydro_fixup_restrict_async( const void *son, void *fa, int ngroup, const int
 int hydro_close_patches_async( const void *pg_out, void *patch, int npg, const int *pid0, int precision,
                                void *stream );
 
+/* ---- Ghost-zone preparation at refinement boundaries ---- */
+
+/* Coarse -> fine interpolation of a batch of equally shaped boxes: Interpolate() with Interpolate_Iterate
+   (src/Interpolation/Interpolate.cpp:90-330) for INT_MINMOD1D (Int_MinMod1D.cpp) and INT_CQUAD
+   (Int_CQuadratic.cpp).  Boxes are cdata[nbox][5][csize[2]][csize[1]][csize[0]] and fdata[nbox][5][fsize...];
+   coarse cells cstart..cstart+crange-1 (one coarse ghost cell needed on each side) fill fine cells
+   fstart..fstart+2*crange-1.  With int_prim (OPT__INT_PRIM) or reduce_mono the result is checked and failing
+   2x2x2 blocks are redone in primitive variables, then with lower monotonic coefficients (MONO_MAX_ITER); the
+   primitive retry converts cdata in place, as the reference does.  *nfail += number of boxes still unphysical. */
+typedef struct HydroInterpArgs {
+   void  *cdata, *fdata;
+   int    nbox;
+   int    csize[3], cstart[3], crange[3], fsize[3], fstart[3];
+   int    scheme;              /* 2 = INT_MINMOD1D, 4 = INT_CQUAD (include/Typedef.h:173-179) */
+   int    monotonic;           /* bit v set: Monotonic[v] */
+   double mono_coeff;          /* INT_MONO_COEFF */
+   int    opp_sign_0th;        /* OppSign0thOrder */
+   int    int_prim;            /* OPT__INT_PRIM */
+   int    reduce_mono;         /* ReduceMonoCoeff */
+   int    mono_max_iter;       /* MONO_MAX_ITER */
+   double gamma, min_pres;
+   int    precision;
+   int   *nfail;               /* device int */
+} HydroInterpArgs;
+int hydro_interpolate_async( const HydroInterpArgs *args, void *stream );
+
+/* Same-level part of Prepare_PatchData for the fluid solver (src/Main/Prepare_PatchData.cpp:166): cell (x,y,z) of
+   patch-group input pg_in[npg][5][N^3] (N = 16 + 2*ghost) is copied from patch slot[pg][sz][sy][sx] (4x4x4 table
+   of patch indices covering the window, patch offset -1..2 per axis; -1 = no patch at this level: the cell keeps
+   its value, e.g. one written by hydro_interpolate_async). */
+int hydro_prepare_patches_async( const void *patch, void *pg_in, int npg, const int *slot, int ghost, int precision,
+                                 void *stream );
+
 /* ---- Differentiation of one MHM_RP step (HYDRO_FP64 only) ----
    The step is the map in[npg][5][22^3] -> out[npg][5][16^3] of hydro_mhm_solve_async with the same arguments
    (dt, dh constant; the MinMod retry is not differentiated).  No counterpart in the reference, which has no

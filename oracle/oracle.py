@@ -50,6 +50,9 @@ def lib():
             getattr(L, "oracle_fixup_flux_" + suf).argtypes = [_P, _I, _P, _P, _D, _D, _D, _D, _D]
             getattr(L, "oracle_fixup_restrict_" + suf).argtypes = [_P, _P, _I, _P, _P, _D]
             getattr(L, "oracle_close_patches_" + suf).argtypes = [_P, _P, _I, _P]
+            getattr(L, "oracle_interpolate_" + suf).argtypes = [_P, _P, _I, _P, _P, _P, _P, _P, _I, _I, _D, _I, _I,
+                                                                  _I, _I, _D, _D, _P]
+            getattr(L, "oracle_prepare_patches_" + suf).argtypes = [_P, _P, _I, _P, _I]
         _lib = L
     return _lib
 
@@ -211,3 +214,27 @@ def close_patches(pg_out, patch, pid0):
     pg_out, patch, pid0 = np.ascontiguousarray(pg_out), np.array(patch, copy=True), _i32(pid0)
     getattr(lib(), "oracle_close_patches_" + _suffix(patch))(_ptr(pg_out), _ptr(patch), pg_out.shape[0], _ptr(pid0))
     return patch
+
+
+_INT = {"MINMOD1D": 2, "CQUAD": 4}
+
+
+def interpolate(cdata, fdata, cstart, crange, fstart=(0, 0, 0), scheme="CQUAD", monotonic=0x1F, mono_coeff=2.0,
+                opp_sign_0th=False, int_prim=True, reduce_mono=True, mono_max_iter=10, gamma=5.0 / 3.0, min_pres=0.0):
+    """Restatement of Interpolate(): returns (cdata after the call, fdata, number of failed boxes); (x, y, z) tuples."""
+    c, f = np.array(cdata, copy=True), np.array(fdata, copy=True)
+    i3 = lambda t: np.ascontiguousarray(t, dtype=np.int32)
+    nf = np.zeros(1, dtype=np.int32)
+    getattr(lib(), "oracle_interpolate_" + _suffix(f))(
+        _ptr(c), _ptr(f), c.shape[0], _ptr(i3(c.shape[:1:-1])), _ptr(i3(cstart)), _ptr(i3(crange)),
+        _ptr(i3(f.shape[:1:-1])), _ptr(i3(fstart)), _INT[scheme], int(monotonic), ctypes.c_double(mono_coeff),
+        int(opp_sign_0th), int(int_prim), int(reduce_mono), int(mono_max_iter), ctypes.c_double(gamma),
+        ctypes.c_double(min_pres), _ptr(nf))
+    return c, f, int(nf[0])
+
+
+def prepare_patches(patch, pg_in, slot, ghost):
+    patch, pg_in, slot = np.ascontiguousarray(patch), np.array(pg_in, copy=True), _i32(slot)
+    getattr(lib(), "oracle_prepare_patches_" + _suffix(pg_in))(_ptr(patch), _ptr(pg_in), pg_in.shape[0], _ptr(slot),
+                                                               int(ghost))
+    return pg_in

@@ -1,0 +1,77 @@
+"""GPU: coarse -> fine interpolation (INT_CQUAD, INT_MINMOD1D, with and without the iterative primitive /
+monotonic-coefficient retries) and the same-level patch gather, against the oracle restatement
+(pinned in test_oracle_interp.py)."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+G = 5.0 / 3.0
+
+
+def _gas_boxes(rng, nbox, cs):
+    shp = (nbox,) + tuple(cs[::-1])
+    rho = np.exp(0.3 * rng.standard_normal(shp))
+    v = 0.3 * rng.standard_normal((3,) + shp)
+    p = np.exp(0.3 * rng.standard_normal(shp))
+    c = np.stack([rho, rho * v[0], rho * v[1], rho * v[2], p / (G - 1) + 0.5 * rho * (v ** 2).sum(0)], axis=1)
+    # every third box gets a fast cold cell next to a slow one: conserved interpolation fails there
+    for q in range(0, nbox, 3):
+        c[q, :, 3, 3, 3] = [1.0, 4.0, 0.0, 0.0, 8.0 + 1e-3]
+        c[q, :, 3, 3, 4] = [1.0, 0.0, 0.0, 0.0, 1e-3]
+    return c
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("scheme,iterate", [("CQUAD", True), ("CQUAD", False), ("MINMOD1D", False),
+                                            ("MINMOD1D", "prim")])
+def test_interpolate_matches_oracle(dtype, scheme, iterate):
+    import oracle
+    from gamer_amd import hydro
+    print(f"[gpu] interpolate {scheme} iterate={iterate} {np.dtype(dtype).name}", flush=True)
+    rng = np.random.default_rng(5)
+    cs, cr = (7, 6, 8), (5, 4, 6)
+    c = _gas_boxes(rng, 12, cs).astype(dtype)
+    f0 = rng.standard_normal((12, 5, 12, 8, 10)).astype(dtype)
+    kw = dict(scheme=scheme, int_prim=bool(iterate), reduce_mono=(iterate is True), mono_coeff=2.0, mono_max_iter=10)
+    rc, rf, rn = oracle.interpolate(c, f0, (1, 1, 1), cr, **kw)
+    gc, gf = torch.from_numpy(c).to(DEV), torch.from_numpy(f0).to(DEV)
+    nf = hydro.interpolate(gc, gf, (1, 1, 1), cr, **kw)
+    torch.cuda.synchronize()
+    assert int(nf.item()) == rn
+    tol = 1e-14 if dtype == np.float64 else 1e-5
+    np.testing.assert_allclose(gf.cpu().numpy(), rf, rtol=tol, atol=tol)
+    np.testing.assert_allclose(gc.cpu().numpy(), rc, rtol=tol, atol=tol)      # primitive retry converts in place
+
+
+def test_interpolate_fine_offset_without_iteration():
+    import oracle
+    from gamer_amd import hydro
+    print("[gpu] interpolate with fstart", flush=True)
+    rng = np.random.default_rng(6)
+    c = _gas_boxes(rng, 4, (6, 6, 6))
+    f0 = rng.standard_normal((4, 5, 11, 11, 11))
+    kw = dict(scheme="CQUAD", int_prim=False, reduce_mono=False, opp_sign_0th=True, monotonic=0b10101)
+    _, rf, _ = oracle.interpolate(c, f0, (1, 2, 1), (4, 3, 4), fstart=(1, 3, 2), **kw)
+    gf = torch.from_numpy(f0).to(DEV)
+    hydro.interpolate(torch.from_numpy(c).to(DEV), gf, (1, 2, 1), (4, 3, 4), fstart=(1, 3, 2), **kw)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(gf.cpu().numpy(), rf, rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.parametrize("ghost", [3, 2])
+def test_prepare_patches_matches_oracle(ghost):
+    import oracle
+    from gamer_amd import hydro
+    print(f"[gpu] prepare_patches ghost={ghost}", flush=True)
+    rng = np.random.default_rng(8)
+    patch = rng.standard_normal((90, 5, 8, 8, 8))
+    slot = rng.integers(-1, 90, size=(6, 4, 4, 4)).astype(np.int32)
+    n = 16 + 2 * ghost
+    pg0 = rng.standard_normal((6, 5, n, n, n))
+    ref = oracle.prepare_patches(patch, pg0, slot, ghost)
+    got = torch.from_numpy(pg0).to(DEV)
+    hydro.prepare_patches(torch.from_numpy(patch).to(DEV), got, torch.from_numpy(slot).to(DEV), ghost)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(got.cpu().numpy(), ref)

@@ -1,0 +1,123 @@
+"""Oracle restatement of the coarse -> fine ghost-zone interpolation (Interpolate / Int_CQuadratic / Int_MinMod1D)
+and of the same-level patch gather, pinned by what the mathematics fixes:
+
+* both schemes are conservative: the mean of the 2x2x2 fine cells equals the coarse cell,
+* both reproduce a linear field exactly; INT_CQUAD without limiting reproduces the cell averages of a separable
+  quadratic exactly (fine average over [i-1/2, i] of x^2 = C_i - (C_{i+1} - C_{i-1})/8),
+* the monotonic limiter keeps the fine values inside the range of the coarse neighbours at a discontinuity,
+* Interpolate_Iterate: a block that fails in conserved variables is redone in primitive variables (the result equals a
+  primitive interpolation converted back), other blocks keep the conserved result,
+* the patch gather copies the right patch octants and leaves slots of -1 untouched.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+G = 5.0 / 3.0
+
+
+def _box(f, cs=(8, 8, 8)):
+    """(1, 5, z, y, x) coarse box whose 5 fields are f(x, y, z) + field offsets, at cell centres 0..cs-1."""
+    z, y, x = np.meshgrid(*(np.arange(n, dtype=np.float64) for n in cs[::-1]), indexing="ij")
+    base = f(x, y, z)
+    return np.stack([base + 10.0 * (v + 1) for v in range(5)])[None]
+
+
+def _fine_centres(n):
+    """Fine cell centres in coarse index units for fine cells 0..2n-1 of coarse cells 1..n."""
+    return 0.75 + 0.5 * np.arange(2 * n)
+
+
+@pytest.mark.parametrize("scheme", ["CQUAD", "MINMOD1D"])
+def test_conservative_and_linear_exact(scheme):
+    c = _box(lambda x, y, z: 0.5 * x - 0.25 * y + 0.125 * z)
+    f0 = np.zeros((1, 5, 12, 12, 12))
+    _, f, nf = oracle.interpolate(c, f0, (1, 1, 1), (6, 6, 6), scheme=scheme, int_prim=False, reduce_mono=False)
+    assert nf == 0
+    xf = _fine_centres(6)
+    zf, yf, xf3 = np.meshgrid(xf, xf, xf, indexing="ij")
+    exp = 0.5 * xf3 - 0.25 * yf + 0.125 * zf
+    for v in range(5):
+        np.testing.assert_allclose(f[0, v], exp + 10.0 * (v + 1), rtol=0, atol=1e-13)
+    mean = f[0].reshape(5, 6, 2, 6, 2, 6, 2).mean(axis=(2, 4, 6))
+    np.testing.assert_allclose(mean, c[0][:, 1:7, 1:7, 1:7], rtol=0, atol=1e-13)
+
+
+def test_cquad_reproduces_separable_quadratic_averages():
+    # coarse cell averages of q(x) = x^2 over [i-1/2, i+1/2] are i^2 + 1/12
+    avg = lambda a: a ** 2 + 1.0 / 12.0
+    c = _box(lambda x, y, z: avg(x) + 2.0 * avg(y) - avg(z))
+    _, f, nf = oracle.interpolate(c, np.zeros((1, 5, 12, 12, 12)), (1, 1, 1), (6, 6, 6), monotonic=0,
+                                  int_prim=False, reduce_mono=False)
+    assert nf == 0
+    xf = _fine_centres(6)
+    favg = xf ** 2 + 1.0 / 48.0                      # fine averages over half-width cells
+    zf, yf, xf3 = np.meshgrid(favg, favg, favg, indexing="ij")
+    np.testing.assert_allclose(f[0, 0], xf3 + 2.0 * yf - zf + 10.0, rtol=0, atol=1e-12)
+
+
+def test_monotonic_limiter_stays_in_range_at_a_jump():
+    c = _box(lambda x, y, z: np.where(x < 4, 1.0, 5.0))
+    _, f, _ = oracle.interpolate(c, np.zeros((1, 5, 12, 12, 12)), (1, 1, 1), (6, 6, 6), int_prim=False,
+                                 reduce_mono=False)
+    assert f[0, 0].min() >= 11.0 - 1e-14 and f[0, 0].max() <= 15.0 + 1e-14
+    _, fu, _ = oracle.interpolate(c, np.zeros((1, 5, 12, 12, 12)), (1, 1, 1), (6, 6, 6), monotonic=0,
+                                  int_prim=False, reduce_mono=False)
+    assert fu[0, 0].min() < 11.0 - 0.1 or fu[0, 0].max() > 15.0 + 0.1        # unlimited over/undershoots
+
+
+def _gas_box(rng, cs=(6, 6, 6)):
+    rho = np.exp(0.2 * rng.standard_normal(cs[::-1]))
+    v = 0.2 * rng.standard_normal((3,) + cs[::-1])
+    p = np.exp(0.2 * rng.standard_normal(cs[::-1]))
+    return np.stack([rho, rho * v[0], rho * v[1], rho * v[2], p / (G - 1) + 0.5 * rho * (v ** 2).sum(0)])[None]
+
+
+def _con2pri(c):
+    rho = c[:, 0]
+    v = c[:, 1:4] / rho[:, None]
+    eint = c[:, 4] - 0.5 * (c[:, 1] ** 2 + c[:, 2] ** 2 + c[:, 3] ** 2) / rho
+    return np.concatenate([rho[:, None], v, (eint * (G - 1))[:, None]], axis=1)
+
+
+def test_iterate_redoes_failing_blocks_in_primitive_variables():
+    rng = np.random.default_rng(7)
+    c = _gas_box(rng)
+    # a fast, cold cell next to a slow one: conserved interpolation of E and m gives Eint < 0 in its fine cells
+    z, y, x = 3, 3, 3
+    c[0, :, z, y, x] = [1.0, 4.0, 0.0, 0.0, 8.0 + 1e-3]
+    c[0, :, z, y, x + 1] = [1.0, 0.0, 0.0, 0.0, 1e-3]
+    f0 = np.zeros((1, 5, 8, 8, 8))
+    cons_only = oracle.interpolate(c, f0, (1, 1, 1), (4, 4, 4), int_prim=False, reduce_mono=False)[1]
+    fine_bad = lambda f: (f[0, 4] - 0.5 * (f[0, 1] ** 2 + f[0, 2] ** 2 + f[0, 3] ** 2) / f[0, 0]) < 0
+    assert fine_bad(cons_only).any()
+    c_after, f, nf = oracle.interpolate(c, f0, (1, 1, 1), (4, 4, 4), int_prim=True, reduce_mono=True)
+    assert nf == 0 and not fine_bad(f).any()
+    np.testing.assert_allclose(c_after, _con2pri(c), rtol=1e-14, atol=1e-14)      # converted in place
+    prim = oracle.interpolate(_con2pri(c), f0, (1, 1, 1), (4, 4, 4), int_prim=False, reduce_mono=False)[1]
+    w = prim[0]
+    cons = np.stack([w[0], w[0] * w[1], w[0] * w[2], w[0] * w[3],
+                     0.5 * ((w[0] * w[1]) ** 2 + (w[0] * w[2]) ** 2 + (w[0] * w[3]) ** 2) / w[0] + w[4] / (G - 1)])
+    bad_blocks = fine_bad(cons_only).reshape(4, 2, 4, 2, 4, 2).any(axis=(1, 3, 5))
+    blk = np.repeat(np.repeat(np.repeat(bad_blocks, 2, 0), 2, 1), 2, 2)
+    np.testing.assert_allclose(f[0][:, blk], cons[:, blk], rtol=1e-14, atol=1e-14)
+    np.testing.assert_array_equal(f[0][:, ~blk], cons_only[0][:, ~blk])
+
+
+def test_prepare_patches_window_and_keep():
+    rng = np.random.default_rng(9)
+    patch = rng.standard_normal((70, 5, 8, 8, 8))
+    slot = rng.integers(-1, 70, size=(2, 4, 4, 4)).astype(np.int32)
+    ghost = 3
+    n = 16 + 2 * ghost
+    pg0 = rng.standard_normal((2, 5, n, n, n))
+    got = oracle.prepare_patches(patch, pg0, slot, ghost)
+    for p in range(2):
+        for z in range(n):
+            for y in range(0, n, 5):
+                for x in range(n):
+                    X, Y, Z = x - ghost + 8, y - ghost + 8, z - ghost + 8
+                    s = slot[p, Z // 8, Y // 8, X // 8]
+                    exp = pg0[p, :, z, y, x] if s < 0 else patch[s, :, Z % 8, Y % 8, X % 8]
+                    np.testing.assert_array_equal(got[p, :, z, y, x], exp)
