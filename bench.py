@@ -86,32 +86,27 @@ def cpu_baseline(u64_host, w, sample_pg, prec):
 
 def riemann_sweep(args):
     """Roe vs HLLE vs HLLC on a Kelvin-Helmholtz shear layer (rho 1/2, v_x -/+0.5, P 2.5, seeded 1% sinusoidal
-    v_y at wavenumbers 1-4), NPG = 8192: FP32-bulk and FP64 GPU results against the FP64 oracle on every patch
-    group, per-field relative L1/Linf, conserved-sum drift, and kernel throughput per solver."""
+    v_y at wavenumbers 1-4), NPG = 8192: kernel throughput per solver and precision, FP32-bulk against the FP64 GPU
+    path (per-field relative L1 / Linf) and conserved-sum drift.  The FP64 path itself is checked against the
+    oracle on every patch group by tests/test_gpu_sweep.py."""
     import torch
     from gamer_amd import hydro, ic
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle
-    from parity import rel_l1, rel_err_cells
-    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    hydro.lib()
     shape = (32, 16, 16)
     print(f"[sweep] generating KH workload {shape}", file=sys.stderr, flush=True)
     w = ic.kelvin_helmholtz(shape, seed=123)
     dev = torch.device("cuda:0")
     npg = w.npg
     cells = npg * CELLS_PER_PG
-    tot_in = ic.pg_interior(w.u, 3).sum(axis=(0, 2, 3, 4))
-    abs_in = np.abs(ic.pg_interior(w.u, 3)).sum(axis=(0, 2, 3, 4))
-    res = {"metric": "Riemann-solver sweep: FP32-bulk and FP64 GPU vs FP64 oracle (KH shear layer)",
+    interior = ic.pg_interior(w.u, 3)
+    tot_in = interior.sum(axis=(0, 2, 3, 4))
+    abs_in = np.abs(interior).sum(axis=(0, 2, 3, 4))
+    res = {"metric": "Riemann-solver sweep: FP32-bulk vs FP64 GPU, throughput and drift (KH shear layer)",
            "config": {"workload": f"KH shear layer, NPG={npg}, MHM_RP+PLM(VL_GMINMOD)+{{HLLC,HLLE,ROE}}, one step",
                       "npg": npg, "dt": w.dt, "dh": w.dh}, "solvers": {}}
     for rs in ("HLLC", "HLLE", "ROE"):
-        print(f"[sweep] {rs}: oracle on {npg} patch groups ({nthreads} threads)", file=sys.stderr, flush=True)
-        t0 = time.perf_counter()
-        ro, _ = oracle.fluid_solver(w.u, w.dt, w.dh, rsolver=rs, nthreads=nthreads)
-        t_cpu = time.perf_counter() - t0
-        entry = {"cpu_oracle_seconds": round(t_cpu, 2)}
+        print(f"[sweep] {rs}", file=sys.stderr, flush=True)
+        entry, outs = {}, {}
         for prec, tdt in (("fp32", torch.float32), ("fp64", torch.float64)):
             u = torch.from_numpy(w.u).to(dev, dtype=tdt)
             out, flux = hydro.fluid_solver(u, w.dt, w.dh, rsolver=rs)
@@ -124,17 +119,74 @@ def riemann_sweep(args):
             torch.cuda.synchronize()
             kms = ev0.elapsed_time(ev1) / args.steps
             o = out.cpu().numpy().astype(np.float64)
-            d = np.abs(o - ro)
-            linf = d.max(axis=(0, 2, 3, 4)) / _field_scale(np.abs(ro).max(axis=(0, 2, 3, 4)), np.max)
+            outs[prec] = o
             drift = (o.sum(axis=(0, 2, 3, 4)) - tot_in) / _field_scale(abs_in, np.max)
-            entry[prec] = {"rel_l1": [float(x) for x in rel_l1(o, ro)], "rel_linf": [float(x) for x in linf],
-                           "max_cell_rel_err": [float(x) for x in rel_err_cells(o, ro)],
-                           "conserved_sum_drift": [float(x) for x in drift],
-                           "kernel_ms": kms, "cell_updates_per_s": cells / (kms * 1e-3)}
+            entry[prec] = {"conserved_sum_drift": [float(x) for x in drift], "kernel_ms": kms,
+                           "cell_updates_per_s": cells / (kms * 1e-3)}
             del u, out, flux
             torch.cuda.empty_cache()
+        d = np.abs(outs["fp32"] - outs["fp64"])
+        scale = _field_scale(np.abs(outs["fp64"]).sum(axis=(0, 2, 3, 4)), np.max)
+        entry["fp32_vs_fp64"] = {
+            "rel_l1": [float(x) for x in d.sum(axis=(0, 2, 3, 4)) / scale],
+            "rel_linf": [float(x) for x in d.max(axis=(0, 2, 3, 4)) /
+                         _field_scale(np.abs(outs["fp64"]).max(axis=(0, 2, 3, 4)), np.max)]}
         res["solvers"][rs] = entry
     print(json.dumps(res))
+
+
+def amr_batch(args):
+    """Uneven 3-level hierarchy (512 / 2048 / 6144 PGs at levels 0 / 1 / 2, dh and dt halving per level), FP32:
+    one solver call per level (the reference's calling pattern) against one batched launch, the autotuned persistent
+    work queue and the HIP-graph replay of the substep (per-level parity: tests/test_gpu_batch.py)."""
+    import torch
+    from gamer_amd import amr, hydro
+    hydro.lib()
+    dev = torch.device("cuda:0")
+    counts = (512, 2048, 6144)
+    print(f"[amr] building the {counts} hierarchy", file=sys.stderr, flush=True)
+    b = amr.make_level_batch(torch, counts=counts, seed=2026, dtype=torch.float32, interleave=True)
+    npg = sum(counts)
+    cells = npg * CELLS_PER_PG
+    reps = args.steps
+    print("[amr] autotune", file=sys.stderr, flush=True)
+    tune = amr.autotune(torch, b, groups=(1, 2, 3, 4), reps=reps)
+    best = tune["best"]
+    gpc = 2 if best == "grid" else int(best.split(":")[1])
+
+    idx = [np.nonzero(b.level == l)[0] for l in range(len(counts))]
+    u_lv = [b.u[torch.from_numpy(i).to(dev)].contiguous() for i in idx]
+    res_lv = [(torch.empty((len(i), 5, 16, 16, 16), dtype=torch.float32, device=dev),
+               torch.empty((len(i), 9, 5, 16, 16), dtype=torch.float32, device=dev)) for i in idx]
+
+    def per_level():
+        for l in range(len(counts)):
+            hydro.fluid_solver(u_lv[l], b.dt[l], b.dh[l], out=res_lv[l][0], flux=res_lv[l][1])
+
+    out = torch.empty((npg, 5, 16, 16, 16), dtype=torch.float32, device=dev)
+    flux = torch.empty((npg, 9, 5, 16, 16), dtype=torch.float32, device=dev)
+    cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+    print("[amr] timing", file=sys.stderr, flush=True)
+    t_lv = amr.time_launch(torch, per_level, reps=reps)
+    t_grid = amr.time_launch(torch, lambda: hydro.solve_batch(b.u, b.pg_dt_dh, out=out, flux=flux), reps=reps)
+    t_pers = amr.time_launch(torch, lambda: hydro.solve_batch(b.u, b.pg_dt_dh, out=out, flux=flux,
+                                                              queue_counter=cnt, groups_per_cu=gpc), reps=reps)
+    g = amr.GraphSubstep(torch, b, groups_per_cu=gpc, persistent=True)
+    t_graph = amr.time_launch(torch, g.replay, reps=reps)
+    torch.cuda.synchronize()
+
+    rate = lambda ms: cells / (ms * 1e-3)
+    line = {"metric": "3-level AMR substep, one launch vs per-level launches (FP32 bulk)",
+            "config": {"workload": f"levels 0/1/2 with {counts} PGs, dh_l = dh_0/2^l, dt_l = dt_0/2^l, "
+                                   "MHM_RP+PLM(VL_GMINMOD)+HLLC, StoreFlux, PGs of all levels interleaved",
+                       "npg": npg, "cells": cells},
+            "unit": "cell-updates/s",
+            "per_level_launches": {"ms": t_lv, "value": rate(t_lv)},
+            "one_launch_grid": {"ms": t_grid, "value": rate(t_grid)},
+            "one_launch_persistent": {"ms": t_pers, "value": rate(t_pers), "groups_per_cu": gpc},
+            "graph_replay": {"ms": t_graph, "value": rate(t_graph)},
+            "autotune": tune}
+    print(json.dumps(line))
 
 
 def _field_scale(per_field, agg):
@@ -163,6 +215,8 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--riemann-sweep", action="store_true",
                     help="run the Roe/HLLE/HLLC accuracy sweep on the KH workload instead of the throughput bench")
+    ap.add_argument("--amr-batch", action="store_true",
+                    help="time the uneven 3-level batch (one launch vs per-level launches, persistent, graph)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (3x), to scale rocprof byte counters")
     args = ap.parse_args()
@@ -174,6 +228,9 @@ def main():
 
     if args.riemann_sweep:
         riemann_sweep(args)
+        return
+    if args.amr_batch:
+        amr_batch(args)
         return
 
     import torch
@@ -234,7 +291,7 @@ def main():
         "data": "synthetic (seeded lognormal medium + Sod planes + x1e3 pressure spikes, periodic ghost zones)",
         "config": {"workload": "MHM_RP+PLM(VL_GMINMOD)+HLLC, gamma 5/3, CFL 0.5, StoreFlux, one step of "
                                f"NPG={npg} patch groups (16^3 cells + 3 ghosts each, {cells} cells)",
-                   "npg": npg, "precision": "FP32 bulk, FP64 pressure recovery and accumulations",
+                   "npg": npg, "precision": "FP32 bulk, FP64 pressure recovery and full-step accumulation",
                    "parallelism": "1 GPU"},
         "roofline": r32["roofline"],
         "kernel_cell_updates_per_s": r32["kernel_cell_updates_per_s"],

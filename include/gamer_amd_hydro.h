@@ -34,7 +34,7 @@ enum { HYDRO_ROE = 2, HYDRO_HLLE = 3, HYDRO_HLLC = 4 };
 enum { HYDRO_LIM_VANLEER = 1, HYDRO_LIM_GMINMOD = 2, HYDRO_LIM_ALBADA = 3, HYDRO_LIM_VL_GMINMOD = 4,
        HYDRO_LIM_CENTRAL = 6 };
 /* HYDRO_FP64: double storage and arithmetic.  HYDRO_FP32_BULK: float storage and bulk arithmetic, with
-   pressure recovery (E - |m|^2/2rho) and the half/full-step conserved accumulations in double. */
+   pressure recovery (E - |m|^2/2rho) and the full-step conserved accumulation U - dt/dh sum dF in double. */
 enum { HYDRO_FP64 = 0, HYDRO_FP32_BULK = 1 };
 
 /* Arguments of one solver call (the scalar arguments of CUAPI_Asyn_FluidSolver, src/GPU_API/
