@@ -65,8 +65,10 @@ def time_path(torch, hydro, u, w, steps, warmup):
     return wall, kernel_ms, out, flux
 
 
-def cpu_baseline(u64_host, w, sample_pg, prec):
-    """GAMER's own CPU_FluidSolver_MHM (oracle/_ref) on the first `sample_pg` patch groups, all host threads."""
+def cpu_baseline(u64_host, w, sample_pg, prec, reps=3, calibrate=False):
+    """GAMER's own CPU_FluidSolver_MHM (oracle/_ref) on the first `sample_pg` patch groups, all host threads,
+    best of `reps` runs.  With `calibrate`, the oracle's restatement is timed on the same sample and threads
+    (one run) and its time ratio to the reference build is reported."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
@@ -74,14 +76,43 @@ def cpu_baseline(u64_host, w, sample_pg, prec):
     u = u64_host[:sample_pg]
     u = u if prec == "f64" else u.astype(np.float32)
     fn = oracle.ref_fluid_solver if kind == "reference" else oracle.fluid_solver
-    fn(u[:min(8, sample_pg)], w["dt"], w["dh"], nthreads=nthreads)            # warm caches / threads
-    t0 = time.perf_counter()
-    fn(u, w["dt"], w["dh"], nthreads=nthreads)
-    dt = time.perf_counter() - t0
-    return {"value": sample_pg * CELLS_PER_PG / dt, "unit": "cell-updates/s", "cores": nthreads, "kind": kind,
-            "seconds": round(dt, 3),
-            "sample": f"first {sample_pg} of the 4096 patch groups, one MHM_RP/HLLC step, "
-                      f"{'FP64' if prec == 'f64' else 'FP32'} build, OpenMP dynamic,1"}
+
+    def timed(f):
+        f(u[:min(8, sample_pg)], w["dt"], w["dh"], nthreads=nthreads)          # warm caches / threads
+        t0 = time.perf_counter()
+        f(u, w["dt"], w["dh"], nthreads=nthreads)
+        return time.perf_counter() - t0
+
+    dt = min(timed(fn) for _ in range(reps))
+    res = {"value": sample_pg * CELLS_PER_PG / dt, "unit": "cell-updates/s", "cores": nthreads, "kind": kind,
+           "seconds": round(dt, 3),
+           "sample": f"first {sample_pg} of the 4096 patch groups, one MHM_RP/HLLC step, "
+                     f"{'FP64' if prec == 'f64' else 'FP32'} build, OpenMP dynamic,1, best of {reps}"}
+    if calibrate and kind == "reference":
+        tp = timed(oracle.fluid_solver)
+        res["port_seconds"] = round(tp, 3)
+        res["port_over_reference"] = round(tp / dt, 3)
+    return res
+
+
+def host_array_rate(torch, hydro, u, w, steps, precision, nstream=4):
+    """The host-array drop-in (hydro_asyn_fluid_solver: pinned [2] buffers, H2D -> kernel -> D2H split over
+    `nstream` streams) on the same workload: cell-updates/s including the PCIe copies."""
+    npg = u.shape[0]
+    api = hydro.GpuFluidApi(npg, gpu_nstream=nstream, precision=precision)
+    try:
+        h_in, _, _ = api.host_arrays(0)
+        h_in[...] = u.cpu().numpy()
+        api.asyn_fluid_solver(0, npg, w["dt"], w["dh"])
+        api.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            api.asyn_fluid_solver(0, npg, w["dt"], w["dh"])
+        api.synchronize()
+        wall = (time.perf_counter() - t0) / steps
+    finally:
+        api.free()
+    return {"value": npg * CELLS_PER_PG / wall, "ms_per_call": wall * 1e3, "gpu_nstream": nstream}
 
 
 def riemann_sweep(args):
@@ -211,7 +242,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--npg-axis", type=int, default=16, help="patch groups per axis (16 -> NPG 4096)")
-    ap.add_argument("--cpu-sample", type=int, default=1024, help="patch groups in the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=4096, help="patch groups in the CPU baseline sample (0: skip)")
+    ap.add_argument("--skip-host-arrays", action="store_true", help="skip the PCIe-inclusive host-array figure")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--riemann-sweep", action="store_true",
                     help="run the Roe/HLLE/HLLC accuracy sweep on the KH workload instead of the throughput bench")
@@ -268,9 +300,14 @@ def main():
                          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(prec),
                          "algorithmic_bytes_per_launch": bpl},
         }
+        del out, flux
+        torch.cuda.empty_cache()
+        if not args.skip_host_arrays:
+            print(f"[bench] {prec}: host-array drop-in, {args.steps} calls", file=sys.stderr, flush=True)
+            results[prec]["host_arrays"] = host_array_rate(torch, hydro, u, w, args.steps, prec)
         if prec == "fp64":
             u64_host = u.cpu().numpy()
-        del u, out, flux
+        del u
         torch.cuda.empty_cache()
 
     cpu = None
@@ -279,8 +316,11 @@ def main():
             u64_host = cut_on_device(cons, 3, torch, dev, torch.float64).cpu().numpy()
         print(f"[bench] CPU baseline on {args.cpu_sample} patch groups", file=sys.stderr, flush=True)
         cpu = cpu_baseline(u64_host, w, args.cpu_sample, "f32")
-        cpu64 = cpu_baseline(u64_host, w, args.cpu_sample, "f64")
+        cpu64 = cpu_baseline(u64_host, w, args.cpu_sample, "f64", calibrate=True)
         cpu["value_fp64_build"] = cpu64["value"]
+        for k in ("port_seconds", "port_over_reference"):
+            if k in cpu64:
+                cpu[k + "_fp64"] = cpu64[k]
 
     r32 = results["fp32"]
     line = {
@@ -296,10 +336,14 @@ def main():
         "roofline": r32["roofline"],
         "kernel_cell_updates_per_s": r32["kernel_cell_updates_per_s"],
     }
+    if "host_arrays" in r32:
+        line["host_arrays_pcie_inclusive"] = r32["host_arrays"]
     if "fp64" in results:
         r64 = results["fp64"]
         line["fp64"] = {"value": r64["value"], "ms_per_step": r64["ms_per_step"],
                         "kernel_cell_updates_per_s": r64["kernel_cell_updates_per_s"], "roofline": r64["roofline"]}
+        if "host_arrays" in r64:
+            line["fp64"]["host_arrays_pcie_inclusive"] = r64["host_arrays"]
         line["fp32_over_fp64"] = r32["kernel_cell_updates_per_s"] / r64["kernel_cell_updates_per_s"]
     if cpu is not None:
         line["cpu_baseline"] = cpu
