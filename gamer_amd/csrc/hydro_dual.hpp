@@ -69,6 +69,14 @@ struct Dual {
    friend __device__ __forceinline__ bool operator!=( const Dual &a, const Dual &b ) { return a.v != b.v; }
 };
 
+// a*b + c with the value rounded once, as the plain-type fmad
+template <int N> __device__ __forceinline__ Dual<N> fmad( const Dual<N> &a, const Dual<N> &b, const Dual<N> &c )
+{
+   Dual<N> r;  r.v = fmad( a.v, b.v, c.v );
+#  pragma unroll
+   for (int k=0; k<N; k++) r.d[k] = a.d[k]*b.v + a.v*b.d[k] + c.d[k];
+   return r;
+}
 // d(1/x) = -dx/x^2 ; the value is the solver's own reciprocal
 template <int N> __device__ __forceinline__ Dual<N> rcp( const Dual<N> &a )
 {

@@ -28,6 +28,10 @@ __device__ __forceinline__ double rcp( double x )
 }
 __device__ __forceinline__ float  vsqrt( float x )  { return __builtin_amdgcn_sqrtf( x ); }
 __device__ __forceinline__ double vsqrt( double x ) { return __builtin_sqrt( x ); }
+// fused multiply-add written out where the rounding of an expression must not depend on how the compiler
+// contracts it (the same face state is built by different code paths)
+__device__ __forceinline__ float  fmad( float a, float b, float c )    { return __builtin_fmaf( a, b, c ); }
+__device__ __forceinline__ double fmad( double a, double b, double c ) { return __builtin_fma( a, b, c ); }
 template <typename R> __device__ __forceinline__ R vmax( R a, R b ) { return __builtin_fmax( a, b ); }
 template <typename R> __device__ __forceinline__ R vmin( R a, R b ) { return __builtin_fmin( a, b ); }
 template <typename R> __device__ __forceinline__ R vabs( R a )      { return __builtin_fabs( a ); }
@@ -250,7 +254,25 @@ __device__ __forceinline__ St<R> face_state( const R w[NV], const Prm<R> &P )
    s.u  = vn;
    s.ir = rcp( w[0] );
    s.p  = floor_pres( w[4], P.min_pres );
-   s.e  = (R)0.5*w[0]*( w[1]*w[1] + w[2]*w[2] + w[3]*w[3] ) + w[4]*P.inv_gm1;
+   s.e  = fmad( (R)0.5*w[0], fmad( w[3], w[3], fmad( w[2], w[2], w[1]*w[1] ) ), w[4]*P.inv_gm1 );
+   s.cs = vsqrt( P.gamma*s.p*s.ir );
+   return s;
+}
+
+// face_state<0> or face_state<1> chosen per lane (dy = y-normal face): the same arithmetic, the rotation by select
+template <typename R>
+__device__ __forceinline__ St<R> face_state_xy( const R w[NV], bool dy, const Prm<R> &P )
+{
+   St<R> s;
+   const R vn = dy ? w[2] : w[1], v1 = dy ? w[3] : w[2], v2 = dy ? w[1] : w[3];
+   s.d  = w[0];
+   s.mn = w[0]*vn;
+   s.m1 = w[0]*v1;
+   s.m2 = w[0]*v2;
+   s.u  = vn;
+   s.ir = rcp( w[0] );
+   s.p  = floor_pres( w[4], P.min_pres );
+   s.e  = fmad( (R)0.5*w[0], fmad( w[3], w[3], fmad( w[2], w[2], w[1]*w[1] ) ), w[4]*P.inv_gm1 );
    s.cs = vsqrt( P.gamma*s.p*s.ir );
    return s;
 }
@@ -288,6 +310,25 @@ __device__ __forceinline__ void riemann( const St<R> &L, const St<R> &Q, const P
    out[1 + D]             = f[1];
    out[1 + ( D + 1 ) % 3] = f[2];
    out[1 + ( D + 2 ) % 3] = f[3];
+}
+
+// riemann<RS, 0> or riemann<RS, 1> chosen per lane (dy = y-normal face)
+template <int RS, typename R>
+__device__ __forceinline__ void riemann_xy( const St<R> &L, const St<R> &Q, const Prm<R> &P, bool dy, R out[NV] )
+{
+   R f[NV];
+   if ( RS == 4 )
+   {
+      rs_hllc( L, Q, P.min_pres, f );
+      if ( f[0] != f[0] || f[1] != f[1] || f[2] != f[2] || f[3] != f[3] || f[4] != f[4] ) rs_hlle( L, Q, f );
+   }
+   else if ( RS == 3 ) rs_hlle( L, Q, f );
+   else                rs_roe( L, Q, P, f );
+   out[0] = f[0];
+   out[4] = f[4];
+   out[1] = dy ? f[3] : f[1];
+   out[2] = dy ? f[1] : f[2];
+   out[3] = dy ? f[2] : f[3];
 }
 
 template <typename IO> struct IOLim;

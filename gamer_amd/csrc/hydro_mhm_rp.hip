@@ -6,20 +6,25 @@
 // scratch per patch group in FP64).  Here the workgroup marches along z, one input plane per step, and keeps
 // every intermediate on chip:
 //
-//   * Lane t owns the output column (i,j) = (3 + t%16, 3 + t/16) and at most one "outer" ghost column
-//     (outer_column()).  Everything that couples planes along z — the previous input planes, the half-step
-//     z-flux, the last three half-step primitive planes, the lower z-face state and the last full-step
-//     z-flux — stays in that lane's registers.
+//   * Lane t owns the output column (i,j) = (3 + t%16, 3 + t/16).  Lanes 0-183 (waves 0-2) also own one
+//     "outer" ghost column for the half step (half_step_column()); lanes 192-255 (wave 3) each reconstruct one
+//     ghost column next to the output block along x or y, and half of them compute the full-step flux on the
+//     face between that column and the block (edge_column()).  Everything that couples planes along z — the
+//     previous input planes, the half-step z-flux, the last three half-step primitive planes, the lower z-face
+//     state and the last full-step z-flux — stays in the owning lane's registers.
 //   * Only x/y neighbours go through LDS, one plane at a time: the input plane with its pressure, 1/rho and
-//     sound speed, one direction of half-step fluxes, the half-step primitive plane, the upper x/y face
-//     states and x/y full-step fluxes.  A lane-private LDS ring keeps the input cells of the own output column
-//     for the three planes between load and full-step update, so every input value is read from HBM once
-//     (125 KB of LDS per workgroup in FP64, 63 KB in FP32).
+//     sound speed, x and y fluxes (half step, then full step), the half-step primitive plane and the upper
+//     x/y face states.  A lane-private LDS ring keeps the input cells of the own output column for the planes
+//     between load and full-step update, so every input value is read from HBM once (148 KB of LDS per
+//     workgroup in FP64, 74 KB in FP32: two workgroups per CU).
 //
-// Plane schedule at step t (input plane t just loaded; all indices in input coordinates):
-//   half-step z-flux at t-1/2, half-step cell update of plane t-1, half-step x/y fluxes of plane t,
-//   PLM reconstruction of plane r = t-2, full-step fluxes of plane r (x/y) and at r-1/2 (z),
-//   full-step update and output of plane t-3.
+// Plane schedule at step t (input plane t just loaded; all indices in input coordinates), with the four
+// barriers of a step:
+//   [A] half-step z-flux at t-1/2, half-step cell update of plane t-1, half-step x/y fluxes of plane t,
+//       PLM reconstruction of plane r = t-2
+//   [H] x/y half-step flux differences of plane t, half-step plane t-1 into LDS
+//   [R] full-step fluxes of plane r (x/y) and at r-1/2 (z)
+//   [F] full-step update and output of plane t-3, input plane t+1 into LDS
 // All branches that contain a barrier depend only on t, so every barrier is reached by the whole workgroup.
 
 #include "hydro_math.hpp"
@@ -51,20 +56,35 @@ __device__ __forceinline__ void ring2( int k, int &i, int &j )
    else               { i = 20;           j = 2 + k - 58; }
 }
 
-__device__ __forceinline__ bool outer_column( int t, int &i, int &j )
+// ghost column of the half step served by lane t (waves 0-2) besides its output column; false = none
+__device__ __forceinline__ bool half_step_column( int t, int &i, int &j )
 {
-   if ( t <  16 ) { i = 2;          j = 3 + t;       return true; }     // wave 0: x-reconstruction ring
+   if ( t <  16 ) { i = 2;          j = 3 + t;       return true; }     // wave 0
    if ( t <  32 ) { i = 19;         j = 3 + t - 16;  return true; }
    if ( t <  64 ) { ring2( t - 32, i, j );           return true; }
-   if ( t <  80 ) { i = 3 + t - 64; j = 2;           return true; }     // wave 1: y-reconstruction ring
+   if ( t <  80 ) { i = 3 + t - 64; j = 2;           return true; }     // wave 1
    if ( t <  96 ) { i = 3 + t - 80; j = 19;          return true; }
    if ( t < 128 ) { ring2( t - 96 + 32, i, j );      return true; }
-   if ( t < 132 ) { const int c = t - 128; i = ( c & 1 ) ? 19 : 2; j = ( c & 2 ) ? 19 : 2; return true; }
-   if ( t < 144 ) { ring2( t - 132 + 64, i, j );     return true; }     // wave 2
-   if ( t < 164 ) { i = 21;         j = 1 + t - 144; return true; }
-   if ( t >= 192 && t < 212 ) { i = 1 + t - 192; j = 21; return true; }  // wave 3
+   if ( t < 132 ) { const int c = t - 128; i = ( c & 1 ) ? 19 : 2; j = ( c & 2 ) ? 19 : 2; return true; }  // wave 2
+   if ( t < 144 ) { ring2( t - 132 + 64, i, j );     return true; }
+   if ( t < 164 ) { i = 21;         j = 1 + t - 144; return true; }     // x half-step face only
+   if ( t < 184 ) { i = 1 + t - 164; j = 21;         return true; }     // y half-step face only
    i = j = -1;
    return false;
+}
+
+// ghost column next to the output block reconstructed by lane t of wave 3 (along y if dy), and whether the
+// lane also owns the full-step flux on the face between that column and the block (columns 19 only)
+__device__ __forceinline__ bool edge_column( int t, int &i, int &j, bool &dy, bool &face )
+{
+   const int k = t - 192;
+   dy = ( k >= 32 );
+   const int e = k & 31;
+   face = ( e >= 16 );
+   const int a = face ? 19 : 2, b = 3 + ( e & 15 );
+   i = dy ? b : a;
+   j = dy ? a : b;
+   return ( k >= 0 && k < 64 );
 }
 
 // roles of a column (i,j) of the 22x22 footprint
@@ -159,33 +179,48 @@ __device__ __forceinline__ St<R> st_face( const R (*a)[NC], int v0, int c, const
    return face_state<D>( w, P );
 }
 
-// LDS of one workgroup.  XY2: x and y fluxes have separate buffers (one barrier per flux stage instead of three).
-template <typename R, bool XY2>
+template <typename R>
+__device__ __forceinline__ St<R> st_face_xy( const R (*a)[NC], int v0, int c, bool dy, const Prm<R> &P )
+{
+   R w[NV];
+#  pragma unroll
+   for (int v=0; v<NV; v++) w[v] = a[v0 + v][c];
+   return face_state_xy( w, dy, P );
+}
+
+// LDS of one workgroup
+template <typename R>
 struct SmemRP {
-   R a[10][NC];                 // input plane (rho,mx,my,mz,E,P,1/rho,c_s) | upper x (0-4) and y (5-9) face states
-   R b[XY2 ? 2*NV : NV][NC];    // half-step or full-step x (0-4) [and y (5-9)] fluxes
+   R a[8][NC];                  // input plane (rho,mx,my,mz,E,P,1/rho,c_s)
+   R f[2*NV][NC];               // upper x (0-4) and y (5-9) face states of the reconstructed plane
+   R b[2*NV][NC];               // half-step or full-step x (0-4) and y (5-9) fluxes
    R w[NV][NC];                 // half-step primitive plane read by the reconstruction
-   R u3[3][NV][NT];             // lane-private ring: input cells of the own output column, planes t-3..t-1
+   R u3[2][NV][NT];             // lane-private ring: input cells of the own output column, planes t-3 and t-2
    int fail, next;
    float cfl[8][4];   // fused CFL: per-patch maxima of the 4 waves
 };
 
-template <typename R, typename IO, int RS, int LIM, bool XY2, bool PERSIST>
+template <typename R, typename IO, int RS, int LIM, bool PERSIST>
 __global__ void __launch_bounds__( NT ) __attribute__(( amdgpu_waves_per_eu( sizeof(R) == 4 ? WPE32 : WPE64 ) ))
 mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict__ g_flux,
                int npg, int store_flux, int minmod_maxiter, R minmod_coeff, Prm<R> P0,
                IO *__restrict__ g_dt, double dh_safety0, Batch B )
 {
    static_assert( sizeof( R ) == sizeof( IO ), "the input ring holds IO values in R storage" );
-   __shared__ SmemRP<R, XY2> S;
-   constexpr int YB = XY2 ? NV : 0;         // offset of the y fluxes in S.b
+   __shared__ SmemRP<R> S;
 
    const int tid = threadIdx.x;
 
    const Col K = make_col( 3 + tid % PS, 3 + tid / PS );
    int oi, oj;
-   const bool has_o = outer_column( tid, oi, oj );
+   const bool has_o = half_step_column( tid, oi, oj );
    const Col O = make_col( has_o ? oi : 0, has_o ? oj : 0 );
+   int ei, ej;
+   bool e_dy, e_face;
+   const bool has_e = edge_column( tid, ei, ej, e_dy, e_face );
+   const int  e_c  = ei + NX*ej;
+   const int  e_st = e_dy ? NX : 1;             // neighbour stride of the edge column's direction
+   const int  e_vo = e_dy ? NV : 0;             // its face states / fluxes in S.f / S.b
    const int  lim = P0.limiter;
 
    for (int item = 0; ; item++)
@@ -227,12 +262,12 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
       if ( tid == 0 ) S.fail = 0;
       IO cfl[2] = { (IO)0, (IO)0 };            // fused CFL: max(|v_d|+c_s) of this column in patches z=0,1
 
-      // lane state carried along z (core column K, outer column O)
+      // lane state carried along z (output column K, half-step ghost column O)
       R kU[8], kU1[8];                            // input cell of planes t and t-1 (+ pressure, 1/rho, c_s)
       R kHz[NV], kDh[NV];                         // half-step z-flux at t-3/2 ; half-step dFx+dFy of plane t-1
       R kW1[NV], kW2[NV], kW3[NV];                // half-step primitives of planes t-1, t-2, t-3
       R kFzR[NV], kFz[NV], kDf[NV];               // upper z face of plane r-1 ; full z-flux at r-3/2 ; full dFx+dFy of plane r-1
-      R oU[8], oU1[8], oHz[NV], oDh[NV], oW[NV];  // outer column
+      R oU[8], oU1[8], oHz[NV], oDh[NV], oW[NV];  // half-step ghost column
 
       // prefetch of the next input plane (two cells per lane, coalesced)
       IO pf[2][NV];
@@ -247,12 +282,9 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
                for (int v=0; v<NV; v++) pf[k][v] = in[ (size_t)v*NC*NX + (size_t)t*NC + c ];
          }
       };
-      issue( 0 );
-
-      for (int t = 0; t < NX; t++)
+      // input plane (prefetched) -> LDS, with pressure, 1/rho and c_s once per cell
+      auto stage = [&]()
       {
-         // ---- A: input plane t -> LDS, pressure/1/rho/c_s once per cell; then prefetch plane t+1
-         __syncthreads();
 #        pragma unroll
          for (int k=0; k<2; k++)
          {
@@ -265,10 +297,17 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
                for (int v=0; v<8; v++) S.a[v][c] = q[v];
             }
          }
-         if ( t + 1 < NX ) issue( t + 1 );
+      };
+      __syncthreads();                            // the previous patch group / iteration is done with S.a
+      issue( 0 );
+      stage();
+      issue( 1 );
+
+      for (int t = 0; t < NX; t++)
+      {
          const int o = t - 3;                    // output plane of this step
-         const int ring = t % 3;                 // u3 slot holding input plane t-3 until the end of this step
-         __syncthreads();
+         const int ring = ( t + 1 ) & 1;         // u3 slot: input plane t-3 (read), then plane t-1 (written)
+         __syncthreads();                        // [A] input plane t visible
 
 #        pragma unroll
          for (int v=0; v<8; v++) { kU1[v] = kU[v];  oU1[v] = oU[v]; }
@@ -299,58 +338,27 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
          }
 
          // ---- half-step x and y fluxes of plane t (low face owned by each column)
+         R fkx[NV], fox[NV], fky[NV], foy[NV];
          if ( t >= 1 && t <= 20 )
          {
-            R fkx[NV], fox[NV], fky[NV], foy[NV];
             riemann<RS, 0>( st_of<0>( S.a, K.c - 1 ), st_of<0>( kU ), Q, fkx );
             if ( has_o && O.hx ) riemann<RS, 0>( st_of<0>( S.a, O.c - 1 ), st_of<0>( oU ), Q, fox );
-            if ( XY2 )
-            {
-               riemann<RS, 1>( st_of<1>( S.a, K.c - NX ), st_of<1>( kU ), Q, fky );
-               if ( has_o && O.hy ) riemann<RS, 1>( st_of<1>( S.a, O.c - NX ), st_of<1>( oU ), Q, foy );
-            }
+            riemann<RS, 1>( st_of<1>( S.a, K.c - NX ), st_of<1>( kU ), Q, fky );
+            if ( has_o && O.hy ) riemann<RS, 1>( st_of<1>( S.a, O.c - NX ), st_of<1>( oU ), Q, foy );
 #           pragma unroll
             for (int v=0; v<NV; v++)
             {
                S.b[v][K.c] = fkx[v];
                if ( has_o && O.hx ) S.b[v][O.c] = fox[v];
-               if ( XY2 )
-               {
-                  S.b[YB + v][K.c] = fky[v];
-                  if ( has_o && O.hy ) S.b[YB + v][O.c] = foy[v];
-               }
-            }
-            __syncthreads();
-#           pragma unroll
-            for (int v=0; v<NV; v++)
-            {
-               kDh[v] = S.b[v][K.c + 1] - fkx[v];
-               if ( has_o && O.w ) oDh[v] = S.b[v][O.c + 1] - fox[v];
-            }
-            if ( !XY2 )
-            {
-               __syncthreads();
-               riemann<RS, 1>( st_of<1>( S.a, K.c - NX ), st_of<1>( kU ), Q, fky );
-               if ( has_o && O.hy ) riemann<RS, 1>( st_of<1>( S.a, O.c - NX ), st_of<1>( oU ), Q, foy );
-#              pragma unroll
-               for (int v=0; v<NV; v++)
-               {
-                  S.b[v][K.c] = fky[v];
-                  if ( has_o && O.hy ) S.b[v][O.c] = foy[v];
-               }
-               __syncthreads();
-            }
-#           pragma unroll
-            for (int v=0; v<NV; v++)
-            {
-               kDh[v] += S.b[YB + v][K.c + NX] - fky[v];
-               if ( has_o && O.w ) oDh[v] += S.b[YB + v][O.c + NX] - foy[v];
+               S.b[NV + v][K.c] = fky[v];
+               if ( has_o && O.hy ) S.b[NV + v][O.c] = foy[v];
             }
          }
 
-         // ---- PLM reconstruction of plane r = t-2 (x/y upper faces to LDS, z faces stay in the lane)
+         // ---- PLM reconstruction of plane r = t-2 (x/y upper faces to LDS, z faces stay in the lane); the
+         //      x/y neighbours come from the half-step plane r in S.w
          const int r = t - 2;
-         R kFL[3][NV], kFRz[NV], oFL[NV];
+         R kFL[3][NV], kFRz[NV], eFL[NV];
          if ( r >= 2 && r <= 19 )
          {
 #           pragma unroll
@@ -362,24 +370,31 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
                {
                   R fr;
                   plm_face<LIM>( S.w[v][K.c - 1], kW2[v], S.w[v][K.c + 1], lim, Q.coeff, kFL[0][v], fr );
-                  S.a[v][K.c] = fr;
+                  S.f[v][K.c] = fr;
                   plm_face<LIM>( S.w[v][K.c - NX], kW2[v], S.w[v][K.c + NX], lim, Q.coeff, kFL[1][v], fr );
-                  S.a[NV + v][K.c] = fr;
+                  S.f[NV + v][K.c] = fr;
                }
-               if ( has_o && ( O.rx || O.ry ) )
+               if ( has_e )
                {
-                  const int st = O.rx ? 1 : NX, vo = O.rx ? 0 : NV;
 #                 pragma unroll
                   for (int v=0; v<NV; v++)
                   {
                      R fr;
-                     plm_face<LIM>( S.w[v][O.c - st], S.w[v][O.c], S.w[v][O.c + st], lim, Q.coeff, oFL[v], fr );
-                     S.a[vo + v][O.c] = fr;
+                     plm_face<LIM>( S.w[v][e_c - e_st], S.w[v][e_c], S.w[v][e_c + e_st], lim, Q.coeff, eFL[v], fr );
+                     S.f[e_vo + v][e_c] = fr;
                   }
                }
             }
          }
-         __syncthreads();
+         __syncthreads();                        // [H] half-step x/y fluxes visible; S.w plane r no longer read
+
+         if ( t >= 1 && t <= 20 )
+#           pragma unroll
+            for (int v=0; v<NV; v++)
+            {
+               kDh[v] = ( S.b[v][K.c + 1] - fkx[v] ) + ( S.b[NV + v][K.c + NX] - fky[v] );
+               if ( has_o && O.w ) oDh[v] = ( S.b[v][O.c + 1] - fox[v] ) + ( S.b[NV + v][O.c + NX] - foy[v] );
+            }
          // the half-step plane t-1 replaces plane t-2 in LDS for the next reconstruction
          if ( t >= 2 )
          {
@@ -408,64 +423,37 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
          R dfn[NV];
          if ( r >= 3 && r <= 18 )
          {
-            R fkx[NV], fky[NV], fo[NV];
-            riemann<RS, 0>( st_face<0>( S.a, 0, K.c - 1, Q ), face_state<0>( kFL[0], Q ), Q, fkx );
-            if ( XY2 ) riemann<RS, 1>( st_face<1>( S.a, NV, K.c - NX, Q ), face_state<1>( kFL[1], Q ), Q, fky );
-            if ( has_o && O.fx ) riemann<RS, 0>( st_face<0>( S.a, 0,  O.c - 1,  Q ), face_state<0>( oFL, Q ), Q, fo );
-            if ( XY2 && has_o && O.fy ) riemann<RS, 1>( st_face<1>( S.a, NV, O.c - NX, Q ), face_state<1>( oFL, Q ), Q, fo );
+            __syncthreads();                     // [R] the half-step fluxes in S.b are read
+            R ffx[NV], ffy[NV], fe[NV];
+            riemann<RS, 0>( st_face<0>( S.f, 0, K.c - 1, Q ), face_state<0>( kFL[0], Q ), Q, ffx );
+            riemann<RS, 1>( st_face<1>( S.f, NV, K.c - NX, Q ), face_state<1>( kFL[1], Q ), Q, ffy );
+            if ( has_e && e_face )
+               riemann_xy<RS>( st_face_xy( S.f, e_vo, e_c - e_st, e_dy, Q ), face_state_xy( eFL, e_dy, Q ), Q, e_dy, fe );
 #           pragma unroll
             for (int v=0; v<NV; v++)
             {
-               S.b[v][K.c] = fkx[v];
-               if ( XY2 ) S.b[YB + v][K.c] = fky[v];
-               if ( has_o && O.fx ) S.b[v][O.c] = fo[v];
-               if ( XY2 && has_o && O.fy ) S.b[YB + v][O.c] = fo[v];
+               S.b[v][K.c] = ffx[v];
+               S.b[NV + v][K.c] = ffy[v];
+               if ( has_e && e_face ) S.b[e_vo + v][e_c] = fe[v];
             }
             if ( flx )
             {
                if ( ( K.i - 3 ) % 8 == 0 )
 #                 pragma unroll
                   for (int v=0; v<NV; v++)
-                     flx[ ( ( ( K.i - 3 )/8 )*NV + v )*PS*PS + ( K.j - 3 ) + PS*( r - 3 ) ] = (IO)fkx[v];
-               if ( XY2 && ( K.j - 3 ) % 8 == 0 )
+                     flx[ ( ( ( K.i - 3 )/8 )*NV + v )*PS*PS + ( K.j - 3 ) + PS*( r - 3 ) ] = (IO)ffx[v];
+               if ( ( K.j - 3 ) % 8 == 0 )
 #                 pragma unroll
                   for (int v=0; v<NV; v++)
-                     flx[ ( ( 3 + ( K.j - 3 )/8 )*NV + v )*PS*PS + ( K.i - 3 ) + PS*( r - 3 ) ] = (IO)fky[v];
-               if ( has_o && O.fx )
+                     flx[ ( ( 3 + ( K.j - 3 )/8 )*NV + v )*PS*PS + ( K.i - 3 ) + PS*( r - 3 ) ] = (IO)ffy[v];
+               if ( has_e && e_face )             // planes x = 16 (2) and y = 16 (5)
 #                 pragma unroll
-                  for (int v=0; v<NV; v++) flx[ ( 2*NV + v )*PS*PS + ( O.j - 3 ) + PS*( r - 3 ) ] = (IO)fo[v];
-               if ( XY2 && has_o && O.fy )
-#                 pragma unroll
-                  for (int v=0; v<NV; v++) flx[ ( 5*NV + v )*PS*PS + ( O.i - 3 ) + PS*( r - 3 ) ] = (IO)fo[v];
+                  for (int v=0; v<NV; v++)
+                     flx[ ( ( e_dy ? 5 : 2 )*NV + v )*PS*PS + ( ( e_dy ? ei : ej ) - 3 ) + PS*( r - 3 ) ] = (IO)fe[v];
             }
-            __syncthreads();
+            __syncthreads();                     // [F] full-step fluxes visible
 #           pragma unroll
-            for (int v=0; v<NV; v++) dfn[v] = S.b[v][K.c + 1] - fkx[v];
-            if ( !XY2 )
-            {
-               __syncthreads();
-               riemann<RS, 1>( st_face<1>( S.a, NV, K.c - NX, Q ), face_state<1>( kFL[1], Q ), Q, fky );
-               if ( has_o && O.fy ) riemann<RS, 1>( st_face<1>( S.a, NV, O.c - NX, Q ), face_state<1>( oFL, Q ), Q, fo );
-#              pragma unroll
-               for (int v=0; v<NV; v++)
-               {
-                  S.b[v][K.c] = fky[v];
-                  if ( has_o && O.fy ) S.b[v][O.c] = fo[v];
-               }
-               if ( flx )
-               {
-                  if ( ( K.j - 3 ) % 8 == 0 )
-#                    pragma unroll
-                     for (int v=0; v<NV; v++)
-                        flx[ ( ( 3 + ( K.j - 3 )/8 )*NV + v )*PS*PS + ( K.i - 3 ) + PS*( r - 3 ) ] = (IO)fky[v];
-                  if ( has_o && O.fy )
-#                    pragma unroll
-                     for (int v=0; v<NV; v++) flx[ ( 5*NV + v )*PS*PS + ( O.i - 3 ) + PS*( r - 3 ) ] = (IO)fo[v];
-               }
-               __syncthreads();
-            }
-#           pragma unroll
-            for (int v=0; v<NV; v++) dfn[v] += S.b[YB + v][K.c + NX] - fky[v];
+            for (int v=0; v<NV; v++) dfn[v] = ( S.b[v][K.c + 1] - ffx[v] ) + ( S.b[NV + v][K.c + NX] - ffy[v] );
          }
 
          // ---- full-step update and output of plane o = t-3
@@ -488,7 +476,14 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
 #           pragma unroll
             for (int v=0; v<NV; v++) kDf[v] = dfn[v];
 #        pragma unroll
-         for (int v=0; v<NV; v++) S.u3[ring][v][tid] = kU[v];      // input plane t, read back at step t+3
+         for (int v=0; v<NV; v++) S.u3[ring][v][tid] = kU1[v];     // input plane t-1, read back at step t+2
+
+         // ---- input plane t+1 into LDS: every read of S.a of this step came before barrier [H]
+         if ( t + 1 < NX )
+         {
+            stage();
+            if ( t + 2 < NX ) issue( t + 2 );
+         }
       } // for t
 
       if ( g_dt ) fused_dt( cfl, ( K.i - 3 )/8 + 2*( ( K.j - 3 )/8 ), S.cfl, g_dt + (size_t)pg*8, dh_safety );
@@ -502,13 +497,13 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
 
 } // namespace
 
-template <typename R, typename IO, int RS, bool XY2>
+template <typename R, typename IO, int RS>
 static void launch_rs( const IO *in, IO *out, IO *flux, int npg, int store_flux, int minmod_maxiter,
                        double minmod_coeff, const Prm<R> &P, IO *dt, double dh_safety, const Batch &B, int grid_blocks, hipStream_t stream )
 {
    const dim3 grid( grid_blocks > 0 ? grid_blocks : npg ), block( NT );
    const bool pers = ( B.counter != nullptr );
-#  define GA_LAUNCH( LIM_, PERS_ ) hipLaunchKernelGGL( ( mhm_rp_kernel<R, IO, RS, LIM_, XY2, PERS_> ), grid, block, 0, stream, \
+#  define GA_LAUNCH( LIM_, PERS_ ) hipLaunchKernelGGL( ( mhm_rp_kernel<R, IO, RS, LIM_, PERS_> ), grid, block, 0, stream, \
                                                       in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P, dt, dh_safety, B )
    if ( P.limiter == 4 ) { if ( pers ) GA_LAUNCH( 4, true ); else GA_LAUNCH( 4, false ); }
    else                  { if ( pers ) GA_LAUNCH( 0, true ); else GA_LAUNCH( 0, false ); }
@@ -520,12 +515,11 @@ int launch_mhm_rp( const IO *in, IO *out, IO *flux, int npg, int rsolver, int st
                    double minmod_coeff, const Prm<R> &P, IO *dt, double dh_safety, const Batch &B, int grid_blocks, hipStream_t stream )
 {
    if ( npg <= 0 ) return 0;
-   constexpr bool XY2 = true;                  // separate x and y flux buffers (one barrier per flux stage)
    switch ( rsolver )
    {
-      case 4: launch_rs<R, IO, 4, XY2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, B, grid_blocks, stream ); break;
-      case 3: launch_rs<R, IO, 3, XY2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, B, grid_blocks, stream ); break;
-      case 2: launch_rs<R, IO, 2, XY2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, B, grid_blocks, stream ); break;
+      case 4: launch_rs<R, IO, 4>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, B, grid_blocks, stream ); break;
+      case 3: launch_rs<R, IO, 3>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, B, grid_blocks, stream ); break;
+      case 2: launch_rs<R, IO, 2>( in, out, flux, npg, store_flux, minmod_maxiter, minmod_coeff, P, dt, dh_safety, B, grid_blocks, stream ); break;
       default: return -1;
    }
    return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
