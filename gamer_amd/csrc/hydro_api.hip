@@ -323,15 +323,18 @@ int hydro_interpolate_async( const HydroInterpArgs *a, void *stream )
    if ( a == nullptr ) return fail( HYDRO_E_ARG, "args == NULL" );
    if ( a->nbox < 0 || ( a->nbox > 0 && ( a->cdata == nullptr || a->fdata == nullptr || a->nfail == nullptr ) ) )
       return fail( HYDRO_E_ARG, "interpolate: bad pointers/size" );
-   if ( a->scheme != 2 && a->scheme != 4 ) return fail( HYDRO_E_ARG, "interpolate: scheme %d (INT_MINMOD1D = 2, INT_CQUAD = 4)", a->scheme );
-   if ( a->scheme == 2 && a->reduce_mono ) return fail( HYDRO_E_ARG, "interpolate: INT_MINMOD1D has no monotonic coefficient to reduce" );
+   if ( a->scheme < 1 || a->scheme > 7 ) return fail( HYDRO_E_ARG, "interpolate: scheme %d (INT_MINMOD3D = 1 ... INT_QUAR = 7)", a->scheme );
+   // Interpolate.cpp:299-300: these schemes have no monotonic coefficient to reduce
+   if ( ( a->scheme == 1 || a->scheme == 2 || a->scheme == 3 ) && a->reduce_mono )
+      return fail( HYDRO_E_ARG, "interpolate: INT_VANLEER/INT_MINMOD3D/INT_MINMOD1D do not support reduce_mono" );
+   const int ng = ( a->scheme == 6 || a->scheme == 7 ) ? 2 : 1;      // coarse ghost cells (Int_Table.cpp)
    if ( a->reduce_mono && a->mono_max_iter <= 0 ) return fail( HYDRO_E_ARG, "interpolate: mono_max_iter must be > 0" );
    if ( !( a->gamma > 1.0 ) ) return fail( HYDRO_E_ARG, "interpolate: gamma must be > 1" );
    long ncell = 1;
    for (int d=0; d<3; d++)
    {
-      if ( a->crange[d] <= 0 || a->cstart[d] < 1 || a->cstart[d] + a->crange[d] > a->csize[d] - 1 )
-         return fail( HYDRO_E_ARG, "interpolate: coarse range out of the box (1 ghost cell needed) in dim %d", d );
+      if ( a->crange[d] <= 0 || a->cstart[d] < ng || a->cstart[d] + a->crange[d] > a->csize[d] - ng )
+         return fail( HYDRO_E_ARG, "interpolate: coarse range out of the box (%d ghost cells needed) in dim %d", ng, d );
       if ( a->fstart[d] < 0 || a->fstart[d] + 2*a->crange[d] > a->fsize[d] )
          return fail( HYDRO_E_ARG, "interpolate: fine range out of the box in dim %d", d );
       if ( ( a->int_prim || a->reduce_mono ) && a->fstart[d] != 0 )

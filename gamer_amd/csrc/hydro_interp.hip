@@ -2,11 +2,13 @@
 // data: the two halves of Prepare_PatchData for the fluid solver at a refinement boundary
 // (src/Main/Prepare_PatchData.cpp:166, src/Main/InterpolateGhostZone.cpp, src/Interpolation/Interpolate.cpp:90).
 //
-//   interpolate_kernel   Interpolate() with Interpolate_Iterate (Interpolate.cpp:90-330) for INT_CQUAD
-//                        (Int_CQuadratic.cpp) and INT_MINMOD1D (Int_MinMod1D.cpp): one workgroup per coarse box,
-//                        one lane per coarse cell producing its 2x2x2 fine cells.  For INT_CQUAD the lane evaluates
-//                        the x pass on the 3x3 coarse (y,z) neighbourhood, then the y pass, then the z pass: the same
-//                        operations in the same order as the reference's split temporaries.  Iterations (conserved
+//   interpolate_kernel   Interpolate() with Interpolate_Iterate (Interpolate.cpp:90-330) for INT_MINMOD3D, INT_MINMOD1D,
+//                        INT_VANLEER, INT_CQUAD, INT_QUAD, INT_CQUAR and INT_QUAR (src/Interpolation/Int_*.cpp): one
+//                        workgroup per coarse box, one lane per coarse cell producing its 2x2x2 fine cells.  For the
+//                        tensor-product schemes (all but INT_MINMOD1D) the lane evaluates the x pass on the
+//                        (2G+1)x(2G+1) coarse (y,z) neighbourhood (G = 1, or 2 for the quartic schemes), then the y
+//                        pass, then the z pass: the same operations in the same order as the reference's split
+//                        temporaries.  Iterations (conserved
 //                        variables, then primitive variables, then lower monotonic coefficients) redo only coarse
 //                        cells whose fine cells failed the physicality test: the INTERP_MASK rule (include/Macro.h:947)
 //                        at the granularity of 2x2x2 fine blocks.
@@ -48,42 +50,127 @@ __device__ __forceinline__ T cquad_slope( T L, T C, T R, bool mono, T coeff, boo
    return s;
 }
 
+// One split pass of a tensor-product scheme: stencil u = {L2, L1, C, R1, R2} along one direction -> the fine values
+// (lo, hi) of the centre cell (x passes of Int_MinMod3D.cpp:93-103, Int_vanLeer.cpp:93-105, Int_CQuadratic.cpp,
+// Int_Quadratic.cpp:104-160, Int_CQuartic.cpp:100-140, Int_Quartic.cpp:100-165; their y and z passes repeat them)
+template <typename T, int SCHEME>
+__device__ __forceinline__ void pass1d( const T u[5], bool mono, T coeff, bool opp0, T &lo, T &hi )
+{
+   #pragma clang fp contract(off)
+   const T L2 = u[0], L1 = u[1], C = u[2], R1 = u[3], R2 = u[4];
+   T s;
+   if ( SCHEME == 1 || SCHEME == 3 )               // MinMod-3D, van Leer
+   {
+      const T l = C - L1, r = R1 - C;
+      if ( r*l <= (T)0 ) s = (T)0;
+      else s = ( SCHEME == 1 ) ? (T)0.25*( vabs( r ) < vabs( l ) ? r : l ) : (T)0.5*l*r/( l + r );
+      if ( opp0 && L1*R1 < (T)0 ) s = (T)0;
+      lo = C - s;  hi = C + s;
+   }
+   else if ( SCHEME == 4 )                         // conservative quadratic
+   {
+      s = cquad_slope( L1, C, R1, mono, coeff, opp0 );
+      lo = C - s;  hi = C + s;
+   }
+   else if ( SCHEME == 6 )                         // conservative quartic
+   {
+      s = (T)( 3.0/128.0)*L2 + (T)(-22.0/128.0)*L1 + (T)(22.0/128.0)*R1 + (T)(-3.0/128.0)*R2;
+      if ( mono )
+      {
+         T l = (T)0.25*( C - L1 ), r = (T)0.25*( R1 - C );
+         if ( l*r > (T)0 )
+         {
+            if ( s*l < (T)0 ) s = (T)0.125*( R1 - L1 );
+            const T sg = sgn( l );
+            s *= sg;  l *= sg;  r *= sg;
+            if ( l < r ) l *= coeff; else r *= coeff;
+            s = vmin( l, s );
+            s = vmin( r, s );
+            s *= sg;
+         }
+         else s = (T)0;
+      }
+      if ( opp0 && L1*R1 < (T)0 ) s = (T)0;
+      lo = C - s;  hi = C + s;
+   }
+   else                                            // quadratic (5), quartic (7): Lagrange values at -/+ 1/4
+   {
+      if ( SCHEME == 5 )
+      {
+         lo = (T)( 5.0/32.0)*L1 + (T)(30.0/32.0)*C + (T)(-3.0/32.0)*R1;
+         hi = (T)(-3.0/32.0)*L1 + (T)(30.0/32.0)*C + (T)( 5.0/32.0)*R1;
+      }
+      else
+      {
+         lo = (T)( -45.0/2048.0)*L2 + (T)( 420.0/2048.0)*L1 + (T)(1890.0/2048.0)*C + (T)(-252.0/2048.0)*R1
+            + (T)(  35.0/2048.0)*R2;
+         hi = (T)(  35.0/2048.0)*L2 + (T)(-252.0/2048.0)*L1 + (T)(1890.0/2048.0)*C + (T)( 420.0/2048.0)*R1
+            + (T)( -45.0/2048.0)*R2;
+      }
+      if ( mono )
+      {
+         T l = C - L1, r = R1 - C;
+         if ( l*r > (T)0 )
+         {
+            const T cmax = ( l > (T)0 ) ? R1 : L1, cmin = ( l > (T)0 ) ? L1 : R1;
+            bool bad = ( vmax( lo, hi ) > cmax || vmin( lo, hi ) < cmin );
+            if ( SCHEME == 7 ) bad = bad || ( hi - lo )*l < (T)0;
+            if ( bad )
+            {
+               const T c4 = (T)0.25*coeff;
+               s  = (T)0.125*( R1 - L1 );
+               l *= c4;  r *= c4;
+               const T sg = sgn( l );
+               s *= sg;
+               s  = vmin( sg*l, s );
+               s  = vmin( sg*r, s );
+               s *= sg;
+               lo = C - s;  hi = C + s;
+            }
+         }
+         else { lo = C;  hi = C; }
+      }
+      if ( opp0 && L1*R1 < (T)0 ) { lo = C;  hi = C; }
+   }
+}
+
 // the 2x2x2 fine values of coarse cell i of one field: f[(k*2 + j)*2 + i]
 template <typename T, int SCHEME>
 __device__ __forceinline__ void interp_cell( const T *__restrict__ c, int i, int dy, int dz, bool mono, T coeff,
                                              bool opp0, T f[8] )
 {
    #pragma clang fp contract(off)
-   if ( SCHEME == 4 )
+   if ( SCHEME != 2 )
    {
-      T X[3][3][2], Y[3][2][2];
+      constexpr int G = ( SCHEME == 6 || SCHEME == 7 ) ? 2 : 1, W = 2*G + 1;
+      T X[W][W][2], Y[W][2][2], u[5];
 #     pragma unroll
-      for (int a=0; a<3; a++)
+      for (int a=0; a<W; a++)
 #        pragma unroll
-         for (int b=0; b<3; b++)
+         for (int b=0; b<W; b++)
          {
-            const int q = i + ( a - 1 )*dz + ( b - 1 )*dy;
-            const T s = cquad_slope( c[q-1], c[q], c[q+1], mono, coeff, opp0 );
-            X[a][b][0] = c[q] - s;
-            X[a][b][1] = c[q] + s;
+            const int q = i + ( a - G )*dz + ( b - G )*dy;
+#           pragma unroll
+            for (int k=0; k<5; k++) u[k] = ( k - 2 >= -G && k - 2 <= G ) ? c[q + k - 2] : (T)0;
+            pass1d<T, SCHEME>( u, mono, coeff, opp0, X[a][b][0], X[a][b][1] );
          }
 #     pragma unroll
-      for (int a=0; a<3; a++)
+      for (int a=0; a<W; a++)
 #        pragma unroll
          for (int x=0; x<2; x++)
          {
-            const T s = cquad_slope( X[a][0][x], X[a][1][x], X[a][2][x], mono, coeff, opp0 );
-            Y[a][0][x] = X[a][1][x] - s;
-            Y[a][1][x] = X[a][1][x] + s;
+#           pragma unroll
+            for (int k=0; k<5; k++) u[k] = ( k - 2 >= -G && k - 2 <= G ) ? X[a][k - 2 + G][x] : (T)0;
+            pass1d<T, SCHEME>( u, mono, coeff, opp0, Y[a][0][x], Y[a][1][x] );
          }
 #     pragma unroll
       for (int y=0; y<2; y++)
 #        pragma unroll
          for (int x=0; x<2; x++)
          {
-            const T s = cquad_slope( Y[0][y][x], Y[1][y][x], Y[2][y][x], mono, coeff, opp0 );
-            f[( 0*2 + y )*2 + x] = Y[1][y][x] - s;
-            f[( 1*2 + y )*2 + x] = Y[1][y][x] + s;
+#           pragma unroll
+            for (int k=0; k<5; k++) u[k] = ( k - 2 >= -G && k - 2 <= G ) ? Y[k - 2 + G][y][x] : (T)0;
+            pass1d<T, SCHEME>( u, mono, coeff, opp0, f[( 0*2 + y )*2 + x], f[( 1*2 + y )*2 + x] );
          }
    }
    else
@@ -271,8 +358,19 @@ template <typename T>
 int launch_interpolate( T *cdata, T *fdata, int nbox, const InterpPrm &p, int *nfail, hipStream_t stream )
 {
    if ( nbox <= 0 ) return 0;
-   if ( p.scheme == 4 ) hipLaunchKernelGGL( ( interpolate_kernel<T, 4> ), dim3( nbox ), dim3( NT ), 0, stream, cdata, fdata, nbox, p, nfail );
-   else                 hipLaunchKernelGGL( ( interpolate_kernel<T, 2> ), dim3( nbox ), dim3( NT ), 0, stream, cdata, fdata, nbox, p, nfail );
+#  define GA_INT( S ) hipLaunchKernelGGL( ( interpolate_kernel<T, S> ), dim3( nbox ), dim3( NT ), 0, stream, cdata, fdata, nbox, p, nfail )
+   switch ( p.scheme )
+   {
+      case 1: GA_INT( 1 ); break;
+      case 2: GA_INT( 2 ); break;
+      case 3: GA_INT( 3 ); break;
+      case 4: GA_INT( 4 ); break;
+      case 5: GA_INT( 5 ); break;
+      case 6: GA_INT( 6 ); break;
+      case 7: GA_INT( 7 ); break;
+      default: return -1;
+   }
+#  undef GA_INT
    return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
 }
 

@@ -66,7 +66,7 @@ class HydroCorrectArgs(ctypes.Structure):
 
 
 FIRST_FLUX_CORR = {"NONE": 0, "3D": 1, "3D1D": 2}
-INT_SCHEME = {"MINMOD1D": 2, "CQUAD": 4}
+INT_SCHEME = {"MINMOD3D": 1, "MINMOD1D": 2, "VANLEER": 3, "CQUAD": 4, "QUAD": 5, "CQUAR": 6, "QUAR": 7}
 
 
 class HydroInterpArgs(ctypes.Structure):

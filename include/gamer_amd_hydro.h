@@ -149,9 +149,10 @@ int hydro_close_patches_async( const void *pg_out, void *patch, int npg, const i
 /* ---- Ghost-zone preparation at refinement boundaries ---- */
 
 /* Coarse -> fine interpolation of a batch of equally shaped boxes: Interpolate() with Interpolate_Iterate
-   (src/Interpolation/Interpolate.cpp:90-330) for INT_MINMOD1D (Int_MinMod1D.cpp) and INT_CQUAD
-   (Int_CQuadratic.cpp).  Boxes are cdata[nbox][5][csize[2]][csize[1]][csize[0]] and fdata[nbox][5][fsize...];
-   coarse cells cstart..cstart+crange-1 (one coarse ghost cell needed on each side) fill fine cells
+   (src/Interpolation/Interpolate.cpp:90-330) for INT_MINMOD3D, INT_MINMOD1D, INT_VANLEER, INT_CQUAD, INT_QUAD,
+   INT_CQUAR and INT_QUAR (src/Interpolation/Int_*.cpp).  Boxes are cdata[nbox][5][csize[2]][csize[1]][csize[0]]
+   and fdata[nbox][5][fsize...]; coarse cells cstart..cstart+crange-1 (1 coarse ghost cell needed on each side,
+   2 for INT_CQUAR / INT_QUAR: Int_Table.cpp) fill fine cells
    fstart..fstart+2*crange-1.  With int_prim (OPT__INT_PRIM) or reduce_mono the result is checked and failing
    2x2x2 blocks are redone in primitive variables, then with lower monotonic coefficients (MONO_MAX_ITER); the
    primitive retry converts cdata in place, as the reference does.  *nfail += number of boxes still unphysical. */
@@ -159,7 +160,8 @@ typedef struct HydroInterpArgs {
    void  *cdata, *fdata;
    int    nbox;
    int    csize[3], cstart[3], crange[3], fsize[3], fstart[3];
-   int    scheme;              /* 2 = INT_MINMOD1D, 4 = INT_CQUAD (include/Typedef.h:173-179) */
+   int    scheme;              /* 1 INT_MINMOD3D, 2 INT_MINMOD1D, 3 INT_VANLEER, 4 INT_CQUAD, 5 INT_QUAD, 6 INT_CQUAR,
+                                  7 INT_QUAR (include/Typedef.h:173-179); reduce_mono is rejected for 1-3 */
    int    monotonic;           /* bit v set: Monotonic[v] */
    double mono_coeff;          /* INT_MONO_COEFF */
    int    opp_sign_0th;        /* OppSign0thOrder */

@@ -906,42 +906,140 @@ template <typename T> T cquad_slope( T L, T C, T R, bool mono, T coeff, bool opp
 
 struct Box { int cs[3], c0[3], cr[3], fs[3], f0[3]; };
 
-// Int_CQuadratic (src/Interpolation/Int_CQuadratic.cpp): x pass over the coarse rows (with one ghost row in y and
-// z), then y, then z, each producing two fine values C -/+ slope
+// interpolation scheme ids (include/Typedef.h:171-180) and their coarse ghost widths (src/Interpolation/Int_Table.cpp)
+enum { INT_MINMOD3D = 1, INT_MINMOD1D = 2, INT_VANLEER = 3, INT_CQUAD = 4, INT_QUAD = 5, INT_CQUAR = 6, INT_QUAR = 7 };
+inline int int_ghost( int scheme ) { return ( scheme == INT_CQUAR || scheme == INT_QUAR ) ? 2 : 1; }
+
+// One split pass of the tensor-product schemes: the stencil u = {L2, L1, C, R1, R2} along one direction -> the two
+// fine values (lo, hi) of the centre cell.  The per-scheme bodies follow the x passes of
+//   Int_MinMod3D.cpp:93-103   Int_vanLeer.cpp:93-105   Int_CQuadratic.cpp (cquad_slope above)
+//   Int_Quadratic.cpp:104-160  Int_CQuartic.cpp:100-140  Int_Quartic.cpp:100-165
+// (the y and z passes of each file repeat the x pass on the previous pass's output).
 template <typename T>
-void int_cquad( const T *C, T *F, const Box &b, int mono_mask, T coeff, bool opp0 )
+void pass1d( int scheme, const T u[5], bool mono, T coeff, bool opp0, T &lo, T &hi )
 {
+   const T L2 = u[0], L1 = u[1], C = u[2], R1 = u[3], R2 = u[4];
+   T s;
+   switch ( scheme )
+   {
+      case INT_MINMOD3D: case INT_VANLEER:
+      {
+         const T l = C - L1, r = R1 - C;
+         if ( r*l <= (T)0.0 ) s = (T)0.0;
+         else s = ( scheme == INT_MINMOD3D ) ? (T)0.25*( std::fabs( r ) < std::fabs( l ) ? r : l )
+                                             : (T)0.5*l*r/( l + r );
+         if ( opp0 && L1*R1 < (T)0.0 ) s = (T)0.0;
+         lo = C - s;  hi = C + s;
+         return;
+      }
+      case INT_CQUAD:
+         s = cquad_slope( L1, C, R1, mono, coeff, opp0 );
+         lo = C - s;  hi = C + s;
+         return;
+      case INT_CQUAR:
+      {
+         s = (T)( 3.0/128.0)*L2 + (T)(-22.0/128.0)*L1 + (T)(22.0/128.0)*R1 + (T)(-3.0/128.0)*R2;
+         if ( mono )
+         {
+            T l = (T)0.25*( C - L1 ), r = (T)0.25*( R1 - C );
+            if ( l*r > (T)0.0 )
+            {
+               if ( s*l < (T)0.0 ) s = (T)0.125*( R1 - L1 );
+               const T sg = sgn( l );
+               s *= sg;  l *= sg;  r *= sg;
+               if ( l < r ) l *= coeff; else r *= coeff;
+               s = vmin( l, s );
+               s = vmin( r, s );
+               s *= sg;
+            }
+            else s = (T)0.0;
+         }
+         if ( opp0 && L1*R1 < (T)0.0 ) s = (T)0.0;
+         lo = C - s;  hi = C + s;
+         return;
+      }
+      default:          // INT_QUAD, INT_QUAR: Lagrange point values at -/+ 1/4, reverted to a limited slope if not monotonic
+      {
+         if ( scheme == INT_QUAD )
+         {
+            lo = (T)( 5.0/32.0)*L1 + (T)(30.0/32.0)*C + (T)(-3.0/32.0)*R1;
+            hi = (T)(-3.0/32.0)*L1 + (T)(30.0/32.0)*C + (T)( 5.0/32.0)*R1;
+         }
+         else
+         {
+            lo = (T)( -45.0/2048.0)*L2 + (T)( 420.0/2048.0)*L1 + (T)(1890.0/2048.0)*C + (T)(-252.0/2048.0)*R1
+               + (T)(  35.0/2048.0)*R2;
+            hi = (T)(  35.0/2048.0)*L2 + (T)(-252.0/2048.0)*L1 + (T)(1890.0/2048.0)*C + (T)( 420.0/2048.0)*R1
+               + (T)( -45.0/2048.0)*R2;
+         }
+         if ( mono )
+         {
+            T l = C - L1, r = R1 - C;
+            if ( l*r > (T)0.0 )
+            {
+               const T cmax = ( l > (T)0.0 ) ? R1 : L1, cmin = ( l > (T)0.0 ) ? L1 : R1;
+               bool bad = ( vmax( lo, hi ) > cmax || vmin( lo, hi ) < cmin );
+               if ( scheme == INT_QUAR ) bad = bad || ( hi - lo )*l < (T)0.0;
+               if ( bad )
+               {
+                  const T c4 = (T)0.25*coeff;
+                  s  = (T)0.125*( R1 - L1 );
+                  l *= c4;  r *= c4;
+                  const T sg = sgn( l );
+                  s *= sg;
+                  s  = vmin( sg*l, s );
+                  s  = vmin( sg*r, s );
+                  s *= sg;
+                  lo = C - s;  hi = C + s;
+               }
+            }
+            else { lo = C;  hi = C; }
+         }
+         if ( opp0 && L1*R1 < (T)0.0 ) { lo = C;  hi = C; }
+         return;
+      }
+   }
+}
+
+// The tensor-product schemes (all but INT_MINMOD1D): x pass over the coarse rows (with G ghost rows in y and z), then
+// y, then z, through the same temporaries as the reference (TDataX, TDataY)
+template <typename T>
+void int_split( const T *C, T *F, const Box &b, int scheme, int mono_mask, T coeff, bool opp0 )
+{
+   const int G = int_ghost( scheme );
    const int cdy = b.cs[0], cdz = b.cs[0]*b.cs[1], C3 = cdz*b.cs[2], F3 = b.fs[0]*b.fs[1]*b.fs[2];
-   const int nx = 2*b.cr[0], ny = b.cr[1] + 2, nz = b.cr[2] + 2;
-   std::vector<T> X( (size_t)nx*ny*nz ), Y( (size_t)nx*2*b.cr[1]*nz );
+   const int nx = 2*b.cr[0], ny = b.cr[1] + 2*G, nz = b.cr[2] + 2*G, my = 2*b.cr[1];
+   std::vector<T> X( (size_t)nx*ny*nz ), Y( (size_t)nx*my*nz );
    for (int v=0; v<NV; v++)
    {
       const T *c = C + (size_t)v*C3;
       T *f = F + (size_t)v*F3;
       const bool mono = ( mono_mask >> v ) & 1;
+      T u[5], lo, hi;
       for (int z=0; z<nz; z++) for (int y=0; y<ny; y++) for (int x=0; x<b.cr[0]; x++)
       {
-         const int i = ( b.c0[2] - 1 + z )*cdz + ( b.c0[1] - 1 + y )*cdy + b.c0[0] + x;
-         const T sl = cquad_slope( c[i-1], c[i], c[i+1], mono, coeff, opp0 );
-         X[ ( (size_t)z*ny + y )*nx + 2*x     ] = c[i] - sl;
-         X[ ( (size_t)z*ny + y )*nx + 2*x + 1 ] = c[i] + sl;
+         const int i = ( b.c0[2] - G + z )*cdz + ( b.c0[1] - G + y )*cdy + b.c0[0] + x;
+         for (int k=0; k<5; k++) u[k] = ( k - 2 >= -G && k - 2 <= G ) ? c[i + k - 2] : (T)0.0;
+         pass1d( scheme, u, mono, coeff, opp0, lo, hi );
+         X[ ( (size_t)z*ny + y )*nx + 2*x     ] = lo;
+         X[ ( (size_t)z*ny + y )*nx + 2*x + 1 ] = hi;
       }
       for (int z=0; z<nz; z++) for (int y=0; y<b.cr[1]; y++) for (int x=0; x<nx; x++)
       {
-         const T L = X[ ( (size_t)z*ny + y     )*nx + x ], Cc = X[ ( (size_t)z*ny + y + 1 )*nx + x ],
-                 R = X[ ( (size_t)z*ny + y + 2 )*nx + x ];
-         const T sl = cquad_slope( L, Cc, R, mono, coeff, opp0 );
-         Y[ ( (size_t)z*2*b.cr[1] + 2*y     )*nx + x ] = Cc - sl;
-         Y[ ( (size_t)z*2*b.cr[1] + 2*y + 1 )*nx + x ] = Cc + sl;
+         for (int k=0; k<5; k++)
+            u[k] = ( k - 2 >= -G && k - 2 <= G ) ? X[ ( (size_t)z*ny + y + G + k - 2 )*nx + x ] : (T)0.0;
+         pass1d( scheme, u, mono, coeff, opp0, lo, hi );
+         Y[ ( (size_t)z*my + 2*y     )*nx + x ] = lo;
+         Y[ ( (size_t)z*my + 2*y + 1 )*nx + x ] = hi;
       }
-      for (int z=0; z<b.cr[2]; z++) for (int y=0; y<2*b.cr[1]; y++) for (int x=0; x<nx; x++)
+      for (int z=0; z<b.cr[2]; z++) for (int y=0; y<my; y++) for (int x=0; x<nx; x++)
       {
-         const T L = Y[ ( (size_t)z*2*b.cr[1] + y )*nx + x ], Cc = Y[ ( (size_t)( z + 1 )*2*b.cr[1] + y )*nx + x ],
-                 R = Y[ ( (size_t)( z + 2 )*2*b.cr[1] + y )*nx + x ];
-         const T sl = cquad_slope( L, Cc, R, mono, coeff, opp0 );
+         for (int k=0; k<5; k++)
+            u[k] = ( k - 2 >= -G && k - 2 <= G ) ? Y[ ( (size_t)( z + G + k - 2 )*my + y )*nx + x ] : (T)0.0;
+         pass1d( scheme, u, mono, coeff, opp0, lo, hi );
          const int o = ( ( b.f0[2] + 2*z )*b.fs[1] + b.f0[1] + y )*b.fs[0] + b.f0[0] + x;
-         f[o]                       = Cc - sl;
-         f[o + b.fs[0]*b.fs[1]]     = Cc + sl;
+         f[o]                   = lo;
+         f[o + b.fs[0]*b.fs[1]] = hi;
       }
    }
 }
@@ -997,8 +1095,8 @@ int interpolate( T *C, T *F, const Box &b, int scheme, int mono_mask, double mon
    const Gas<T> g( gamma );
    const double inv_gm1 = 1.0/( gamma - 1.0 );
    auto run = [&]( T *out, T coeff ) {
-      if ( scheme == 4 ) int_cquad( C, out, b, mono_mask, coeff, opp0 );
-      else               int_minmod1d( C, out, b, opp0 );
+      if ( scheme == INT_MINMOD1D ) int_minmod1d( C, out, b, opp0 );
+      else                          int_split( C, out, b, scheme, mono_mask, coeff, opp0 );
    };
    if ( !reduce && !int_prim ) { run( F, (T)mono_coeff );  return 0; }
 
@@ -1023,7 +1121,8 @@ int interpolate( T *C, T *F, const Box &b, int scheme, int mono_mask, double mon
       }
       else if ( it > 0 )
       {
-         if ( scheme != 4 ) return 1;               // MINMOD1D has no monotonic coefficient to lower
+         // INT_MINMOD1D / INT_MINMOD3D / INT_VANLEER have no monotonic coefficient to lower (Interpolate.cpp:299-300)
+         if ( scheme == INT_MINMOD1D || scheme == INT_MINMOD3D || scheme == INT_VANLEER ) return 1;
          coeff -= (T)mono_coeff/(T)mono_max;
          coeff  = vmax( coeff, (T)0.0 );
       }

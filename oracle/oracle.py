@@ -216,7 +216,7 @@ def close_patches(pg_out, patch, pid0):
     return patch
 
 
-_INT = {"MINMOD1D": 2, "CQUAD": 4}
+_INT = {"MINMOD3D": 1, "MINMOD1D": 2, "VANLEER": 3, "CQUAD": 4, "QUAD": 5, "CQUAR": 6, "QUAR": 7}
 
 
 def interpolate(cdata, fdata, cstart, crange, fstart=(0, 0, 0), scheme="CQUAD", monotonic=0x1F, mono_coeff=2.0,

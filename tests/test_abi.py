@@ -81,3 +81,27 @@ def test_host_path_requires_allocation():
     rc = hydro.lib().hydro_asyn_fluid_solver(ctypes.cast(buf, ctypes.c_void_p), ctypes.cast(buf, ctypes.c_void_p),
                                              ctypes.cast(buf, ctypes.c_void_p), ctypes.byref(a), 1)
     assert rc == -2 and "hydro_mem_allocate_fluid" in hydro.lib().hydro_last_error().decode()
+
+
+def _interp_args(scheme, cstart, crange, csize, reduce_mono=0):
+    I3 = ctypes.c_int * 3
+    return hydro.HydroInterpArgs(16, 16, 1, I3(*csize), I3(*cstart), I3(*crange), I3(*(2 * r for r in crange)),
+                                 I3(0, 0, 0), scheme, 0x1F, 2.0, 0, 0, reduce_mono, 10, 5.0 / 3.0, 0.0,
+                                 hydro.PRECISION["fp64"], 16)
+
+
+@pytest.mark.parametrize("scheme,cstart,crange,csize,reduce_mono,frag", [
+    (8, (1, 1, 1), (4, 4, 4), (6, 6, 6), 0, "scheme"),
+    (0, (1, 1, 1), (4, 4, 4), (6, 6, 6), 0, "scheme"),
+    (3, (1, 1, 1), (4, 4, 4), (6, 6, 6), 1, "reduce_mono"),          # INT_VANLEER
+    (1, (1, 1, 1), (4, 4, 4), (6, 6, 6), 1, "reduce_mono"),          # INT_MINMOD3D
+    (6, (1, 2, 2), (4, 4, 4), (8, 8, 8), 0, "2 ghost cells"),        # INT_CQUAR needs two coarse ghosts
+    (7, (2, 2, 2), (4, 4, 5), (8, 8, 8), 0, "2 ghost cells"),        # INT_QUAR: upper side
+    (4, (0, 1, 1), (4, 4, 4), (6, 6, 6), 0, "1 ghost cells"),
+])
+def test_interpolate_rejects_bad_arguments(scheme, cstart, crange, csize, reduce_mono, frag):
+    """Interpolate.cpp:103-114 (ghost widths of Int_Table) and :299-300 (no coefficient to reduce) as errors."""
+    L = hydro.lib()
+    a = _interp_args(scheme, cstart, crange, csize, reduce_mono)
+    assert L.hydro_interpolate_async(ctypes.byref(a), None) == hydro.HYDRO_E_ARG
+    assert frag in L.hydro_last_error().decode()

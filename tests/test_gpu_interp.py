@@ -1,5 +1,5 @@
-"""GPU: coarse -> fine interpolation (INT_CQUAD, INT_MINMOD1D, with and without the iterative primitive /
-monotonic-coefficient retries) and the same-level patch gather, against the oracle restatement
+"""GPU: coarse -> fine interpolation (all seven schemes of src/Interpolation except INT_SPECTRAL, with and without the
+iterative primitive / monotonic-coefficient retries) and the same-level patch gather, against the oracle restatement
 (pinned in test_oracle_interp.py)."""
 import numpy as np
 import pytest
@@ -23,21 +23,28 @@ def _gas_boxes(rng, nbox, cs):
     return c
 
 
+CASES = [("CQUAD", True), ("CQUAD", False), ("MINMOD1D", False), ("MINMOD1D", "prim"),
+         ("MINMOD3D", False), ("MINMOD3D", "prim"), ("VANLEER", False), ("VANLEER", "prim"),
+         ("QUAD", True), ("QUAD", False), ("CQUAR", True), ("CQUAR", False), ("QUAR", True), ("QUAR", False)]
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("scheme,iterate", [("CQUAD", True), ("CQUAD", False), ("MINMOD1D", False),
-                                            ("MINMOD1D", "prim")])
+@pytest.mark.parametrize("scheme,iterate", CASES)
 def test_interpolate_matches_oracle(dtype, scheme, iterate):
+    """iterate: True = primitive retry + monotonic-coefficient reduction, "prim" = primitive retry only."""
     import oracle
     from gamer_amd import hydro
     print(f"[gpu] interpolate {scheme} iterate={iterate} {np.dtype(dtype).name}", flush=True)
     rng = np.random.default_rng(5)
-    cs, cr = (7, 6, 8), (5, 4, 6)
+    g = 2 if scheme in ("CQUAR", "QUAR") else 1
+    cr = (5, 4, 6)
+    cs = tuple(r + 2 * g for r in cr)
     c = _gas_boxes(rng, 12, cs).astype(dtype)
     f0 = rng.standard_normal((12, 5, 12, 8, 10)).astype(dtype)
     kw = dict(scheme=scheme, int_prim=bool(iterate), reduce_mono=(iterate is True), mono_coeff=2.0, mono_max_iter=10)
-    rc, rf, rn = oracle.interpolate(c, f0, (1, 1, 1), cr, **kw)
+    rc, rf, rn = oracle.interpolate(c, f0, (g,) * 3, cr, **kw)
     gc, gf = torch.from_numpy(c).to(DEV), torch.from_numpy(f0).to(DEV)
-    nf = hydro.interpolate(gc, gf, (1, 1, 1), cr, **kw)
+    nf = hydro.interpolate(gc, gf, (g,) * 3, cr, **kw)
     torch.cuda.synchronize()
     assert int(nf.item()) == rn
     tol = 1e-14 if dtype == np.float64 else 1e-5

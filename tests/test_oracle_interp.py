@@ -121,3 +121,110 @@ def test_prepare_patches_window_and_keep():
                     s = slot[p, Z // 8, Y // 8, X // 8]
                     exp = pg0[p, :, z, y, x] if s < 0 else patch[s, :, Z % 8, Y % 8, X % 8]
                     np.testing.assert_array_equal(got[p, :, z, y, x], exp)
+
+
+# ---- the other tensor-product schemes: INT_MINMOD3D, INT_VANLEER, INT_QUAD, INT_CQUAR, INT_QUAR ----
+
+def _ghost(scheme):
+    return 2 if scheme in ("CQUAR", "QUAR") else 1
+
+
+def _scheme_box(f, scheme, n=6):
+    """Coarse box with the scheme's ghost width around n^3 interpolated cells; returns (box, c0, cr)."""
+    g = _ghost(scheme)
+    cs = (n + 2 * g,) * 3
+    z, y, x = np.meshgrid(*(np.arange(m, dtype=np.float64) for m in cs), indexing="ij")
+    base = f(x - g + 1, y - g + 1, z - g + 1)          # interpolated coarse cells sit at coordinates 1..n
+    return np.stack([base + 10.0 * (v + 1) for v in range(5)])[None], (g,) * 3, (n,) * 3
+
+
+@pytest.mark.parametrize("scheme", ["MINMOD3D", "VANLEER", "CQUAD", "CQUAR", "MINMOD1D"])
+def test_conservative_schemes_keep_coarse_means(scheme):
+    rng = np.random.default_rng(11)
+    g = _ghost(scheme)
+    c = 5.0 + rng.standard_normal((2, 5, 6 + 2 * g, 7 + 2 * g, 8 + 2 * g))
+    _, f, _ = oracle.interpolate(c, np.zeros((2, 5, 12, 14, 16)), (g,) * 3, (8, 7, 6), scheme=scheme,
+                                 int_prim=False, reduce_mono=False)
+    mean = f.reshape(2, 5, 6, 2, 7, 2, 8, 2).mean(axis=(3, 5, 7))
+    np.testing.assert_allclose(mean, c[:, :, g:g + 6, g:g + 7, g:g + 8], rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("scheme", ["MINMOD3D", "VANLEER", "QUAD", "CQUAR", "QUAR"])
+def test_linear_fields_are_reproduced(scheme):
+    c, c0, cr = _scheme_box(lambda x, y, z: 0.5 * x - 0.25 * y + 0.125 * z, scheme)
+    _, f, nf = oracle.interpolate(c, np.zeros((1, 5, 12, 12, 12)), c0, cr, scheme=scheme, int_prim=False,
+                                  reduce_mono=False)
+    xf = _fine_centres(6)
+    zf, yf, xf3 = np.meshgrid(xf, xf, xf, indexing="ij")
+    for v in range(5):
+        np.testing.assert_allclose(f[0, v], 0.5 * xf3 - 0.25 * yf + 0.125 * zf + 10.0 * (v + 1), rtol=0, atol=1e-12)
+
+
+def test_quad_interpolates_quadratic_point_values():
+    q = lambda x, y, z: x ** 2 + 2.0 * y ** 2 - z ** 2 + 0.5 * x * y
+    c, c0, cr = _scheme_box(q, "QUAD")
+    _, f, _ = oracle.interpolate(c, np.zeros((1, 5, 12, 12, 12)), c0, cr, scheme="QUAD", monotonic=0,
+                                 int_prim=False, reduce_mono=False)
+    xf = _fine_centres(6)
+    zf, yf, xf3 = np.meshgrid(xf, xf, xf, indexing="ij")
+    np.testing.assert_allclose(f[0, 0], q(xf3, yf, zf) + 10.0, rtol=0, atol=1e-12)
+
+
+def test_quar_interpolates_quartic_point_values():
+    q = lambda x, y, z: 0.01 * x ** 4 - 0.05 * y ** 3 + z ** 2 + 0.1 * x * z ** 2
+    c, c0, cr = _scheme_box(q, "QUAR")
+    _, f, _ = oracle.interpolate(c, np.zeros((1, 5, 12, 12, 12)), c0, cr, scheme="QUAR", monotonic=0,
+                                 int_prim=False, reduce_mono=False)
+    xf = _fine_centres(6)
+    zf, yf, xf3 = np.meshgrid(xf, xf, xf, indexing="ij")
+    np.testing.assert_allclose(f[0, 0], q(xf3, yf, zf) + 10.0, rtol=0, atol=1e-11)
+
+
+def test_cquar_reproduces_quartic_cell_averages():
+    # cell averages of a^4 over [i-1/2, i+1/2] and over the fine halves [i-1/2, i], [i, i+1/2]
+    A = lambda i: ((i + 0.5) ** 5 - (i - 0.5) ** 5) / 5.0
+    c, c0, cr = _scheme_box(lambda x, y, z: 0.01 * A(x) + 0.02 * A(y) - 0.01 * A(z), "CQUAR")
+    _, f, _ = oracle.interpolate(c, np.zeros((1, 5, 12, 12, 12)), c0, cr, scheme="CQUAR", monotonic=0,
+                                 int_prim=False, reduce_mono=False)
+    lo = np.arange(12) * 0.5 + 0.5                       # fine cell [lo, lo + 1/2] in coarse units
+    fa = 2.0 * ((lo + 0.5) ** 5 - lo ** 5) / 5.0
+    zf, yf, xf = np.meshgrid(fa, fa, fa, indexing="ij")
+    np.testing.assert_allclose(f[0, 0], 0.01 * xf + 0.02 * yf - 0.01 * zf + 10.0, rtol=0, atol=1e-10)
+
+
+def _row_box(vals, g):
+    """A box whose field 0 varies along x only: vals along x, the same in every (y, z) row."""
+    nx = len(vals)
+    c = np.ones((1, 5, 2 * g + 1, 2 * g + 1, nx))
+    c[0, 0] = np.asarray(vals, dtype=np.float64)[None, None, :]
+    return c
+
+
+@pytest.mark.parametrize("scheme,expect", [("MINMOD3D", (0.75, 1.25)), ("VANLEER", (1.0 - 1.0 / 3.0, 1.0 + 1.0 / 3.0))])
+def test_slope_formulas_by_hand(scheme, expect):
+    # neighbours 0, 1, 3: left slope 1, right slope 2 -> minmod 1/4, van Leer 0.5*1*2/3
+    _, f, _ = oracle.interpolate(_row_box([0.0, 1.0, 3.0], 1), np.zeros((1, 5, 2, 2, 2)), (1, 1, 1), (1, 1, 1),
+                                 scheme=scheme, int_prim=False, reduce_mono=False)
+    np.testing.assert_allclose(f[0, 0, :, :, 0], expect[0], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(f[0, 0, :, :, 1], expect[1], rtol=0, atol=1e-15)
+
+
+def test_quartic_monotonic_order_check():
+    # L2 L1 C R1 R2 = -10 0 0.5 1 0: both quartic values lie inside [0, 1] but in the wrong order
+    # (hi - lo = -128/2048 < 0), which only INT_QUAR's extra test rejects; the limited slope is then
+    # min(0.125 (R1 - L1), coeff/4 (C - L1), coeff/4 (R1 - C)) = 0.125
+    c = _row_box([-10.0, 0.0, 0.5, 1.0, 0.0], 2)
+    kw = dict(int_prim=False, reduce_mono=False, mono_coeff=2.0)
+    _, f, _ = oracle.interpolate(c, np.zeros((1, 5, 2, 2, 2)), (2, 2, 2), (1, 1, 1), scheme="QUAR", **kw)
+    np.testing.assert_allclose(f[0, 0, :, :, 0], 0.375, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(f[0, 0, :, :, 1], 0.625, rtol=0, atol=1e-15)
+    _, fu, _ = oracle.interpolate(c, np.zeros((1, 5, 2, 2, 2)), (2, 2, 2), (1, 1, 1), scheme="QUAR", monotonic=0, **kw)
+    np.testing.assert_allclose(fu[0, 0, 0, 0], [1143.0 / 2048.0, 1015.0 / 2048.0], rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("scheme", ["MINMOD3D", "VANLEER", "QUAD", "CQUAR", "QUAR"])
+def test_limited_schemes_stay_in_range_at_a_jump(scheme):
+    c, c0, cr = _scheme_box(lambda x, y, z: np.where(x < 4, 1.0, 5.0), scheme)
+    _, f, _ = oracle.interpolate(c, np.zeros((1, 5, 12, 12, 12)), c0, cr, scheme=scheme, int_prim=False,
+                                 reduce_mono=False)
+    assert f[0, 0].min() >= 11.0 - 1e-14 and f[0, 0].max() <= 15.0 + 1e-14
