@@ -366,6 +366,49 @@ int hydro_prepare_patches_async( const void *patch, void *pg_in, int npg, const 
    return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "prepare_patches launch failed" );
 }
 
+int hydro_gather_boxes_async( const void *patch, void *box, int nbox, const int size[3], const int origin[3],
+                              const int *slot, const int tdim[3], int precision, void *stream )
+{
+   if ( nbox < 0 || size == nullptr || origin == nullptr || tdim == nullptr ||
+        ( nbox > 0 && ( patch == nullptr || box == nullptr || slot == nullptr ) ) )
+      return fail( HYDRO_E_ARG, "gather_boxes: bad pointers/size" );
+   BoxGeom g;
+   for (int d=0; d<3; d++)
+   {
+      if ( size[d] <= 0 || tdim[d] <= 0 || origin[d] < 0 || origin[d] + size[d] > 8*tdim[d] )
+         return fail( HYDRO_E_ARG, "gather_boxes: box [%d, %d) outside the %d-patch table in dim %d", origin[d],
+                      origin[d] + size[d], tdim[d], d );
+      g.size[d] = size[d];  g.origin[d] = origin[d];  g.tdim[d] = tdim[d];
+   }
+   int rc;
+   if      ( precision == HYDRO_FP64 )      rc = launch_gather_boxes<double>( (const double*)patch, (double*)box, nbox, g, slot, (hipStream_t)stream );
+   else if ( precision == HYDRO_FP32_BULK ) rc = launch_gather_boxes<float> ( (const float*)patch,  (float*)box,  nbox, g, slot, (hipStream_t)stream );
+   else return fail( HYDRO_E_ARG, "unsupported precision %d", precision );
+   return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "gather_boxes launch failed" );
+}
+
+int hydro_assemble_patch_groups_async( const void *patch, const void *fbox, void *pg_in, int npg, const int *slot,
+                                       int ghost, const int fb_size[3], const int fb_origin[3], int precision,
+                                       void *stream )
+{
+   if ( npg < 0 || fb_size == nullptr || fb_origin == nullptr ||
+        ( npg > 0 && ( patch == nullptr || fbox == nullptr || pg_in == nullptr || slot == nullptr ) ) )
+      return fail( HYDRO_E_ARG, "assemble_patch_groups: bad pointers/size" );
+   if ( ghost < 0 || ghost > 8 ) return fail( HYDRO_E_ARG, "assemble_patch_groups: ghost %d out of range 0..8", ghost );
+   BoxGeom fb;
+   for (int d=0; d<3; d++)
+   {
+      if ( fb_origin[d] < 0 || fb_origin[d] + 16 + 2*ghost > fb_size[d] )
+         return fail( HYDRO_E_ARG, "assemble_patch_groups: window outside the fine box in dim %d", d );
+      fb.size[d] = fb_size[d];  fb.origin[d] = fb_origin[d];  fb.tdim[d] = 4;
+   }
+   int rc;
+   if      ( precision == HYDRO_FP64 )      rc = launch_assemble<double>( (const double*)patch, (const double*)fbox, (double*)pg_in, npg, slot, ghost, fb, (hipStream_t)stream );
+   else if ( precision == HYDRO_FP32_BULK ) rc = launch_assemble<float> ( (const float*)patch,  (const float*)fbox,  (float*)pg_in,  npg, slot, ghost, fb, (hipStream_t)stream );
+   else return fail( HYDRO_E_ARG, "unsupported precision %d", precision );
+   return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "assemble_patch_groups launch failed" );
+}
+
 int hydro_mhm_solve_batch_async( const HydroSolveArgs *a, const double *pg_dt_dh, unsigned *queue_counter,
                                  int groups_per_cu, void *stream )
 {

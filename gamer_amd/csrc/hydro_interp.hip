@@ -14,6 +14,11 @@
 //                        at the granularity of 2x2x2 fine blocks.
 //   prepare_patches_kernel  the same-level copies: window cell (x,y,z) of a patch group from the patch in its
 //                        4x4x4 slot table (-1: no patch at this level, the cell keeps the interpolated value)
+//   gather_boxes_kernel  the coarse data of InterpolateGhostZone (src/Main/InterpolateGhostZone.cpp): box cell (x,y,z)
+//                        from the patch covering cell origin + (x,y,z) of the box's slot table (e.g. the father patch
+//                        and its 26 siblings)
+//   assemble_kernel      prepare_patches_kernel with the interpolated fine box as the source of the cells that have no
+//                        patch at this level
 
 #include "hydro_math.hpp"
 #include "hydro_kernels.hpp"
@@ -352,6 +357,53 @@ __global__ void prepare_patches_kernel( const IO *__restrict__ patch, IO *__rest
    }
 }
 
+// box[q][v][z][y][x] (size) = patch[ slot[q][Z/8][Y/8][X/8] ][v][Z%8][Y%8][X%8], (X,Y,Z) = origin + (x,y,z);
+// slot -1 keeps the box value
+template <typename IO>
+__global__ void gather_boxes_kernel( const IO *__restrict__ patch, IO *__restrict__ box, int nbox, BoxGeom g,
+                                     const int *__restrict__ slot )
+{
+   const int n3 = g.size[0]*g.size[1]*g.size[2], t3 = g.tdim[0]*g.tdim[1]*g.tdim[2];
+   const long total = (long)nbox*NV*n3;
+   for (long e = blockIdx.x*(long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x*blockDim.x)
+   {
+      const int c = (int)( e % n3 ), v = (int)( ( e/n3 ) % NV );
+      const long q = e/( (long)NV*n3 );
+      const int X = c % g.size[0] + g.origin[0], Y = ( c/g.size[0] ) % g.size[1] + g.origin[1],
+                Z = c/( g.size[0]*g.size[1] ) + g.origin[2];
+      const int s = slot[ q*t3 + ( Z/H*g.tdim[1] + Y/H )*g.tdim[0] + X/H ];
+      if ( s < 0 ) continue;
+      box[e] = patch[ ( (long)s*NV + v )*P3 + ( ( Z % H )*H + Y % H )*H + X % H ];
+   }
+}
+
+// patch-group input: same-level patch where slot[pg] >= 0 (as prepare_patches_kernel), else the interpolated fine box
+// fbox[pg][v][fz][fy][fx] (size fb.size) at window cell + fb.origin
+template <typename IO>
+__global__ void assemble_kernel( const IO *__restrict__ patch, const IO *__restrict__ fbox, IO *__restrict__ pg_in,
+                                 int npg, const int *__restrict__ slot, int ghost, BoxGeom fb )
+{
+   const int n = PS + 2*ghost, n3 = n*n*n, f3 = fb.size[0]*fb.size[1]*fb.size[2];
+   const long total = (long)npg*NV*n3;
+   for (long e = blockIdx.x*(long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x*blockDim.x)
+   {
+      const int c = (int)( e % n3 ), v = (int)( ( e/n3 ) % NV );
+      const long p = e/( (long)NV*n3 );
+      const int x = c % n, y = ( c/n ) % n, z = c/( n*n );
+      const int X = x - ghost + H, Y = y - ghost + H, Z = z - ghost + H;
+      const int s = slot[ ( ( p*4 + Z/H )*4 + Y/H )*4 + X/H ];
+      if ( s >= 0 ) pg_in[e] = patch[ ( (long)s*NV + v )*P3 + ( ( Z % H )*H + Y % H )*H + X % H ];
+      else          pg_in[e] = fbox[ ( p*NV + v )*f3 + ( ( z + fb.origin[2] )*fb.size[1] + y + fb.origin[1] )*fb.size[0]
+                                     + x + fb.origin[0] ];
+   }
+}
+
+static long copy_blocks( long total )
+{
+   long b = ( total + 255 )/256;
+   return b > 65536 ? 65536 : ( b < 1 ? 1 : b );
+}
+
 } // namespace
 
 template <typename T>
@@ -385,6 +437,29 @@ int launch_prepare_patches( const IO *patch, IO *pg_in, int npg, const int *slot
    return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
 }
 
+template <typename IO>
+int launch_gather_boxes( const IO *patch, IO *box, int nbox, const BoxGeom &g, const int *slot, hipStream_t stream )
+{
+   if ( nbox <= 0 ) return 0;
+   const long total = (long)nbox*NV*g.size[0]*g.size[1]*g.size[2];
+   hipLaunchKernelGGL( gather_boxes_kernel<IO>, dim3( (int)copy_blocks( total ) ), dim3( 256 ), 0, stream, patch, box, nbox, g, slot );
+   return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
+}
+
+template <typename IO>
+int launch_assemble( const IO *patch, const IO *fbox, IO *pg_in, int npg, const int *slot, int ghost, const BoxGeom &fb,
+                     hipStream_t stream )
+{
+   if ( npg <= 0 ) return 0;
+   const long total = (long)npg*NV*( PS + 2*ghost )*( PS + 2*ghost )*( PS + 2*ghost );
+   hipLaunchKernelGGL( assemble_kernel<IO>, dim3( (int)copy_blocks( total ) ), dim3( 256 ), 0, stream, patch, fbox, pg_in, npg, slot, ghost, fb );
+   return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
+}
+
+template int launch_gather_boxes<double>( const double*, double*, int, const BoxGeom&, const int*, hipStream_t );
+template int launch_gather_boxes<float> ( const float*,  float*,  int, const BoxGeom&, const int*, hipStream_t );
+template int launch_assemble<double>( const double*, const double*, double*, int, const int*, int, const BoxGeom&, hipStream_t );
+template int launch_assemble<float> ( const float*,  const float*,  float*,  int, const int*, int, const BoxGeom&, hipStream_t );
 template int launch_interpolate<double>( double*, double*, int, const InterpPrm&, int*, hipStream_t );
 template int launch_interpolate<float> ( float*,  float*,  int, const InterpPrm&, int*, hipStream_t );
 template int launch_prepare_patches<double>( const double*, double*, int, const int*, int, hipStream_t );

@@ -60,7 +60,7 @@ int launch_close_patches( const IO *pg_out, IO *patch, int npg, const int *pid0,
 constexpr int INTERP_MAX_CELLS = 4096;      // coarse cells per box (CRange product)
 struct InterpPrm {
    int    cs[3], c0[3], cr[3], fs[3], f0[3];   // CSize, CStart, CRange, FSize, FStart
-   int    scheme;                             // 2 = INT_MINMOD1D, 4 = INT_CQUAD
+   int    scheme;                             // IntScheme_t 1-7 (INT_MINMOD3D ... INT_QUAR)
    int    mono;                               // bit v: Monotonic[v]
    int    opp0, int_prim, reduce, mono_max_iter;
    double mono_coeff, gm1, inv_gm1, min_pres;
@@ -70,6 +70,13 @@ int launch_interpolate( T *cdata, T *fdata, int nbox, const InterpPrm &p, int *n
 // same-level part of the patch-group input preparation: slot[npg][4][4][4] of patch indices (-1 = keep)
 template <typename IO>
 int launch_prepare_patches( const IO *patch, IO *pg_in, int npg, const int *slot, int ghost, hipStream_t stream );
+// a box of `size` cells at `origin` inside a tdim[0] x tdim[1] x tdim[2] table of 8^3 patches
+struct BoxGeom { int size[3], origin[3], tdim[3]; };
+template <typename IO>
+int launch_gather_boxes( const IO *patch, IO *box, int nbox, const BoxGeom &g, const int *slot, hipStream_t stream );
+template <typename IO>
+int launch_assemble( const IO *patch, const IO *fbox, IO *pg_in, int npg, const int *slot, int ghost, const BoxGeom &fb,
+                     hipStream_t stream );
 
 // MHM_RP step, FP64: adjoint lam_in = J^T lam_out, and tangent-linear tout = J tin (+ out = U_out; tin == NULL:
 // plain phase-wise forward).  ws: adjoint_workspace_bytes(npg) of device memory.

@@ -35,7 +35,7 @@ EXPORTS = ("hydro_init", "hydro_mhm_solve_async", "hydro_synchronize", "hydro_la
            "hydro_mhm_tangent_async", "hydro_prepare_uniform_adjoint_async", "hydro_correct_unphysical_async",
            "hydro_store_flux_async", "hydro_correct_flux_async", "hydro_fixup_flux_async",
            "hydro_fixup_restrict_async", "hydro_close_patches_async", "hydro_interpolate_async",
-           "hydro_prepare_patches_async")
+           "hydro_prepare_patches_async", "hydro_gather_boxes_async", "hydro_assemble_patch_groups_async")
 
 
 class HydroError(RuntimeError):
@@ -121,6 +121,11 @@ def lib():
         L.hydro_interpolate_async.argtypes = [ctypes.POINTER(HydroInterpArgs), ctypes.c_void_p]
         L.hydro_prepare_patches_async.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                                   ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+        I3p = ctypes.POINTER(ctypes.c_int * 3)
+        L.hydro_gather_boxes_async.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, I3p, I3p,
+                                               ctypes.c_void_p, I3p, ctypes.c_int, ctypes.c_void_p]
+        L.hydro_assemble_patch_groups_async.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_void_p,
+                                                        ctypes.c_int, I3p, I3p, ctypes.c_int, ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -430,6 +435,58 @@ def prepare_patches(patch, pg_in, slot, ghost, stream=None):
                                              slot.data_ptr(), int(ghost), _prec(pg_in),
                                              _stream(pg_in.device, stream)))
     return pg_in
+
+
+def gather_boxes(patch, box, slot, origin, stream=None):
+    """Coarse boxes (nbox, 5, sz, sy, sx) from patches (npatch, 5, 8, 8, 8), in place: box cell (x,y,z) comes from the
+    patch covering cell origin + (x,y,z) of the box's slot table slot (nbox, tz, ty, tx) (-1 = keep).  (x,y,z) tuples."""
+    nbox = box.shape[0]
+    tdim = tuple(reversed(slot.shape[1:]))
+    _int_table(slot, (nbox,) + tuple(slot.shape[1:]), "slot")
+    if slot.dim() != 4:
+        raise HydroError("gather_boxes: slot must be (nbox, tz, ty, tx)")
+    I3 = ctypes.c_int * 3
+    _check(lib().hydro_gather_boxes_async(patch.contiguous().data_ptr(), box.data_ptr(), nbox,
+                                          ctypes.byref(I3(*reversed(box.shape[2:]))), ctypes.byref(I3(*origin)),
+                                          slot.data_ptr(), ctypes.byref(I3(*tdim)), _prec(box),
+                                          _stream(box.device, stream)))
+    return box
+
+
+def assemble_patch_groups(patch, fbox, pg_in, slot, ghost, fb_origin, stream=None):
+    """Patch-group inputs: same-level patches where slot (npg, 4, 4, 4) >= 0, else the interpolated fine boxes
+    fbox (npg, 5, fz, fy, fx) at window cell + fb_origin (x, y, z)."""
+    npg = pg_in.shape[0]
+    _int_table(slot, (npg, 4, 4, 4), "slot")
+    if fbox.shape[0] != npg or fbox.dtype != pg_in.dtype or not fbox.is_contiguous():
+        raise HydroError("assemble_patch_groups: fbox must be a contiguous (npg, 5, fz, fy, fx) tensor of pg_in's dtype")
+    I3 = ctypes.c_int * 3
+    _check(lib().hydro_assemble_patch_groups_async(patch.contiguous().data_ptr(), fbox.data_ptr(), pg_in.data_ptr(),
+                                                   npg, slot.data_ptr(), int(ghost),
+                                                   ctypes.byref(I3(*reversed(fbox.shape[2:]))),
+                                                   ctypes.byref(I3(*fb_origin)), _prec(pg_in),
+                                                   _stream(pg_in.device, stream)))
+    return pg_in
+
+
+def prepare_refined_patch_groups(coarse_patch, fine_patch, coarse_slot, fine_slot, pg_in, ghost=3, scheme="CQUAD",
+                                 stream=None, **interp_kw):
+    """Prepare_PatchData for fine patch groups at a refinement boundary, on the device: the coarse box of each patch
+    group (father patch + 26 siblings: coarse_slot (npg, 3, 3, 3)) -> Interpolate() over coarse cells -2..9 around the
+    father (the 22^3 window of ghost 3 plus one fine cell) -> same-level fine patches (fine_slot (npg, 4, 4, 4)) and
+    the interpolated cells into pg_in (npg, 5, 16+2g, ...).  Returns the interpolation failure counter."""
+    import torch
+    if ghost != 3:
+        raise HydroError("prepare_refined_patch_groups: ghost 3 (the MHM_RP window) only")
+    g = 2 if scheme in ("CQUAR", "QUAR") else 1
+    npg = pg_in.shape[0]
+    n = 12 + 2 * g
+    cbox = torch.empty((npg, 5, n, n, n), dtype=pg_in.dtype, device=pg_in.device)
+    gather_boxes(coarse_patch, cbox, coarse_slot, (6 - g,) * 3, stream=stream)
+    fbox = torch.empty((npg, 5, 24, 24, 24), dtype=pg_in.dtype, device=pg_in.device)
+    nfail = interpolate(cbox, fbox, (g,) * 3, (12, 12, 12), scheme=scheme, stream=stream, **interp_kw)
+    assemble_patch_groups(fine_patch, fbox, pg_in, fine_slot, ghost, (1, 1, 1), stream=stream)
+    return nfail
 
 
 class UniformLevel:

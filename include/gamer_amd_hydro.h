@@ -181,6 +181,23 @@ int hydro_interpolate_async( const HydroInterpArgs *args, void *stream );
 int hydro_prepare_patches_async( const void *patch, void *pg_in, int npg, const int *slot, int ghost, int precision,
                                  void *stream );
 
+/* Coarse data of InterpolateGhostZone (src/Main/InterpolateGhostZone.cpp, the CData it hands to Interpolate()):
+   box[nbox][5][size[2]][size[1]][size[0]], cell (x,y,z) of box q taken from the patch that covers cell
+   origin + (x,y,z) of the box's table slot[q][tdim[2]][tdim[1]][tdim[0]] of patch indices into patch[][5][8^3]
+   (-1: cell left unchanged).  For a fine patch group the table is its father patch and the 26 siblings
+   (tdim = 3) and the box is coarse cells -2-G .. 9+G around the father (size 12+2G, origin 6-G), G the
+   scheme's coarse ghost width. */
+int hydro_gather_boxes_async( const void *patch, void *box, int nbox, const int size[3], const int origin[3],
+                              const int *slot, const int tdim[3], int precision, void *stream );
+
+/* Patch-group inputs at a refinement boundary (Prepare_PatchData, src/Main/Prepare_PatchData.cpp:166): cells
+   with a same-level patch in slot[npg][4][4][4] copy it, as hydro_prepare_patches_async; the others take the
+   interpolated fine box fbox[npg][5][fb_size...] at window cell + fb_origin (for the box above, interpolated
+   over coarse cells G .. 11+G: fb_size 24, fb_origin 1 with ghost 3). */
+int hydro_assemble_patch_groups_async( const void *patch, const void *fbox, void *pg_in, int npg, const int *slot,
+                                       int ghost, const int fb_size[3], const int fb_origin[3], int precision,
+                                       void *stream );
+
 /* ---- Differentiation of one MHM_RP step (HYDRO_FP64 only) ----
    The step is the map in[npg][5][22^3] -> out[npg][5][16^3] of hydro_mhm_solve_async with the same arguments
    (dt, dh constant; the MinMod retry is not differentiated).  No counterpart in the reference, which has no

@@ -1178,6 +1178,45 @@ void prepare_patches( const T *patch, T *pg_in, int npg, const int *slot, int gh
 
 } // namespace
 
+// the coarse boxes of InterpolateGhostZone: box cell (x,y,z) <- patch covering cell origin + (x,y,z) of the box's
+// tdim table (-1 keeps the box value)
+template <typename T>
+void gather_boxes( const T *patch, T *box, int nbox, const int size[3], const int origin[3], const int *slot,
+                   const int tdim[3] )
+{
+   const int n3 = size[0]*size[1]*size[2], t3 = tdim[0]*tdim[1]*tdim[2];
+   for (int q=0; q<nbox; q++)
+   for (int z=0; z<size[2]; z++) for (int y=0; y<size[1]; y++) for (int x=0; x<size[0]; x++)
+   {
+      const int X = origin[0] + x, Y = origin[1] + y, Z = origin[2] + z;
+      const int s = slot[ (size_t)q*t3 + ( Z/8*tdim[1] + Y/8 )*tdim[0] + X/8 ];
+      if ( s < 0 ) continue;
+      for (int v=0; v<NV; v++)
+         box[ ( (size_t)q*NV + v )*n3 + ( z*size[1] + y )*size[0] + x ] =
+            patch[ ( (size_t)s*NV + v )*512 + ( ( Z % 8 )*8 + Y % 8 )*8 + X % 8 ];
+   }
+}
+
+// patch-group input at a refinement boundary: same-level patch where the 4x4x4 slot is >= 0, else the interpolated
+// fine box at window cell + fb_origin
+template <typename T>
+void assemble_patch_groups( const T *patch, const T *fbox, T *pg_in, int npg, const int *slot, int ghost,
+                            const int fb_size[3], const int fb_origin[3] )
+{
+   const int n = 16 + 2*ghost, n3 = n*n*n, f3 = fb_size[0]*fb_size[1]*fb_size[2];
+   for (int p=0; p<npg; p++)
+   for (int z=0; z<n; z++) for (int y=0; y<n; y++) for (int x=0; x<n; x++)
+   {
+      const int X = x - ghost + 8, Y = y - ghost + 8, Z = z - ghost + 8;
+      const int s = slot[ ( ( p*4 + Z/8 )*4 + Y/8 )*4 + X/8 ];
+      for (int v=0; v<NV; v++)
+         pg_in[ ( (size_t)p*NV + v )*n3 + ( z*n + y )*n + x ] = ( s >= 0 )
+            ? patch[ ( (size_t)s*NV + v )*512 + ( ( Z % 8 )*8 + Y % 8 )*8 + X % 8 ]
+            : fbox[ ( (size_t)p*NV + v )*f3 + ( ( z + fb_origin[2] )*fb_size[1] + y + fb_origin[1] )*fb_size[0]
+                    + x + fb_origin[0] ];
+   }
+}
+
 extern "C" {
 
 int oracle_fluid_solver_f64( const double *in, double *out, double *flux, int npg, double dt, double dh,
@@ -1294,7 +1333,13 @@ int oracle_interpolate_##SUF( T *cdata, T *fdata, int nbox, const int csize[3], 
    return 0;                                                                                                           \
 }                                                                                                                      \
 int oracle_prepare_patches_##SUF( const T *patch, T *pg_in, int npg, const int *slot, int ghost )                    \
-{ prepare_patches( patch, pg_in, npg, slot, ghost );  return 0; }
+{ prepare_patches( patch, pg_in, npg, slot, ghost );  return 0; }                                                      \
+int oracle_gather_boxes_##SUF( const T *patch, T *box, int nbox, const int size[3], const int origin[3],             \
+                               const int *slot, const int tdim[3] )                                                   \
+{ gather_boxes( patch, box, nbox, size, origin, slot, tdim );  return 0; }                                           \
+int oracle_assemble_patch_groups_##SUF( const T *patch, const T *fbox, T *pg_in, int npg, const int *slot,          \
+                                        int ghost, const int fb_size[3], const int fb_origin[3] )                     \
+{ assemble_patch_groups( patch, fbox, pg_in, npg, slot, ghost, fb_size, fb_origin );  return 0; }
 
 ORC_INTERP_API( f64, double )
 ORC_INTERP_API( f32, float )

@@ -53,6 +53,8 @@ def lib():
             getattr(L, "oracle_interpolate_" + suf).argtypes = [_P, _P, _I, _P, _P, _P, _P, _P, _I, _I, _D, _I, _I,
                                                                   _I, _I, _D, _D, _P]
             getattr(L, "oracle_prepare_patches_" + suf).argtypes = [_P, _P, _I, _P, _I]
+            getattr(L, "oracle_gather_boxes_" + suf).argtypes = [_P, _P, _I, _P, _P, _P, _P]
+            getattr(L, "oracle_assemble_patch_groups_" + suf).argtypes = [_P, _P, _P, _I, _P, _I, _P, _P]
         _lib = L
     return _lib
 
@@ -237,4 +239,24 @@ def prepare_patches(patch, pg_in, slot, ghost):
     patch, pg_in, slot = np.ascontiguousarray(patch), np.array(pg_in, copy=True), _i32(slot)
     getattr(lib(), "oracle_prepare_patches_" + _suffix(pg_in))(_ptr(patch), _ptr(pg_in), pg_in.shape[0], _ptr(slot),
                                                                int(ghost))
+    return pg_in
+
+
+def gather_boxes(patch, box, slot, origin):
+    """Restatement of the coarse-box gather: returns a copy of box (nbox, 5, sz, sy, sx) filled from the patches;
+    slot (nbox, tz, ty, tx); origin (x, y, z)."""
+    patch, box, slot = np.ascontiguousarray(patch), np.array(box, copy=True), _i32(slot)
+    i3 = lambda t: np.ascontiguousarray(t, dtype=np.int32)
+    getattr(lib(), "oracle_gather_boxes_" + _suffix(box))(_ptr(patch), _ptr(box), box.shape[0],
+                                                          _ptr(i3(box.shape[:1:-1])), _ptr(i3(origin)), _ptr(slot),
+                                                          _ptr(i3(slot.shape[:0:-1])))
+    return box
+
+
+def assemble_patch_groups(patch, fbox, pg_in, slot, ghost, fb_origin):
+    patch, fbox, pg_in, slot = np.ascontiguousarray(patch), np.ascontiguousarray(fbox), np.array(pg_in, copy=True), _i32(slot)
+    i3 = lambda t: np.ascontiguousarray(t, dtype=np.int32)
+    getattr(lib(), "oracle_assemble_patch_groups_" + _suffix(pg_in))(
+        _ptr(patch), _ptr(fbox), _ptr(pg_in), pg_in.shape[0], _ptr(slot), int(ghost), _ptr(i3(fbox.shape[:1:-1])),
+        _ptr(i3(fb_origin)))
     return pg_in

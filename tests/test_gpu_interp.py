@@ -82,3 +82,39 @@ def test_prepare_patches_matches_oracle(ghost):
     hydro.prepare_patches(torch.from_numpy(patch).to(DEV), got, torch.from_numpy(slot).to(DEV), ghost)
     torch.cuda.synchronize()
     np.testing.assert_array_equal(got.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("scheme", ["CQUAD", "QUAR", "MINMOD1D"])
+def test_refined_patch_groups_on_device(dtype, scheme):
+    """Prepare_PatchData of 8 fine patch groups at a refinement boundary entirely on the device (coarse boxes from the
+    father + 26 siblings -> Interpolate -> same-level fine patches), against the oracle chain; then one MHM_RP step
+    of the assembled inputs against the oracle solver."""
+    import oracle
+    import amr_mesh as M
+    from gamer_amd import hydro
+    print(f"[gpu] refined patch groups {scheme} {np.dtype(dtype).name}", flush=True)
+    rng = np.random.default_rng(15)
+    cpatch = M.patches_of(M.gas((32, 32, 32), rng, smooth=False)).astype(dtype)
+    fpatch = M.patches_of(M.gas((32, 32, 32), rng, smooth=False)).astype(dtype)
+    cslot, fslot = M.coarse_slots(), M.fine_slots()
+    npg, g = len(M.fathers()), (2 if scheme in ("CQUAR", "QUAR") else 1)
+    kw = dict(int_prim=True, reduce_mono=(scheme != "MINMOD1D"))
+    cbox = oracle.gather_boxes(cpatch, np.zeros((npg, 5) + (12 + 2 * g,) * 3, dtype), cslot, (6 - g,) * 3)
+    _, fbox, rn = oracle.interpolate(cbox, np.zeros((npg, 5, 24, 24, 24), dtype), (g,) * 3, (12, 12, 12), scheme=scheme,
+                                     **kw)
+    ref = oracle.assemble_patch_groups(fpatch, fbox, np.zeros((npg, 5, 22, 22, 22), dtype), fslot, 3, (1, 1, 1))
+    pg = torch.zeros((npg, 5, 22, 22, 22), dtype=torch.from_numpy(cpatch).dtype, device=DEV)
+    nf = hydro.prepare_refined_patch_groups(torch.from_numpy(cpatch).to(DEV), torch.from_numpy(fpatch).to(DEV),
+                                            torch.from_numpy(cslot).to(DEV), torch.from_numpy(fslot).to(DEV), pg,
+                                            scheme=scheme, **kw)
+    torch.cuda.synchronize()
+    assert int(nf.item()) == rn
+    tol = 1e-14 if dtype == np.float64 else 1e-5
+    np.testing.assert_allclose(pg.cpu().numpy(), ref, rtol=tol, atol=tol)
+    if dtype == np.float64:
+        dt, dh = 2e-3, 1.0 / 64.0
+        out, _ = hydro.fluid_solver(pg, dt, dh)
+        rout, _ = oracle.fluid_solver(ref, dt, dh)
+        torch.cuda.synchronize()
+        np.testing.assert_allclose(out.cpu().numpy(), rout, rtol=1e-12, atol=1e-12)

@@ -228,3 +228,70 @@ def test_limited_schemes_stay_in_range_at_a_jump(scheme):
     _, f, _ = oracle.interpolate(c, np.zeros((1, 5, 12, 12, 12)), c0, cr, scheme=scheme, int_prim=False,
                                  reduce_mono=False)
     assert f[0, 0].min() >= 11.0 - 1e-14 and f[0, 0].max() <= 15.0 + 1e-14
+
+
+# ---- coarse-box gather and patch-group assembly at a refinement boundary ----
+
+def test_gather_boxes_brute_force():
+    rng = np.random.default_rng(12)
+    patch = rng.standard_normal((30, 5, 8, 8, 8))
+    slot = rng.integers(-1, 30, size=(3, 2, 3, 3)).astype(np.int32)       # (nbox, tz, ty, tx)
+    box0 = rng.standard_normal((3, 5, 9, 10, 11))                          # (z, y, x) = 9, 10, 11
+    origin = (7, 5, 4)
+    got = oracle.gather_boxes(patch, box0, slot, origin)
+    for q in range(3):
+        for z in range(9):
+            for y in range(10):
+                for x in range(0, 11, 3):
+                    X, Y, Z = x + origin[0], y + origin[1], z + origin[2]
+                    s = slot[q, Z // 8, Y // 8, X // 8]
+                    exp = box0[q, :, z, y, x] if s < 0 else patch[s, :, Z % 8, Y % 8, X % 8]
+                    np.testing.assert_array_equal(got[q, :, z, y, x], exp)
+
+
+def test_assemble_takes_patches_then_the_fine_box():
+    rng = np.random.default_rng(13)
+    patch = rng.standard_normal((40, 5, 8, 8, 8))
+    slot = rng.integers(-1, 40, size=(2, 4, 4, 4)).astype(np.int32)
+    fbox = rng.standard_normal((2, 5, 24, 24, 24))
+    got = oracle.assemble_patch_groups(patch, fbox, np.zeros((2, 5, 22, 22, 22)), slot, 3, (1, 1, 1))
+    for p in range(2):
+        for z in range(22):
+            for y in range(0, 22, 7):
+                for x in range(22):
+                    X, Y, Z = x + 5, y + 5, z + 5
+                    s = slot[p, Z // 8, Y // 8, X // 8]
+                    exp = fbox[p, :, z + 1, y + 1, x + 1] if s < 0 else patch[s, :, Z % 8, Y % 8, X % 8]
+                    np.testing.assert_array_equal(got[p, :, z, y, x], exp)
+
+
+@pytest.mark.parametrize("scheme", ["CQUAD", "MINMOD3D", "CQUAR", "QUAR"])
+def test_refined_patch_group_geometry(scheme):
+    """On a linear coarse field (cell averages = centre values) every interpolated ghost cell of the assembled
+    22^3 window equals the linear function at its fine cell centre, and every same-level cell its fine patch: pins
+    the coarse-box origin (6-G), the 3x3x3 sibling table, the interpolated range and the 24^3 -> 22^3 offset."""
+    import amr_mesh as M
+    n = 8 * M.NPC
+    z, y, x = np.meshgrid(*(np.arange(n, dtype=np.float64) for _ in range(3)), indexing="ij")
+    lin = lambda X, Y, Z: 0.5 * X - 0.25 * Y + 0.125 * Z
+    grid = np.stack([lin(x, y, z) + 10.0 * (v + 1) for v in range(5)])
+    cpatch = M.patches_of(grid)
+    fpatch = np.random.default_rng(14).standard_normal((64, 5, 8, 8, 8))
+    g = 2 if scheme in ("CQUAR", "QUAR") else 1
+    npg = len(M.fathers())
+    cbox = oracle.gather_boxes(cpatch, np.zeros((npg, 5) + (12 + 2 * g,) * 3), M.coarse_slots(), (6 - g,) * 3)
+    _, fbox, _ = oracle.interpolate(cbox, np.zeros((npg, 5, 24, 24, 24)), (g,) * 3, (12, 12, 12), scheme=scheme,
+                                    int_prim=False, reduce_mono=False)
+    fslot = M.fine_slots()
+    pg = oracle.assemble_patch_groups(fpatch, fbox, np.zeros((npg, 5, 22, 22, 22)), fslot, 3, (1, 1, 1))
+    w = np.arange(22)
+    for q, (I, J, K) in enumerate(M.fathers()):
+        cx, cy, cz = ((16 * np.array([I, J, K])[:, None] + w[None, :] - 3) + 0.5) / 2.0 - 0.5   # coarse units
+        Zc, Yc, Xc = np.meshgrid(cz, cy, cx, indexing="ij")
+        W = (w - 3 + 8)
+        has = fslot[q][np.ix_(W // 8, W // 8, W // 8)] >= 0
+        for v in range(5):
+            np.testing.assert_allclose(pg[q, v][~has], (lin(Xc, Yc, Zc) + 10.0 * (v + 1))[~has], rtol=0, atol=1e-12)
+        s, Wm = fslot[q][np.ix_(W // 8, W // 8, W // 8)], W % 8
+        same = fpatch[s, 0, Wm[:, None, None], Wm[None, :, None], Wm[None, None, :]]
+        np.testing.assert_array_equal(pg[q, 0][has], same[has])
