@@ -220,6 +220,46 @@ def amr_batch(args):
     print(json.dumps(line))
 
 
+def scheme_compare(args):
+    """MHM_RP (Riemann-solver half step, 22^3 inputs) against MHM (Hancock half step, 20^3 inputs) on the 4096-PG
+    workload: kernel throughput and algorithmic %HBM per scheme and precision (parity: tests/test_gpu_parity.py)."""
+    import torch
+    from gamer_amd import hydro, ic
+    hydro.lib()
+    dev = torch.device("cuda:0")
+    n = args.npg_axis
+    cons, dt, dh = ic.random_sod_grid((n, n, n), seed=20260, sedov=True)
+    npg = n ** 3
+    cells = npg * CELLS_PER_PG
+    res = {"metric": "MHM_RP vs MHM patch-group solver throughput (HLLC, VL_GMINMOD, StoreFlux)",
+           "config": {"workload": f"NPG={npg} patch groups of the default bench workload", "npg": npg},
+           "unit": "cell-updates/s", "schemes": {}}
+    for scheme, ghost in (("MHM_RP", 3), ("MHM", 2)):
+        nx = 16 + 2 * ghost
+        entry = {}
+        for prec, tdt, elem in (("fp32", torch.float32, 4), ("fp64", torch.float64, 8)):
+            print(f"[schemes] {scheme} {prec}", file=sys.stderr, flush=True)
+            u = cut_on_device(cons, ghost, torch, dev, tdt)
+            out, flux = hydro.fluid_solver(u, dt, dh, scheme=scheme)
+            for _ in range(args.warmup):
+                hydro.fluid_solver(u, dt, dh, scheme=scheme, out=out, flux=flux)
+            torch.cuda.synchronize()
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ev0.record()
+            for _ in range(args.steps):
+                hydro.fluid_solver(u, dt, dh, scheme=scheme, out=out, flux=flux)
+            ev1.record()
+            torch.cuda.synchronize()
+            kms = ev0.elapsed_time(ev1) / args.steps
+            gbs = algorithmic_bytes_per_pg(elem, nx=nx) * npg / (kms * 1e-3) / 1e9
+            entry[prec] = {"kernel_ms": kms, "cell_updates_per_s": cells / (kms * 1e-3),
+                           "algorithmic_GBps": gbs, "frac_hbm": gbs / HBM_PEAK_GBS}
+            del u, out, flux
+            torch.cuda.empty_cache()
+        res["schemes"][scheme] = entry
+    print(json.dumps(res))
+
+
 def _field_scale(per_field, agg):
     """Per-field normalisers where the three momentum components share one scale (a component that is zero
     everywhere, e.g. m_z of a 2D shear layer, is measured against the momentum magnitude, not against 0)."""
@@ -249,6 +289,8 @@ def main():
                     help="run the Roe/HLLE/HLLC accuracy sweep on the KH workload instead of the throughput bench")
     ap.add_argument("--amr-batch", action="store_true",
                     help="time the uneven 3-level batch (one launch vs per-level launches, persistent, graph)")
+    ap.add_argument("--schemes", action="store_true",
+                    help="time MHM_RP against MHM (both precisions) on the default workload")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (3x), to scale rocprof byte counters")
     args = ap.parse_args()
@@ -263,6 +305,9 @@ def main():
         return
     if args.amr_batch:
         amr_batch(args)
+        return
+    if args.schemes:
+        scheme_compare(args)
         return
 
     import torch

@@ -308,6 +308,9 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
          const int o = t - 3;                    // output plane of this step
          const int ring = ( t + 1 ) & 1;         // u3 slot: input plane t-3 (read), then plane t-1 (written)
          __syncthreads();                        // [A] input plane t visible
+         // waves 0-2 carry the ghost half-step solves of this interval and set its length: with two workgroups per
+         // CU sharing the SIMDs, letting them issue ahead of wave 3 (and of waves in other intervals) shortens it
+         if ( tid < 192 ) __builtin_amdgcn_s_setprio( 3 ); else __builtin_amdgcn_s_setprio( 1 );
 
 #        pragma unroll
          for (int v=0; v<8; v++) { kU1[v] = kU[v];  oU1[v] = oU[v]; }
@@ -387,6 +390,7 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
             }
          }
          __syncthreads();                        // [H] half-step x/y fluxes visible; S.w plane r no longer read
+         __builtin_amdgcn_s_setprio( 0 );
 
          if ( t >= 1 && t <= 20 )
 #           pragma unroll
