@@ -160,3 +160,193 @@ def autotune(torch, batch: LevelBatch, groups=(1, 2, 3, 4), reps=5, cache=None, 
         except (OSError, ValueError):
             pass
     return res
+
+
+# ---------------------------------------------------------------------------------------------------------------
+# A refined level on top of a periodic base level, advanced as GAMER's EvolveLevel does for two levels
+# (src/Main/EvolveLevel.cpp: Flu_AdvanceDt of the base level, two substeps of the refined level, then
+# Flu_FixUp_Restrict (12-1) and Flu_FixUp_Flux (12-3)), with every per-cell operation on the device.
+# ---------------------------------------------------------------------------------------------------------------
+
+LOCAL_XYZ = ((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (0, 1, 1), (1, 0, 1), (1, 1, 1))  # Table_02
+_LID = {o: l for l, o in enumerate(LOCAL_XYZ)}
+_FACE = ((-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1))                           # s = 0..5
+
+
+@dataclasses.dataclass
+class TwoLevelTables:
+    """Patch relations of a two-level mesh (host int32 arrays; what GAMER's AMR tree provides).
+
+    Base level: ``ncp`` = (nx, ny, nz) patches, periodic, stored by patch group: patch 8*pg + LocalID, patch group
+    pg = (PZ*npgy + PY)*npgx + PX covering patches (2PX + a, 2PY + b, 2PZ + c).  Refined level: one patch group per
+    refined base patch (``fathers``, (x, y, z) patch coordinates), its 8 patches at 8*q + LocalID."""
+    ncp: tuple
+    fathers: list
+    base_pg_slot: np.ndarray        # (npg_b, 4, 4, 4)  base patches around each base patch group (all present)
+    fine_coarse_slot: np.ndarray    # (nf, 3, 3, 3)     father + 26 siblings of each refined patch group
+    fine_pg_slot: np.ndarray        # (nf, 4, 4, 4)     refined-level patches around each refined patch group, -1 = none
+    base_store_slot: np.ndarray     # (npg_b, 8, 6)     StoreFlux register of each base patch face, -1 = none
+    fine_correct_slot: np.ndarray   # (nf, 6)           CorrectFlux register of each refined patch-group face, -1 = none
+    fixup_slot: np.ndarray          # (npatch_b, 6)     Flu_FixUp_Flux register of each base patch face, -1 = none
+    son0: np.ndarray                # (nf,)             first son patch of each father
+    fa_idx: np.ndarray              # (nf,)             base patch index of each father
+    base_pid0: np.ndarray           # (npg_b,)
+    nreg: int
+
+    @property
+    def npg_base(self):
+        return self.base_pg_slot.shape[0]
+
+
+def base_patch_index(ncp, i, j, k):
+    """Index of base patch (i, j, k) (periodic) in the patch-group ordering."""
+    nx, ny, nz = ncp
+    i, j, k = i % nx, j % ny, k % nz
+    pg = ((k // 2) * (ny // 2) + j // 2) * (nx // 2) + i // 2
+    return 8 * pg + _LID[(i % 2, j % 2, k % 2)]
+
+
+def build_two_level_tables(ncp, fathers) -> TwoLevelTables:
+    nx, ny, nz = ncp
+    if nx % 2 or ny % 2 or nz % 2:
+        raise ValueError("the base level needs an even number of patches per axis")
+    fathers = [tuple(int(c) for c in f) for f in fathers]
+    fidx = {f: q for q, f in enumerate(fathers)}
+    if len(fidx) != len(fathers):
+        raise ValueError("duplicate fathers")
+    npg = (nx // 2) * (ny // 2) * (nz // 2)
+    P = lambda i, j, k: base_patch_index(ncp, i, j, k)
+    base_pg_slot = np.empty((npg, 4, 4, 4), dtype=np.int32)
+    for PZ in range(nz // 2):
+        for PY in range(ny // 2):
+            for PX in range(nx // 2):
+                pg = (PZ * (ny // 2) + PY) * (nx // 2) + PX
+                for c in range(4):
+                    for b in range(4):
+                        for a in range(4):
+                            base_pg_slot[pg, c, b, a] = P(2 * PX - 1 + a, 2 * PY - 1 + b, 2 * PZ - 1 + c)
+    nf = len(fathers)
+    fine_coarse_slot = np.empty((nf, 3, 3, 3), dtype=np.int32)
+    fine_pg_slot = np.full((nf, 4, 4, 4), -1, dtype=np.int32)
+    refined = {f[0] % nx + nx * (f[1] % ny + ny * (f[2] % nz)) for f in fathers}
+    is_ref = lambda i, j, k: (i % nx + nx * (j % ny + ny * (k % nz))) in refined
+    for q, (I, J, K) in enumerate(fathers):
+        for c in range(3):
+            for b in range(3):
+                for a in range(3):
+                    fine_coarse_slot[q, c, b, a] = P(I + a - 1, J + b - 1, K + c - 1)
+        for c in range(4):
+            for b in range(4):
+                for a in range(4):
+                    fx, fy, fz = 2 * I - 1 + a, 2 * J - 1 + b, 2 * K - 1 + c          # refined patch coordinates
+                    fa = ((fx // 2) % nx, (fy // 2) % ny, (fz // 2) % nz)
+                    if fa in fidx:
+                        fine_pg_slot[q, c, b, a] = 8 * fidx[fa] + _LID[(fx % 2, fy % 2, fz % 2)]
+    # registers: (base patch, face) with the base patch not refined and its neighbour across the face refined
+    reg_of = {}
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                if is_ref(i, j, k):
+                    continue
+                for s, (dx, dy, dz) in enumerate(_FACE):
+                    if is_ref(i + dx, j + dy, k + dz):
+                        reg_of[(P(i, j, k), s)] = len(reg_of)
+    npatch = 8 * npg
+    base_store_slot = np.full((npg, 8, 6), -1, dtype=np.int32)
+    fixup_slot = np.full((npatch, 6), -1, dtype=np.int32)
+    for (p, s), r in reg_of.items():
+        base_store_slot[p // 8, p % 8, s] = r
+        fixup_slot[p, s] = r
+    fine_correct_slot = np.full((nf, 6), -1, dtype=np.int32)
+    for q, (I, J, K) in enumerate(fathers):
+        for s, (dx, dy, dz) in enumerate(_FACE):
+            if not is_ref(I + dx, J + dy, K + dz):
+                fine_correct_slot[q, s] = reg_of[(P(I + dx, J + dy, K + dz), s ^ 1)]
+    return TwoLevelTables(ncp=tuple(ncp), fathers=fathers, base_pg_slot=base_pg_slot, fine_coarse_slot=fine_coarse_slot,
+                          fine_pg_slot=fine_pg_slot, base_store_slot=base_store_slot,
+                          fine_correct_slot=fine_correct_slot, fixup_slot=fixup_slot,
+                          son0=(8 * np.arange(nf)).astype(np.int32),
+                          fa_idx=np.array([P(*f) for f in fathers], dtype=np.int32),
+                          base_pid0=(8 * np.arange(npg)).astype(np.int32), nreg=len(reg_of))
+
+
+def patches_from_grid(grid):
+    """(5, Z, Y, X) base-level array -> (npatch, 5, 8, 8, 8) patches in the patch-group ordering (numpy or torch)."""
+    _, Z, Y, X = grid.shape
+    nx, ny, nz = X // 8, Y // 8, Z // 8
+    g = grid.reshape(5, nz // 2, 2, 8, ny // 2, 2, 8, nx // 2, 2, 8)
+    # -> (PZ, PY, PX, c, b, a, v, z, y, x)
+    g = g.permute(1, 4, 7, 2, 5, 8, 0, 3, 6, 9) if hasattr(g, "permute") else g.transpose(1, 4, 7, 2, 5, 8, 0, 3, 6, 9)
+    perm = [c * 4 + b * 2 + a for (a, b, c) in LOCAL_XYZ]                 # LocalID -> (c, b, a) position
+    g = g.reshape(-1, 8, 5, 8, 8, 8)[:, perm]
+    return g.reshape(-1, 5, 8, 8, 8)
+
+
+def grid_from_patches(patch, ncp):
+    """Inverse of patches_from_grid."""
+    nx, ny, nz = ncp
+    inv = [0] * 8
+    for l, (a, b, c) in enumerate(LOCAL_XYZ):
+        inv[c * 4 + b * 2 + a] = l
+    g = patch.reshape(-1, 8, 5, 8, 8, 8)[:, inv].reshape(nz // 2, ny // 2, nx // 2, 2, 2, 2, 5, 8, 8, 8)
+    g = g.permute(6, 0, 3, 7, 1, 4, 8, 2, 5, 9) if hasattr(g, "permute") else g.transpose(6, 0, 3, 7, 1, 4, 8, 2, 5, 9)
+    return g.reshape(5, 8 * nz, 8 * ny, 8 * nx)
+
+
+class TwoLevel:
+    """A periodic base level with one refined level, advanced on the device (GAMER's EvolveLevel for two levels):
+
+    base step   prepare (same-level, periodic) -> MHM_RP solve (StoreFlux) -> CorrectUnphysical -> StoreFlux into FP64
+                registers -> copy back (Flu_AdvanceDt / Flu_Close, src/Fluid/Flu_Close.cpp:71-85);
+    2 substeps  coarse data at the substep's start time (linear in time between the old and new base level,
+                src/Main/Prepare_PatchData.cpp:2282-2283) -> coarse boxes -> Interpolate -> same-level patches ->
+                solve -> CorrectUnphysical -> CorrectFlux -> copy back;
+    sync        Flu_FixUp_Restrict, then Flu_FixUp_Flux (src/Main/EvolveLevel.cpp:757, 815).
+
+    base, fine: device patch arrays (npatch, 5, 8, 8, 8) in the orderings of ``TwoLevelTables``, updated in place."""
+
+    def __init__(self, torch, base, fine, tables: TwoLevelTables, dh_base, rsolver="HLLC", int_scheme="CQUAD",
+                 gamma=5.0 / 3.0, **solver_kw):
+        self.torch = torch
+        self.base, self.fine, self.tab = base, fine, tables
+        self.dh = float(dh_base)
+        self.rsolver, self.int_scheme, self.gamma, self.kw = rsolver, int_scheme, gamma, solver_kw
+        dev = base.device
+        T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+        self.t = {k: T(getattr(tables, k)) for k in ("base_pg_slot", "fine_coarse_slot", "fine_pg_slot",
+                                                      "base_store_slot", "fine_correct_slot", "fixup_slot", "son0",
+                                                      "fa_idx", "base_pid0")}
+        self.reg = torch.zeros((max(tables.nreg, 1), 5, 8, 8), dtype=torch.float64, device=dev)
+        nf = len(tables.fathers)
+        self.fine_pid0 = T((8 * np.arange(nf)).astype(np.int32))
+        self.counters = torch.zeros(2, dtype=torch.int32, device=dev)
+
+    def _close(self, pg_in, dt, dh):
+        out, flux = hydro.fluid_solver(pg_in, dt, dh, rsolver=self.rsolver, gamma=self.gamma, **self.kw)
+        self.counters += hydro.correct_unphysical(pg_in, out, flux, dt, dh, gamma=self.gamma)
+        return out, flux
+
+    def step(self, dt_base):
+        torch, t = self.torch, self.t
+        dh_f, dt_f = 0.5 * self.dh, 0.5 * dt_base
+        npg_b = self.tab.npg_base
+        pg_b = torch.empty((npg_b, 5, 22, 22, 22), dtype=self.base.dtype, device=self.base.device)
+        hydro.prepare_patches(self.base, pg_b, t["base_pg_slot"], 3)
+        out, flux = self._close(pg_b, dt_base, self.dh)
+        self.reg.zero_()
+        hydro.store_flux(flux, t["base_store_slot"], dt_base, self.reg)
+        old = self.base.clone()
+        hydro.close_patches(out, self.base, t["base_pid0"])
+        nf = len(self.tab.fathers)
+        for sub in range(2):
+            coarse = old if sub == 0 else 0.5 * old + 0.5 * self.base
+            pg_f = torch.empty((nf, 5, 22, 22, 22), dtype=self.fine.dtype, device=self.fine.device)
+            hydro.prepare_refined_patch_groups(coarse, self.fine, t["fine_coarse_slot"], t["fine_pg_slot"], pg_f,
+                                               scheme=self.int_scheme, gamma=self.gamma)
+            out_f, flux_f = self._close(pg_f, dt_f, dh_f)
+            hydro.correct_flux(flux_f, t["fine_correct_slot"], dt_f, self.reg)
+            hydro.close_patches(out_f, self.fine, self.fine_pid0)
+        hydro.fixup_restrict(self.fine, self.base, t["son0"], t["fa_idx"])
+        hydro.fixup_flux(self.base, t["fixup_slot"], self.reg, self.dh, gamma=self.gamma)
+        return self

@@ -92,7 +92,6 @@ mhm_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict_
    int oi, oj;
    const bool has_o = ring_column( tid, oi, oj );
    const int  oc = oi + NX*oj;
-   const bool orx = has_o && ( oi == 1 || oi == 18 );      // ring column on an x side: its x faces are needed
    const bool ofx = has_o && oi == 18;                      // owns the full-step x face (17,18)
    const bool ofy = has_o && oj == 18;
 
@@ -286,7 +285,6 @@ mhm_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict_
          if ( r >= 2 && r <= 17 )
 #           pragma unroll
             for (int v=0; v<NV; v++) kDf[v] = dfn[v];
-         (void)orx;
       } // for t
 
       if ( g_dt ) fused_dt( cfl, ( ki - 2 )/8 + 2*( ( kj - 2 )/8 ), S.cfl, g_dt + (size_t)pg*8, dh_safety );

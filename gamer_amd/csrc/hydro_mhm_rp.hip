@@ -25,7 +25,9 @@
 //   [H] x/y half-step flux differences of plane t, half-step plane t-1 into LDS
 //   [R] full-step fluxes of plane r (x/y) and at r-1/2 (z)
 //   [F] full-step update and output of plane t-3, input plane t+1 into LDS
-// All branches that contain a barrier depend only on t, so every barrier is reached by the whole workgroup.
+// [A] is the longest interval and waves 0-2 (ghost half-step solves) its critical path; they raise their issue
+// priority inside it.  All branches that contain a barrier depend only on t, so every barrier is reached by the
+// whole workgroup.
 
 #include "hydro_math.hpp"
 #include "hydro_kernels.hpp"

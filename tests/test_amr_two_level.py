@@ -1,0 +1,85 @@
+"""The two-level step (base level + one refined level, GAMER's EvolveLevel for two levels) composed from the oracle:
+
+* the patch tables of gamer_amd.amr.build_two_level_tables describe the mesh (every coarse-fine interface face has
+  one register, stored by its coarse patch, corrected by exactly one refined patch-group face, applied once);
+* on a periodic box the conserved totals of the base level are unchanged by a step to round-off: the flux fix-up
+  replaces each interface flux by the time-averaged fine fluxes, and restriction carries the fine totals down.
+  Without the fix-up the totals drift, so the test sees the flux registers at work.
+"""
+import numpy as np
+import pytest
+
+from gamer_amd import amr
+import amr_chain
+
+
+def test_tables_describe_the_mesh():
+    fathers = [(i, j, k) for k in (1, 2) for j in (1, 2) for i in (1, 2)] + [(3, 0, 0)]
+    t = amr.build_two_level_tables((4, 4, 4), fathers)
+    refined = {tuple(f) for f in fathers}
+    # patch indices: base_patch_index is a bijection onto 0..63
+    idx = {amr.base_patch_index((4, 4, 4), i, j, k) for i in range(4) for j in range(4) for k in range(4)}
+    assert idx == set(range(64))
+    # every register: stored once by its base patch, applied once by the fix-up, corrected by exactly one face
+    stored = t.base_store_slot[t.base_store_slot >= 0]
+    assert sorted(stored.tolist()) == list(range(t.nreg))
+    assert sorted(t.fixup_slot[t.fixup_slot >= 0].tolist()) == list(range(t.nreg))
+    assert sorted(t.fine_correct_slot[t.fine_correct_slot >= 0].tolist()) == list(range(t.nreg))
+    # brute force: count coarse-fine faces from coordinates
+    n = 0
+    for (i, j, k) in refined:
+        for d in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+            nb = ((i + d[0]) % 4, (j + d[1]) % 4, (k + d[2]) % 4)
+            n += nb not in refined
+    assert n == t.nreg
+    # the refined-level slot tables: window patch (a, b, c) of group q is the son with those fine coordinates
+    for q, (I, J, K) in enumerate(fathers):
+        for c in range(4):
+            for b in range(4):
+                for a in range(4):
+                    s = t.fine_pg_slot[q, c, b, a]
+                    fx, fy, fz = 2 * I - 1 + a, 2 * J - 1 + b, 2 * K - 1 + c
+                    fa = ((fx // 2) % 4, (fy // 2) % 4, (fz // 2) % 4)
+                    if fa in refined:
+                        qq = fathers.index(fa)
+                        assert s == 8 * qq + amr.LOCAL_XYZ.index((fx % 2, fy % 2, fz % 2))
+                    else:
+                        assert s == -1
+
+
+def test_patch_grid_round_trip():
+    g = np.random.default_rng(3).standard_normal((5, 32, 16, 48))
+    p = amr.patches_from_grid(g)
+    assert np.array_equal(amr.grid_from_patches(p, (6, 2, 4)), g)
+    assert np.array_equal(p[amr.base_patch_index((6, 2, 4), 5, 1, 2)], g[:, 16:24, 8:16, 40:48])
+
+
+def _totals(base):
+    return base.sum(axis=(0, 2, 3, 4)), np.abs(base).sum(axis=(0, 2, 3, 4))
+
+
+@pytest.mark.parametrize("int_scheme", ["CQUAD", "QUAR"])
+def test_two_level_step_conserves(int_scheme):
+    tab, base, fine, dh, dt = amr_chain.setup()
+    t0, s0 = _totals(base)
+    b1, f1, _ = amr_chain.two_level_step(base, fine, tab, dh, dt, int_scheme=int_scheme)
+    t1, _ = _totals(b1)
+    assert np.all(np.abs(t1 - t0) <= 1e-13 * s0), (t1 - t0) / s0
+    # the refined region changed and the base cells over it are the restricted fine data
+    assert np.abs(f1 - fine).max() > 1e-4
+    fa = f1.reshape(-1, 8, 5, 8, 8, 8)
+    for q, p in enumerate(tab.fa_idx):
+        sons = {l: fa[q, l] for l in range(8)}
+        for l, (a, b, c) in enumerate(amr.LOCAL_XYZ):
+            blk = b1[p][:, 4 * c:4 * c + 4, 4 * b:4 * b + 4, 4 * a:4 * a + 4]
+            np.testing.assert_allclose(blk, sons[l].reshape(5, 4, 2, 4, 2, 4, 2).mean(axis=(2, 4, 6)),
+                                       rtol=1e-14, atol=1e-14)
+
+
+def test_without_flux_fixup_the_totals_drift():
+    tab, base, fine, dh, dt = amr_chain.setup()
+    t0, s0 = _totals(base)
+    b1, _, reg = amr_chain.two_level_step(base, fine, tab, dh, dt, fixup=False)
+    t1, _ = _totals(b1)
+    assert np.abs(reg).max() > 0
+    assert np.max(np.abs(t1 - t0) / s0) > 1e-9
