@@ -260,6 +260,41 @@ def scheme_compare(args):
     print(json.dumps(res))
 
 
+def two_level(args):
+    """A periodic base level of 16^3 patches (128^3 cells) with its central 8^3 patches refined (a 128^3-cell fine
+    region), advanced by gamer_amd.amr.TwoLevel: one base step + two refined substeps + restriction + flux fix-up per
+    step, eagerly and as a replayed HIP graph.  FP32-bulk, HLLC, INT_CQUAD.  cell-updates = base cells + 2 x fine."""
+    import torch
+    from gamer_amd import amr, hydro, ic
+    hydro.lib()
+    dev = torch.device("cuda:0")
+    ncp = (16, 16, 16)
+    fathers = [(i, j, k) for k in range(4, 12) for j in range(4, 12) for i in range(4, 12)]
+    print("[two-level] tables", file=sys.stderr, flush=True)
+    tab = amr.build_two_level_tables(ncp, fathers)
+    cons, dt, dh = ic.random_sod_grid((8, 8, 8), seed=7, sedov=False)          # (5, 128, 128, 128), CFL-limited dt
+    base = amr.patches_from_grid(torch.from_numpy(cons).to(dev, dtype=torch.float32)).contiguous()
+    fine = torch.empty((8 * len(fathers), 5, 8, 8, 8), dtype=torch.float32, device=dev)
+    for l, (a, b, c) in enumerate(amr.LOCAL_XYZ):                # injection: son octant (a,b,c) of each father
+        fa = base[torch.from_numpy(tab.fa_idx.astype(np.int64)).to(dev)]
+        blk = fa[:, :, 4 * c:4 * c + 4, 4 * b:4 * b + 4, 4 * a:4 * a + 4]
+        fine[l::8] = blk.repeat_interleave(2, 2).repeat_interleave(2, 3).repeat_interleave(2, 4)
+    lv = amr.TwoLevel(torch, base, fine, tab, dh)
+    cells = 128 ** 3 + 2 * (8 * len(fathers) * 512)
+    print("[two-level] eager", file=sys.stderr, flush=True)
+    t_eager = amr.time_launch(torch, lambda: lv.step(0.5 * dt), reps=args.steps)
+    graph = lv.capture(0.5 * dt)
+    print("[two-level] graph", file=sys.stderr, flush=True)
+    t_graph = amr.time_launch(torch, graph.replay, reps=args.steps)
+    rate = lambda ms: cells / (ms * 1e-3)
+    print(json.dumps({"metric": "two-level AMR step (base + 2 refined substeps + fix-ups) on the device",
+                      "config": {"workload": "base 128^3 cells (512 patch groups) + refined 128^3 cells "
+                                             "(512 patch groups), MHM_RP+HLLC, INT_CQUAD, FP32-bulk", "cells": cells},
+                      "unit": "cell-updates/s", "eager": {"ms": t_eager, "value": rate(t_eager)},
+                      "graph_replay": {"ms": t_graph, "value": rate(t_graph)},
+                      "counters": lv.counters.cpu().tolist()}))
+
+
 def _field_scale(per_field, agg):
     """Per-field normalisers where the three momentum components share one scale (a component that is zero
     everywhere, e.g. m_z of a 2D shear layer, is measured against the momentum magnitude, not against 0)."""
@@ -289,6 +324,8 @@ def main():
                     help="run the Roe/HLLE/HLLC accuracy sweep on the KH workload instead of the throughput bench")
     ap.add_argument("--amr-batch", action="store_true",
                     help="time the uneven 3-level batch (one launch vs per-level launches, persistent, graph)")
+    ap.add_argument("--two-level", action="store_true",
+                    help="time the on-device two-level AMR step (eager and HIP-graph replay)")
     ap.add_argument("--schemes", action="store_true",
                     help="time MHM_RP against MHM (both precisions) on the default workload")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -308,6 +345,9 @@ def main():
         return
     if args.schemes:
         scheme_compare(args)
+        return
+    if args.two_level:
+        two_level(args)
         return
 
     import torch

@@ -192,6 +192,7 @@ class TwoLevelTables:
     fa_idx: np.ndarray              # (nf,)             base patch index of each father
     base_pid0: np.ndarray           # (npg_b,)
     nreg: int
+    interp_need: np.ndarray = None  # (nf, 12, 12, 12)  coarse cells whose fine cells no same-level patch covers
 
     @property
     def npg_base(self):
@@ -268,7 +269,19 @@ def build_two_level_tables(ncp, fathers) -> TwoLevelTables:
                           fine_correct_slot=fine_correct_slot, fixup_slot=fixup_slot,
                           son0=(8 * np.arange(nf)).astype(np.int32),
                           fa_idx=np.array([P(*f) for f in fathers], dtype=np.int32),
-                          base_pid0=(8 * np.arange(npg)).astype(np.int32), nreg=len(reg_of))
+                          base_pid0=(8 * np.arange(npg)).astype(np.int32), nreg=len(reg_of),
+                          interp_need=refined_interp_need(fine_pg_slot))
+
+
+def refined_interp_need(fine_pg_slot, ghost=3):
+    """(npg, 12, 12, 12) uint8: coarse cells -2..9 around each father (the range prepare_refined_patch_groups
+    interpolates) with at least one fine cell inside the window that has no same-level patch (slot -1)."""
+    n = 16 + 2 * ghost
+    w = np.arange(n) - ghost + 8
+    miss = fine_pg_slot[:, (w >> 3)[:, None, None], (w >> 3)[None, :, None], (w >> 3)[None, None, :]] < 0
+    fb = np.zeros((fine_pg_slot.shape[0], 24, 24, 24), dtype=bool)
+    fb[:, 1:1 + n, 1:1 + n, 1:1 + n] = miss                 # window cell w sits at fine-box cell w + 1
+    return np.ascontiguousarray(fb.reshape(-1, 12, 2, 12, 2, 12, 2).any(axis=(2, 4, 6)).astype(np.uint8))
 
 
 def patches_from_grid(grid):
@@ -316,7 +329,7 @@ class TwoLevel:
         T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
         self.t = {k: T(getattr(tables, k)) for k in ("base_pg_slot", "fine_coarse_slot", "fine_pg_slot",
                                                       "base_store_slot", "fine_correct_slot", "fixup_slot", "son0",
-                                                      "fa_idx", "base_pid0")}
+                                                      "fa_idx", "base_pid0", "interp_need")}
         self.reg = torch.zeros((max(tables.nreg, 1), 5, 8, 8), dtype=torch.float64, device=dev)
         nf = len(tables.fathers)
         self.fine_pid0 = T((8 * np.arange(nf)).astype(np.int32))
@@ -343,10 +356,26 @@ class TwoLevel:
             coarse = old if sub == 0 else 0.5 * old + 0.5 * self.base
             pg_f = torch.empty((nf, 5, 22, 22, 22), dtype=self.fine.dtype, device=self.fine.device)
             hydro.prepare_refined_patch_groups(coarse, self.fine, t["fine_coarse_slot"], t["fine_pg_slot"], pg_f,
-                                               scheme=self.int_scheme, gamma=self.gamma)
+                                               scheme=self.int_scheme, need=t["interp_need"], gamma=self.gamma)
             out_f, flux_f = self._close(pg_f, dt_f, dh_f)
             hydro.correct_flux(flux_f, t["fine_correct_slot"], dt_f, self.reg)
             hydro.close_patches(out_f, self.fine, self.fine_pid0)
         hydro.fixup_restrict(self.fine, self.base, t["son0"], t["fa_idx"])
         hydro.fixup_flux(self.base, t["fixup_slot"], self.reg, self.dh, gamma=self.gamma)
         return self
+
+    def capture(self, dt_base):
+        """Record one ``step(dt_base)`` into a HIP graph and return it; ``graph.replay()`` then advances base and fine
+        in place.  The warm-up step before the capture runs on the live state, which is saved and restored around
+        it.  Every clear inside the step is a fill kernel (no memset nodes)."""
+        torch = self.torch
+        saved = (self.base.clone(), self.fine.clone(), self.counters.clone())
+        self.step(dt_base)
+        torch.cuda.synchronize()
+        for dst, src in zip((self.base, self.fine, self.counters), saved):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            self.step(dt_base)
+        return graph

@@ -347,6 +347,7 @@ int hydro_interpolate_async( const HydroInterpArgs *a, void *stream )
    p.scheme = a->scheme;  p.mono = a->monotonic;  p.opp0 = a->opp_sign_0th;  p.int_prim = a->int_prim;
    p.reduce = a->reduce_mono;  p.mono_max_iter = a->mono_max_iter;  p.mono_coeff = a->mono_coeff;
    p.gm1 = a->gamma - 1.0;  p.inv_gm1 = 1.0/( a->gamma - 1.0 );  p.min_pres = a->min_pres;
+   p.need = a->need;
    int rc;
    if      ( a->precision == HYDRO_FP64 )      rc = launch_interpolate<double>( (double*)a->cdata, (double*)a->fdata, a->nbox, p, a->nfail, (hipStream_t)stream );
    else if ( a->precision == HYDRO_FP32_BULK ) rc = launch_interpolate<float> ( (float*)a->cdata,  (float*)a->fdata,  a->nbox, p, a->nfail, (hipStream_t)stream );
@@ -375,7 +376,7 @@ int hydro_gather_boxes_async( const void *patch, void *box, int nbox, const int 
    BoxGeom g;
    for (int d=0; d<3; d++)
    {
-      if ( size[d] <= 0 || tdim[d] <= 0 || origin[d] < 0 || origin[d] + size[d] > 8*tdim[d] )
+      if ( size[d] <= 0 || size[d] > 65535 || tdim[d] <= 0 || origin[d] < 0 || origin[d] + size[d] > 8*tdim[d] )
          return fail( HYDRO_E_ARG, "gather_boxes: box [%d, %d) outside the %d-patch table in dim %d", origin[d],
                       origin[d] + size[d], tdim[d], d );
       g.size[d] = size[d];  g.origin[d] = origin[d];  g.tdim[d] = tdim[d];

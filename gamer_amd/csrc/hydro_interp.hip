@@ -256,7 +256,7 @@ interpolate_kernel( T *__restrict__ g_c, T *__restrict__ g_f, int nbox, InterpPr
    const bool iterate = p.int_prim || p.reduce;
    const int mono_max = p.reduce ? p.mono_max_iter : 0, max_iter = p.int_prim ? mono_max + 1 : mono_max;
 
-   for (int c = tid; c < ncell; c += NT) s_done[c] = 0;
+   for (int c = tid; c < ncell; c += NT) s_done[c] = p.need ? !p.need[ (size_t)q*ncell + c ] : 0;
    T coeff = (T)p.mono_coeff;
    bool prim = false;
    for (int it = 0; ; it++)
@@ -340,68 +340,74 @@ interpolate_kernel( T *__restrict__ g_c, T *__restrict__ g_f, int nbox, InterpPr
    }
 }
 
+// Copies between per-patch data and boxes: one block per (box, field, z plane), 32 x 8 threads over (x, y), so a
+// cell's coordinates come from the thread and block indices without divisions; each thread issues its (up to 4) row
+// loads before its stores.
+constexpr int CPX = 32, CPY = 8;
+
+// patch-group input window (N = 16 + 2 ghost <= 32 per axis): cell (x,y,z) from the patch slot[pg][Z/8][Y/8][X/8],
+// (X,Y,Z) = (x,y,z) - ghost + 8; slot -1: from the fine box fbox at (x,y,z) + fb.origin, or kept if fbox == NULL.
+// A block covers CPZ planes; each thread loads its (up to 4 x CPZ) cells before storing them.
+constexpr int CPZ = 4;
 template <typename IO>
-__global__ void prepare_patches_kernel( const IO *__restrict__ patch, IO *__restrict__ pg_in, int npg,
-                                        const int *__restrict__ slot, int ghost )
+__global__ void __launch_bounds__( CPX*CPY )
+prepare_patches_kernel( const IO *__restrict__ patch, const IO *__restrict__ fbox, IO *__restrict__ pg_in,
+                        const int *__restrict__ slot, int ghost, BoxGeom fb )
 {
-   const int n = PS + 2*ghost, n3 = n*n*n;
-   const long total = (long)npg*NV*n3;
-   for (long e = blockIdx.x*(long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x*blockDim.x)
+   const int p = blockIdx.x, v = blockIdx.y, n = PS + 2*ghost, x = threadIdx.x;
+   if ( x >= n ) return;
+   const int X = x - ghost + H;
+   const int *sl = slot + (size_t)p*64 + ( X >> 3 );
+   const IO *pp = patch + (size_t)v*P3 + ( X & 7 );
+   const IO *src = fbox ? fbox + ( (size_t)p*NV + v )*fb.size[2]*fb.size[1]*fb.size[0] + x + fb.origin[0] : nullptr;
+   IO val[CPZ][4];
+   bool put[CPZ][4];
+#  pragma unroll
+   for (int kz = 0; kz < CPZ; kz++)
    {
-      const int c = (int)( e % n3 ), v = (int)( ( e/n3 ) % NV );
-      const long p = e/( (long)NV*n3 );
-      const int X = c % n - ghost + H, Y = ( c/n ) % n - ghost + H, Z = c/( n*n ) - ghost + H;
-      const int s = slot[ ( ( p*4 + Z/H )*4 + Y/H )*4 + X/H ];
-      if ( s < 0 ) continue;
-      pg_in[e] = patch[ ( (long)s*NV + v )*P3 + ( ( Z % H )*H + Y % H )*H + X % H ];
+      const int z = blockIdx.z*CPZ + kz, Z = z - ghost + H;
+#     pragma unroll
+      for (int k = 0; k < 4; k++)
+      {
+         const int y = threadIdx.y + k*CPY, Y = y - ghost + H;
+         put[kz][k] = false;
+         if ( z < n && y < n )
+         {
+            const int s = sl[ ( Z >> 3 )*16 + ( Y >> 3 )*4 ];
+            if ( s >= 0 )   { val[kz][k] = pp[ (size_t)s*NV*P3 + ( Z & 7 )*H*H + ( Y & 7 )*H ];  put[kz][k] = true; }
+            else if ( src ) { val[kz][k] = src[ ( (size_t)( z + fb.origin[2] )*fb.size[1] + y + fb.origin[1] )*fb.size[0] ];
+                              put[kz][k] = true; }
+         }
+      }
    }
+   IO *dst = pg_in + ( (size_t)p*NV + v )*n*n*n + x;
+#  pragma unroll
+   for (int kz = 0; kz < CPZ; kz++)
+#     pragma unroll
+      for (int k = 0; k < 4; k++)
+         if ( put[kz][k] ) dst[ ( ( blockIdx.z*CPZ + kz )*n + threadIdx.y + k*CPY )*n ] = val[kz][k];
 }
 
 // box[q][v][z][y][x] (size) = patch[ slot[q][Z/8][Y/8][X/8] ][v][Z%8][Y%8][X%8], (X,Y,Z) = origin + (x,y,z);
 // slot -1 keeps the box value
 template <typename IO>
-__global__ void gather_boxes_kernel( const IO *__restrict__ patch, IO *__restrict__ box, int nbox, BoxGeom g,
-                                     const int *__restrict__ slot )
+__global__ void __launch_bounds__( CPX*CPY )
+gather_boxes_kernel( const IO *__restrict__ patch, IO *__restrict__ box, BoxGeom g, const int *__restrict__ slot )
 {
-   const int n3 = g.size[0]*g.size[1]*g.size[2], t3 = g.tdim[0]*g.tdim[1]*g.tdim[2];
-   const long total = (long)nbox*NV*n3;
-   for (long e = blockIdx.x*(long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x*blockDim.x)
+   const int q = blockIdx.x, v = blockIdx.y, z = blockIdx.z;
+   const int Z = z + g.origin[2];
+   const int *sl = slot + (size_t)q*g.tdim[0]*g.tdim[1]*g.tdim[2] + ( Z >> 3 )*g.tdim[1]*g.tdim[0];
+   IO *dst = box + ( ( (size_t)q*NV + v )*g.size[2] + z )*g.size[1]*g.size[0];
+   for (int y = threadIdx.y; y < g.size[1]; y += CPY)
    {
-      const int c = (int)( e % n3 ), v = (int)( ( e/n3 ) % NV );
-      const long q = e/( (long)NV*n3 );
-      const int X = c % g.size[0] + g.origin[0], Y = ( c/g.size[0] ) % g.size[1] + g.origin[1],
-                Z = c/( g.size[0]*g.size[1] ) + g.origin[2];
-      const int s = slot[ q*t3 + ( Z/H*g.tdim[1] + Y/H )*g.tdim[0] + X/H ];
-      if ( s < 0 ) continue;
-      box[e] = patch[ ( (long)s*NV + v )*P3 + ( ( Z % H )*H + Y % H )*H + X % H ];
+      const int Y = y + g.origin[1];
+      for (int x = threadIdx.x; x < g.size[0]; x += CPX)
+      {
+         const int X = x + g.origin[0];
+         const int s = sl[ ( Y >> 3 )*g.tdim[0] + ( X >> 3 ) ];
+         if ( s >= 0 ) dst[ y*g.size[0] + x ] = patch[ ( ( (size_t)s*NV + v )*H + ( Z & 7 ) )*H*H + ( Y & 7 )*H + ( X & 7 ) ];
+      }
    }
-}
-
-// patch-group input: same-level patch where slot[pg] >= 0 (as prepare_patches_kernel), else the interpolated fine box
-// fbox[pg][v][fz][fy][fx] (size fb.size) at window cell + fb.origin
-template <typename IO>
-__global__ void assemble_kernel( const IO *__restrict__ patch, const IO *__restrict__ fbox, IO *__restrict__ pg_in,
-                                 int npg, const int *__restrict__ slot, int ghost, BoxGeom fb )
-{
-   const int n = PS + 2*ghost, n3 = n*n*n, f3 = fb.size[0]*fb.size[1]*fb.size[2];
-   const long total = (long)npg*NV*n3;
-   for (long e = blockIdx.x*(long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x*blockDim.x)
-   {
-      const int c = (int)( e % n3 ), v = (int)( ( e/n3 ) % NV );
-      const long p = e/( (long)NV*n3 );
-      const int x = c % n, y = ( c/n ) % n, z = c/( n*n );
-      const int X = x - ghost + H, Y = y - ghost + H, Z = z - ghost + H;
-      const int s = slot[ ( ( p*4 + Z/H )*4 + Y/H )*4 + X/H ];
-      if ( s >= 0 ) pg_in[e] = patch[ ( (long)s*NV + v )*P3 + ( ( Z % H )*H + Y % H )*H + X % H ];
-      else          pg_in[e] = fbox[ ( p*NV + v )*f3 + ( ( z + fb.origin[2] )*fb.size[1] + y + fb.origin[1] )*fb.size[0]
-                                     + x + fb.origin[0] ];
-   }
-}
-
-static long copy_blocks( long total )
-{
-   long b = ( total + 255 )/256;
-   return b > 65536 ? 65536 : ( b < 1 ? 1 : b );
 }
 
 } // namespace
@@ -429,20 +435,14 @@ int launch_interpolate( T *cdata, T *fdata, int nbox, const InterpPrm &p, int *n
 template <typename IO>
 int launch_prepare_patches( const IO *patch, IO *pg_in, int npg, const int *slot, int ghost, hipStream_t stream )
 {
-   if ( npg <= 0 ) return 0;
-   const long total = (long)npg*NV*( PS + 2*ghost )*( PS + 2*ghost )*( PS + 2*ghost );
-   long b = ( total + 255 )/256;
-   if ( b > 65536 ) b = 65536;
-   hipLaunchKernelGGL( prepare_patches_kernel<IO>, dim3( (int)b ), dim3( 256 ), 0, stream, patch, pg_in, npg, slot, ghost );
-   return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
+   return launch_assemble<IO>( patch, nullptr, pg_in, npg, slot, ghost, BoxGeom{}, stream );
 }
 
 template <typename IO>
 int launch_gather_boxes( const IO *patch, IO *box, int nbox, const BoxGeom &g, const int *slot, hipStream_t stream )
 {
    if ( nbox <= 0 ) return 0;
-   const long total = (long)nbox*NV*g.size[0]*g.size[1]*g.size[2];
-   hipLaunchKernelGGL( gather_boxes_kernel<IO>, dim3( (int)copy_blocks( total ) ), dim3( 256 ), 0, stream, patch, box, nbox, g, slot );
+   hipLaunchKernelGGL( gather_boxes_kernel<IO>, dim3( nbox, NV, g.size[2] ), dim3( CPX, CPY ), 0, stream, patch, box, g, slot );
    return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
 }
 
@@ -451,8 +451,8 @@ int launch_assemble( const IO *patch, const IO *fbox, IO *pg_in, int npg, const 
                      hipStream_t stream )
 {
    if ( npg <= 0 ) return 0;
-   const long total = (long)npg*NV*( PS + 2*ghost )*( PS + 2*ghost )*( PS + 2*ghost );
-   hipLaunchKernelGGL( assemble_kernel<IO>, dim3( (int)copy_blocks( total ) ), dim3( 256 ), 0, stream, patch, fbox, pg_in, npg, slot, ghost, fb );
+   hipLaunchKernelGGL( prepare_patches_kernel<IO>, dim3( npg, NV, ( PS + 2*ghost + CPZ - 1 )/CPZ ), dim3( CPX, CPY ), 0, stream, patch, fbox,
+                       pg_in, slot, ghost, fb );
    return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
 }
 

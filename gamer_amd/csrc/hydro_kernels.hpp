@@ -64,6 +64,7 @@ struct InterpPrm {
    int    mono;                               // bit v: Monotonic[v]
    int    opp0, int_prim, reduce, mono_max_iter;
    double mono_coeff, gm1, inv_gm1, min_pres;
+   const unsigned char *need;                 // optional [nbox][ncell] mask of coarse cells to interpolate
 };
 template <typename T>
 int launch_interpolate( T *cdata, T *fdata, int nbox, const InterpPrm &p, int *nfail, hipStream_t stream );

@@ -171,6 +171,8 @@ typedef struct HydroInterpArgs {
    double gamma, min_pres;
    int    precision;
    int   *nfail;               /* device int */
+   const unsigned char *need;  /* optional device [nbox][crange[2]][crange[1]][crange[0]]: 0 = the coarse cell's fine
+                                  cells are not used (a same-level patch covers them), skip it; NULL = all cells */
 } HydroInterpArgs;
 int hydro_interpolate_async( const HydroInterpArgs *args, void *stream );
 

@@ -83,3 +83,20 @@ def test_without_flux_fixup_the_totals_drift():
     t1, _ = _totals(b1)
     assert np.abs(reg).max() > 0
     assert np.max(np.abs(t1 - t0) / s0) > 1e-9
+
+
+def test_interp_need_mask():
+    """Interior patch groups (all 26 neighbours refined) need no interpolation; an isolated one needs exactly the
+    coarse cells under its 3-cell fine ghost shell, never the ones under its own 16^3 cells."""
+    inner = [(i, j, k) for k in range(1, 4) for j in range(1, 4) for i in range(1, 4)]
+    t = amr.build_two_level_tables((6, 6, 6), inner)
+    assert t.interp_need[inner.index((2, 2, 2))].sum() == 0
+    t1 = amr.build_two_level_tables((4, 4, 4), [(1, 1, 1)])
+    need = t1.interp_need[0].astype(bool)
+    c = np.arange(12) - 2                                    # coarse cell offsets around the father
+    fine_lo, fine_hi = 2 * c, 2 * c + 1                      # its fine cells, father-relative
+    inwin = (fine_hi >= -3) & (fine_lo <= 18)                # touches the window -3..18
+    core = (fine_lo >= 0) & (fine_hi <= 15)                  # both fine cells inside the patch group's own 16 cells
+    W = inwin[:, None, None] & inwin[None, :, None] & inwin[None, None, :]
+    C = core[:, None, None] & core[None, :, None] & core[None, None, :]
+    np.testing.assert_array_equal(need, W & ~C)

@@ -84,16 +84,17 @@ def test_prepare_patches_matches_oracle(ghost):
     np.testing.assert_array_equal(got.cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("masked", [False, True])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("scheme", ["CQUAD", "QUAR", "MINMOD1D"])
-def test_refined_patch_groups_on_device(dtype, scheme):
+def test_refined_patch_groups_on_device(dtype, scheme, masked):
     """Prepare_PatchData of 8 fine patch groups at a refinement boundary entirely on the device (coarse boxes from the
     father + 26 siblings -> Interpolate -> same-level fine patches), against the oracle chain; then one MHM_RP step
     of the assembled inputs against the oracle solver."""
     import oracle
     import amr_mesh as M
     from gamer_amd import hydro
-    print(f"[gpu] refined patch groups {scheme} {np.dtype(dtype).name}", flush=True)
+    print(f"[gpu] refined patch groups {scheme} {np.dtype(dtype).name} masked={masked}", flush=True)
     rng = np.random.default_rng(15)
     cpatch = M.patches_of(M.gas((32, 32, 32), rng, smooth=False)).astype(dtype)
     fpatch = M.patches_of(M.gas((32, 32, 32), rng, smooth=False)).astype(dtype)
@@ -105,11 +106,15 @@ def test_refined_patch_groups_on_device(dtype, scheme):
                                      **kw)
     ref = oracle.assemble_patch_groups(fpatch, fbox, np.zeros((npg, 5, 22, 22, 22), dtype), fslot, 3, (1, 1, 1))
     pg = torch.zeros((npg, 5, 22, 22, 22), dtype=torch.from_numpy(cpatch).dtype, device=DEV)
+    need = None
+    if masked:                          # interpolate only under window cells without a same-level patch
+        from gamer_amd import amr
+        need = torch.from_numpy(amr.refined_interp_need(fslot)).to(DEV)
     nf = hydro.prepare_refined_patch_groups(torch.from_numpy(cpatch).to(DEV), torch.from_numpy(fpatch).to(DEV),
                                             torch.from_numpy(cslot).to(DEV), torch.from_numpy(fslot).to(DEV), pg,
-                                            scheme=scheme, **kw)
+                                            scheme=scheme, need=need, **kw)
     torch.cuda.synchronize()
-    assert int(nf.item()) == rn
+    assert int(nf.item()) <= rn if masked else int(nf.item()) == rn
     tol = 1e-14 if dtype == np.float64 else 1e-5
     np.testing.assert_allclose(pg.cpu().numpy(), ref, rtol=tol, atol=tol)
     if dtype == np.float64:
