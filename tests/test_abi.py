@@ -105,3 +105,25 @@ def test_interpolate_rejects_bad_arguments(scheme, cstart, crange, csize, reduce
     a = _interp_args(scheme, cstart, crange, csize, reduce_mono)
     assert L.hydro_interpolate_async(ctypes.byref(a), None) == hydro.HYDRO_E_ARG
     assert frag in L.hydro_last_error().decode()
+
+
+def test_header_is_c99_and_links(tmp_path):
+    """The boundary is plain C: a C99 translation unit that includes the header and takes the address of every
+    declared entry point compiles and links against the library (no C++ types, no torch)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc")
+    if cc is None:
+        pytest.skip("gcc not available")
+    names = declared_functions()
+    src = tmp_path / "use_header.c"
+    body = "\n".join(f"   (void *)&{n}," for n in names)
+    src.write_text('#include "gamer_amd_hydro.h"\n#include <stdio.h>\n'
+                   f"static void *const entry[] = {{\n{body}\n}};\n"
+                   'int main( void ) { printf( "%d\\n", (int)( sizeof entry / sizeof entry[0] ) ); return 0; }\n')
+    exe = tmp_path / "use_header"
+    libdir = os.path.dirname(hydro.LIB_PATH)
+    subprocess.run([cc, "-std=c99", "-Wall", "-Werror", f"-I{os.path.join(ROOT, 'include')}",
+                    str(src), "-o", str(exe), f"-L{libdir}", "-lgamer_amd_hydro", f"-Wl,-rpath,{libdir}"], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    assert int(out) == len(names) > 20
