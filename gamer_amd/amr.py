@@ -119,12 +119,14 @@ def time_launch(torch, fn, reps=5, warmup=1):
     return e0.elapsed_time(e1) / reps
 
 
-def autotune(torch, batch: LevelBatch, groups=(1, 2, 3, 4), reps=5, cache=None, **kw):
+def autotune(torch, batch: LevelBatch, groups=(1, 2, 3, 4), reps=5, rounds=3, cache=None, **kw):
     """Measure every launch configuration of a batched substep and return the fastest.
 
     Configurations: one workgroup per patch group ("grid"), and the persistent work queue with g workgroups
-    per compute unit for g in ``groups``.  Results are cached in ``cache`` (a JSON path) keyed by the batch
-    shape and precision when given."""
+    per compute unit for g in ``groups``.  The configurations are timed in ``rounds`` interleaved rounds of
+    ``reps`` launches and each keeps its best round, so that a clock or thermal drift during the sweep does not
+    pick the winner.  Results are cached in ``cache`` (a JSON path) keyed by the batch shape and precision when
+    given."""
     npg = batch.u.shape[0]
     key = f"{torch.cuda.get_device_name(0)}|npg={npg}|{batch.u.dtype}|{kw.get('scheme', 'MHM_RP')}|{kw.get('rsolver', 'HLLC')}"
     if cache and os.path.exists(cache):
@@ -138,13 +140,14 @@ def autotune(torch, batch: LevelBatch, groups=(1, 2, 3, 4), reps=5, cache=None, 
     out = torch.empty((npg, 5, 16, 16, 16), dtype=batch.u.dtype, device=batch.u.device)
     flux = torch.empty((npg, 9, 5, 16, 16), dtype=batch.u.dtype, device=batch.u.device)
     counter = torch.zeros(1, dtype=torch.int32, device=batch.u.device)
-    trials = {}
-    trials["grid"] = time_launch(torch, lambda: hydro.solve_batch(batch.u, batch.pg_dt_dh, out=out, flux=flux, **kw),
-                                 reps)
+    configs = {"grid": lambda: hydro.solve_batch(batch.u, batch.pg_dt_dh, out=out, flux=flux, **kw)}
     for g in groups:
-        trials[f"persistent:{g}"] = time_launch(
-            torch, lambda g=g: hydro.solve_batch(batch.u, batch.pg_dt_dh, out=out, flux=flux, queue_counter=counter,
-                                                 groups_per_cu=g, **kw), reps)
+        configs[f"persistent:{g}"] = (lambda g=g: hydro.solve_batch(batch.u, batch.pg_dt_dh, out=out, flux=flux,
+                                                                    queue_counter=counter, groups_per_cu=g, **kw))
+    trials = {name: float("inf") for name in configs}
+    for _ in range(max(1, rounds)):
+        for name, fn in configs.items():
+            trials[name] = min(trials[name], time_launch(torch, fn, reps))
     best = min(trials, key=trials.get)
     res = {"best": best, "ms": trials[best], "trials_ms": trials, "measured_at": time.time()}
     if cache:
