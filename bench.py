@@ -300,6 +300,37 @@ def two_level(args):
                       "counters": lv.counters.cpu().tolist()}))
 
 
+def adjoint_bench(args):
+    """C5 kernels at NPG = 512 of the default workload (FP64, MHM_RP + HLLC + VL_GMINMOD): the production forward
+    step, the tangent-linear step (J v, with the forward) and the adjoint step (J^T lambda, recomputing the forward
+    phases), each timed with HIP events over ``--steps`` launches."""
+    import torch
+    from gamer_amd import amr, hydro, ic
+    hydro.lib()
+    dev = torch.device("cuda:0")
+    cons, dt, dh = ic.random_sod_grid((8, 8, 8), seed=20260, sedov=True)
+    u = cut_on_device(cons, 3, torch, dev, torch.float64)
+    npg = u.shape[0]
+    gen = torch.Generator(device=dev).manual_seed(5)
+    v = torch.randn(u.shape, dtype=torch.float64, device=dev, generator=gen)
+    lam = torch.randn((npg, 5, 16, 16, 16), dtype=torch.float64, device=dev, generator=gen)
+    ws = hydro.adjoint_workspace(npg, dev)
+    out = torch.empty((npg, 5, 16, 16, 16), dtype=torch.float64, device=dev)
+    fns = {"forward": lambda: hydro.fluid_solver(u, dt, dh, out=out, store_flux=False),
+           "tangent": lambda: hydro.mhm_tangent(u, v, dt, dh, workspace=ws),
+           "adjoint": lambda: hydro.mhm_adjoint(u, lam, dt, dh, workspace=ws)}
+    res = {"metric": "MHM_RP FP64 step, tangent-linear and adjoint kernels", "unit": "ms per step",
+           "config": {"workload": f"NPG={npg} patch groups of the default bench workload, FP64, HLLC, VL_GMINMOD",
+                      "npg": npg, "workspace_bytes": int(ws.numel() * 8)}}
+    for name, fn in fns.items():
+        print(f"[adjoint] {name}", file=sys.stderr, flush=True)
+        ms = min(amr.time_launch(torch, fn, reps=args.steps) for _ in range(3))
+        res[name] = {"ms": ms, "cell_updates_per_s": npg * CELLS_PER_PG / (ms * 1e-3)}
+    res["tangent_over_forward"] = res["tangent"]["ms"] / res["forward"]["ms"]
+    res["adjoint_over_forward"] = res["adjoint"]["ms"] / res["forward"]["ms"]
+    print(json.dumps(res))
+
+
 def _field_scale(per_field, agg):
     """Per-field normalisers where the three momentum components share one scale (a component that is zero
     everywhere, e.g. m_z of a 2D shear layer, is measured against the momentum magnitude, not against 0)."""
@@ -329,6 +360,8 @@ def main():
                     help="run the Roe/HLLE/HLLC accuracy sweep on the KH workload instead of the throughput bench")
     ap.add_argument("--amr-batch", action="store_true",
                     help="time the uneven 3-level batch (one launch vs per-level launches, persistent, graph)")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="time the FP64 forward, tangent-linear and adjoint kernels (C5) at NPG = 512")
     ap.add_argument("--two-level", action="store_true",
                     help="time the on-device two-level AMR step (eager and HIP-graph replay)")
     ap.add_argument("--schemes", action="store_true",
@@ -353,6 +386,9 @@ def main():
         return
     if args.two_level:
         two_level(args)
+        return
+    if args.adjoint:
+        adjoint_bench(args)
         return
 
     import torch
