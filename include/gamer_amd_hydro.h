@@ -246,7 +246,9 @@ int hydro_host_buffers( int array_id, void **h_in, void **h_out, void **h_flux )
 
 /* Asynchronous solve over host arrays: per stream H2D copy, kernel, D2H copy of out (and flux), the
    NPG/NStream split with the remainder on the last stream (src/GPU_API/CUAPI_Asyn_FluidSolver.cu:433-722).
-   Host arrays should be the pinned buffers from hydro_host_buffers.  Returns before the GPU finishes. */
+   Host arrays should be the pinned buffers from hydro_host_buffers.  Returns before the GPU finishes.  As in
+   GAMER (src/Main/InvokeSolver.cpp:227-267), one call is in flight at a time: both host buffers share the device
+   arrays, so call hydro_synchronize before the next call; the other host buffer can be filled or read meanwhile. */
 int hydro_asyn_fluid_solver( const void *h_in, void *h_out, void *h_flux, const HydroSolveArgs *args,
                              int gpu_nstream );
 
