@@ -16,17 +16,29 @@
 //             R1 lambda of every half-step face, then J_RS^T of the cell-centred half-step solve per side
 //             R0 lambda_in = lambda_out (core) + lambda_U_half + the half-step face contributions
 //
-// The reverse sweep is written out by hand.  The local Jacobians of the nonlinear pieces (Riemann solver,
-// limiter + clamp, conversions with floors) come from forward-mode dual numbers (hydro_dual.hpp) pushed
-// through the same templated device functions the production kernels use, one input direction at a time:
-// they take exactly the branches of the forward evaluation.  The reverse phases are gathers: every workspace
-// entry has one writer, so there are no atomics and the result is deterministic.
+// The reverse sweep is written out by hand.  For HLLC and HLLE the Riemann solves run backwards as hand-written
+// reverse-mode sweeps (hydro_reverse.hpp: solver + interface-state builders, one pass per face for all ten
+// inputs).  The other local Jacobians (the Roe solver, limiter + clamp, conversions with floors) come from
+// forward-mode dual numbers (hydro_dual.hpp) pushed through the same templated device functions the production
+// kernels use, one input direction at a time.  Both forms take exactly the branches of the forward evaluation.
+// The reverse phases are gathers: every workspace entry has one writer, so there are no atomics and the result
+// is deterministic.
 //
 // Scope: MHM_RP, FP64, any Riemann solver and limiter of the forward path.  The MinMod retry is not
 // differentiated (the first-iteration coefficient is used); dt and dh are constants.
+//
+// No floating-point contraction in this translation unit (it holds only the derivative kernels, and the
+// pragma comes before the solver headers it instantiates).  Every value is then the rounded result of its own
+// source expression, with fused multiply-adds only where fmad() is written, so the dual-number evaluation, the
+// plain recomputation inside the reverse sweeps and the forward phases all produce bitwise equal values and
+// split ties the same way.  Ties are common: a uniform floored region puts the HLLC star pressure on the
+// pressure floor up to rounding, and the floor decides whether its derivative passes.
+
+#pragma clang fp contract(off)
 
 #include "hydro_dual.hpp"
 #include "hydro_kernels.hpp"
+#include "hydro_reverse.hpp"
 
 namespace gamer_amd {
 
@@ -87,6 +99,45 @@ __device__ __forceinline__ void full_flux( int d, const T wL[NV], const T wR[NV]
    if      ( d == 0 ) riemann<RS, 0>( face_state<0>( wL, P ), face_state<0>( wR, P ), P, f );
    else if ( d == 1 ) riemann<RS, 1>( face_state<1>( wL, P ), face_state<1>( wR, P ), P, f );
    else               riemann<RS, 2>( face_state<2>( wL, P ), face_state<2>( wR, P ), P, f );
+}
+
+// reverse sweeps of half_flux / full_flux (HLLC, HLLE): bL, bR = (d flux / d input side)^T bf, one pass
+template <int RS, int D>
+__device__ __forceinline__ void half_flux_rev_d( const double uL[NV], const double uR[NV], const Prm<double> &P,
+                                                 const double bf[NV], double bL[NV], double bR[NV] )
+{
+   const StD L = cell_st<D>( uL, P ), Q = cell_st<D>( uR, P );
+   StD sL, sQ;
+   riemann_rev<RS, D>( L, Q, P, bf, sL, sQ );
+   cell_st_rev<D>( uL, L, sL, P, bL );
+   cell_st_rev<D>( uR, Q, sQ, P, bR );
+}
+template <int RS>
+__device__ __forceinline__ void half_flux_rev( int d, const double uL[NV], const double uR[NV], const Prm<double> &P,
+                                               const double bf[NV], double bL[NV], double bR[NV] )
+{
+   if      ( d == 0 ) half_flux_rev_d<RS, 0>( uL, uR, P, bf, bL, bR );
+   else if ( d == 1 ) half_flux_rev_d<RS, 1>( uL, uR, P, bf, bL, bR );
+   else               half_flux_rev_d<RS, 2>( uL, uR, P, bf, bL, bR );
+}
+
+template <int RS, int D>
+__device__ __forceinline__ void full_flux_rev_d( const double wL[NV], const double wR[NV], const Prm<double> &P,
+                                                 const double bf[NV], double bL[NV], double bR[NV] )
+{
+   const StD L = face_state<D>( wL, P ), Q = face_state<D>( wR, P );
+   StD sL, sQ;
+   riemann_rev<RS, D>( L, Q, P, bf, sL, sQ );
+   face_state_rev<D>( wL, L, sL, P, bL );
+   face_state_rev<D>( wR, Q, sQ, P, bR );
+}
+template <int RS>
+__device__ __forceinline__ void full_flux_rev( int d, const double wL[NV], const double wR[NV], const Prm<double> &P,
+                                               const double bf[NV], double bL[NV], double bR[NV] )
+{
+   if      ( d == 0 ) full_flux_rev_d<RS, 0>( wL, wR, P, bf, bL, bR );
+   else if ( d == 1 ) full_flux_rev_d<RS, 1>( wL, wR, P, bf, bL, bR );
+   else               full_flux_rev_d<RS, 2>( wL, wR, P, bf, bL, bR );
 }
 
 // half-step conserved state -> density floor, internal-energy floor, primitive variables with pressure floor
@@ -325,7 +376,17 @@ adjoint_kernel( const double *__restrict__ g_in, const double *__restrict__ g_lo
          const double lh = hi ? lout[ (size_t)v*PS*PS*PS + out_index( i2, j2, k2 ) ] : 0.0;
          lf[v] = P.dt_dh*( lh - ll );
       }
-      for (int q=0; q<2*NV; q++)                  // input direction: R face of c (q < 5), L face of c + e_d
+      if constexpr ( RS != 2 )                    // HLLC / HLLE: one reverse sweep for both sides
+      {
+         double wl[NV], wr[NV], gl[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 }, gr[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+#        pragma unroll
+         for (int v=0; v<NV; v++) { wl[v] = F.fc( d, 1, v, c );  wr[v] = F.fc( d, 0, v, c + s ); }
+         full_flux_rev<RS>( d, wl, wr, P, lf, gl, gr );
+#        pragma unroll
+         for (int v=0; v<NV; v++) { A.lfc( d, 1, v, c ) = gl[v];  A.lfc( d, 0, v, c + s ) = gr[v]; }
+         continue;
+      }
+      for (int q=0; q<2*NV; q++)                  // Roe: input direction q, R face of c (q < 5), L face of c + e_d
       {
          D1 wl[NV], wr[NV], f[NV];
 #        pragma unroll
@@ -423,8 +484,10 @@ adjoint_kernel( const double *__restrict__ g_in, const double *__restrict__ g_lo
       double uL[NV], uR[NV];
 #     pragma unroll
       for (int v=0; v<NV; v++) { uL[v] = in[ v*N3 + c ];  uR[v] = in[ v*N3 + c + s ]; }
-      double gl[NV], gr[NV];
-      for (int q=0; q<2*NV; q++)
+      double gl[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 }, gr[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+      if constexpr ( RS != 2 ) half_flux_rev<RS>( d, uL, uR, P, lf, gl, gr );
+      else
+      for (int q=0; q<2*NV; q++)                  // Roe: one dual evaluation per input direction
       {
          D1 dl[NV], dr[NV], f[NV];
 #        pragma unroll

@@ -4,6 +4,8 @@
 // (hydro_math.hpp) are templates on their arithmetic type, so instantiating them with Dual<N> evaluates the
 // very same operation sequence on the value part while propagating derivatives.  Every comparison, min/max
 // and select acts on the value part, so derivatives follow the branch the plain FP64 evaluation takes.
+// The derivative parts use explicit fused multiply-adds: the file that instantiates them is compiled without
+// contraction.
 #pragma once
 #include "hydro_math.hpp"
 
@@ -46,7 +48,7 @@ struct Dual {
    {
       Dual r;  r.v = a.v*b.v;
 #     pragma unroll
-      for (int k=0; k<N; k++) r.d[k] = a.d[k]*b.v + a.v*b.d[k];
+      for (int k=0; k<N; k++) r.d[k] = __builtin_fma( a.d[k], b.v, a.v*b.d[k] );
       return r;
    }
    friend __device__ __forceinline__ Dual operator/( const Dual &a, const Dual &b )
@@ -54,7 +56,7 @@ struct Dual {
       Dual r;  r.v = a.v/b.v;
       const double ib = 1.0/b.v;
 #     pragma unroll
-      for (int k=0; k<N; k++) r.d[k] = ( a.d[k] - r.v*b.d[k] )*ib;
+      for (int k=0; k<N; k++) r.d[k] = __builtin_fma( -r.v, b.d[k], a.d[k] )*ib;
       return r;
    }
    __device__ __forceinline__ Dual &operator+=( const Dual &b ) { *this = *this + b;  return *this; }
@@ -74,7 +76,7 @@ template <int N> __device__ __forceinline__ Dual<N> fmad( const Dual<N> &a, cons
 {
    Dual<N> r;  r.v = fmad( a.v, b.v, c.v );
 #  pragma unroll
-   for (int k=0; k<N; k++) r.d[k] = a.d[k]*b.v + a.v*b.d[k] + c.d[k];
+   for (int k=0; k<N; k++) r.d[k] = __builtin_fma( a.d[k], b.v, __builtin_fma( a.v, b.d[k], c.d[k] ) );
    return r;
 }
 // d(1/x) = -dx/x^2 ; the value is the solver's own reciprocal

@@ -1,0 +1,250 @@
+// Reverse-mode local sweeps (FP64) of the interface-state builders and of the HLLC / HLLE interface solvers,
+// for the adjoint of the MHM_RP step.
+//
+// Each *_rev function takes the primal inputs, recomputes the forward intermediates and accumulates the input
+// adjoints for one output adjoint in a single backward pass: the row vector bf^T J of the piece, which the
+// dual-number form (hydro_dual.hpp) needs one evaluation per input direction for (10 for a Riemann solve).
+//
+// Branches: every min / max / select / floor takes the branch of the forward evaluation, with the tie rules
+// of the dual numbers (a tie of min or max goes to the first operand, a NaN second operand loses), so this
+// adjoint and the dual-number tangent agree at the kinks too (Davis speeds, HLLC upwind side, pressure
+// floors, the HLLE rescue of a NaN HLLC flux).
+//
+// Forward definitions being reversed:
+//   rs_hllc, rs_hlle, riemann, face_state          hydro_math.hpp (CPU_Shared_RiemannSolver_HLLC.cpp:391-655,
+//                                                  _HLLE.cpp:301-698, CPU_FluidSolver_MHM.cpp:737-786)
+//   conserved cell -> interface state (cell_st)    hydro_adjoint.hip (Hydro_Con2Pres, FluUtility.cpp:1259-1300)
+#pragma once
+#include "hydro_math.hpp"
+
+namespace gamer_amd {
+
+typedef St<double> StD;
+
+__device__ __forceinline__ void st_zero( StD &b ) { b.d = b.mn = b.m1 = b.m2 = b.e = b.p = b.ir = b.cs = b.u = 0.0; }
+
+// branch of the dual-number min(a, b) / max(a, b): true = the first operand
+__device__ __forceinline__ bool min_first( double a, double b ) { return b != b || a <= b; }
+__device__ __forceinline__ bool max_first( double a, double b ) { return b != b || a >= b; }
+
+// F = x-flux(s) - sp*U(s)  (xflux_minus)
+__device__ __forceinline__ void xflux_minus_rev( const StD &s, double sp, const double bF[NV], StD &bs, double &bsp )
+{
+   const double us = s.u - sp;
+   bs.d  -= sp*bF[0];
+   bs.mn += bF[0] + us*bF[1];
+   bs.m1 += us*bF[2];
+   bs.m2 += us*bF[3];
+   bs.e  += us*bF[4];
+   bs.p  += bF[1] + s.u*bF[4];
+   bs.u  += s.mn*bF[1] + s.m1*bF[2] + s.m2*bF[3] + ( s.e + s.p )*bF[4];
+   bsp   -= s.d*bF[0] + s.mn*bF[1] + s.m1*bF[2] + s.m2*bF[3] + s.e*bF[4];
+}
+
+__device__ __forceinline__ void add_upwind( StD &b, const StD &bs )
+{
+   b.d += bs.d;  b.mn += bs.mn;  b.m1 += bs.m1;  b.m2 += bs.m2;  b.e += bs.e;  b.p += bs.p;  b.u += bs.u;
+}
+
+// HLLE: sl = min(min(uL-cL, uR-cR), 0), sr = max(max(uL+cL, uR+cR), 0),
+//       f = (sr (F_L - sl U_L) - sl (F_R - sr U_R)) / (sr - sl), or F_L - sl U_L when sl = sr = 0
+__device__ __forceinline__ void rs_hlle_rev( const StD &L, const StD &Q, const double bf[NV], StD &bL, StD &bQ )
+{
+   const double a1 = L.u - L.cs, a2 = Q.u - Q.cs, b1 = L.u + L.cs, b2 = Q.u + Q.cs;
+   const bool   k1 = min_first( a1, a2 ), k2 = max_first( b1, b2 );
+   const double slr = k1 ? a1 : a2, srr = k2 ? b1 : b2;
+   const bool   kl = min_first( slr, 0.0 ), kr = max_first( srr, 0.0 );
+   const double sl = kl ? slr : 0.0, sr = kr ? srr : 0.0;
+   double fl[NV], fr[NV], bfl[NV], bfr[NV], bsl = 0.0, bsr = 0.0;
+   xflux_minus( L, sl, fl );
+   xflux_minus( Q, sr, fr );
+   if ( sl == 0.0 && sr == 0.0 )
+   {
+#     pragma unroll
+      for (int v=0; v<NV; v++) { bfl[v] = bf[v];  bfr[v] = 0.0; }
+   }
+   else
+   {
+      const double inv = rcp( sr - sl );
+      double binv = 0.0;
+#     pragma unroll
+      for (int v=0; v<NV; v++)
+      {
+         const double bD = bf[v]*inv;
+         binv += bf[v]*( sr*fl[v] - sl*fr[v] );
+         bsr  += bD*fl[v];
+         bsl  -= bD*fr[v];
+         bfl[v] =  bD*sr;
+         bfr[v] = -bD*sl;
+      }
+      const double bdf = -inv*inv*binv;
+      bsr += bdf;
+      bsl -= bdf;
+   }
+   xflux_minus_rev( L, sl, bfl, bL, bsl );
+   xflux_minus_rev( Q, sr, bfr, bQ, bsr );
+   const double bslr = kl ? bsl : 0.0, bsrr = kr ? bsr : 0.0;
+   if ( k1 ) { bL.u += bslr;  bL.cs -= bslr; } else { bQ.u += bslr;  bQ.cs -= bslr; }
+   if ( k2 ) { bL.u += bsrr;  bL.cs += bsrr; } else { bQ.u += bsrr;  bQ.cs += bsrr; }
+}
+
+// HLLC (rs_hllc).  Returns false, with nothing accumulated, when the forward flux has a NaN component: the
+// caller then differentiates the HLLE rescue instead.
+__device__ __forceinline__ bool rs_hllc_rev( const StD &L, const StD &Q, double pmin, const double bf[NV], StD &bL, StD &bQ )
+{
+   // forward
+   const double wl1 = L.u - L.cs, wl2 = Q.u - Q.cs, wr1 = L.u + L.cs, wr2 = Q.u + Q.cs;
+   const bool   kl = min_first( wl1, wl2 ), kr = max_first( wr1, wr2 );
+   const double wl = kl ? wl1 : wl2, wr = kr ? wr1 : wr2;
+   const double du = L.u - Q.u;
+   const bool   cl = ( wl1 < wl2 ), cr = ( wr2 > wr1 );
+   const double XL = cl ? L.cs : du + Q.cs, XR = cr ? Q.cs : du + L.cs;
+   const double tl = L.d*XL, tr = -Q.d*XR;
+   const double t2 = rcp( tl - tr );
+   const double A  = L.p - Q.p + tl*L.u - tr*Q.u;
+   const double B  = tl*( Q.p + tr*Q.u ) - tr*( L.p + tl*L.u );
+   const double vs = t2*A, psr = t2*B;
+   const bool   pk = !( psr < pmin );
+   const double ps = pk ? psr : pmin;
+   const bool   left = ( vs >= 0.0 );
+   const StD   &s = left ? L : Q;
+   const bool   spk = left ? min_first( wl, 0.0 ) : max_first( wr, 0.0 );
+   const double sp = spk ? ( left ? wl : wr ) : 0.0;
+   double F[NV];
+   xflux_minus( s, sp, F );
+   const bool   deg = left && ( vs == 0.0 ) && ( sp == 0.0 );
+   const double t4 = rcp( vs - sp );
+   const double cf = deg ? 1.0 : t4*vs;
+   const double cs = deg ? 0.0 : -t4*sp*ps;
+   bool nan = false;
+#  pragma unroll
+   for (int v=0; v<NV; v++)
+   {
+      const double f = cf*F[v] + ( v == 1 ? cs : v == 4 ? cs*vs : 0.0 );
+      nan |= ( f != f );
+   }
+   if ( nan ) return false;
+
+   // reverse: f_k = cf F_k (+ cs, + cs vs)
+   double bF[NV], bcf = 0.0;
+#  pragma unroll
+   for (int v=0; v<NV; v++) { bF[v] = cf*bf[v];  bcf += bf[v]*F[v]; }
+   const double bcs = bf[1] + bf[4]*vs;
+   double bvs = bf[4]*cs, bsp = 0.0, bps = 0.0;
+   if ( !deg )
+   {
+      // cf = t4 vs, cs = (-t4) sp ps, t4 = 1/(vs - sp)
+      const double bt4 = bcf*vs - bcs*sp*ps;
+      bvs += bcf*t4;
+      bsp -= bcs*t4*ps;
+      bps -= bcs*t4*sp;
+      const double bdf = -t4*t4*bt4;
+      bvs += bdf;
+      bsp -= bdf;
+   }
+   StD bs;
+   st_zero( bs );
+   xflux_minus_rev( s, sp, bF, bs, bsp );
+   add_upwind( left ? bL : bQ, bs );
+   const double bwl = ( spk &&  left ) ? bsp : 0.0;
+   const double bwr = ( spk && !left ) ? bsp : 0.0;
+
+   // ps = floor(psr), vs = t2 A, psr = t2 B
+   const double bpsr = pk ? bps : 0.0;
+   const double bt2 = bvs*A + bpsr*B, bA = bvs*t2, bB = bpsr*t2;
+   double btl = 0.0, btr = 0.0;
+   // A = L.p - Q.p + tl L.u - tr Q.u
+   bL.p += bA;  bQ.p -= bA;
+   btl  += bA*L.u;  bL.u += bA*tl;
+   btr  -= bA*Q.u;  bQ.u -= bA*tr;
+   // B = tl (Q.p + tr Q.u) - tr (L.p + tl L.u)
+   btl  += bB*( Q.p + tr*Q.u ) - bB*tr*L.u;
+   btr  += bB*tl*Q.u - bB*( L.p + tl*L.u );
+   bQ.p += bB*tl;     bQ.u += bB*tl*tr;
+   bL.p -= bB*tr;     bL.u -= bB*tr*tl;
+   // t2 = 1/(tl - tr)
+   const double bd2 = -t2*t2*bt2;
+   btl += bd2;
+   btr -= bd2;
+   // tl = L.d XL, tr = (-Q.d) XR
+   bL.d += btl*XL;
+   bQ.d -= btr*XR;
+   const double bXL = btl*L.d, bXR = -btr*Q.d;
+   double bdu = 0.0;
+   if ( cl ) bL.cs += bXL; else { bdu += bXL;  bQ.cs += bXL; }
+   if ( cr ) bQ.cs += bXR; else { bdu += bXR;  bL.cs += bXR; }
+   bL.u += bdu;
+   bQ.u -= bdu;
+   // wl = min(wl1, wl2), wr = max(wr1, wr2)
+   if ( kl ) { bL.u += bwl;  bL.cs -= bwl; } else { bQ.u += bwl;  bQ.cs -= bwl; }
+   if ( kr ) { bL.u += bwr;  bL.cs += bwr; } else { bQ.u += bwr;  bQ.cs += bwr; }
+   return true;
+}
+
+// riemann<RS, D> (HLLC with the HLLE rescue, or HLLE): bout is the adjoint of the lab-frame flux
+template <int RS, int D>
+__device__ __forceinline__ void riemann_rev( const StD &L, const StD &Q, const Prm<double> &P, const double bout[NV],
+                                             StD &bL, StD &bQ )
+{
+   static_assert( RS == 3 || RS == 4, "reverse sweep written for HLLE and HLLC" );
+   double bf[NV];
+   bf[0] = bout[0];
+   bf[1] = bout[1 + D];
+   bf[2] = bout[1 + ( D + 1 ) % 3];
+   bf[3] = bout[1 + ( D + 2 ) % 3];
+   bf[4] = bout[4];
+   st_zero( bL );
+   st_zero( bQ );
+   if ( RS == 4 && rs_hllc_rev( L, Q, P.min_pres, bf, bL, bQ ) ) return;
+   rs_hlle_rev( L, Q, bf, bL, bQ );
+}
+
+// face_state<D>: primitive face (rho, vx, vy, vz, P) -> interface state; bw accumulates
+template <int D>
+__device__ __forceinline__ void face_state_rev( const double w[NV], const StD &s, const StD &bs, const Prm<double> &P,
+                                                double bw[NV] )
+{
+   constexpr int IN = 1 + D, I1 = 1 + ( D + 1 ) % 3, I2 = 1 + ( D + 2 ) % 3;
+   const double K = fmad( w[3], w[3], fmad( w[2], w[2], w[1]*w[1] ) );
+   // cs = sqrt(gamma p ir)
+   const double bx = bs.cs*( 0.5/s.cs );
+   const double bp = bs.p + bx*P.gamma*s.ir;
+   const double bir = bs.ir + bx*P.gamma*s.p;
+   // e = (0.5 rho) K + P/(gamma-1)
+   const double bK = bs.e*0.5*w[0];
+   bw[0]  += bs.d + bs.mn*w[IN] + bs.m1*w[I1] + bs.m2*w[I2] + bs.e*0.5*K - s.ir*s.ir*bir;
+   bw[IN] += bs.mn*w[0] + bs.u;
+   bw[I1] += bs.m1*w[0];
+   bw[I2] += bs.m2*w[0];
+   bw[1]  += 2.0*bK*w[1];
+   bw[2]  += 2.0*bK*w[2];
+   bw[3]  += 2.0*bK*w[3];
+   bw[4]  += bs.e*P.inv_gm1 + ( ( w[4] < P.min_pres ) ? 0.0 : bp );
+}
+
+// conserved cell -> interface state: ir = 1/rho, p = max((gamma-1)(E - |m|^2 ir/2), MinPres),
+// cs = sqrt(gamma p ir), u = ir m_n; bu accumulates
+template <int D>
+__device__ __forceinline__ void cell_st_rev( const double u[NV], const StD &s, const StD &bs, const Prm<double> &P,
+                                             double bu[NV] )
+{
+   constexpr int IN = 1 + D, I1 = 1 + ( D + 1 ) % 3, I2 = 1 + ( D + 2 ) % 3;
+   const double S = 0.5*( u[1]*u[1] + u[2]*u[2] + u[3]*u[3] );
+   const double eint = u[4] - S*s.ir;
+   const double bx = bs.cs*( 0.5/s.cs );
+   const double bp = bs.p + bx*P.gamma*s.ir;
+   double bir = bs.ir + bx*P.gamma*s.p + bs.u*s.mn;
+   const double beint = ( eint*P.gm1 < P.min_pres ) ? 0.0 : bp*P.gm1;
+   const double bS = -beint*s.ir;
+   bir -= beint*S;
+   bu[0]  += bs.d - s.ir*s.ir*bir;
+   bu[IN] += bs.mn + bs.u*s.ir;
+   bu[I1] += bs.m1;
+   bu[I2] += bs.m2;
+   bu[1]  += bS*u[1];
+   bu[2]  += bS*u[2];
+   bu[3]  += bS*u[3];
+   bu[4]  += bs.e + beint;
+}
+
+} // namespace gamer_amd

@@ -78,12 +78,15 @@ def test_tangent_matches_central_differences(rsolver):
             assert ((fd - jv).norm() / jv.norm()).item() <= 1e-6
 
 
-@pytest.mark.parametrize("case", ["smooth", "sod"])
+@pytest.mark.parametrize("case", ["smooth", "sod", "floored"])
 @pytest.mark.parametrize("rsolver,limiter", [("HLLC", "VL_GMINMOD"), ("HLLE", "VL_GMINMOD"), ("ROE", "VL_GMINMOD"),
                                              ("HLLC", "VANLEER"), ("HLLC", "GMINMOD"), ("HLLC", "ALBADA")])
 def test_step_dot_product(case, rsolver, limiter):
     """<lam, J v> and <J^T lam, v> agree to round-off: the reverse sweep is the exact transpose of the
-    tangent map, branch choices included."""
+    tangent map, branch choices included.  The adjoint of HLLC / HLLE is a hand-written reverse sweep and the
+    tangent runs dual numbers, so this compares two independent differentiations.  "floored" is the Sod +
+    Sedov input with a pressure floor of 0.5, which engages the floors of the cell and face states and of
+    the HLLC star pressure on the low-pressure side."""
     from gamer_amd import hydro
     print(f"[gpu] dot test {case} {rsolver} {limiter}", flush=True)
     u, dt, dh = _smooth_pg() if case == "smooth" else _sod_pg()
@@ -91,6 +94,8 @@ def test_step_dot_product(case, rsolver, limiter):
     v = torch.randn(u.shape, generator=g, dtype=torch.float64).to(DEV)
     lam = torch.randn((u.shape[0], 5, 16, 16, 16), generator=g, dtype=torch.float64).to(DEV)
     kw = dict(rsolver=rsolver, limiter=limiter)
+    if case == "floored":
+        kw["min_pres"] = 0.5
     ws = hydro.adjoint_workspace(u.shape[0])
     _, jv = hydro.mhm_tangent(u, v, dt, dh, workspace=ws, **kw)
     lin = hydro.mhm_adjoint(u, lam, dt, dh, workspace=ws, **kw)
