@@ -6,10 +6,12 @@
 //
 //   forward   F1 half-step fluxes Fh (Hydro_RiemannPredict_Flux)   -> ws
 //             F2 half-step update, floors, primitive W (Hydro_RiemannPredict)   -> ws
-//             F3 PLM faces FC for every cell of the 18^3 reconstruction block (Hydro_DataReconstruction)   -> ws
+//             F3 (tangent / plain forward only) PLM faces FC of the 18^3 reconstruction block
+//                (Hydro_DataReconstruction)   -> ws
 //             F4 (tangent / plain forward only) full-step fluxes and update of the 16^3 output cells
 //   reverse   R4 lambda_F of each full-step face from lambda_out (transpose of Hydro_FullStepUpdate),
-//                lambda_FC = J_RS^T lambda_F (transpose of Hydro_ComputeFlux)
+//                lambda_FC = J_RS^T lambda_F (transpose of Hydro_ComputeFlux), with the two PLM faces of the
+//                face recomputed from W
 //             R3 lambda_W = sum over the three stencil cells of J_PLM^T lambda_FC (transpose of the limiter,
 //                face clamp and mirror)
 //             R2 lambda_U_half = J_Con2Pri^T lambda_W (transpose of the floors and primitive conversion)
@@ -51,7 +53,7 @@ constexpr int STRIDE[3] = { 1, NX, N2 };
 
 // workspace layout per patch group, in units of N3 doubles
 constexpr int WS_W   = 0;            // [5]       half-step primitives (cells 1..20)
-constexpr int WS_FC  = 5;            // [3][2][5] PLM faces L (0) / R (1) (cells 2..19)
+constexpr int WS_FC  = 5;            // [3][2][5] PLM faces L (0) / R (1) (cells 2..19; tangent kernel only)
 constexpr int WS_LW  = 35;           // [5]       lambda_W
 constexpr int WS_LU  = 40;           // [5]       lambda of the half-step conserved state
 constexpr int WS_LFC = 45;           // [3][2][5] lambda of the faces
@@ -203,7 +205,7 @@ template <> struct Load< Dual<1> >
 };
 
 // F1-F3: forward phases into the workspace
-template <int RS, int LIM, typename T>
+template <int RS, int LIM, typename T, bool STORE_FC = true>
 __device__ void forward_phases( const double *in, const double *tin, Fwd<T> &F, const Prm<T> &P )
 {
    const int tid = threadIdx.x;
@@ -241,6 +243,7 @@ __device__ void forward_phases( const double *in, const double *tin, Fwd<T> &F, 
       for (int v=0; v<NV; v++) F.w( v, c ) = w[v];
    }
    __syncthreads();
+   if ( !STORE_FC ) return;
    // F3 PLM faces of the 18^3 reconstruction block, all three directions
    for (int c = tid; c < N3; c += NT)
    {
@@ -352,12 +355,11 @@ adjoint_kernel( const double *__restrict__ g_in, const double *__restrict__ g_lo
    Fwd<double> F( ws, WS_LS );
    Adj A( ws );
 
-   forward_phases<RS, LIM, double>( in, nullptr, F, P );
+   // F1, F2 (the PLM faces are recomputed where they are needed instead of stored)
+   forward_phases<RS, LIM, double, false>( in, nullptr, F, P );
 
-   // R4 (a) zero lambda_FC: only the faces that feed a full-step flux receive a value
-   for (int e = tid; e < 30*N3; e += NT) A.LFC[e] = 0.0;
-   __syncthreads();
-   // R4 (b) full-step faces: lambda_F = dt/dh (lambda_out(c+e_d) - lambda_out(c)), then J_RS^T per side
+   // R4 full-step faces: lambda_F = dt/dh (lambda_out(c+e_d) - lambda_out(c)), then J_RS^T per side.  Only the
+   // faces that feed a full-step flux receive a value; R3 reads the others as zero.
    for (int e = tid; e < 3*N3; e += NT)
    {
       const int d = e / N3, c = e % N3;
@@ -376,11 +378,19 @@ adjoint_kernel( const double *__restrict__ g_in, const double *__restrict__ g_lo
          const double lh = hi ? lout[ (size_t)v*PS*PS*PS + out_index( i2, j2, k2 ) ] : 0.0;
          lf[v] = P.dt_dh*( lh - ll );
       }
+      // the R face of c and the L face of c + e_d (F3's values, recomputed from the half-step primitives)
+      double wl[NV], wr[NV];
+#     pragma unroll
+      for (int v=0; v<NV; v++)
+      {
+         const double wm = F.w( v, c - s ), w0 = F.w( v, c ), w1 = F.w( v, c + s ), w2 = F.w( v, c + 2*s );
+         double fl, fr;
+         plm<LIM>( wm, w0, w1, P, fl, wl[v] );
+         plm<LIM>( w0, w1, w2, P, wr[v], fr );
+      }
       if constexpr ( RS != 2 )                    // HLLC / HLLE: one reverse sweep for both sides
       {
-         double wl[NV], wr[NV], gl[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 }, gr[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
-#        pragma unroll
-         for (int v=0; v<NV; v++) { wl[v] = F.fc( d, 1, v, c );  wr[v] = F.fc( d, 0, v, c + s ); }
+         double gl[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 }, gr[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
          full_flux_rev<RS>( d, wl, wr, P, lf, gl, gr );
 #        pragma unroll
          for (int v=0; v<NV; v++) { A.lfc( d, 1, v, c ) = gl[v];  A.lfc( d, 0, v, c + s ) = gr[v]; }
@@ -388,14 +398,14 @@ adjoint_kernel( const double *__restrict__ g_in, const double *__restrict__ g_lo
       }
       for (int q=0; q<2*NV; q++)                  // Roe: input direction q, R face of c (q < 5), L face of c + e_d
       {
-         D1 wl[NV], wr[NV], f[NV];
+         D1 dl[NV], dr[NV], f[NV];
 #        pragma unroll
          for (int v=0; v<NV; v++)
          {
-            wl[v] = D1( F.fc( d, 1, v, c ) );      wl[v].d[0] = ( q == v )      ? 1.0 : 0.0;
-            wr[v] = D1( F.fc( d, 0, v, c + s ) );  wr[v].d[0] = ( q == v + NV ) ? 1.0 : 0.0;
+            dl[v] = D1( wl[v] );  dl[v].d[0] = ( q == v )      ? 1.0 : 0.0;
+            dr[v] = D1( wr[v] );  dr[v].d[0] = ( q == v + NV ) ? 1.0 : 0.0;
          }
-         full_flux<RS>( d, wl, wr, PD, f );
+         full_flux<RS>( d, dl, dr, PD, f );
          double g = 0.0;
 #        pragma unroll
          for (int v=0; v<NV; v++) g += lf[v]*f[v].d[0];
@@ -418,13 +428,18 @@ adjoint_kernel( const double *__restrict__ g_in, const double *__restrict__ g_lo
          for (int off = -1; off <= 1; off++)        // stencil centre c' = c - off e_d; c sits at position off
          {
             const int cc = c - off*s;
-            int ci, cj, ck;
+            int ci, cj, ck, n, a, b;
             coords( cc, ci, cj, ck );
             if ( !( in_rng( ci, 2, 19 ) && in_rng( cj, 2, 19 ) && in_rng( ck, 2, 19 ) ) ) continue;
+            // faces of cc that R4 wrote: the L face feeds the full-step face below cc, the R face the one above
+            split( d, ci, cj, ck, n, a, b );
+            const bool tan = in_rng( a, 3, 18 ) && in_rng( b, 3, 18 );
+            const bool vl = tan && in_rng( n, 3, 19 ), vr = tan && in_rng( n, 2, 18 );
+            if ( !vl && !vr ) continue;
 #           pragma unroll
             for (int v=0; v<NV; v++)
             {
-               const double gl = A.lfc( d, 0, v, cc ), gr = A.lfc( d, 1, v, cc );
+               const double gl = vl ? A.lfc( d, 0, v, cc ) : 0.0, gr = vr ? A.lfc( d, 1, v, cc ) : 0.0;
                if ( gl == 0.0 && gr == 0.0 ) continue;
                D1 L( F.w( v, cc - s ) ), C( F.w( v, cc ) ), R( F.w( v, cc + s ) ), fL, fR;
                L.d[0] = ( off == -1 ) ? 1.0 : 0.0;
