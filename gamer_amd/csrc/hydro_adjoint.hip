@@ -13,16 +13,18 @@
 //                lambda_FC = J_RS^T lambda_F (transpose of Hydro_ComputeFlux), with the two PLM faces of the
 //                face recomputed from W
 //             R3 lambda_W = sum over the three stencil cells of J_PLM^T lambda_FC (transpose of the limiter,
-//                face clamp and mirror)
+//                face clamp and mirror), one line march per direction
 //             R2 lambda_U_half = J_Con2Pri^T lambda_W (transpose of the floors and primitive conversion)
 //             R1 lambda of every half-step face, then J_RS^T of the cell-centred half-step solve per side
 //             R0 lambda_in = lambda_out (core) + lambda_U_half + the half-step face contributions
 //
 // The reverse sweep is written out by hand.  For HLLC and HLLE the Riemann solves run backwards as hand-written
 // reverse-mode sweeps (hydro_reverse.hpp: solver + interface-state builders, one pass per face for all ten
-// inputs).  The other local Jacobians (the Roe solver, limiter + clamp, conversions with floors) come from
-// forward-mode dual numbers (hydro_dual.hpp) pushed through the same templated device functions the production
-// kernels use, one input direction at a time.  Both forms take exactly the branches of the forward evaluation.
+// inputs).  The PLM reconstruction (face clamps and mirror) is reversed by hand for every limiter, with the
+// limited slope's two partials from a two-direction dual evaluation.  The other local Jacobians (the Roe solver,
+// the half-step conversion with floors) come from forward-mode dual numbers (hydro_dual.hpp) pushed through the
+// same templated device functions the production kernels use, one input direction at a time.  All forms take
+// exactly the branches of the forward evaluation.
 // The reverse phases are gathers: every workspace entry has one writer, so there are no atomics and the result
 // is deterministic.
 //
@@ -415,45 +417,46 @@ adjoint_kernel( const double *__restrict__ g_in, const double *__restrict__ g_lo
    }
    __syncthreads();
 
-   // R3 lambda_W of the 20^3 half-step cells: gather over the reconstruction stencils that contain the cell
-   for (int c = tid; c < N3; c += NT)
+   // R3 lambda_W of the 20^3 half-step cells, one direction at a time.  Lane t marches line t along d through the
+   // reconstruction stencils (centres 2..19) and reverses each one (plm_face_rev), keeping a window of the three
+   // cell adjoints the stencil touches; a cell is complete once the stencil above it is done.  Direction 0 writes
+   // every cell of the 20^3 block (zero on lines no full-step face reads), directions 1 and 2 add.  The faces of
+   // a centre that R4 wrote: the L face when the centre's normal coordinate is 3..19, the R face when it is 2..18,
+   // with both tangential coordinates in 3..18.
+   for (int d=0; d<3; d++)
    {
-      int i, j, k;
-      coords( c, i, j, k );
-      if ( !( in_rng( i, 1, 20 ) && in_rng( j, 1, 20 ) && in_rng( k, 1, 20 ) ) ) continue;
-      double acc[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
-      for (int d=0; d<3; d++)
+      const int s = STRIDE[d];
+      const int nline = ( d == 0 ) ? 400 : 256;
+      for (int ln = tid; ln < nline; ln += NT)
       {
-         const int s = STRIDE[d];
-         for (int off = -1; off <= 1; off++)        // stencil centre c' = c - off e_d; c sits at position off
+         const int  a = ( d == 0 ) ? 1 + ln % 20 : 3 + ln % 16, b = ( d == 0 ) ? 1 + ln / 20 : 3 + ln / 16;
+         const bool live = in_rng( a, 3, 18 ) && in_rng( b, 3, 18 );
+         const int  c0 = ( d == 0 ) ? NX*a + N2*b : ( d == 1 ) ? a + N2*b : a + NX*b;     // n = 0 of the line
+         for (int v=0; v<NV; v++)
          {
-            const int cc = c - off*s;
-            int ci, cj, ck, n, a, b;
-            coords( cc, ci, cj, ck );
-            if ( !( in_rng( ci, 2, 19 ) && in_rng( cj, 2, 19 ) && in_rng( ck, 2, 19 ) ) ) continue;
-            // faces of cc that R4 wrote: the L face feeds the full-step face below cc, the R face the one above
-            split( d, ci, cj, ck, n, a, b );
-            const bool tan = in_rng( a, 3, 18 ) && in_rng( b, 3, 18 );
-            const bool vl = tan && in_rng( n, 3, 19 ), vr = tan && in_rng( n, 2, 18 );
-            if ( !vl && !vr ) continue;
-#           pragma unroll
-            for (int v=0; v<NV; v++)
+            if ( !live )
             {
-               const double gl = vl ? A.lfc( d, 0, v, cc ) : 0.0, gr = vr ? A.lfc( d, 1, v, cc ) : 0.0;
-               if ( gl == 0.0 && gr == 0.0 ) continue;
-               D1 L( F.w( v, cc - s ) ), C( F.w( v, cc ) ), R( F.w( v, cc + s ) ), fL, fR;
-               L.d[0] = ( off == -1 ) ? 1.0 : 0.0;
-               C.d[0] = ( off ==  0 ) ? 1.0 : 0.0;
-               R.d[0] = ( off ==  1 ) ? 1.0 : 0.0;
-               plm<LIM>( L, C, R, PD, fL, fR );
-               acc[v] += gl*fL.d[0] + gr*fR.d[0];
+               for (int n=1; n<=20; n++) A.lw( v, c0 + n*s ) = 0.0;
+               continue;
             }
+            double wm = F.w( v, c0 + s ), w0 = F.w( v, c0 + 2*s ), am = 0.0, a0 = 0.0;
+            for (int n=2; n<=19; n++)
+            {
+               const int    cc = c0 + n*s;
+               const double wp = F.w( v, cc + s );
+               const double gl = ( n >= 3 )  ? A.lfc( d, 0, v, cc ) : 0.0;
+               const double gr = ( n <= 18 ) ? A.lfc( d, 1, v, cc ) : 0.0;
+               double br = 0.0;
+               plm_face_rev<LIM>( wm, w0, wp, P.limiter, P.coeff, gl, gr, am, a0, br );
+               if ( d == 0 ) A.lw( v, cc - s ) = am; else A.lw( v, cc - s ) += am;
+               am = a0;  a0 = br;  wm = w0;  w0 = wp;
+            }
+            if ( d == 0 ) { A.lw( v, c0 + 19*s ) = am;   A.lw( v, c0 + 20*s ) = a0; }
+            else          { A.lw( v, c0 + 19*s ) += am;  A.lw( v, c0 + 20*s ) += a0; }
          }
       }
-#     pragma unroll
-      for (int v=0; v<NV; v++) A.lw( v, c ) = acc[v];
+      __syncthreads();
    }
-   __syncthreads();
 
    // R2 lambda of the half-step conserved state: J_Con2Pri^T lambda_W (floors included)
    for (int c = tid; c < N3; c += NT)

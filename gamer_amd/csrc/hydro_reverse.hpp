@@ -1,5 +1,5 @@
-// Reverse-mode local sweeps (FP64) of the interface-state builders and of the HLLC / HLLE interface solvers,
-// for the adjoint of the MHM_RP step.
+// Reverse-mode local sweeps (FP64) of the interface-state builders, of the HLLC / HLLE interface solvers and of
+// the PLM face reconstruction, for the adjoint of the MHM_RP step.
 //
 // Each *_rev function takes the primal inputs, recomputes the forward intermediates and accumulates the input
 // adjoints for one output adjoint in a single backward pass: the row vector bf^T J of the piece, which the
@@ -14,8 +14,9 @@
 //   rs_hllc, rs_hlle, riemann, face_state          hydro_math.hpp (CPU_Shared_RiemannSolver_HLLC.cpp:391-655,
 //                                                  _HLLE.cpp:301-698, CPU_FluidSolver_MHM.cpp:737-786)
 //   conserved cell -> interface state (cell_st)    hydro_adjoint.hip (Hydro_Con2Pres, FluUtility.cpp:1259-1300)
+//   plm_face (limited slope, face clamp, mirror)    hydro_math.hpp (DataReconstruction.cpp:403-430, 1917-2032)
 #pragma once
-#include "hydro_math.hpp"
+#include "hydro_dual.hpp"
 
 namespace gamer_amd {
 
@@ -245,6 +246,51 @@ __device__ __forceinline__ void cell_st_rev( const double u[NV], const StD &s, c
    bu[2]  += bS*u[2];
    bu[3]  += bS*u[3];
    bu[4]  += bs.e + beint;
+}
+
+// clamp_between(x, a, b) = min(max(x, min(a, b)), max(a, b)) under the dual tie rules: which operand the result
+// is (0 = x, 1 = a, 2 = b)
+__device__ __forceinline__ int clamp_pick( double x, double a, double b )
+{
+   const bool   lo_a = min_first( a, b ), hi_a = max_first( a, b );
+   const double lo = lo_a ? a : b, hi = hi_a ? a : b;
+   const bool   x1_x = max_first( x, lo );
+   const double x1 = x1_x ? x : lo;
+   if ( min_first( x1, hi ) ) return x1_x ? 0 : ( lo_a ? 1 : 2 );
+   return hi_a ? 1 : 2;
+}
+
+// plm_face<LIM>: faces fL, fR of the centre C from (L, C, R).  Given their adjoints (bfL, bfR), accumulate the
+// adjoints of L, C, R.  The face clamps and the mirror fL = 2C - fR are reversed here; the limited slope's two
+// partials come from a two-direction dual evaluation of the same slope function.
+template <int LIM>
+__device__ __forceinline__ void plm_face_rev( double L, double C, double Rr, int lim, double coeff,
+                                              double bfL, double bfR, double &bL, double &bC, double &bR )
+{
+   Dual<2> sl( C - L ), sr( Rr - C );
+   sl.d[0] = 1.0;
+   sr.d[1] = 1.0;
+   const Dual<2> s = ( LIM == 4 ) ? slope_vl_gminmod( sl, sr, Dual<2>( coeff ) )
+                                  : plm_slope<LIM>( sl, sr, Dual<2>( 0.0 ), lim, Dual<2>( coeff ) );
+   // forward: fl = clamp(C - s/2, C, L), fr = clamp(2C - fl, C, R), fL = 2C - fr, fR = fr
+   const double x1 = C - 0.5*s.v;
+   const int    p1 = clamp_pick( x1, C, L );
+   const double fl = ( p1 == 0 ) ? x1 : ( p1 == 1 ) ? C : L;
+   const int    p2 = clamp_pick( 2.0*C - fl, C, Rr );
+   // reverse
+   const double bfr = bfR - bfL;
+   double bc = 2.0*bfL, bl = 0.0, br = 0.0;
+   double bx2 = 0.0, bx1 = 0.0;
+   if ( p2 == 0 ) bx2 = bfr; else if ( p2 == 1 ) bc += bfr; else br += bfr;
+   bc += 2.0*bx2;
+   const double bfl = -bx2;
+   if ( p1 == 0 ) bx1 = bfl; else if ( p1 == 1 ) bc += bfl; else bl += bfl;
+   bc += bx1;
+   const double bs = -0.5*bx1;
+   const double bsl = bs*s.d[0], bsr = bs*s.d[1];
+   bL += bl - bsl;
+   bC += bc + bsl - bsr;
+   bR += br + bsr;
 }
 
 } // namespace gamer_amd

@@ -80,7 +80,8 @@ def test_tangent_matches_central_differences(rsolver):
 
 @pytest.mark.parametrize("case", ["smooth", "sod", "floored"])
 @pytest.mark.parametrize("rsolver,limiter", [("HLLC", "VL_GMINMOD"), ("HLLE", "VL_GMINMOD"), ("ROE", "VL_GMINMOD"),
-                                             ("HLLC", "VANLEER"), ("HLLC", "GMINMOD"), ("HLLC", "ALBADA")])
+                                             ("HLLC", "VANLEER"), ("HLLC", "GMINMOD"), ("HLLC", "ALBADA"),
+                                             ("HLLC", "CENTRAL")])
 def test_step_dot_product(case, rsolver, limiter):
     """<lam, J v> and <J^T lam, v> agree to round-off: the reverse sweep is the exact transpose of the
     tangent map, branch choices included.  The adjoint of HLLC / HLLE is a hand-written reverse sweep and the
