@@ -340,52 +340,71 @@ interpolate_kernel( T *__restrict__ g_c, T *__restrict__ g_f, int nbox, InterpPr
    }
 }
 
-// Copies between per-patch data and boxes: one block per (box, field, z plane), 32 x 8 threads over (x, y), so a
-// cell's coordinates come from the thread and block indices without divisions; each thread issues its (up to 4) row
-// loads before its stores.
+// Coarse boxes of general size: one block per (box, field, z plane), 32 x 8 threads over (x, y), so a cell's
+// coordinates come from the thread and block indices without divisions.
 constexpr int CPX = 32, CPY = 8;
 
-// patch-group input window (N = 16 + 2 ghost <= 32 per axis): cell (x,y,z) from the patch slot[pg][Z/8][Y/8][X/8],
-// (X,Y,Z) = (x,y,z) - ghost + 8; slot -1: from the fine box fbox at (x,y,z) + fb.origin, or kept if fbox == NULL.
-// A block covers CPZ planes; each thread loads its (up to 4 x CPZ) cells before storing them.
-constexpr int CPZ = 4;
+// patch-group input window (N = 16 + 2 ghost <= 32 per axis, N even): cell (x,y,z) from the patch
+// slot[pg][Z/8][Y/8][X/8], (X,Y,Z) = (x,y,z) - ghost + 8; slot -1: from the fine box fbox at (x,y,z) + fb.origin, or
+// kept if fbox == NULL.  Each thread copies EPT consecutive cells of the linear window index e = x + N (y + N z) of
+// one (PG, field): every lane is busy, the EPT loads are independent, and the store is one aligned vector (N^3 is a
+// multiple of 8, so a group never straddles two windows; an unaligned destination takes scalar stores).  (x,y,z)
+// of the first cell come from a float reciprocal: after the +0.5 shift, e/N^2 and r/N sit at least 1/(2 N^2) away
+// from an integer, far above the float error for N <= 32.  Measured against a 32 x 8-thread (x, y) block layout
+// and a lane-contiguous layout (each instruction over 64 consecutive cells), this one was the fastest.
+constexpr int EPT = 4;
+template <typename IO> struct Vec4;
+template <> struct Vec4<float>  { typedef float  type __attribute__(( ext_vector_type( 4 ) )); };
+template <> struct Vec4<double> { typedef double type __attribute__(( ext_vector_type( 4 ) )); };
+
 template <typename IO>
-__global__ void __launch_bounds__( CPX*CPY )
+__global__ void __launch_bounds__( NT )
 prepare_patches_kernel( const IO *__restrict__ patch, const IO *__restrict__ fbox, IO *__restrict__ pg_in,
                         const int *__restrict__ slot, int ghost, BoxGeom fb )
 {
-   const int p = blockIdx.x, v = blockIdx.y, n = PS + 2*ghost, x = threadIdx.x;
-   if ( x >= n ) return;
-   const int X = x - ghost + H;
-   const int *sl = slot + (size_t)p*64 + ( X >> 3 );
-   const IO *pp = patch + (size_t)v*P3 + ( X & 7 );
-   const IO *src = fbox ? fbox + ( (size_t)p*NV + v )*fb.size[2]*fb.size[1]*fb.size[0] + x + fb.origin[0] : nullptr;
-   IO val[CPZ][4];
-   bool put[CPZ][4];
+   const int p = blockIdx.x, v = blockIdx.y, n = PS + 2*ghost, n2 = n*n, n3 = n2*n;
+   const int e0 = ( blockIdx.z*NT + threadIdx.x )*EPT;
+   if ( e0 >= n3 ) return;
+   int z = (int)( ( (float)e0 + 0.5f )*( 1.0f/(float)n2 ) );
+   const int r = e0 - z*n2;
+   int y = (int)( ( (float)r + 0.5f )*( 1.0f/(float)n ) );
+   int x = r - y*n;
+   const int *sl = slot + (size_t)p*64;
+   const IO  *pv = patch + (size_t)v*P3;
+   const IO  *fv = fbox ? fbox + ( (size_t)p*NV + v )*fb.size[2]*fb.size[1]*fb.size[0] : nullptr;
+   IO   val[EPT];
+   bool put[EPT];
 #  pragma unroll
-   for (int kz = 0; kz < CPZ; kz++)
+   for (int k = 0; k < EPT; k++)
    {
-      const int z = blockIdx.z*CPZ + kz, Z = z - ghost + H;
-#     pragma unroll
-      for (int k = 0; k < 4; k++)
-      {
-         const int y = threadIdx.y + k*CPY, Y = y - ghost + H;
-         put[kz][k] = false;
-         if ( z < n && y < n )
-         {
-            const int s = sl[ ( Z >> 3 )*16 + ( Y >> 3 )*4 ];
-            if ( s >= 0 )   { val[kz][k] = pp[ (size_t)s*NV*P3 + ( Z & 7 )*H*H + ( Y & 7 )*H ];  put[kz][k] = true; }
-            else if ( src ) { val[kz][k] = src[ ( (size_t)( z + fb.origin[2] )*fb.size[1] + y + fb.origin[1] )*fb.size[0] ];
-                              put[kz][k] = true; }
-         }
-      }
+      const int X = x - ghost + H, Y = y - ghost + H, Z = z - ghost + H;
+      const int s = sl[ ( Z >> 3 )*16 + ( Y >> 3 )*4 + ( X >> 3 ) ];
+      put[k] = true;
+      if ( s >= 0 )  val[k] = pv[ (size_t)s*NV*P3 + ( Z & 7 )*H*H + ( Y & 7 )*H + ( X & 7 ) ];
+      else if ( fv ) val[k] = fv[ ( (size_t)( z + fb.origin[2] )*fb.size[1] + y + fb.origin[1] )*fb.size[0] + x + fb.origin[0] ];
+      else           put[k] = false;
+      // next cell of the window
+      x++;
+      const bool wx = ( x == n );
+      x = wx ? 0 : x;
+      y = wx ? y + 1 : y;
+      const bool wy = ( y == n );
+      y = wy ? 0 : y;
+      z = wy ? z + 1 : z;
    }
-   IO *dst = pg_in + ( (size_t)p*NV + v )*n*n*n + x;
-#  pragma unroll
-   for (int kz = 0; kz < CPZ; kz++)
+   IO *dst = pg_in + ( (size_t)p*NV + v )*n3 + e0;
+   const bool aligned = ( reinterpret_cast<size_t>( dst ) % sizeof( typename Vec4<IO>::type ) ) == 0;
+   if ( aligned && put[0] && put[1] && put[2] && put[3] )
+   {
+      typename Vec4<IO>::type q;
+      q.x = val[0];  q.y = val[1];  q.z = val[2];  q.w = val[3];
+      *reinterpret_cast<typename Vec4<IO>::type*>( dst ) = q;
+   }
+   else
+   {
 #     pragma unroll
-      for (int k = 0; k < 4; k++)
-         if ( put[kz][k] ) dst[ ( ( blockIdx.z*CPZ + kz )*n + threadIdx.y + k*CPY )*n ] = val[kz][k];
+      for (int k = 0; k < EPT; k++) if ( put[k] ) dst[k] = val[k];
+   }
 }
 
 // box[q][v][z][y][x] (size) = patch[ slot[q][Z/8][Y/8][X/8] ][v][Z%8][Y%8][X%8], (X,Y,Z) = origin + (x,y,z);
@@ -451,7 +470,8 @@ int launch_assemble( const IO *patch, const IO *fbox, IO *pg_in, int npg, const 
                      hipStream_t stream )
 {
    if ( npg <= 0 ) return 0;
-   hipLaunchKernelGGL( prepare_patches_kernel<IO>, dim3( npg, NV, ( PS + 2*ghost + CPZ - 1 )/CPZ ), dim3( CPX, CPY ), 0, stream, patch, fbox,
+   const int n = PS + 2*ghost, per = NT*EPT;
+   hipLaunchKernelGGL( prepare_patches_kernel<IO>, dim3( npg, NV, ( n*n*n + per - 1 )/per ), dim3( NT ), 0, stream, patch, fbox,
                        pg_in, slot, ghost, fb );
    return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
 }
