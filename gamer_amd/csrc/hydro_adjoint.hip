@@ -18,13 +18,13 @@
 //             R1 lambda of every half-step face, then J_RS^T of the cell-centred half-step solve per side
 //             R0 lambda_in = lambda_out (core) + lambda_U_half + the half-step face contributions
 //
-// The reverse sweep is written out by hand.  For HLLC and HLLE the Riemann solves run backwards as hand-written
+// The reverse sweep is written out by hand.  The Riemann solves (HLLC, HLLE and Roe) run backwards as hand-written
 // reverse-mode sweeps (hydro_reverse.hpp: solver + interface-state builders, one pass per face for all ten
-// inputs).  The PLM reconstruction (face clamps and mirror) is reversed by hand for every limiter, with the
-// limited slope's two partials from a two-direction dual evaluation.  The other local Jacobians (the Roe solver,
-// the half-step conversion with floors) come from forward-mode dual numbers (hydro_dual.hpp) pushed through the
-// same templated device functions the production kernels use, one input direction at a time.  All forms take
-// exactly the branches of the forward evaluation.
+// inputs).  The PLM reconstruction (face clamps and mirror) is reversed by hand for every
+// limiter, with the limited slope's two partials from a two-direction dual evaluation.  The half-step conversion
+// with floors takes its local Jacobian from forward-mode dual numbers (hydro_dual.hpp) pushed through the same
+// templated device function, one input direction at a time.  All forms take exactly the branches of the forward
+// evaluation.
 // The reverse phases are gathers: every workspace entry has one writer, so there are no atomics and the result
 // is deterministic.
 //
@@ -105,7 +105,7 @@ __device__ __forceinline__ void full_flux( int d, const T wL[NV], const T wR[NV]
    else               riemann<RS, 2>( face_state<2>( wL, P ), face_state<2>( wR, P ), P, f );
 }
 
-// reverse sweeps of half_flux / full_flux (HLLC, HLLE): bL, bR = (d flux / d input side)^T bf, one pass
+// reverse sweeps of half_flux / full_flux: bL, bR = (d flux / d input side)^T bf, one pass
 template <int RS, int D>
 __device__ __forceinline__ void half_flux_rev_d( const double uL[NV], const double uR[NV], const Prm<double> &P,
                                                  const double bf[NV], double bL[NV], double bR[NV] )
@@ -390,30 +390,11 @@ adjoint_kernel( const double *__restrict__ g_in, const double *__restrict__ g_lo
          plm<LIM>( wm, w0, w1, P, fl, wl[v] );
          plm<LIM>( w0, w1, w2, P, wr[v], fr );
       }
-      if constexpr ( RS != 2 )                    // HLLC / HLLE: one reverse sweep for both sides
-      {
-         double gl[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 }, gr[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
-         full_flux_rev<RS>( d, wl, wr, P, lf, gl, gr );
-#        pragma unroll
-         for (int v=0; v<NV; v++) { A.lfc( d, 1, v, c ) = gl[v];  A.lfc( d, 0, v, c + s ) = gr[v]; }
-         continue;
-      }
-      for (int q=0; q<2*NV; q++)                  // Roe: input direction q, R face of c (q < 5), L face of c + e_d
-      {
-         D1 dl[NV], dr[NV], f[NV];
-#        pragma unroll
-         for (int v=0; v<NV; v++)
-         {
-            dl[v] = D1( wl[v] );  dl[v].d[0] = ( q == v )      ? 1.0 : 0.0;
-            dr[v] = D1( wr[v] );  dr[v].d[0] = ( q == v + NV ) ? 1.0 : 0.0;
-         }
-         full_flux<RS>( d, dl, dr, PD, f );
-         double g = 0.0;
-#        pragma unroll
-         for (int v=0; v<NV; v++) g += lf[v]*f[v].d[0];
-         if ( q < NV ) A.lfc( d, 1, q, c ) = g;
-         else          A.lfc( d, 0, q - NV, c + s ) = g;
-      }
+      // one reverse sweep for both sides
+      double gl[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 }, gr[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+      full_flux_rev<RS>( d, wl, wr, P, lf, gl, gr );
+#     pragma unroll
+      for (int v=0; v<NV; v++) { A.lfc( d, 1, v, c ) = gl[v];  A.lfc( d, 0, v, c + s ) = gr[v]; }
    }
    __syncthreads();
 
@@ -503,23 +484,7 @@ adjoint_kernel( const double *__restrict__ g_in, const double *__restrict__ g_lo
 #     pragma unroll
       for (int v=0; v<NV; v++) { uL[v] = in[ v*N3 + c ];  uR[v] = in[ v*N3 + c + s ]; }
       double gl[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 }, gr[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
-      if constexpr ( RS != 2 ) half_flux_rev<RS>( d, uL, uR, P, lf, gl, gr );
-      else
-      for (int q=0; q<2*NV; q++)                  // Roe: one dual evaluation per input direction
-      {
-         D1 dl[NV], dr[NV], f[NV];
-#        pragma unroll
-         for (int v=0; v<NV; v++)
-         {
-            dl[v] = D1( uL[v] );  dl[v].d[0] = ( q == v )      ? 1.0 : 0.0;
-            dr[v] = D1( uR[v] );  dr[v].d[0] = ( q == v + NV ) ? 1.0 : 0.0;
-         }
-         half_flux<RS>( d, dl, dr, PD, f );
-         double g = 0.0;
-#        pragma unroll
-         for (int v=0; v<NV; v++) g += lf[v]*f[v].d[0];
-         if ( q < NV ) gl[q] = g; else gr[q - NV] = g;
-      }
+      half_flux_rev<RS>( d, uL, uR, P, lf, gl, gr );
 #     pragma unroll
       for (int v=0; v<NV; v++) { A.ls( d, 0, v, c ) = gl[v];  A.ls( d, 1, v, c + s ) = gr[v]; }
    }

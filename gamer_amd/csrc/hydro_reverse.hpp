@@ -1,5 +1,5 @@
-// Reverse-mode local sweeps (FP64) of the interface-state builders, of the HLLC / HLLE interface solvers and of
-// the PLM face reconstruction, for the adjoint of the MHM_RP step.
+// Reverse-mode local sweeps (FP64) of the interface-state builders, of the HLLC / HLLE / Roe interface solvers and
+// of the PLM face reconstruction, for the adjoint of the MHM_RP step.
 //
 // Each *_rev function takes the primal inputs, recomputes the forward intermediates and accumulates the input
 // adjoints for one output adjoint in a single backward pass: the row vector bf^T J of the piece, which the
@@ -11,8 +11,8 @@
 // floors, the HLLE rescue of a NaN HLLC flux).
 //
 // Forward definitions being reversed:
-//   rs_hllc, rs_hlle, riemann, face_state          hydro_math.hpp (CPU_Shared_RiemannSolver_HLLC.cpp:391-655,
-//                                                  _HLLE.cpp:301-698, CPU_FluidSolver_MHM.cpp:737-786)
+//   rs_hllc, rs_hlle, rs_roe, riemann, face_state  hydro_math.hpp (CPU_Shared_RiemannSolver_HLLC.cpp:391-655,
+//                                                  _HLLE.cpp:301-698, _Roe.cpp:87-711, CPU_FluidSolver_MHM.cpp:737-786)
 //   conserved cell -> interface state (cell_st)    hydro_adjoint.hip (Hydro_Con2Pres, FluUtility.cpp:1259-1300)
 //   plm_face (limited slope, face clamp, mirror)    hydro_math.hpp (DataReconstruction.cpp:403-430, 1917-2032)
 #pragma once
@@ -182,12 +182,151 @@ __device__ __forceinline__ bool rs_hllc_rev( const StD &L, const StD &Q, double 
    return true;
 }
 
-// riemann<RS, D> (HLLC with the HLLE rescue, or HLLE): bout is the adjoint of the lab-frame flux
+// Roe solver with its positivity fallback to HLLE (rs_roe).  The Roe-averaged state, the eigenvalues, the wave
+// amplitudes and the upwinded dissipation are recomputed in the forward's operation order, then reversed.
+__device__ __forceinline__ void rs_roe_rev( const StD &L, const StD &Q, const Prm<double> &P, const double bf[NV],
+                                            StD &bL, StD &bQ )
+{
+   const double HALF = 0.5, gm1 = P.gm1;
+   // forward
+   const double hl = L.ir*( L.e + L.p ), hr = Q.ir*( Q.e + Q.p );
+   const double sql = vsqrt( L.d ), sqr = vsqrt( Q.d );
+   const double rho = sql*sqr, irho = rcp( rho );
+   const double isql = rcp( sql ), isqr = rcp( sqr ), isum = rcp( sql + sqr );
+   const double NU = isql*L.mn + isqr*Q.mn, NVv = isql*L.m1 + isqr*Q.m1, NW = isql*L.m2 + isqr*Q.m2;
+   const double u = isum*NU, v = isum*NVv, w = isum*NW;
+   const double V2 = u*u + v*v + w*w;
+   const double NH = sql*hl + sqr*hr, H = isum*NH;
+   const double g0 = gm1*( H - HALF*V2 ), rg = rcp( P.gamma ), g1 = g0*rho*rg;
+   const bool   kf = !( g1 < P.min_pres );
+   const double gf = kf ? g1 : P.min_pres;
+   const double a2 = P.gamma*irho*gf, a = vsqrt( a2 );
+   const double e0 = u - a, e4 = u + a;
+
+   double bu = 0.0, bv = 0.0, bw = 0.0, bH = 0.0, bV2 = 0.0, ba = 0.0;
+   StD bs;
+   double bsp = 0.0;
+   if ( e0 >= 0.0 || e4 <= 0.0 )                 // supersonic: the upwind physical flux
+   {
+      st_zero( bs );
+      xflux_minus_rev( ( e0 >= 0.0 ) ? L : Q, 0.0, bf, bs, bsp );
+      add_upwind( ( e0 >= 0.0 ) ? bL : bQ, bs );
+      return;
+   }
+   const double j0 = Q.d - L.d, j1 = Q.mn - L.mn, j2 = Q.m1 - L.m1, j3 = Q.m2 - L.m2, j4 = Q.e - L.e;
+   const double a2_ = j2 - v*j0, a3_ = j3 - w*j0;
+   const double c1 = gm1*rcp( a2 ), T1 = j0*( H - u*u ) + u*j1 - j4 + v*a2_ + w*a3_, a1_ = c1*T1;
+   const double ra = rcp( a ), c0 = HALF*ra, T0 = j0*( u + a ) - j1 - a*a1_, a0_ = c0*T0;
+   const double a4_ = j0 - a0_ - a1_;
+   {
+      const double d1 = L.d + a0_, m1 = L.mn + a0_*e0, t1 = L.m1 + a0_*v, t2 = L.m2 + a0_*w, en1 = L.e + a0_*( H - u*a );
+      const double ie1 = en1 - HALF*( m1*m1 + t1*t1 + t2*t2 )*rcp( d1 );
+      bool bad = ( u > e0 ) && ( d1 <= 0.0 || ie1*gm1 <= 0.0 );
+      const double d4 = d1 + a1_, m4 = m1 + a1_*u, s4 = t1 + a1_*v + a2_, r4 = t2 + a1_*w + a3_,
+                   en4 = en1 + a1_*( HALF*V2 ) + a2_*v + a3_*w;
+      const double ie4 = en4 - HALF*( m4*m4 + s4*s4 + r4*r4 )*rcp( d4 );
+      bad = bad || ( ( e4 > u ) && ( d4 <= 0.0 || ie4*gm1 <= 0.0 ) );
+      if ( bad ) { rs_hlle_rev( L, Q, bf, bL, bQ );  return; }
+   }
+   const double s0 = __builtin_signbit( e0 ) ? -1.0 : 1.0, su = __builtin_signbit( u ) ? -1.0 : 1.0,
+                s4 = __builtin_signbit( e4 ) ? -1.0 : 1.0;
+   const double A0 = a0_*( s0*e0 ), A1 = a1_*( su*u ), A2 = a2_*( su*u ), A3 = a3_*( su*u ), A4 = a4_*( s4*e4 );
+
+   // reverse: f = (F_L + F_R - sum_k A_k R_k) / 2
+   double bh[NV];
+#  pragma unroll
+   for (int k=0; k<NV; k++) bh[k] = HALF*bf[k];
+   st_zero( bs );
+   xflux_minus_rev( L, 0.0, bh, bs, bsp );
+   add_upwind( bL, bs );
+   st_zero( bs );
+   xflux_minus_rev( Q, 0.0, bh, bs, bsp );
+   add_upwind( bQ, bs );
+   const double bA0 = -( bh[0] + bh[1]*e0 + bh[2]*v + bh[3]*w + bh[4]*( H - u*a ) );
+   const double bA1 = -( bh[0] + bh[1]*u  + bh[2]*v + bh[3]*w + bh[4]*( HALF*V2 ) );
+   const double bA2 = -( bh[2] + bh[4]*v );
+   const double bA3 = -( bh[3] + bh[4]*w );
+   const double bA4 = -( bh[0] + bh[1]*e4 + bh[2]*v + bh[3]*w + bh[4]*( H + u*a ) );
+   double be0 = -bh[1]*A0, be4 = -bh[1]*A4;
+   bu  -= bh[1]*A1 + bh[4]*a*( A4 - A0 );
+   bv  -= bh[2]*( A0 + A1 + A4 ) + bh[4]*A2;
+   bw  -= bh[3]*( A0 + A1 + A4 ) + bh[4]*A3;
+   bH  -= bh[4]*( A0 + A4 );
+   ba  -= bh[4]*u*( A4 - A0 );
+   bV2 -= bh[4]*HALF*A1;
+   // A_k = alpha_k |lambda_k|
+   double ba0_ = bA0*( s0*e0 ), ba1_ = bA1*( su*u ), ba2_ = bA2*( su*u ), ba3_ = bA3*( su*u ), ba4_ = bA4*( s4*e4 );
+   be0 += bA0*a0_*s0;
+   be4 += bA4*a4_*s4;
+   bu  += ( bA1*a1_ + bA2*a2_ + bA3*a3_ )*su;
+   // alpha_4 = j0 - alpha_0 - alpha_1
+   double bj0 = ba4_, bj1 = 0.0, bj2 = 0.0, bj3 = 0.0, bj4 = 0.0;
+   ba0_ -= ba4_;
+   ba1_ -= ba4_;
+   // alpha_0 = (1/(2a)) T0, T0 = j0 (u + a) - j1 - a alpha_1
+   const double bT0 = ba0_*c0;
+   ba  -= ra*ra*( HALF*ba0_*T0 );
+   bj0 += bT0*( u + a );
+   bu  += bT0*j0;
+   ba  += bT0*( j0 - a1_ );
+   bj1 -= bT0;
+   ba1_ -= bT0*a;
+   // alpha_1 = ((gamma-1)/a^2) T1
+   const double bT1 = ba1_*c1, ra2 = rcp( a2 );
+   double ba2 = -ra2*ra2*( gm1*ba1_*T1 );
+   bj0 += bT1*( H - u*u );
+   bH  += bT1*j0;
+   bu  += bT1*( j1 - 2.0*u*j0 );
+   bj1 += bT1*u;
+   bj4 -= bT1;
+   bv  += bT1*a2_;
+   ba2_ += bT1*v;
+   bw  += bT1*a3_;
+   ba3_ += bT1*w;
+   // alpha_2 = j2 - v j0, alpha_3 = j3 - w j0
+   bj3 += ba3_;  bw -= ba3_*j0;  bj0 -= ba3_*w;
+   bj2 += ba2_;  bv -= ba2_*j0;  bj0 -= ba2_*v;
+   bQ.d  += bj0;  bL.d  -= bj0;
+   bQ.mn += bj1;  bL.mn -= bj1;
+   bQ.m1 += bj2;  bL.m1 -= bj2;
+   bQ.m2 += bj3;  bL.m2 -= bj3;
+   bQ.e  += bj4;  bL.e  -= bj4;
+   // eigenvalues u -/+ a, a = sqrt(a2), a2 = (gamma/rho) max((gamma-1)(H - V2/2) rho/gamma, MinPres)
+   bu += be0 + be4;
+   ba += be4 - be0;
+   ba2 += ba*( 0.5/a );
+   const double birho = ba2*P.gamma*gf, bgf = ba2*P.gamma*irho;
+   const double bg1 = kf ? bgf : 0.0;
+   const double bg0 = bg1*rho*rg;
+   double brho = bg1*g0*rg - irho*irho*birho;
+   bH  += bg0*gm1;
+   bV2 -= HALF*gm1*bg0;
+   // Roe averages
+   bu += 2.0*u*bV2;
+   bv += 2.0*v*bV2;
+   bw += 2.0*w*bV2;
+   const double bNH = bH*isum, bNU = bu*isum, bNV = bv*isum, bNW = bw*isum;
+   const double bisum = bH*NH + bu*NU + bv*NVv + bw*NW;
+   const double bhl = bNH*sql, bhr = bNH*sqr;
+   double bsql = bNH*hl, bsqr = bNH*hr;
+   const double bisql = bNU*L.mn + bNV*L.m1 + bNW*L.m2, bisqr = bNU*Q.mn + bNV*Q.m1 + bNW*Q.m2;
+   bL.mn += bNU*isql;  bL.m1 += bNV*isql;  bL.m2 += bNW*isql;
+   bQ.mn += bNU*isqr;  bQ.m1 += bNV*isqr;  bQ.m2 += bNW*isqr;
+   const double bss = -isum*isum*bisum;
+   bsql += bss - isql*isql*bisql + brho*sqr;
+   bsqr += bss - isqr*isqr*bisqr + brho*sql;
+   bL.d += bsql*( 0.5/sql );
+   bQ.d += bsqr*( 0.5/sqr );
+   bL.ir += bhl*( L.e + L.p );  bL.e += bhl*L.ir;  bL.p += bhl*L.ir;
+   bQ.ir += bhr*( Q.e + Q.p );  bQ.e += bhr*Q.ir;  bQ.p += bhr*Q.ir;
+}
+
+// riemann<RS, D> (HLLC with the HLLE rescue, HLLE, or Roe): bout is the adjoint of the lab-frame flux
 template <int RS, int D>
 __device__ __forceinline__ void riemann_rev( const StD &L, const StD &Q, const Prm<double> &P, const double bout[NV],
                                              StD &bL, StD &bQ )
 {
-   static_assert( RS == 3 || RS == 4, "reverse sweep written for HLLE and HLLC" );
+   static_assert( RS == 2 || RS == 3 || RS == 4, "reverse sweeps exist for Roe, HLLE and HLLC" );
    double bf[NV];
    bf[0] = bout[0];
    bf[1] = bout[1 + D];
@@ -196,6 +335,7 @@ __device__ __forceinline__ void riemann_rev( const StD &L, const StD &Q, const P
    bf[4] = bout[4];
    st_zero( bL );
    st_zero( bQ );
+   if ( RS == 2 ) { rs_roe_rev( L, Q, P, bf, bL, bQ );  return; }
    if ( RS == 4 && rs_hllc_rev( L, Q, P.min_pres, bf, bL, bQ ) ) return;
    rs_hlle_rev( L, Q, bf, bL, bQ );
 }
