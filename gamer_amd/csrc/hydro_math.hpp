@@ -292,6 +292,20 @@ __device__ __forceinline__ St<R> cell_state( const R c[NV], R p, R ir, R cs )
    return s;
 }
 
+// whether a flux component is NaN (an infinite component is not)
+template <typename R>
+__device__ __forceinline__ bool any_nan5( const R f[NV] )
+{
+   return f[0] != f[0] || f[1] != f[1] || f[2] != f[2] || f[3] != f[3] || f[4] != f[4];
+}
+// float: gfx950's v_maximum3_f32 propagates NaN, so two of them and one unordered compare test all five components
+__device__ __forceinline__ bool any_nan5( const float f[NV] )
+{
+   const float m = __builtin_elementwise_maximum( __builtin_elementwise_maximum( f[0], f[1] ),
+                                                  __builtin_elementwise_maximum( f[2], __builtin_elementwise_maximum( f[3], f[4] ) ) );
+   return m != m;
+}
+
 // Riemann solve across a D-normal face; flux returned in the lab frame.  HLLC falls back to HLLE when a flux
 // component is NaN (RSOLVER_RESCUE, CPU_FluidSolver_MHM.cpp:737-786, ComputeFlux.cpp:313-362).
 template <int RS, int D, typename R>
@@ -301,7 +315,7 @@ __device__ __forceinline__ void riemann( const St<R> &L, const St<R> &Q, const P
    if ( RS == 4 )
    {
       rs_hllc( L, Q, P.min_pres, f );
-      if ( f[0] != f[0] || f[1] != f[1] || f[2] != f[2] || f[3] != f[3] || f[4] != f[4] ) rs_hlle( L, Q, f );
+      if ( any_nan5( f ) ) rs_hlle( L, Q, f );
    }
    else if ( RS == 3 ) rs_hlle( L, Q, f );
    else                rs_roe( L, Q, P, f );
@@ -320,7 +334,7 @@ __device__ __forceinline__ void riemann_xy( const St<R> &L, const St<R> &Q, cons
    if ( RS == 4 )
    {
       rs_hllc( L, Q, P.min_pres, f );
-      if ( f[0] != f[0] || f[1] != f[1] || f[2] != f[2] || f[3] != f[3] || f[4] != f[4] ) rs_hlle( L, Q, f );
+      if ( any_nan5( f ) ) rs_hlle( L, Q, f );
    }
    else if ( RS == 3 ) rs_hlle( L, Q, f );
    else                rs_roe( L, Q, P, f );

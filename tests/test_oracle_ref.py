@@ -26,13 +26,41 @@ def test_bit_exact_vs_reference_build(scheme, rsolver, prec, limiter):
     np.testing.assert_array_equal(f1, f2)
 
 
-@pytest.mark.parametrize("case", ["retry", "floors", "kh"])
+def _cold_flow(npg, n=22):
+    """Pressureless uniform flow rho = 1, v = (1, 0, 0), E = 1/2: the sound speed is 0, so every HLLC solve
+    divides by temp1_L - temp1_R = 0 and returns NaN, and RSOLVER_RESCUE falls back to HLLE."""
+    u = np.zeros((npg, 5, n, n, n))
+    u[:, 0] = 1.0
+    u[:, 1] = 1.0
+    u[:, 4] = 0.5
+    return u
+
+
+@pytest.mark.parametrize("scheme", ["MHM_RP", "MHM"])
+def test_hlle_rescue_closed_form(scheme):
+    """The rescued update of a uniform cold flow is exact: output = input, x fluxes (1, 1, 0, 0, 1/2), the
+    others 0 (no reference build needed)."""
+    n = 22 if scheme == "MHM_RP" else 20
+    g = (n - 16) // 2
+    u = _cold_flow(2, n)
+    o, f = oracle.fluid_solver(u, 0.01, 1.0 / 64, scheme=scheme, rsolver="HLLC")
+    np.testing.assert_array_equal(o, u[:, :, g:g + 16, g:g + 16, g:g + 16])
+    np.testing.assert_array_equal(f[:, 0:3], np.broadcast_to(np.array([1.0, 1.0, 0.0, 0.0, 0.5])[None, None, :, None, None],
+                                                             f[:, 0:3].shape))
+    np.testing.assert_array_equal(f[:, 3:9], 0.0)
+
+
+@pytest.mark.parametrize("case", ["retry", "floors", "kh", "rescue"])
 def test_bit_exact_special_paths(case):
     if not oracle.ref_available("MHM_RP", "HLLC", "f64"):
         pytest.skip("reference build (oracle/_ref) not present")
     if case == "kh":
         w = ic.kelvin_helmholtz((1, 2, 2), seed=9)
         kw = dict(rsolver="ROE")
+    elif case == "rescue":
+        w = ic.random_sod((1, 1, 2), seed=78, sedov=True)
+        w.u = np.concatenate([w.u[:1], _cold_flow(1), w.u[1:]], axis=0)
+        kw = dict(rsolver="HLLC")
     else:
         w = ic.random_sod((1, 1, 2), seed=77, sedov=True, cfl=1.3, sigma_v=1.5)
         kw = dict(minmod_maxiter=5) if case == "retry" else dict(min_dens=1e-2, min_pres=1e-3, min_eint=1e-3)
