@@ -182,9 +182,9 @@ def amr_batch(args):
     cells = npg * CELLS_PER_PG
     reps = args.steps
     print("[amr] autotune", file=sys.stderr, flush=True)
-    tune = amr.autotune(torch, b, groups=(1, 2, 3, 4), reps=reps)
+    tune = amr.autotune(torch, b, groups=(1, 2, 3, 4), tickets=(1, 2, 4), reps=reps)
     best = tune["best"]
-    gpc = 2 if best == "grid" else int(best.split(":")[1])
+    gpc, tk = (2, 1) if best == "grid" else (int(best.split(":")[1]), int(best.split(":")[2]))
 
     idx = [np.nonzero(b.level == l)[0] for l in range(len(counts))]
     u_lv = [b.u[torch.from_numpy(i).to(dev)].contiguous() for i in idx]
@@ -199,10 +199,11 @@ def amr_batch(args):
     flux = torch.empty((npg, 9, 5, 16, 16), dtype=torch.float32, device=dev)
     cnt = torch.zeros(1, dtype=torch.int32, device=dev)
     print("[amr] timing", file=sys.stderr, flush=True)
-    g = amr.GraphSubstep(torch, b, groups_per_cu=gpc, persistent=True)
+    g = amr.GraphSubstep(torch, b, groups_per_cu=gpc, pgs_per_ticket=tk, persistent=True)
     fns = {"lv": per_level,
            "grid": lambda: hydro.solve_batch(b.u, b.pg_dt_dh, out=out, flux=flux),
-           "pers": lambda: hydro.solve_batch(b.u, b.pg_dt_dh, out=out, flux=flux, queue_counter=cnt, groups_per_cu=gpc),
+           "pers": lambda: hydro.solve_batch(b.u, b.pg_dt_dh, out=out, flux=flux, queue_counter=cnt, groups_per_cu=gpc,
+                                             pgs_per_ticket=tk),
            "graph": g.replay}
     best = {k: float("inf") for k in fns}
     for _ in range(3):                                   # interleaved rounds, best of each
@@ -219,7 +220,7 @@ def amr_batch(args):
             "unit": "cell-updates/s",
             "per_level_launches": {"ms": t_lv, "value": rate(t_lv)},
             "one_launch_grid": {"ms": t_grid, "value": rate(t_grid)},
-            "one_launch_persistent": {"ms": t_pers, "value": rate(t_pers), "groups_per_cu": gpc},
+            "one_launch_persistent": {"ms": t_pers, "value": rate(t_pers), "groups_per_cu": gpc, "pgs_per_ticket": tk},
             "graph_replay": {"ms": t_graph, "value": rate(t_graph)},
             "autotune": tune}
     print(json.dumps(line))

@@ -119,16 +119,27 @@ def time_launch(torch, fn, reps=5, warmup=1):
     return e0.elapsed_time(e1) / reps
 
 
-def autotune(torch, batch: LevelBatch, groups=(1, 2, 3, 4), reps=5, rounds=3, cache=None, **kw):
+def launch_kwargs(torch, config, device):
+    """solve_batch keyword arguments of an autotuner configuration name ("grid" or "persistent:g:k")."""
+    if config == "grid":
+        return {}
+    _, g, k = config.split(":")
+    return dict(queue_counter=torch.zeros(1, dtype=torch.int32, device=device), groups_per_cu=int(g),
+                pgs_per_ticket=int(k))
+
+
+def autotune(torch, batch: LevelBatch, groups=(1, 2, 3, 4), tickets=(1, 2, 4), reps=5, rounds=3, cache=None, **kw):
     """Measure every launch configuration of a batched substep and return the fastest.
 
-    Configurations: one workgroup per patch group ("grid"), and the persistent work queue with g workgroups
-    per compute unit for g in ``groups``.  The configurations are timed in ``rounds`` interleaved rounds of
+    Configurations: one workgroup per patch group ("grid"), and the persistent work queue ("persistent:g:k")
+    with g workgroups per compute unit for g in ``groups``, each taking k consecutive patch groups per queue
+    ticket for k in ``tickets`` (fewer queue atomics and barriers against a coarser tail).  The configurations are timed in ``rounds`` interleaved rounds of
     ``reps`` launches and each keeps its best round, so that a clock or thermal drift during the sweep does not
     pick the winner.  Results are cached in ``cache`` (a JSON path) keyed by the batch shape and precision when
     given."""
     npg = batch.u.shape[0]
-    key = f"{torch.cuda.get_device_name(0)}|npg={npg}|{batch.u.dtype}|{kw.get('scheme', 'MHM_RP')}|{kw.get('rsolver', 'HLLC')}"
+    key = (f"{torch.cuda.get_device_name(0)}|npg={npg}|{batch.u.dtype}|{kw.get('scheme', 'MHM_RP')}|"
+           f"{kw.get('rsolver', 'HLLC')}|groups={tuple(groups)}|tickets={tuple(tickets)}")
     if cache and os.path.exists(cache):
         try:
             with open(cache) as f:
@@ -142,8 +153,10 @@ def autotune(torch, batch: LevelBatch, groups=(1, 2, 3, 4), reps=5, rounds=3, ca
     counter = torch.zeros(1, dtype=torch.int32, device=batch.u.device)
     configs = {"grid": lambda: hydro.solve_batch(batch.u, batch.pg_dt_dh, out=out, flux=flux, **kw)}
     for g in groups:
-        configs[f"persistent:{g}"] = (lambda g=g: hydro.solve_batch(batch.u, batch.pg_dt_dh, out=out, flux=flux,
-                                                                    queue_counter=counter, groups_per_cu=g, **kw))
+        for k in tickets:
+            configs[f"persistent:{g}:{k}"] = (lambda g=g, k=k: hydro.solve_batch(
+                batch.u, batch.pg_dt_dh, out=out, flux=flux, queue_counter=counter, groups_per_cu=g,
+                pgs_per_ticket=k, **kw))
     trials = {name: float("inf") for name in configs}
     for _ in range(max(1, rounds)):
         for name, fn in configs.items():

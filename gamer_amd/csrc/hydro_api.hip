@@ -92,7 +92,7 @@ int enqueue( const HydroSolveArgs *a, const void *in, void *out, void *flux, voi
              const Batch *batch = nullptr, int grid_blocks = 0 )
 {
    Batch nb;
-   nb.pg_dt_dh = nullptr;  nb.counter = nullptr;  nb.safety = 0.0;
+   nb.pg_dt_dh = nullptr;  nb.counter = nullptr;  nb.safety = 0.0;  nb.claim = 1;
    const Batch &B = batch ? *batch : nb;
    int rc;
    const bool rp = ( a->scheme == HYDRO_MHM_RP );
@@ -411,7 +411,7 @@ int hydro_assemble_patch_groups_async( const void *patch, const void *fbox, void
 }
 
 int hydro_mhm_solve_batch_async( const HydroSolveArgs *a, const double *pg_dt_dh, unsigned *queue_counter,
-                                 int groups_per_cu, void *stream )
+                                 int groups_per_cu, int pgs_per_ticket, void *stream )
 {
    int rc = check_args( a );
    if ( rc != HYDRO_OK ) return rc;
@@ -419,15 +419,19 @@ int hydro_mhm_solve_batch_async( const HydroSolveArgs *a, const double *pg_dt_dh
    B.pg_dt_dh = pg_dt_dh;
    B.counter  = queue_counter;
    B.safety   = a->safety;
+   B.claim    = 1;
    int grid = 0;
    if ( queue_counter != nullptr )
    {
       if ( groups_per_cu < 1 || groups_per_cu > 16 ) return fail( HYDRO_E_ARG, "batch: groups_per_cu must be 1..16" );
+      if ( pgs_per_ticket < 1 || pgs_per_ticket > 64 ) return fail( HYDRO_E_ARG, "batch: pgs_per_ticket must be 1..64" );
+      B.claim = pgs_per_ticket;
       int dev = 0, ncu = 0;
       HIP_TRY( hipGetDevice( &dev ) );
       HIP_TRY( hipDeviceGetAttribute( &ncu, hipDeviceAttributeMultiprocessorCount, dev ) );
       grid = groups_per_cu*ncu;
-      if ( grid > a->npg ) grid = a->npg;
+      const int tickets = ( a->npg + B.claim - 1 )/B.claim;      // workgroups beyond this would find the queue empty
+      if ( grid > tickets ) grid = tickets;
       if ( a->npg > 0 && launch_reset_counter( queue_counter, (hipStream_t)stream ) != 0 )
          return fail( HYDRO_E_DEVICE, "queue reset launch failed" );
    }

@@ -50,13 +50,14 @@ struct Prm {
 };
 
 // Multi-level batch description: optional per-PG (dt, dh) pairs in device memory (uneven AMR batches, local
-// time steps) and an optional global work-queue counter for the persistent mode (each workgroup pulls patch
-// groups until the queue is exhausted).  pg_dt_dh == nullptr: every PG uses Prm's dt/dh;
+// time steps) and an optional global work-queue counter for the persistent mode (each workgroup pulls tickets of
+// `claim` consecutive patch groups until the queue is exhausted).  pg_dt_dh == nullptr: every PG uses Prm's dt/dh;
 // counter == nullptr: one PG per workgroup.
 struct Batch {
    const double *pg_dt_dh;
    unsigned     *counter;
    double        safety;
+   int           claim;
 };
 
 // an interface state rotated to the face normal: conserved fields + derived quantities

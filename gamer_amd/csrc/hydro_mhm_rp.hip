@@ -225,16 +225,24 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
    const int  e_vo = e_dy ? NV : 0;             // its face states / fluxes in S.f / S.b
    const int  lim = P0.limiter;
 
+   int base = 0;
    for (int item = 0; ; item++)
    {
-   // next patch group: blockIdx.x, or the next ticket of the persistent work queue
+   // next patch group: blockIdx.x, or the next PG of this workgroup's ticket of the persistent work queue (a ticket
+   // covers B.claim consecutive PGs; the barrier before every PG also keeps the previous PG's LDS reads ahead of
+   // the next PG's writes)
    int pg;
    if ( PERSIST )
    {
+      const int sub = item % B.claim;
       __syncthreads();
-      if ( tid == 0 ) S.next = (int)atomicAdd( B.counter, 1u );
-      __syncthreads();
-      pg = __builtin_amdgcn_readfirstlane( S.next );      // uniform: keep pointers and dt in SGPRs
+      if ( sub == 0 )
+      {
+         if ( tid == 0 ) S.next = (int)atomicAdd( B.counter, 1u );
+         __syncthreads();
+         base = __builtin_amdgcn_readfirstlane( S.next )*B.claim;   // uniform: keep pointers and dt in SGPRs
+      }
+      pg = base + sub;
    }
    else
    {

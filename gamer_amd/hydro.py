@@ -102,7 +102,7 @@ def lib():
         L.hydro_prepare_uniform_async.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 5 + [ctypes.c_void_p]
         L.hydro_close_uniform_async.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 4 + [ctypes.c_void_p]
         L.hydro_mhm_solve_batch_async.argtypes = [ctypes.POINTER(HydroSolveArgs), ctypes.c_void_p, ctypes.c_void_p,
-                                                  ctypes.c_int, ctypes.c_void_p]
+                                                  ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
         L.hydro_mhm_adjoint_workspace_bytes.argtypes = [ctypes.c_int]
         L.hydro_mhm_adjoint_workspace_bytes.restype = ctypes.c_size_t
         L.hydro_mhm_adjoint_async.argtypes = [ctypes.POINTER(HydroSolveArgs)] + [ctypes.c_void_p] * 4
@@ -194,13 +194,14 @@ def fluid_solver(u, dt, dh, gamma=5.0 / 3.0, scheme="MHM_RP", rsolver="HLLC", li
     return out, (flux if store_flux else None)
 
 
-def solve_batch(u, pg_dt_dh=None, out=None, flux=None, queue_counter=None, groups_per_cu=2, stream=None,
-                dt_patch=None, safety=0.5, dt=0.0, dh=1.0, **kw):
+def solve_batch(u, pg_dt_dh=None, out=None, flux=None, queue_counter=None, groups_per_cu=2, pgs_per_ticket=1,
+                stream=None, dt_patch=None, safety=0.5, dt=0.0, dh=1.0, **kw):
     """One launch over patch groups with per-PG time steps (e.g. several AMR levels at once).
 
     pg_dt_dh: float64 device tensor (npg, 2) of (dt, dh) per patch group (None: scalar dt, dh for all).
     queue_counter: a 1-element int32 device tensor enables the persistent work-queue mode with ``groups_per_cu``
-    workgroups per compute unit; None = one workgroup per PG.
+    workgroups per compute unit, each taking ``pgs_per_ticket`` consecutive patch groups per queue ticket;
+    None = one workgroup per PG.
     """
     import torch
     scheme = kw.pop("scheme", "MHM_RP")
@@ -231,7 +232,7 @@ def solve_batch(u, pg_dt_dh=None, out=None, flux=None, queue_counter=None, group
     _check(lib().hydro_mhm_solve_batch_async(ctypes.byref(args),
                                              pg_dt_dh.data_ptr() if pg_dt_dh is not None else None,
                                              queue_counter.data_ptr() if queue_counter is not None else None,
-                                             int(groups_per_cu), _stream(u.device, stream)))
+                                             int(groups_per_cu), int(pgs_per_ticket), _stream(u.device, stream)))
     return out, (flux if store_flux else None)
 
 

@@ -70,12 +70,13 @@ int hydro_mhm_solve_async( const HydroSolveArgs *args, void *stream );
 /* One launch over patch groups that need different time steps, e.g. the PGs of several AMR levels at once:
    PG p advances with dt = pg_dt_dh[2p], dh = pg_dt_dh[2p+1] (device double array [npg][2]); args->dt/dh are
    ignored when pg_dt_dh != NULL.  With queue_counter (a device unsigned scratch word) the kernel runs
-   persistently: groups_per_cu workgroups per compute unit pull patch groups from a global work queue whose
-   counter is reset by a kernel on the same stream (graph-capturable, no memset).  With queue_counter == NULL
-   there is one workgroup per PG.  No counterpart in the reference, which issues one CUAPI_Asyn_FluidSolver call
+   persistently: groups_per_cu (1..16) workgroups per compute unit pull tickets of pgs_per_ticket (1..64)
+   consecutive patch groups from a global work queue whose counter is reset by a kernel on the same stream
+   (graph-capturable, no memset).  With queue_counter == NULL there is one workgroup per PG and groups_per_cu,
+   pgs_per_ticket are ignored.  No counterpart in the reference, which issues one CUAPI_Asyn_FluidSolver call
    per level and substep (src/Main/EvolveLevel.cpp:203). */
 int hydro_mhm_solve_batch_async( const HydroSolveArgs *args, const double *pg_dt_dh, unsigned *queue_counter,
-                                 int groups_per_cu, void *stream );
+                                 int groups_per_cu, int pgs_per_ticket, void *stream );
 
 /* Per-patch hydro CFL time step on device arrays: in[npatch][5][8][8][8] -> dt[npatch],
    dt = safety*dh / max_cells max_d(|v_d| + c_s) (replaces CUAPI_Asyn_dtSolver for DT_FLU_SOLVER,

@@ -53,10 +53,10 @@ def test_persistent_equals_grid_bitwise(dtype):
     b = _batch(dtype=dtype, counts=(64, 128, 192))
     o1, f1 = hydro.solve_batch(b.u, b.pg_dt_dh)
     cnt = torch.zeros(1, dtype=torch.int32, device=DEV)
-    for g in (1, 3):
-        o2, f2 = hydro.solve_batch(b.u, b.pg_dt_dh, queue_counter=cnt, groups_per_cu=g)
+    for g, k in ((1, 1), (3, 1), (2, 3), (1, 4), (4, 64)):
+        o2, f2 = hydro.solve_batch(b.u, b.pg_dt_dh, queue_counter=cnt, groups_per_cu=g, pgs_per_ticket=k)
         torch.cuda.synchronize()
-        assert torch.equal(o1, o2) and torch.equal(f1, f2), g
+        assert torch.equal(o1, o2) and torch.equal(f1, f2), (g, k)
 
 
 def test_batch_uniform_equals_fluid_solver():
@@ -119,13 +119,10 @@ def test_autotune_picks_a_measured_config(tmp_path):
     print("[gpu] autotune", flush=True)
     b = _batch(dtype=torch.float32, counts=(64, 128, 192))
     cache = str(tmp_path / "tune.json")
-    res = amr.autotune(torch, b, groups=(1, 2), reps=2, cache=cache)
-    assert res["best"] in res["trials_ms"] and res["ms"] > 0
-    assert amr.autotune(torch, b, groups=(1, 2), reps=2, cache=cache)["best"] == res["best"]   # cached
-    best = res["best"]
-    cnt = None if best == "grid" else torch.zeros(1, dtype=torch.int32, device=DEV)
-    gpc = 2 if best == "grid" else int(best.split(":")[1])
-    o, _ = hydro.solve_batch(b.u, b.pg_dt_dh, queue_counter=cnt, groups_per_cu=gpc)
+    res = amr.autotune(torch, b, groups=(1, 2), tickets=(1, 2), reps=2, cache=cache)
+    assert res["best"] in res["trials_ms"] and res["ms"] > 0 and len(res["trials_ms"]) == 5
+    assert amr.autotune(torch, b, groups=(1, 2), tickets=(1, 2), reps=2, cache=cache)["best"] == res["best"]   # cached
+    o, _ = hydro.solve_batch(b.u, b.pg_dt_dh, **amr.launch_kwargs(torch, res["best"], DEV))
     ref, _ = hydro.solve_batch(b.u, b.pg_dt_dh)
     torch.cuda.synchronize()
     assert torch.equal(o, ref)
