@@ -74,9 +74,30 @@ __device__ __forceinline__ void split( int d, int i, int j, int k, int &n, int &
    b = ( d == 2 ) ? j : k;
 }
 __device__ __forceinline__ bool half_face( int n, int a, int b ) { return in_rng( n, 0, 20 ) && in_rng( a, 1, 20 ) && in_rng( b, 1, 20 ); }
-__device__ __forceinline__ bool full_face( int n, int a, int b ) { return in_rng( n, 2, 18 ) && in_rng( a, 3, 18 ) && in_rng( b, 3, 18 ); }
 __device__ __forceinline__ bool core3( int i, int j, int k ) { return in_rng( i, 3, 18 ) && in_rng( j, 3, 18 ) && in_rng( k, 3, 18 ); }
 __device__ __forceinline__ int  out_index( int i, int j, int k ) { return ( i - 3 ) + PS*( ( j - 3 ) + PS*( k - 3 ) ); }
+
+// The phases enumerate only the cells or faces they work on (a box [lo, lo + len) per axis, x fastest, so that
+// neighbouring lanes read neighbouring cells) instead of striding over the whole 22^3 block and skipping.
+struct Box { int lx, ly, lz, nx, ny, nz; };
+__device__ __forceinline__ int box_cell( const Box &b, int e, int &i, int &j, int &k )
+{
+   const int x = e % b.nx, r = e / b.nx, y = r % b.ny, z = r / b.ny;
+   i = b.lx + x;  j = b.ly + y;  k = b.lz + z;
+   return i + NX*j + N2*k;
+}
+// d-normal faces c + 1/2 e_d with normal coordinate in [n0, n0 + nn) and both tangential ones in [t0, t0 + nt)
+__device__ __forceinline__ Box face_box( int d, int n0, int nn, int t0, int nt )
+{
+   Box b;
+   b.lx = ( d == 0 ) ? n0 : t0;  b.nx = ( d == 0 ) ? nn : nt;
+   b.ly = ( d == 1 ) ? n0 : t0;  b.ny = ( d == 1 ) ? nn : nt;
+   b.lz = ( d == 2 ) ? n0 : t0;  b.nz = ( d == 2 ) ? nn : nt;
+   return b;
+}
+constexpr int NHF = 21*20*20;        // half-step faces per direction: normal 0..20, tangential 1..20
+constexpr int NFF = 17*16*16;        // full-step faces per direction: normal 2..18, tangential 3..18
+constexpr Box HALF_CELLS = { 1, 1, 1, 20, 20, 20 }, RECON_CELLS = { 2, 2, 2, 18, 18, 18 }, CORE_CELLS = { 3, 3, 3, 16, 16, 16 };
 
 // ---------------- pieces of the step, generic in the arithmetic type ----------------
 // cell-centred interface state of the half step (pressure recovery + floor, 1/rho, sound speed)
@@ -212,12 +233,12 @@ __device__ void forward_phases( const double *in, const double *tin, Fwd<T> &F, 
 {
    const int tid = threadIdx.x;
    // F1 half-step fluxes at the 25,200 faces (stored at the low cell)
-   for (int e = tid; e < 3*N3; e += NT)
+   for (int e = tid; e < 3*NHF; e += NT)
    {
-      const int d = e / N3, c = e % N3;
+      const int d = e / NHF;
       int i, j, k, n, a, b;
-      coords( c, i, j, k );  split( d, i, j, k, n, a, b );
-      if ( !half_face( n, a, b ) ) continue;
+      const int c = box_cell( face_box( d, 0, 21, 1, 20 ), e - d*NHF, i, j, k );
+      split( d, i, j, k, n, a, b );
       T uL[NV], uR[NV], f[NV];
       Load<T>::cell( in, tin, c, uL );
       Load<T>::cell( in, tin, c + STRIDE[d], uR );
@@ -227,11 +248,10 @@ __device__ void forward_phases( const double *in, const double *tin, Fwd<T> &F, 
    }
    __syncthreads();
    // F2 half-step update of the 20^3 cells (x and y flux differences, then z, as the production kernel)
-   for (int c = tid; c < N3; c += NT)
+   for (int e = tid; e < 8000; e += NT)
    {
       int i, j, k;
-      coords( c, i, j, k );
-      if ( !( in_rng( i, 1, 20 ) && in_rng( j, 1, 20 ) && in_rng( k, 1, 20 ) ) ) continue;
+      const int c = box_cell( HALF_CELLS, e, i, j, k );
       T u[NV], h[NV], w[NV];
       Load<T>::cell( in, tin, c, u );
 #     pragma unroll
@@ -247,11 +267,10 @@ __device__ void forward_phases( const double *in, const double *tin, Fwd<T> &F, 
    __syncthreads();
    if ( !STORE_FC ) return;
    // F3 PLM faces of the 18^3 reconstruction block, all three directions
-   for (int c = tid; c < N3; c += NT)
+   for (int e = tid; e < 18*18*18; e += NT)
    {
       int i, j, k;
-      coords( c, i, j, k );
-      if ( !( in_rng( i, 2, 19 ) && in_rng( j, 2, 19 ) && in_rng( k, 2, 19 ) ) ) continue;
+      const int c = box_cell( RECON_CELLS, e, i, j, k );
       for (int d=0; d<3; d++)
       {
          const int s = STRIDE[d];
@@ -282,11 +301,10 @@ template <> struct Store< Dual<1> >
 template <int RS, typename T>
 __device__ void full_update( const double *in, const double *tin, Fwd<T> &F, const Prm<T> &P, double *out, double *tout )
 {
-   for (int c = threadIdx.x; c < N3; c += NT)
+   for (int e = threadIdx.x; e < PS*PS*PS; e += NT)
    {
       int i, j, k;
-      coords( c, i, j, k );
-      if ( !core3( i, j, k ) ) continue;
+      const int c = box_cell( CORE_CELLS, e, i, j, k );
       T u[NV], dF[3][NV];
       Load<T>::cell( in, tin, c, u );
       for (int d=0; d<3; d++)
@@ -362,12 +380,12 @@ adjoint_kernel( const double *__restrict__ g_in, const double *__restrict__ g_lo
 
    // R4 full-step faces: lambda_F = dt/dh (lambda_out(c+e_d) - lambda_out(c)), then J_RS^T per side.  Only the
    // faces that feed a full-step flux receive a value; R3 reads the others as zero.
-   for (int e = tid; e < 3*N3; e += NT)
+   for (int e = tid; e < 3*NFF; e += NT)
    {
-      const int d = e / N3, c = e % N3;
+      const int d = e / NFF;
       int i, j, k, n, a, b;
-      coords( c, i, j, k );  split( d, i, j, k, n, a, b );
-      if ( !full_face( n, a, b ) ) continue;
+      const int c = box_cell( face_box( d, 2, 17, 3, 16 ), e - d*NFF, i, j, k );
+      split( d, i, j, k, n, a, b );
       const int s = STRIDE[d];
       int i2 = i, j2 = j, k2 = k;
       if ( d == 0 ) i2++; else if ( d == 1 ) j2++; else k2++;
@@ -440,11 +458,10 @@ adjoint_kernel( const double *__restrict__ g_in, const double *__restrict__ g_lo
    }
 
    // R2 lambda of the half-step conserved state: J_Con2Pri^T lambda_W (floors included)
-   for (int c = tid; c < N3; c += NT)
+   for (int e = tid; e < 8000; e += NT)
    {
       int i, j, k;
-      coords( c, i, j, k );
-      if ( !( in_rng( i, 1, 20 ) && in_rng( j, 1, 20 ) && in_rng( k, 1, 20 ) ) ) continue;
+      const int c = box_cell( HALF_CELLS, e, i, j, k );
       double h[NV], lw[NV];
 #     pragma unroll
       for (int v=0; v<NV; v++)
@@ -469,12 +486,12 @@ adjoint_kernel( const double *__restrict__ g_in, const double *__restrict__ g_lo
 
    // R1 half-step faces: lambda_Fh = dt/(2dh) (lambda_U(c+e_d) - lambda_U(c)), then J_RS^T per side.
    // (The Fh arrays alias the low-side slots written here; every reader of Fh has passed the barrier above.)
-   for (int e = tid; e < 3*N3; e += NT)
+   for (int e = tid; e < 3*NHF; e += NT)
    {
-      const int d = e / N3, c = e % N3;
+      const int d = e / NHF;
       int i, j, k, n, a, b;
-      coords( c, i, j, k );  split( d, i, j, k, n, a, b );
-      if ( !half_face( n, a, b ) ) continue;
+      const int c = box_cell( face_box( d, 0, 21, 1, 20 ), e - d*NHF, i, j, k );
+      split( d, i, j, k, n, a, b );
       const int s = STRIDE[d];
       const bool lo = in_rng( n, 1, 20 ), hi = in_rng( n + 1, 1, 20 );   // tangential range is 1..20 already
       double lf[NV];
