@@ -369,7 +369,12 @@ class TwoLevel:
         hydro.close_patches(out, self.base, t["base_pid0"])
         nf = len(self.tab.fathers)
         for sub in range(2):
-            coarse = old if sub == 0 else 0.5 * old + 0.5 * self.base
+            if sub == 0:
+                coarse = old
+            else:
+                # 0.5 old + 0.5 new (GAMER's temporal weights), as (old + new) * 0.5: scaling by 0.5 is exact, so the
+                # two agree bit for bit (short of overflow), in two kernels instead of three and into the clone's buffer
+                coarse = old.add_(self.base).mul_(0.5)
             pg_f = torch.empty((nf, 5, 22, 22, 22), dtype=self.fine.dtype, device=self.fine.device)
             hydro.prepare_refined_patch_groups(coarse, self.fine, t["fine_coarse_slot"], t["fine_pg_slot"], pg_f,
                                                scheme=self.int_scheme, need=t["interp_need"], gamma=self.gamma)

@@ -239,6 +239,66 @@ __device__ __forceinline__ bool prim_bad( const T w[NV], double inv_gm1 )
    return bad || e < (T)0 || e > huge || e != e;
 }
 
+// the 2x2x2 fine values of one coarse cell: physicality test (when iterating), conversion back from primitive
+// variables, store; false when the test failed (nothing stored)
+template <typename T>
+__device__ __forceinline__ bool store_fine( T f[NV][8], bool iterate, bool prim, const InterpPrm &p, T *F, int F3, int fo,
+                                           int fdy, int fdz )
+{
+   #pragma clang fp contract(off)
+   bool bad = false;
+   if ( iterate )
+#     pragma unroll
+      for (int b=0; b<8; b++)
+      {
+         T t[NV];
+#        pragma unroll
+         for (int v=0; v<NV; v++) t[v] = f[v][b];
+         bad |= prim ? prim_bad( t, p.inv_gm1 ) : cons_bad( t );
+      }
+   if ( bad ) return false;
+#  pragma unroll
+   for (int b=0; b<8; b++)
+   {
+      T t[NV];
+#     pragma unroll
+      for (int v=0; v<NV; v++) t[v] = f[v][b];
+      if ( prim )
+      {
+         // Hydro_Pri2Con with EoS_DensPres2Eint
+         const T ei = (T)( (double)t[4]*p.inv_gm1 );
+         t[1] = t[0]*t[1];  t[2] = t[0]*t[2];  t[3] = t[0]*t[3];
+         t[4] = (T)0.5*( t[1]*t[1] + t[2]*t[2] + t[3]*t[3] )/t[0] + ei;
+      }
+      const int o = fo + ( b >> 2 )*fdz + ( ( b >> 1 ) & 1 )*fdy + ( b & 1 );
+#     pragma unroll
+      for (int v=0; v<NV; v++) F[ (size_t)v*F3 + o ] = t[v];
+   }
+   return true;
+}
+
+// x pass of the rows y-1, y, y+1 of coarse plane z' around index q, then the y pass: Y[y-lo/hi][x-lo/hi] of that
+// plane, exactly as interp_cell computes its Y[a] (3-point schemes)
+template <typename T, int SCHEME>
+__device__ __forceinline__ void xy_pass( const T *__restrict__ c, int q, int dy, bool mono, T coeff, bool opp0, T Y[2][2] )
+{
+   #pragma clang fp contract(off)
+   T X[3][2], u[5];
+#  pragma unroll
+   for (int b=0; b<3; b++)
+   {
+      const int qq = q + ( b - 1 )*dy;
+      u[0] = (T)0;  u[1] = c[qq - 1];  u[2] = c[qq];  u[3] = c[qq + 1];  u[4] = (T)0;
+      pass1d<T, SCHEME>( u, mono, coeff, opp0, X[b][0], X[b][1] );
+   }
+#  pragma unroll
+   for (int x=0; x<2; x++)
+   {
+      u[0] = (T)0;  u[1] = X[0][x];  u[2] = X[1][x];  u[3] = X[2][x];  u[4] = (T)0;
+      pass1d<T, SCHEME>( u, mono, coeff, opp0, Y[0][x], Y[1][x] );
+   }
+}
+
 template <typename T, int SCHEME>
 __global__ void __launch_bounds__( NT )
 interpolate_kernel( T *__restrict__ g_c, T *__restrict__ g_f, int nbox, InterpPrm p, int *__restrict__ nfail )
@@ -289,6 +349,61 @@ interpolate_kernel( T *__restrict__ g_c, T *__restrict__ g_f, int nbox, InterpPr
       if ( tid == 0 ) s_fail = 0;
       __syncthreads();
 
+      constexpr bool MARCH = ( SCHEME == 1 || SCHEME == 3 || SCHEME == 4 || SCHEME == 5 );
+      if ( MARCH && it == 0 )
+      {
+         // first pass of a 3-point tensor-product scheme: one lane per (x, y) column of the coarse range, marching
+         // along z with the y-pass results of planes z-1, z, z+1 in registers, so each plane's x and y passes run
+         // once per column instead of once per cell that reads them (9 pass1d per cell and field instead of 19).
+         // The operations and their order are interp_cell's, so the values are bitwise the same.
+         const int ncol = p.cr[0]*p.cr[1], cz = p.cr[0]*p.cr[1];
+         for (int col = tid; col < ncol; col += NT)
+         {
+            const int x = col % p.cr[0], y = col / p.cr[0];
+            bool todo = false;
+            for (int z = 0; z < p.cr[2]; z++) todo |= !s_done[ col + z*cz ];
+            if ( !todo ) continue;
+            const int q0 = p.c0[2]*dz + ( p.c0[1] + y )*dy + p.c0[0] + x;      // coarse index of (x, y, z = 0)
+            T Yw[NV][3][2][2];
+#           pragma unroll
+            for (int v=0; v<NV; v++)
+            {
+               xy_pass<T, SCHEME>( C + (size_t)v*C3, q0 - dz, dy, ( p.mono >> v ) & 1, coeff, p.opp0, Yw[v][0] );
+               xy_pass<T, SCHEME>( C + (size_t)v*C3, q0,      dy, ( p.mono >> v ) & 1, coeff, p.opp0, Yw[v][1] );
+            }
+            for (int z = 0; z < p.cr[2]; z++)
+            {
+               T f[NV][8], u[5];
+#              pragma unroll
+               for (int v=0; v<NV; v++)
+               {
+                  xy_pass<T, SCHEME>( C + (size_t)v*C3, q0 + ( z + 1 )*dz, dy, ( p.mono >> v ) & 1, coeff, p.opp0, Yw[v][2] );
+#                 pragma unroll
+                  for (int yy=0; yy<2; yy++)
+#                    pragma unroll
+                     for (int xx=0; xx<2; xx++)
+                     {
+                        u[0] = (T)0;  u[1] = Yw[v][0][yy][xx];  u[2] = Yw[v][1][yy][xx];  u[3] = Yw[v][2][yy][xx];  u[4] = (T)0;
+                        pass1d<T, SCHEME>( u, ( p.mono >> v ) & 1, coeff, p.opp0, f[v][( 0*2 + yy )*2 + xx],
+                                           f[v][( 1*2 + yy )*2 + xx] );
+                     }
+               }
+               const int c = col + z*cz;
+               if ( !s_done[c] )
+               {
+                  const int fo = ( p.f0[2] + 2*z )*fdz + ( p.f0[1] + 2*y )*fdy + p.f0[0] + 2*x;
+                  if ( store_fine( f, iterate, prim, p, F, F3, fo, fdy, fdz ) ) s_done[c] = 1; else s_fail = 1;
+               }
+#              pragma unroll
+               for (int v=0; v<NV; v++)
+#                 pragma unroll
+                  for (int yy=0; yy<2; yy++)
+#                    pragma unroll
+                     for (int xx=0; xx<2; xx++) { Yw[v][0][yy][xx] = Yw[v][1][yy][xx];  Yw[v][1][yy][xx] = Yw[v][2][yy][xx]; }
+            }
+         }
+      }
+      else
       for (int c = tid; c < ncell; c += NT)
       {
          if ( s_done[c] ) continue;
@@ -297,36 +412,8 @@ interpolate_kernel( T *__restrict__ g_c, T *__restrict__ g_f, int nbox, InterpPr
          T f[NV][8];
 #        pragma unroll
          for (int v=0; v<NV; v++) interp_cell<T, SCHEME>( C + (size_t)v*C3, ic, dy, dz, ( p.mono >> v ) & 1, coeff, p.opp0, f[v] );
-         bool bad = false;
-         if ( iterate )
-#           pragma unroll
-            for (int b=0; b<8; b++)
-            {
-               T t[NV];
-#              pragma unroll
-               for (int v=0; v<NV; v++) t[v] = f[v][b];
-               bad |= prim ? prim_bad( t, p.inv_gm1 ) : cons_bad( t );
-            }
-         if ( bad ) { s_fail = 1;  continue; }
          const int fo = ( p.f0[2] + 2*z )*fdz + ( p.f0[1] + 2*y )*fdy + p.f0[0] + 2*x;
-#        pragma unroll
-         for (int b=0; b<8; b++)
-         {
-            T t[NV];
-#           pragma unroll
-            for (int v=0; v<NV; v++) t[v] = f[v][b];
-            if ( prim )
-            {
-               // Hydro_Pri2Con with EoS_DensPres2Eint
-               const T ei = (T)( (double)t[4]*p.inv_gm1 );
-               t[1] = t[0]*t[1];  t[2] = t[0]*t[2];  t[3] = t[0]*t[3];
-               t[4] = (T)0.5*( t[1]*t[1] + t[2]*t[2] + t[3]*t[3] )/t[0] + ei;
-            }
-            const int o = fo + ( b >> 2 )*fdz + ( ( b >> 1 ) & 1 )*fdy + ( b & 1 );
-#           pragma unroll
-            for (int v=0; v<NV; v++) F[ (size_t)v*F3 + o ] = t[v];
-         }
-         s_done[c] = 1;
+         if ( store_fine( f, iterate, prim, p, F, F3, fo, fdy, fdz ) ) s_done[c] = 1; else s_fail = 1;
       }
       __syncthreads();
       const bool any = ( s_fail != 0 );
