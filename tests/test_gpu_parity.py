@@ -105,6 +105,20 @@ def test_fp64_retry_and_floors(scheme, case):
     assert np.all(err <= FP64_TOL), err
 
 
+@pytest.mark.parametrize("scheme", ["MHM_RP", "MHM"])
+@pytest.mark.parametrize("case", ["retry", "floors"])
+def test_fp32_retry_and_floors(scheme, case):
+    """The same special paths in FP32-bulk against the FP64 oracle, at the FP32-bulk bar."""
+    w = ic.random_sod((1, 1, 2), seed=31, sedov=True, cfl=1.2, sigma_v=1.5, scheme_ghost=GHOST[scheme])
+    kw = dict(minmod_maxiter=4) if case == "retry" else dict(min_dens=1e-2, min_pres=1e-3, min_eint=1e-3)
+    print(f"[gpu] fp32 {case} scheme={scheme}", flush=True)
+    o, f = _gpu(w.u.astype(np.float32), w, scheme=scheme, **kw)
+    ro, rf = _oracle(w.u, w, scheme=scheme, **kw)
+    l1 = rel_l1(o.astype(np.float64), ro)
+    print(f"  per-field rel L1 {l1}")
+    assert np.all(l1 <= FP32_L1_TOL), l1
+
+
 def test_conservation_and_determinism():
     """Periodic domain: domain sums of the conserved fields are unchanged to round-off, and two runs agree bitwise."""
     w = ic.random_sod((2, 2, 2), seed=5, sedov=True)
