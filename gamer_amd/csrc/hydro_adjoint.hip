@@ -6,9 +6,8 @@
 //
 //   forward   F1 half-step fluxes Fh (Hydro_RiemannPredict_Flux)   -> ws
 //             F2 half-step update, floors, primitive W (Hydro_RiemannPredict)   -> ws
-//             F3 (tangent / plain forward only) PLM faces FC of the 18^3 reconstruction block
-//                (Hydro_DataReconstruction)   -> ws
-//             F4 (tangent / plain forward only) full-step fluxes and update of the 16^3 output cells
+//             F4 (tangent / plain forward only) PLM faces (Hydro_DataReconstruction), full-step fluxes and update
+//                of the 16^3 output cells
 //   reverse   R4 lambda_F of each full-step face from lambda_out (transpose of Hydro_FullStepUpdate),
 //                lambda_FC = J_RS^T lambda_F (transpose of Hydro_ComputeFlux), with the two PLM faces of the
 //                face recomputed from W
@@ -55,7 +54,7 @@ constexpr int STRIDE[3] = { 1, NX, N2 };
 
 // workspace layout per patch group, in units of N3 doubles
 constexpr int WS_W   = 0;            // [5]       half-step primitives (cells 1..20)
-constexpr int WS_FC  = 5;            // [3][2][5] PLM faces L (0) / R (1) (cells 2..19; tangent kernel only)
+                                     // [5..35)   unused (the PLM faces are recomputed where needed)
 constexpr int WS_LW  = 35;           // [5]       lambda_W
 constexpr int WS_LU  = 40;           // [5]       lambda of the half-step conserved state
 constexpr int WS_LFC = 45;           // [3][2][5] lambda of the faces
@@ -195,16 +194,15 @@ __device__ __forceinline__ void plm( const T &L, const T &C, const T &R, const P
 
 // ---------------- workspace views ----------------
 // The adjoint keeps Fh in the lambda-side slots (read until R2, overwritten in R1); the tangent kernel stores
-// dual numbers (two doubles each) and packs Fh right after the faces: 50 duals = 100 doubles per cell.
-constexpr int WS_FH_TAN = 35;
+// dual numbers (two doubles each) and packs Fh right after W: 20 duals = 40 doubles per cell.
+constexpr int WS_FH_TAN = 5;
 static_assert( 2*( WS_FH_TAN + 15 ) <= WS_ADJ, "tangent workspace must fit the per-PG slot" );
 
 template <typename T>
 struct Fwd {
-   T *W, *FC, *Fh;
-   __device__ Fwd( T *base, int fh_off ) : W( base + WS_W*N3 ), FC( base + WS_FC*N3 ), Fh( base + (size_t)fh_off*N3 ) {}
+   T *W, *Fh;
+   __device__ Fwd( T *base, int fh_off ) : W( base + WS_W*N3 ), Fh( base + (size_t)fh_off*N3 ) {}
    __device__ T &w( int v, int c )                { return W[ v*N3 + c ]; }
-   __device__ T &fc( int d, int s, int v, int c ) { return FC[ ( ( d*2 + s )*NV + v )*N3 + c ]; }
    __device__ T &fh( int d, int v, int c )        { return Fh[ ( d*NV + v )*N3 + c ]; }
 };
 
@@ -228,7 +226,7 @@ template <> struct Load< Dual<1> >
 };
 
 // F1-F3: forward phases into the workspace
-template <int RS, int LIM, typename T, bool STORE_FC = true>
+template <int RS, int LIM, typename T>
 __device__ void forward_phases( const double *in, const double *tin, Fwd<T> &F, const Prm<T> &P )
 {
    const int tid = threadIdx.x;
@@ -265,26 +263,6 @@ __device__ void forward_phases( const double *in, const double *tin, Fwd<T> &F, 
       for (int v=0; v<NV; v++) F.w( v, c ) = w[v];
    }
    __syncthreads();
-   if ( !STORE_FC ) return;
-   // F3 PLM faces of the 18^3 reconstruction block, all three directions
-   for (int e = tid; e < 18*18*18; e += NT)
-   {
-      int i, j, k;
-      const int c = box_cell( RECON_CELLS, e, i, j, k );
-      for (int d=0; d<3; d++)
-      {
-         const int s = STRIDE[d];
-#        pragma unroll
-         for (int v=0; v<NV; v++)
-         {
-            T fL, fR;
-            plm<LIM>( F.w( v, c - s ), F.w( v, c ), F.w( v, c + s ), P, fL, fR );
-            F.fc( d, 0, v, c ) = fL;
-            F.fc( d, 1, v, c ) = fR;
-         }
-      }
-   }
-   __syncthreads();
 }
 
 template <typename T> struct Store;
@@ -297,8 +275,9 @@ template <> struct Store< Dual<1> >
    __device__ static void put( double *out, double *tout, size_t o, const Dual<1> &x ) { if ( out ) out[o] = x.v;  tout[o] = x.d[0]; }
 };
 
-// F4: full-step update of the 16^3 output cells, each of its six face fluxes evaluated from the stored faces
-template <int RS, typename T>
+// F4: full-step update of the 16^3 output cells.  The PLM faces of each face are recomputed from the half-step
+// primitives W (centres c - e_d, c, c + e_d), as the adjoint's R4 does, instead of being stored and re-read.
+template <int RS, int LIM, typename T>
 __device__ void full_update( const double *in, const double *tin, Fwd<T> &F, const Prm<T> &P, double *out, double *tout )
 {
    for (int e = threadIdx.x; e < PS*PS*PS; e += NT)
@@ -310,13 +289,18 @@ __device__ void full_update( const double *in, const double *tin, Fwd<T> &F, con
       for (int d=0; d<3; d++)
       {
          const int s = STRIDE[d];
-         T wl[NV], wr[NV], fm[NV], fp[NV];
+         T rm[NV], l0[NV], r0[NV], lp[NV], fm[NV], fp[NV];
 #        pragma unroll
-         for (int v=0; v<NV; v++) { wl[v] = F.fc( d, 1, v, c - s );  wr[v] = F.fc( d, 0, v, c ); }
-         full_flux<RS>( d, wl, wr, P, fm );
-#        pragma unroll
-         for (int v=0; v<NV; v++) { wl[v] = F.fc( d, 1, v, c );  wr[v] = F.fc( d, 0, v, c + s ); }
-         full_flux<RS>( d, wl, wr, P, fp );
+         for (int v=0; v<NV; v++)
+         {
+            const T w2 = F.w( v, c - 2*s ), w1 = F.w( v, c - s ), w0 = F.w( v, c ), p1 = F.w( v, c + s ), p2 = F.w( v, c + 2*s );
+            T dummy;
+            plm<LIM>( w2, w1, w0, P, dummy, rm[v] );       // R face of c - e_d
+            plm<LIM>( w1, w0, p1, P, l0[v], r0[v] );       // both faces of c
+            plm<LIM>( w0, p1, p2, P, lp[v], dummy );       // L face of c + e_d
+         }
+         full_flux<RS>( d, rm, l0, P, fm );
+         full_flux<RS>( d, r0, lp, P, fp );
 #        pragma unroll
          for (int v=0; v<NV; v++) dF[d][v] = fp[v] - fm[v];
       }
@@ -342,7 +326,7 @@ tangent_kernel( const double *__restrict__ g_in, const double *__restrict__ g_ti
    const double *tin = g_tin ? g_tin + pg*nin : nullptr;
    Fwd<T> F( reinterpret_cast<T*>( g_ws + (size_t)pg*WS_ADJ*N3 ), WS_FH_TAN );
    forward_phases<RS, LIM, T>( in, tin, F, P );
-   full_update<RS, T>( in, tin, F, P, g_out ? g_out + pg*nout : nullptr, g_tout ? g_tout + pg*nout : nullptr );
+   full_update<RS, LIM, T>( in, tin, F, P, g_out ? g_out + pg*nout : nullptr, g_tout ? g_tout + pg*nout : nullptr );
 }
 
 // ---------------- reverse phases ----------------
@@ -376,7 +360,7 @@ adjoint_kernel( const double *__restrict__ g_in, const double *__restrict__ g_lo
    Adj A( ws );
 
    // F1, F2 (the PLM faces are recomputed where they are needed instead of stored)
-   forward_phases<RS, LIM, double, false>( in, nullptr, F, P );
+   forward_phases<RS, LIM, double>( in, nullptr, F, P );
 
    // R4 full-step faces: lambda_F = dt/dh (lambda_out(c+e_d) - lambda_out(c)), then J_RS^T per side.  Only the
    // faces that feed a full-step flux receive a value; R3 reads the others as zero.
