@@ -54,13 +54,12 @@ constexpr int STRIDE[3] = { 1, NX, N2 };
 
 // workspace layout per patch group, in units of N3 doubles
 constexpr int WS_W   = 0;            // [5]       half-step primitives (cells 1..20)
-                                     // [5..35)   unused (the PLM faces are recomputed where needed)
-constexpr int WS_LW  = 35;           // [5]       lambda_W
-constexpr int WS_LU  = 40;           // [5]       lambda of the half-step conserved state
-constexpr int WS_LFC = 45;           // [3][2][5] lambda of the faces
-constexpr int WS_LS  = 75;           // [3][2][5] half-step face contributions to lambda_in (low / high cell);
+constexpr int WS_LW  = 5;            // [5]       lambda_W
+constexpr int WS_LU  = 10;           // [5]       lambda of the half-step conserved state
+constexpr int WS_LFC = 15;           // [3][2][5] lambda of the faces
+constexpr int WS_LS  = 45;           // [3][2][5] half-step face contributions to lambda_in (low / high cell);
                                      //           the first 15 arrays hold Fh [3][5] during the forward phases
-constexpr int WS_ADJ = 105;
+constexpr int WS_ADJ = 75;
 
 __device__ __forceinline__ void coords( int c, int &i, int &j, int &k ) { i = c % NX;  j = ( c / NX ) % NX;  k = c / N2; }
 __device__ __forceinline__ bool in_rng( int x, int lo, int hi ) { return x >= lo && x <= hi; }
