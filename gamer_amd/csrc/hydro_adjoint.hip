@@ -95,7 +95,7 @@ __device__ __forceinline__ Box face_box( int d, int n0, int nn, int t0, int nt )
 }
 constexpr int NHF = 21*20*20;        // half-step faces per direction: normal 0..20, tangential 1..20
 constexpr int NFF = 17*16*16;        // full-step faces per direction: normal 2..18, tangential 3..18
-constexpr Box HALF_CELLS = { 1, 1, 1, 20, 20, 20 }, RECON_CELLS = { 2, 2, 2, 18, 18, 18 }, CORE_CELLS = { 3, 3, 3, 16, 16, 16 };
+constexpr Box HALF_CELLS = { 1, 1, 1, 20, 20, 20 }, CORE_CELLS = { 3, 3, 3, 16, 16, 16 };
 
 // ---------------- pieces of the step, generic in the arithmetic type ----------------
 // cell-centred interface state of the half step (pressure recovery + floor, 1/rho, sound speed)

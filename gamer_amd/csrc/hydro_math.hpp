@@ -142,7 +142,7 @@ __device__ __forceinline__ void rs_hllc( const St<R> &L, const St<R> &Q, R pmin,
 template <typename R>
 __device__ __forceinline__ void rs_roe( const St<R> &L, const St<R> &Q, const Prm<R> &P, R f[NV] )
 {
-   const R Z = (R)0, ONE = (R)1, HALF = (R)0.5;
+   const R Z = (R)0, HALF = (R)0.5;
    const R hl = L.ir*( L.e + L.p ), hr = Q.ir*( Q.e + Q.p );
    const R sql = vsqrt( L.d ), sqr = vsqrt( Q.d );
    const R rho = sql*sqr, irho = rcp( rho );
