@@ -352,7 +352,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=25,
+                    help="untimed launches before the timed ones (the first ~20 run while the clocks ramp up)")
     ap.add_argument("--npg-axis", type=int, default=16, help="patch groups per axis (16 -> NPG 4096)")
     ap.add_argument("--cpu-sample", type=int, default=4096, help="patch groups in the CPU baseline sample (0: skip)")
     ap.add_argument("--skip-host-arrays", action="store_true", help="skip the PCIe-inclusive host-array figure")
