@@ -37,6 +37,14 @@ template <typename R> __device__ __forceinline__ R vmin( R a, R b ) { return __b
 template <typename R> __device__ __forceinline__ R vabs( R a )      { return __builtin_fabs( a ); }
 template <typename R> __device__ __forceinline__ R vcopysign( R m, R s ) { return __builtin_copysign( m, s ); }
 
+// reciprocal for the FP64 pressure recovery of the FP32-bulk kernels: hardware estimate + one Newton step, a
+// relative error near 1e-14, far below what the float result keeps (two steps, as in rcp(double), for FP64)
+__device__ __forceinline__ double rcp_recover( double x )
+{
+   const double r = __builtin_amdgcn_rcp( x );
+   return __builtin_fma( r, __builtin_fma( -x, r, 1.0 ), r );
+}
+
 // pressure floor that lets NaN through (FluUtility.cpp:681-690): a NaN compares false and is returned as is.
 // One compare + select; max() would first have to quiet operands read from memory (IEEE mode).
 template <typename R> __device__ __forceinline__ R floor_pres( R p, R pmin ) { return ( p < pmin ) ? pmin : p; }
@@ -361,15 +369,13 @@ __device__ __forceinline__ bool full_cell( const IO u[NV], const R dfxy[NV], con
 #  pragma unroll
    for (int v=0; v<NV; v++)
       o[v] = (IO)( (double)u[v] - P.dt_dh*( (double)dfxy[v] + ( (double)fzu[v] - (double)fzl[v] ) ) );
+   // each test is one ordered compare, false for NaN: "not finite-and-in-range" covers the NaN test too
    const IO huge = IOLim<IO>::huge, Z = (IO)0;
-   bool bad = false;
-#  pragma unroll
-   for (int v=0; v<NV; v++) bad |= ( o[v] != o[v] );
-   bad |= ( vabs( o[1] ) > huge || vabs( o[2] ) > huge || vabs( o[3] ) > huge );
-   bad |= ( o[0] <= Z || o[0] > huge || o[4] <= Z || o[4] > huge );
+   bool ok = ( vabs( o[1] ) <= huge ) && ( vabs( o[2] ) <= huge ) && ( vabs( o[3] ) <= huge );
+   ok = ok && ( o[0] > Z && o[0] <= huge ) && ( o[4] > Z && o[4] <= huge );
    const IO ei = o[4] - (IO)0.5*( o[1]*o[1] + o[2]*o[2] + o[3]*o[3] )*rcp( o[0] );
-   bad |= ( ei < (IO)-3.0*o[4]*IOLim<IO>::eps || ei > huge || ei != ei );
-   return bad;
+   ok = ok && ( ei >= (IO)-3.0*o[4]*IOLim<IO>::eps && ei <= huge );
+   return !ok;
 }
 
 // Hancock half-step advance of the six face states of one cell (MHM only;
