@@ -45,6 +45,8 @@ __device__ __forceinline__ double rcp_recover( double x )
    return __builtin_fma( r, __builtin_fma( -x, r, 1.0 ), r );
 }
 
+template <typename R> __device__ __forceinline__ double rcp_pres( double x ) { return sizeof( R ) == 4 ? rcp_recover( x ) : rcp( x ); }
+
 // pressure floor that lets NaN through (FluUtility.cpp:681-690): a NaN compares false and is returned as is.
 // One compare + select; max() would first have to quiet operands read from memory (IEEE mode).
 template <typename R> __device__ __forceinline__ R floor_pres( R p, R pmin ) { return ( p < pmin ) ? pmin : p; }
@@ -392,7 +394,7 @@ __device__ __forceinline__ void hancock( R fc[6][NV], const R fcp[6][NV], const 
       const int d = f/2;
       const R mn = fc[f][1 + d];
       const double eint = (double)fc[f][4] - 0.5*( (double)fc[f][1]*fc[f][1] + (double)fc[f][2]*fc[f][2]
-                                                    + (double)fc[f][3]*fc[f][3] )*rcp( (double)fc[f][0] );
+                                                    + (double)fc[f][3]*fc[f][3] )*rcp_pres<R>( (double)fc[f][0] );
       const R p  = floor_pres( (R)( eint*P.gm1_d ), P.min_pres );
       const R vn = rcp( fc[f][0] )*mn;
       F[f][0] = mn;
@@ -434,7 +436,7 @@ __device__ __forceinline__ void hancock( R fc[6][NV], const R fcp[6][NV], const 
       {
          fc[f][0] = vmax( fc[f][0], P.min_dens );
          const double ke  = 0.5*( (double)fc[f][1]*fc[f][1] + (double)fc[f][2]*fc[f][2] + (double)fc[f][3]*fc[f][3] )
-                            *rcp( (double)fc[f][0] );
+                            *rcp_pres<R>( (double)fc[f][0] );
          const double ein = (double)fc[f][4] - ke;
          const double eo  = ( ein == ein ) ? __builtin_fmax( ein, P.min_eint_d ) : ein;
          if ( ein != eo ) fc[f][4] = (R)( (double)fc[f][4] - ein + eo );
@@ -457,7 +459,7 @@ __device__ __forceinline__ St<R> cons_state( const R c[NV], const Prm<R> &P )
 {
    St<R> s;
    const double eint = (double)c[4] - 0.5*( (double)c[1]*c[1] + (double)c[2]*c[2] + (double)c[3]*c[3] )
-                                      *rcp( (double)c[0] );
+                                      *rcp_pres<R>( (double)c[0] );
    s.d  = c[0];
    s.mn = c[1 + D];
    s.m1 = c[1 + ( D + 1 ) % 3];

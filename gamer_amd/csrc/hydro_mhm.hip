@@ -51,7 +51,7 @@ __device__ __forceinline__ bool ring_column( int t, int &i, int &j )
 template <typename R, typename IO>
 __device__ __forceinline__ void prim_of( const IO u[NV], R w[NV], const Prm<R> &P )
 {
-   const double d = (double)u[0], ir = rcp( d );
+   const double d = (double)u[0], ir = rcp_pres<R>( d );
    const double eint = (double)u[4] - 0.5*( (double)u[1]*u[1] + (double)u[2]*u[2] + (double)u[3]*u[3] )*ir;
    const R rr = rcp( (R)u[0] );
    w[0] = (R)u[0];

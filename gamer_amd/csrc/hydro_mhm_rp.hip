@@ -122,7 +122,7 @@ template <typename R, typename IO>
 __device__ __forceinline__ void derive_cell( const IO u[NV], R q[8], const Prm<R> &P )
 {
    const double d = (double)u[0], mx = (double)u[1], my = (double)u[2], mz = (double)u[3], e = (double)u[4];
-   const double eint = e - 0.5*( mx*mx + my*my + mz*mz )*( sizeof( R ) == 4 ? rcp_recover( d ) : rcp( d ) );
+   const double eint = e - 0.5*( mx*mx + my*my + mz*mz )*rcp_pres<R>( d );
    const R p = floor_pres( (R)( eint*P.gm1_d ), P.min_pres );
    const R ir = rcp( (R)u[0] );
 #  pragma unroll
@@ -145,7 +145,7 @@ __device__ __forceinline__ void half_cell( const R u[NV], const R dfxy[NV], cons
    for (int v=0; v<NV; v++) c[v] = u[v] - k*( dfxy[v] + ( hzu[v] - hzl[v] ) );
    c[0] = vmax( c[0], P.min_dens );
    const double d = (double)c[0], m1 = (double)c[1], m2 = (double)c[2], m3 = (double)c[3];
-   const double ke = 0.5*( m1*m1 + m2*m2 + m3*m3 )*( sizeof( R ) == 4 ? rcp_recover( d ) : rcp( d ) );
+   const double ke = 0.5*( m1*m1 + m2*m2 + m3*m3 )*rcp_pres<R>( d );
    double e = (double)c[4];
    double ein = e - ke;
    const double eout = ( ein == ein ) ? __builtin_fmax( ein, P.min_eint_d ) : ein;
