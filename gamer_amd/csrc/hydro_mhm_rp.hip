@@ -6,7 +6,7 @@
 // scratch per patch group in FP64).  Here the workgroup marches along z, one input plane per step, and keeps
 // every intermediate on chip:
 //
-//   * Lane t owns the output column (i,j) = (3 + t%16, 3 + t/16).  Lanes 0-183 (waves 0-2) also own one
+//   * Lane t owns the output column (i,j) = (3 + t%16, 3 + output_row(t)).  Lanes 0-183 (waves 0-2) also own one
 //     "outer" ghost column for the half step (half_step_column()); lanes 192-255 (wave 3) each reconstruct one
 //     ghost column next to the output block along x or y, and half of them compute the full-step flux on the
 //     face between that column and the block (edge_column()).  Everything that couples planes along z — the
@@ -47,6 +47,15 @@ constexpr int NT  = 256;         // lanes per workgroup
 #define WPE64 1                  // ... and of the FP64 kernels
 #endif
 
+
+// output row (j - 3) of lane t: wave w takes rows w, w+8, w+4, w+12, 16 lanes each.  The neighbour reads of a
+// plane in LDS (row stride NX = 22 words) then put the two rows of each half wave 176 words apart, 16 banks
+// modulo 32, so they meet no bank conflict, where rows 4w..4w+3 collided on 6 of 16 lanes per row pair.
+__device__ __forceinline__ int output_row( int t )
+{
+   const int q = ( t & 63 ) >> 4;
+   return ( t >> 6 ) + 8*( q & 1 ) + 4*( q >> 1 );
+}
 
 // ghost columns served by lane t besides its output column; -1 = none
 __device__ __forceinline__ void ring2( int k, int &i, int &j )
@@ -213,7 +222,7 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
 
    const int tid = threadIdx.x;
 
-   const Col K = make_col( 3 + tid % PS, 3 + tid / PS );
+   const Col K = make_col( 3 + tid % PS, 3 + output_row( tid ) );
    int oi, oj;
    const bool has_o = half_step_column( tid, oi, oj );
    const Col O = make_col( has_o ? oi : 0, has_o ? oj : 0 );
