@@ -51,6 +51,34 @@ template <typename R> __device__ __forceinline__ double rcp_pres( double x ) { r
 // One compare + select; max() would first have to quiet operands read from memory (IEEE mode).
 template <typename R> __device__ __forceinline__ R floor_pres( R p, R pmin ) { return ( p < pmin ) ? pmin : p; }
 
+// ---------------- raw buffer access ----------------
+// Global loads and stores of the z-marching kernels go through a buffer descriptor per patch group: the 64-bit
+// base stays in SGPRs and every access takes a lane-constant 32-bit byte offset plus a wave-uniform byte offset,
+// so the march does no 64-bit address arithmetic per access.  The hardware range-checks each access against
+// `bytes`: a read past it returns 0 and a write past it is dropped.  Word 3 = 0x00020000 is the gfx9-family
+// raw-buffer format (32-bit data format, no swizzle).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_buf( const void *base, unsigned bytes )
+{
+   return __builtin_amdgcn_make_buffer_rsrc( const_cast<void *>( base ), (short)0, (int)bytes, 0x00020000 );
+}
+__device__ __forceinline__ void buf_load( __amdgpu_buffer_rsrc_t r, int voff, int soff, float &x )
+{
+   x = __builtin_bit_cast( float, __builtin_amdgcn_raw_buffer_load_b32( r, voff, soff, 0 ) );
+}
+__device__ __forceinline__ void buf_load( __amdgpu_buffer_rsrc_t r, int voff, int soff, double &x )
+{
+   x = __builtin_bit_cast( double, __builtin_amdgcn_raw_buffer_load_b64( r, voff, soff, 0 ) );
+}
+__device__ __forceinline__ void buf_store( float x, __amdgpu_buffer_rsrc_t r, int voff, int soff )
+{
+   __builtin_amdgcn_raw_buffer_store_b32( __builtin_bit_cast( unsigned, x ), r, voff, soff, 0 );
+}
+__device__ __forceinline__ void buf_store( double x, __amdgpu_buffer_rsrc_t r, int voff, int soff )
+{
+   using U2 = decltype( __builtin_amdgcn_raw_buffer_load_b64( __amdgpu_buffer_rsrc_t(), 0, 0, 0 ) );
+   __builtin_amdgcn_raw_buffer_store_b64( __builtin_bit_cast( U2, x ), r, voff, soff, 0 );
+}
+
 // ---------------- solver constants ----------------
 template <typename R>
 struct Prm {

@@ -232,6 +232,12 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
    const int  e_c  = ei + NX*ej;
    const int  e_st = e_dy ? NX : 1;             // neighbour stride of the edge column's direction
    const int  e_vo = e_dy ? NV : 0;             // its face states / fluxes in S.f / S.b
+   // lane-constant byte offsets of the global stores: output column in a 16^2 plane, and the coarse-fine flux
+   // planes x = 0/8 (ffx), y = 0/8 (ffy) and x or y = 16 (edge lanes) at row K.j, K.i or the edge column
+   const int  kxy    = ( K.i - 3 ) + PS*( K.j - 3 );
+   const int  fx_off = ( ( ( K.i - 3 )/8 )*NV*PS*PS + ( K.j - 3 ) )*(int)sizeof( IO );
+   const int  fy_off = ( ( 3 + ( K.j - 3 )/8 )*NV*PS*PS + ( K.i - 3 ) )*(int)sizeof( IO );
+   const int  fe_off = ( ( e_dy ? 5 : 2 )*NV*PS*PS + ( ( e_dy ? ei : ej ) - 3 ) )*(int)sizeof( IO );
    const int  lim = P0.limiter;
 
    int base = 0;
@@ -260,9 +266,13 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
    }
    if ( pg >= npg ) break;
 
+   constexpr int SZ = sizeof( IO );
    const IO *in  = g_in  + (size_t)pg*NV*NC*NX;
    IO       *out = g_out + (size_t)pg*NV*PS*PS*PS;
    IO       *flx = store_flux ? g_flux + (size_t)pg*9*NV*PS*PS : nullptr;
+   const __amdgpu_buffer_rsrc_t b_in  = make_buf( in,  NV*NC*NX*SZ );
+   const __amdgpu_buffer_rsrc_t b_out = make_buf( out, NV*PS*PS*PS*SZ );
+   const __amdgpu_buffer_rsrc_t b_flx = make_buf( flx, flx ? 9*NV*PS*PS*SZ : 0 );
    Prm<R> P = P0;
    double dh_safety = dh_safety0;
    if ( B.pg_dt_dh )
@@ -298,7 +308,7 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
             const int c = tid + k*NT;
             if ( c < NC )
 #              pragma unroll
-               for (int v=0; v<NV; v++) pf[k][v] = in[ (size_t)v*NC*NX + (size_t)t*NC + c ];
+               for (int v=0; v<NV; v++) buf_load( b_in, c*SZ, ( v*NC*NX + t*NC )*SZ, pf[k][v] );
          }
       };
       // input plane (prefetched) -> LDS, with pressure, 1/rho and c_s once per cell
@@ -437,7 +447,7 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
             if ( flx && ( r - 3 ) % 8 == 0 )
 #              pragma unroll
                for (int v=0; v<NV; v++)
-                  flx[ ( ( 6 + ( r - 3 )/8 )*NV + v )*PS*PS + ( K.i - 3 ) + PS*( K.j - 3 ) ] = (IO)fzn[v];
+                  buf_store( (IO)fzn[v], b_flx, kxy*SZ, ( ( 6 + ( r - 3 )/8 )*NV + v )*PS*PS*SZ );
          }
          if ( r >= 2 )
 #           pragma unroll
@@ -464,15 +474,15 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
                if ( ( K.i - 3 ) % 8 == 0 )
 #                 pragma unroll
                   for (int v=0; v<NV; v++)
-                     flx[ ( ( ( K.i - 3 )/8 )*NV + v )*PS*PS + ( K.j - 3 ) + PS*( r - 3 ) ] = (IO)ffx[v];
+                     buf_store( (IO)ffx[v], b_flx, fx_off, ( v*PS*PS + PS*( r - 3 ) )*SZ );
                if ( ( K.j - 3 ) % 8 == 0 )
 #                 pragma unroll
                   for (int v=0; v<NV; v++)
-                     flx[ ( ( 3 + ( K.j - 3 )/8 )*NV + v )*PS*PS + ( K.i - 3 ) + PS*( r - 3 ) ] = (IO)ffy[v];
+                     buf_store( (IO)ffy[v], b_flx, fy_off, ( v*PS*PS + PS*( r - 3 ) )*SZ );
                if ( has_e && e_face )             // planes x = 16 (2) and y = 16 (5)
 #                 pragma unroll
                   for (int v=0; v<NV; v++)
-                     flx[ ( ( e_dy ? 5 : 2 )*NV + v )*PS*PS + ( ( e_dy ? ei : ej ) - 3 ) + PS*( r - 3 ) ] = (IO)fe[v];
+                     buf_store( (IO)fe[v], b_flx, fe_off, ( v*PS*PS + PS*( r - 3 ) )*SZ );
             }
             __syncthreads();                     // [F] full-step fluxes visible
 #           pragma unroll
@@ -486,9 +496,8 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
 #           pragma unroll
             for (int v=0; v<NV; v++) uo[v] = S.u3[ring][v][tid];
             const bool bad = full_cell( uo, kDf, kFz, fzn, Q, res );
-            const size_t co = (size_t)( K.i - 3 ) + PS*( K.j - 3 ) + (size_t)PS*PS*( o - 3 );
 #           pragma unroll
-            for (int v=0; v<NV; v++) out[ (size_t)v*PS*PS*PS + co ] = res[v];
+            for (int v=0; v<NV; v++) buf_store( res[v], b_out, kxy*SZ, ( v*PS*PS*PS + PS*PS*( o - 3 ) )*SZ );
             if ( bad ) S.fail = 1;
             if ( g_dt ) cfl[ ( o - 3 )/8 ] = vmax( cfl[ ( o - 3 )/8 ], cell_cfl( res, (double)Q.gamma, Q.gm1_d, Q.min_pres_d ) );
          }
