@@ -332,6 +332,9 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
       stage();
       issue( 1 );
 
+      // two steps per iteration let the compiler rename the plane-to-plane register hand-offs (kU1 = kU, the
+      // z-flux and face carries) instead of copying them; the Roe kernels stay rolled, their code is twice as large
+#     pragma unroll ( RS != 2 ? 2 : 1 )
       for (int t = 0; t < NX; t++)
       {
          const int o = t - 3;                    // output plane of this step
