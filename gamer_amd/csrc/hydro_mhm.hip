@@ -96,6 +96,13 @@ mhm_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict_
    const int  oc = oi + NX*oj;
    const bool ofx = has_o && oi == 18;                      // owns the full-step x face (17,18)
    const bool ofy = has_o && oj == 18;
+   // lane-constant byte offsets of the global stores (output column in a 16^2 plane; coarse-fine flux planes
+   // x = 0/8, y = 0/8 at row kj / ki; x = 16 and y = 16 for the ring lanes that own those faces)
+   const int  kxy    = ( ki - 2 ) + PS*( kj - 2 );
+   const int  fx_off = ( ( ( ki - 2 )/8 )*NV*PS*PS + ( kj - 2 ) )*(int)sizeof( IO );
+   const int  fy_off = ( ( 3 + ( kj - 2 )/8 )*NV*PS*PS + ( ki - 2 ) )*(int)sizeof( IO );
+   const int  ox_off = ( 2*NV*PS*PS + ( oj - 2 ) )*(int)sizeof( IO );
+   const int  oy_off = ( 5*NV*PS*PS + ( oi - 2 ) )*(int)sizeof( IO );
 
    int base = 0;
    for (int item = 0; ; item++)
@@ -123,9 +130,13 @@ mhm_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict_
    }
    if ( pg >= npg ) break;
 
+   constexpr int SZ = sizeof( IO );
    const IO *in  = g_in  + (size_t)pg*NV*NC*NX;
    IO       *out = g_out + (size_t)pg*NV*PS*PS*PS;
    IO       *flx = store_flux ? g_flux + (size_t)pg*9*NV*PS*PS : nullptr;
+   const __amdgpu_buffer_rsrc_t b_in  = make_buf( in,  NV*NC*NX*SZ );
+   const __amdgpu_buffer_rsrc_t b_out = make_buf( out, NV*PS*PS*PS*SZ );
+   const __amdgpu_buffer_rsrc_t b_flx = make_buf( flx, flx ? 9*NV*PS*PS*SZ : 0 );
    Prm<R> P = P0;
    double dh_safety = dh_safety0;
    if ( B.pg_dt_dh )
@@ -157,11 +168,14 @@ mhm_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict_
             const int c = tid + k*NT;
             if ( c < NC )
 #              pragma unroll
-               for (int v=0; v<NV; v++) pf[k][v] = in[ (size_t)v*NC*NX + (size_t)t*NC + c ];
+               for (int v=0; v<NV; v++) buf_load( b_in, c*SZ, ( v*NC*NX + t*NC )*SZ, pf[k][v] );
          }
       };
       issue( 0 );
 
+      // two steps per iteration: the LDS slot (t & 1) becomes a constant and the plane-to-plane register hand-offs
+      // are renamed instead of copied; the Roe kernels stay rolled, their code is twice as large
+#     pragma unroll ( RS != 2 ? 2 : 1 )
       for (int t = 0; t < NX; t++)
       {
          const int slot = t & 1;
@@ -218,7 +232,7 @@ mhm_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict_
             if ( flx && ( r - 2 ) % 8 == 0 )
 #              pragma unroll
                for (int v=0; v<NV; v++)
-                  flx[ ( ( 6 + ( r - 2 )/8 )*NV + v )*PS*PS + ( ki - 2 ) + PS*( kj - 2 ) ] = (IO)fzn[v];
+                  buf_store( (IO)fzn[v], b_flx, kxy*SZ, ( ( 6 + ( r - 2 )/8 )*NV + v )*PS*PS*SZ );
          }
          if ( r >= 1 )
 #           pragma unroll
@@ -259,17 +273,17 @@ mhm_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict_
                if ( ( ki - 2 ) % 8 == 0 )
 #                 pragma unroll
                   for (int v=0; v<NV; v++)
-                     flx[ ( ( ( ki - 2 )/8 )*NV + v )*PS*PS + ( kj - 2 ) + PS*( r - 2 ) ] = (IO)fx[v];
+                     buf_store( (IO)fx[v], b_flx, fx_off, ( v*PS*PS + PS*( r - 2 ) )*SZ );
                if ( ( kj - 2 ) % 8 == 0 )
 #                 pragma unroll
                   for (int v=0; v<NV; v++)
-                     flx[ ( ( 3 + ( kj - 2 )/8 )*NV + v )*PS*PS + ( ki - 2 ) + PS*( r - 2 ) ] = (IO)fy[v];
+                     buf_store( (IO)fy[v], b_flx, fy_off, ( v*PS*PS + PS*( r - 2 ) )*SZ );
                if ( ofx )
 #                 pragma unroll
-                  for (int v=0; v<NV; v++) flx[ ( 2*NV + v )*PS*PS + ( oj - 2 ) + PS*( r - 2 ) ] = (IO)fo[v];
+                  for (int v=0; v<NV; v++) buf_store( (IO)fo[v], b_flx, ox_off, ( v*PS*PS + PS*( r - 2 ) )*SZ );
                if ( ofy )
 #                 pragma unroll
-                  for (int v=0; v<NV; v++) flx[ ( 5*NV + v )*PS*PS + ( oi - 2 ) + PS*( r - 2 ) ] = (IO)fo[v];
+                  for (int v=0; v<NV; v++) buf_store( (IO)fo[v], b_flx, oy_off, ( v*PS*PS + PS*( r - 2 ) )*SZ );
             }
             __syncthreads();
 #           pragma unroll
@@ -283,9 +297,8 @@ mhm_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict_
 #           pragma unroll
             for (int v=0; v<NV; v++) uo[v] = S.u2[v][tid];
             const bool bad = full_cell( uo, kDf, kFz, fzn, Q, res );
-            const size_t co = (size_t)( ki - 2 ) + PS*( kj - 2 ) + (size_t)PS*PS*( o - 2 );
 #           pragma unroll
-            for (int v=0; v<NV; v++) out[ (size_t)v*PS*PS*PS + co ] = res[v];
+            for (int v=0; v<NV; v++) buf_store( res[v], b_out, kxy*SZ, ( v*PS*PS*PS + PS*PS*( o - 2 ) )*SZ );
             if ( bad ) S.fail = 1;
             if ( g_dt ) cfl[ ( o - 2 )/8 ] = vmax( cfl[ ( o - 2 )/8 ], cell_cfl( res, (double)Q.gamma, Q.gm1_d, Q.min_pres_d ) );
          }
