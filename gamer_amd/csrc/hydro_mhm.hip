@@ -88,8 +88,9 @@ mhm_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict_
 
    const int tid = threadIdx.x;
 
-   // wave w takes output rows w, w+4, w+8, w+12 (16 lanes each): 80 words apart in LDS (row stride NX = 20), so
-   // each wave's neighbour reads of a plane hit 64 distinct banks, where rows 4w..4w+3 partly overlapped
+   // wave w takes output rows w, w+4, w+8, w+12 (16 lanes each), 80 words apart in LDS (row stride NX = 20): the
+   // two rows of each half wave lie 16 banks apart modulo 32 and the four rows of the wave cover 64 consecutive
+   // banks modulo 64, so a neighbour read of a plane meets no bank conflict; rows 4w..4w+3 partly overlapped
    const int ki = 2 + tid % PS, kj = 2 + ( tid >> 6 ) + 4*( ( tid & 63 ) >> 4 ), kc = ki + NX*kj;
    int oi, oj;
    const bool has_o = ring_column( tid, oi, oj );
