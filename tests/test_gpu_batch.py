@@ -45,16 +45,19 @@ def test_batch_levels_match_oracle_fp64(scheme, persistent):
     assert flux_rel_err(flux.cpu().numpy(), ref_f, ref_o) <= 1e-12
 
 
+@pytest.mark.parametrize("scheme,rsolver", [("MHM_RP", "HLLC"), ("MHM", "HLLC"), ("MHM_RP", "ROE")])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-def test_persistent_equals_grid_bitwise(dtype):
-    """The work-queue kernel runs the same per-PG code as the one-workgroup-per-PG kernel: identical bits."""
+def test_persistent_equals_grid_bitwise(dtype, scheme, rsolver):
+    """The work-queue kernel runs the same per-PG code as the one-workgroup-per-PG kernel: identical bits (HLLC
+    kernels march two planes per loop iteration, Roe kernels one)."""
     from gamer_amd import hydro
-    print(f"[gpu] persistent vs grid {dtype}", flush=True)
-    b = _batch(dtype=dtype, counts=(64, 128, 192))
-    o1, f1 = hydro.solve_batch(b.u, b.pg_dt_dh)
+    print(f"[gpu] persistent vs grid {dtype} {scheme} {rsolver}", flush=True)
+    b = _batch(scheme, dtype=dtype, counts=(64, 128, 192))
+    kw = dict(scheme=scheme, rsolver=rsolver)
+    o1, f1 = hydro.solve_batch(b.u, b.pg_dt_dh, **kw)
     cnt = torch.zeros(1, dtype=torch.int32, device=DEV)
     for g, k in ((1, 1), (3, 1), (2, 3), (1, 4), (4, 64)):
-        o2, f2 = hydro.solve_batch(b.u, b.pg_dt_dh, queue_counter=cnt, groups_per_cu=g, pgs_per_ticket=k)
+        o2, f2 = hydro.solve_batch(b.u, b.pg_dt_dh, queue_counter=cnt, groups_per_cu=g, pgs_per_ticket=k, **kw)
         torch.cuda.synchronize()
         assert torch.equal(o1, o2) and torch.equal(f1, f2), (g, k)
 
