@@ -353,7 +353,7 @@ class TwoLevel:
 
     def _close(self, pg_in, dt, dh):
         out, flux = hydro.fluid_solver(pg_in, dt, dh, rsolver=self.rsolver, gamma=self.gamma, **self.kw)
-        self.counters += hydro.correct_unphysical(pg_in, out, flux, dt, dh, gamma=self.gamma)
+        hydro.correct_unphysical(pg_in, out, flux, dt, dh, gamma=self.gamma, counters=self.counters)
         return out, flux
 
     def step(self, dt_base):

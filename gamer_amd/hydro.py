@@ -346,16 +346,23 @@ def _int_table(t, shape, name):
 
 def correct_unphysical(u_in, out, flux, dt, dh, mode="3D1D", rsolver="HLLE", scheme="MHM_RP", gamma=5.0 / 3.0,
                        last_resort_floor=True, auto_reduce_dt=False, fixup_flux=True, check_pres=False,
-                       min_dens=0.0, min_pres=0.0, min_eint=0.0, stream=None):
+                       min_dens=0.0, min_pres=0.0, min_eint=0.0, stream=None, counters=None):
     """CorrectUnphysical on device: corrects ``out`` (and the coarse-fine planes of ``flux``) in place.
-    Returns a 2-element int32 device tensor: [corrected cells, cells still unphysical]."""
+    Returns a 2-element int32 device tensor: [corrected cells, cells still unphysical].  With ``counters`` (an
+    int32 device tensor of 2 elements) the kernel adds its counts to it and returns it, with no extra launches."""
     import torch
     n = NX[scheme]
     if tuple(u_in.shape[1:]) != (5, n, n, n) or tuple(out.shape) != (u_in.shape[0], 5, 16, 16, 16):
         raise HydroError("correct_unphysical: shapes must be (npg, 5, N^3) and (npg, 5, 16^3)")
     if u_in.dtype != out.dtype or (flux is not None and flux.dtype != out.dtype):
         raise HydroError("correct_unphysical: dtypes differ")
-    cnt = torch.zeros(2, dtype=torch.int32, device=out.device)
+    if counters is None:
+        cnt = torch.zeros(2, dtype=torch.int32, device=out.device)
+    elif (counters.dtype != torch.int32 or counters.numel() != 2 or not counters.is_contiguous()
+          or counters.device != out.device):
+        raise HydroError("correct_unphysical: counters must be a contiguous int32 tensor of 2 elements on out's device")
+    else:
+        cnt = counters
     a = HydroCorrectArgs(u_in.contiguous().data_ptr(), out.data_ptr(), flux.data_ptr() if flux is not None else None,
                          u_in.shape[0], float(dt), float(dh), float(gamma), SCHEME[scheme], FIRST_FLUX_CORR[mode],
                          RSOLVER[rsolver], int(last_resort_floor), int(auto_reduce_dt), int(fixup_flux),

@@ -59,6 +59,27 @@ def test_correct_unphysical_matches_oracle(dtype, mode, rsolver):
     np.testing.assert_allclose(gf, rf, rtol=tol, atol=tol * np.nanmax(np.abs(rf)))
 
 
+def test_correct_unphysical_accumulates_into_given_counters():
+    """With ``counters`` the kernel adds to the caller's tensor (the AMR step keeps one running count): two calls
+    leave the sum of the oracle's counts, and the outputs equal those of the call without it."""
+    import oracle
+    from gamer_amd import hydro
+    print("[gpu] correct_unphysical running counters", flush=True)
+    u, out, flux, dt, dh = _corrupted_case(11, np.float64)
+    kw = dict(mode="3D1D", rsolver="HLLE", min_dens=1e-6, min_pres=1e-8, min_eint=1e-8)
+    _, _, rc = oracle.correct_unphysical(u, out, flux, dt, dh, **kw)
+    run = torch.tensor([5, 7], dtype=torch.int32, device="cuda:0")
+    go1, gf1, go2, gf2 = _dev(out), _dev(flux), _dev(out), _dev(flux)
+    assert hydro.correct_unphysical(_dev(u), go1, gf1, dt, dh, counters=run, **kw) is run
+    hydro.correct_unphysical(_dev(u), _dev(out), _dev(flux), dt, dh, counters=run, **kw)
+    hydro.correct_unphysical(_dev(u), go2, gf2, dt, dh, **kw)
+    torch.cuda.synchronize()
+    assert run.cpu().tolist() == [5 + 2*int(rc[0]), 7 + 2*int(rc[1])]
+    assert torch.equal(go1, go2) and torch.equal(gf1, gf2)
+    with pytest.raises(hydro.HydroError):
+        hydro.correct_unphysical(_dev(u), go1, gf1, dt, dh, counters=run.to(torch.int64), **kw)
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_correct_unphysical_split_sweep_matches_oracle(dtype):
     """The 3D1D sweep path (unsplit update rejected by MIN_EINT, split accepted; see test_oracle_sync)."""
