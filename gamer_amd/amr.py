@@ -350,6 +350,7 @@ class TwoLevel:
         nf = len(tables.fathers)
         self.fine_pid0 = T((8 * np.arange(nf)).astype(np.int32))
         self.counters = torch.zeros(2, dtype=torch.int32, device=dev)
+        self.interp_fail = torch.zeros(1, dtype=torch.int32, device=dev)    # boxes still unphysical, running count
 
     def _close(self, pg_in, dt, dh):
         out, flux = hydro.fluid_solver(pg_in, dt, dh, rsolver=self.rsolver, gamma=self.gamma, **self.kw)
@@ -377,7 +378,8 @@ class TwoLevel:
                 coarse = old.add_(self.base).mul_(0.5)
             pg_f = torch.empty((nf, 5, 22, 22, 22), dtype=self.fine.dtype, device=self.fine.device)
             hydro.prepare_refined_patch_groups(coarse, self.fine, t["fine_coarse_slot"], t["fine_pg_slot"], pg_f,
-                                               scheme=self.int_scheme, need=t["interp_need"], gamma=self.gamma)
+                                               scheme=self.int_scheme, need=t["interp_need"], gamma=self.gamma,
+                                               nfail=self.interp_fail)
             out_f, flux_f = self._close(pg_f, dt_f, dh_f)
             hydro.correct_flux(flux_f, t["fine_correct_slot"], dt_f, self.reg)
             hydro.close_patches(out_f, self.fine, self.fine_pid0)
@@ -390,10 +392,10 @@ class TwoLevel:
         in place.  The warm-up step before the capture runs on the live state, which is saved and restored around
         it.  Every clear inside the step is a fill kernel (no memset nodes)."""
         torch = self.torch
-        saved = (self.base.clone(), self.fine.clone(), self.counters.clone())
+        saved = (self.base.clone(), self.fine.clone(), self.counters.clone(), self.interp_fail.clone())
         self.step(dt_base)
         torch.cuda.synchronize()
-        for dst, src in zip((self.base, self.fine, self.counters), saved):
+        for dst, src in zip((self.base, self.fine, self.counters, self.interp_fail), saved):
             dst.copy_(src)
         torch.cuda.synchronize()
         graph = torch.cuda.CUDAGraph()

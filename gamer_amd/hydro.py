@@ -417,18 +417,24 @@ def close_patches(pg_out, patch, pid0, stream=None):
 
 def interpolate(cdata, fdata, cstart, crange, fstart=(0, 0, 0), scheme="CQUAD", monotonic=0x1F, mono_coeff=2.0,
                 opp_sign_0th=False, int_prim=True, reduce_mono=True, mono_max_iter=10, gamma=5.0 / 3.0, min_pres=0.0,
-                need=None, stream=None):
+                need=None, stream=None, nfail=None):
     """Coarse -> fine interpolation of boxes cdata (nbox, 5, cz, cy, cx) into fdata (nbox, 5, fz, fy, fx), in place.
     Index tuples are (x, y, z).  ``need``: optional uint8 device mask (nbox, crz, cry, crx) of the coarse cells whose
     fine cells are used (the others are skipped and their fine cells left as they are).
-    Returns an int32 device tensor [number of boxes still unphysical]."""
+    Returns an int32 device tensor [number of boxes still unphysical].  With ``nfail`` (an int32 device tensor of
+    1 element) the kernel adds its count to it and returns it, with no extra launch."""
     import torch
     nbox = cdata.shape[0]
     if cdata.dim() != 5 or fdata.dim() != 5 or fdata.shape[0] != nbox or cdata.shape[1] != 5 or fdata.shape[1] != 5:
         raise HydroError("interpolate: cdata/fdata must be (nbox, 5, z, y, x)")
     if cdata.dtype != fdata.dtype or not cdata.is_contiguous() or not fdata.is_contiguous():
         raise HydroError("interpolate: contiguous tensors of one dtype required")
-    nf = torch.zeros(1, dtype=torch.int32, device=fdata.device)
+    if nfail is None:
+        nf = torch.zeros(1, dtype=torch.int32, device=fdata.device)
+    elif (nfail.dtype != torch.int32 or nfail.numel() != 1 or nfail.device != fdata.device):
+        raise HydroError("interpolate: nfail must be an int32 tensor of 1 element on fdata's device")
+    else:
+        nf = nfail
     I3 = ctypes.c_int * 3
     a = HydroInterpArgs(cdata.data_ptr(), fdata.data_ptr(), nbox, I3(*reversed(cdata.shape[2:])), I3(*cstart),
                         I3(*crange), I3(*reversed(fdata.shape[2:])), I3(*fstart), INT_SCHEME[scheme], int(monotonic),
@@ -485,13 +491,13 @@ def assemble_patch_groups(patch, fbox, pg_in, slot, ghost, fb_origin, stream=Non
 
 
 def prepare_refined_patch_groups(coarse_patch, fine_patch, coarse_slot, fine_slot, pg_in, ghost=3, scheme="CQUAD",
-                                 need=None, stream=None, **interp_kw):
+                                 need=None, stream=None, nfail=None, **interp_kw):
     """Prepare_PatchData for fine patch groups at a refinement boundary, on the device: the coarse box of each patch
     group (father patch + 26 siblings: coarse_slot (npg, 3, 3, 3)) -> Interpolate() over coarse cells -2..9 around the
     father (the 22^3 window of ghost 3 plus one fine cell) -> same-level fine patches (fine_slot (npg, 4, 4, 4)) and
     the interpolated cells into pg_in (npg, 5, 16+2g, ...).  ``need`` (npg, 12, 12, 12) uint8, optional: the coarse
     cells whose fine cells have no same-level patch (gamer_amd.amr.refined_interp_need); only those are
-    interpolated.  Returns the interpolation failure counter."""
+    interpolated.  Returns the interpolation failure counter (``nfail``, when given, is added to and returned)."""
     import torch
     if ghost != 3:
         raise HydroError("prepare_refined_patch_groups: ghost 3 (the MHM_RP window) only")
@@ -501,7 +507,8 @@ def prepare_refined_patch_groups(coarse_patch, fine_patch, coarse_slot, fine_slo
     cbox = torch.empty((npg, 5, n, n, n), dtype=pg_in.dtype, device=pg_in.device)
     gather_boxes(coarse_patch, cbox, coarse_slot, (6 - g,) * 3, stream=stream)
     fbox = torch.empty((npg, 5, 24, 24, 24), dtype=pg_in.dtype, device=pg_in.device)
-    nfail = interpolate(cbox, fbox, (g,) * 3, (12, 12, 12), scheme=scheme, need=need, stream=stream, **interp_kw)
+    nfail = interpolate(cbox, fbox, (g,) * 3, (12, 12, 12), scheme=scheme, need=need, stream=stream, nfail=nfail,
+                        **interp_kw)
     assemble_patch_groups(fine_patch, fbox, pg_in, fine_slot, ghost, (1, 1, 1), stream=stream)
     return nfail
 

@@ -50,6 +50,12 @@ def test_interpolate_matches_oracle(dtype, scheme, iterate):
     tol = 1e-14 if dtype == np.float64 else 1e-5
     np.testing.assert_allclose(gf.cpu().numpy(), rf, rtol=tol, atol=tol)
     np.testing.assert_allclose(gc.cpu().numpy(), rc, rtol=tol, atol=tol)      # primitive retry converts in place
+    # a caller-held running count (the AMR step's) is added to, with the same result
+    run = torch.full((1,), 3, dtype=torch.int32, device=DEV)
+    gf2 = torch.from_numpy(f0).to(DEV)
+    assert hydro.interpolate(torch.from_numpy(c).to(DEV), gf2, (g,) * 3, cr, nfail=run, **kw) is run
+    torch.cuda.synchronize()
+    assert int(run.item()) == 3 + rn and torch.equal(gf2, gf)
 
 
 def test_interpolate_fine_offset_without_iteration():
