@@ -340,6 +340,9 @@ def _field_scale(per_field, agg):
     return sc
 
 
+VALU_PEAK_GINST = 1024 * 2.4 / 4       # 256 CUs x 4 SIMDs, 2.4 GHz, one wave64 VALU instruction per 4 cycles
+
+
 def load_traffic(name):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -428,6 +431,13 @@ def main():
                          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(prec),
                          "algorithmic_bytes_per_launch": bpl},
         }
+        valu = load_traffic("valu_" + prec)
+        if valu:
+            # the limit the kernel actually runs into: VALU issue, one wave64 instruction per SIMD every 4 cycles
+            # at the nominal clock, with the profiled instruction count of one launch over this run's kernel time
+            rate = valu / (kms * 1e-3) / 1e9
+            results[prec]["valu_issue"] = {"achieved": round(rate, 1), "peak": VALU_PEAK_GINST, "frac": round(rate / VALU_PEAK_GINST, 4),
+                                           "unit": "G wave64 VALU instructions/s", "instructions_per_launch": valu}
         del out, flux
         torch.cuda.empty_cache()
         if not args.skip_host_arrays:
@@ -464,12 +474,16 @@ def main():
         "roofline": r32["roofline"],
         "kernel_cell_updates_per_s": r32["kernel_cell_updates_per_s"],
     }
+    if "valu_issue" in r32:
+        line["valu_issue"] = r32["valu_issue"]
     if "host_arrays" in r32:
         line["host_arrays_pcie_inclusive"] = r32["host_arrays"]
     if "fp64" in results:
         r64 = results["fp64"]
         line["fp64"] = {"value": r64["value"], "ms_per_step": r64["ms_per_step"],
                         "kernel_cell_updates_per_s": r64["kernel_cell_updates_per_s"], "roofline": r64["roofline"]}
+        if "valu_issue" in r64:
+            line["fp64"]["valu_issue"] = r64["valu_issue"]
         if "host_arrays" in r64:
             line["fp64"]["host_arrays_pcie_inclusive"] = r64["host_arrays"]
         line["fp32_over_fp64"] = r32["kernel_cell_updates_per_s"] / r64["kernel_cell_updates_per_s"]
