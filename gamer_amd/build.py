@@ -18,6 +18,10 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
 ]
+# per-source additions.  The MHM_RP kernels run two waves per SIMD with four barriers per z-plane; LLVM's
+# iterative ILP scheduler gives them a schedule that is 1.3% (FP32) and 1.9% (FP64) faster than the default one
+# (interleaved A/B on the 4096-PG workload; MHM measured mixed, and the adjoint does not compile with it).
+EXTRA = {"hydro_mhm_rp.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"]}
 
 
 def build(verbose: bool = False) -> str:
@@ -26,7 +30,7 @@ def build(verbose: bool = False) -> str:
     procs = []
     for src in SOURCES:
         obj = os.path.join(CSRC, src.replace(".hip", ".o"))
-        cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC, *FLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
         cmd.remove("-shared")
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
         objs.append(obj)
