@@ -89,6 +89,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise HydroError(f"{LIB_PATH} not built: run `python gamer_amd/build.py`")
+        # torch ships its own HIP runtime (soname libamdhip64.so.7, requested by torch as libamdhip64.so).  With
+        # torch loaded first, this library binds to that copy; loaded the other way round, the process would map
+        # two HIP runtimes and torch's device memory would be foreign to this library's launches.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         L.hydro_last_error.restype = ctypes.c_char_p
         L.hydro_init.argtypes = [ctypes.c_int]

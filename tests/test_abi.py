@@ -127,3 +127,19 @@ def test_header_is_c99_and_links(tmp_path):
                     str(src), "-o", str(exe), f"-L{libdir}", "-lgamer_amd_hydro", f"-Wl,-rpath,{libdir}"], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
     assert int(out) == len(names) > 20
+
+
+def test_one_hip_runtime_whatever_the_import_order():
+    """torch ships its own HIP runtime; ``hydro.lib()`` loads torch first so that the library binds to that copy.
+    In a fresh process that asks for the library before importing torch, exactly one libamdhip64 is mapped."""
+    import subprocess
+    import sys
+    pytest.importorskip("torch")
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from gamer_amd import hydro\n"
+            "hydro.lib()\n"
+            "import torch\n"
+            "maps = open('/proc/self/maps').read().splitlines()\n"
+            "print(len({l.split()[-1] for l in maps if 'libamdhip64' in l}))\n") % ROOT
+    out = subprocess.run([sys.executable, "-c", code], check=True, capture_output=True, text=True, timeout=600)
+    assert out.stdout.strip().splitlines()[-1] == "1", out.stdout + out.stderr
