@@ -5,7 +5,8 @@
 // 188-689, 2063-2160) -> Hydro_ComputeFlux -> Hydro_StoreIntFlux -> Hydro_FullStepUpdate.
 //
 // Same organisation as the MHM_RP kernel: one 256-lane workgroup per patch group marching along z; lane t owns
-// output column (2 + t%16, 2 + t/16) and, on waves 0-1, one column of the reconstruction ring (i or j in {1,18}).
+// output column (2 + t%16, 2 + row), wave w taking rows w, w+4, w+8, w+12, and, on waves 0-1, one column of the
+// reconstruction ring (i or j in {1,18}).
 // Every lane that reconstructs a cell keeps that column's primitive planes r-1, r, r+1 and the conserved cell
 // value of plane r in registers; x/y neighbours come from a double-buffered primitive plane in LDS, the conserved
 // input plane is staged in LDS too, and a lane-private LDS slot keeps the own output column's input until its update
@@ -350,7 +351,13 @@ int launch_mhm( const IO *in, IO *out, IO *flux, int npg, int rsolver, int store
    return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
 }
 
+// The build compiles this file twice, once per precision (MHM_PRECISION 64 or 32), so that each gets the
+// instruction scheduler that suits it; without the macro both precisions are instantiated.
+#if !defined( MHM_PRECISION ) || MHM_PRECISION == 64
 template int launch_mhm<double, double>( const double*, double*, double*, int, int, int, int, double, const Prm<double>&, double*, double, const Batch&, int, hipStream_t );
+#endif
+#if !defined( MHM_PRECISION ) || MHM_PRECISION == 32
 template int launch_mhm<float,  float >( const float*,  float*,  float*,  int, int, int, int, double, const Prm<float>&,  float*, double, const Batch&, int, hipStream_t );
+#endif
 
 } // namespace gamer_amd
