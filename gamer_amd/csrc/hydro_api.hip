@@ -37,6 +37,10 @@ struct FluidBuffers {
    size_t elem = 8;
    void  *d_in = nullptr, *d_out = nullptr, *d_flux = nullptr;
    void  *h_in[2] = { nullptr, nullptr }, *h_out[2] = { nullptr, nullptr }, *h_flux[2] = { nullptr, nullptr };
+   // CFL dt solver: 8 patches of 5 x 8^3 cells per patch group (d/h_Flu_Array_T, d/h_dt_Array_T,
+   // src/GPU_API/CUAPI_MemAllocate_Fluid.cu:238-239, 315-316)
+   void  *d_flu_t = nullptr, *d_dt = nullptr;
+   void  *h_flu_t[2] = { nullptr, nullptr }, *h_dt[2] = { nullptr, nullptr };
    std::vector<hipStream_t> streams;
 };
 FluidBuffers g_buf;
@@ -113,11 +117,15 @@ void free_buffers()
    if ( g_buf.d_in )   (void)hipFree( g_buf.d_in );
    if ( g_buf.d_out )  (void)hipFree( g_buf.d_out );
    if ( g_buf.d_flux ) (void)hipFree( g_buf.d_flux );
+   if ( g_buf.d_flu_t ) (void)hipFree( g_buf.d_flu_t );
+   if ( g_buf.d_dt )    (void)hipFree( g_buf.d_dt );
    for (int k=0; k<2; k++)
    {
-      if ( g_buf.h_in[k] )   (void)hipHostFree( g_buf.h_in[k] );
-      if ( g_buf.h_out[k] )  (void)hipHostFree( g_buf.h_out[k] );
-      if ( g_buf.h_flux[k] ) (void)hipHostFree( g_buf.h_flux[k] );
+      if ( g_buf.h_in[k] )    (void)hipHostFree( g_buf.h_in[k] );
+      if ( g_buf.h_out[k] )   (void)hipHostFree( g_buf.h_out[k] );
+      if ( g_buf.h_flux[k] )  (void)hipHostFree( g_buf.h_flux[k] );
+      if ( g_buf.h_flu_t[k] ) (void)hipHostFree( g_buf.h_flu_t[k] );
+      if ( g_buf.h_dt[k] )    (void)hipHostFree( g_buf.h_dt[k] );
    }
    for (hipStream_t s : g_buf.streams) (void)hipStreamDestroy( s );
    g_buf = FluidBuffers();
@@ -456,13 +464,19 @@ int hydro_mem_allocate_fluid( int flu_npg, int nstream, int scheme, int precisio
    const size_t in_b   = (size_t)flu_npg*5*nx*nx*nx*g_buf.elem;
    const size_t out_b  = (size_t)flu_npg*5*16*16*16*g_buf.elem;
    const size_t flux_b = (size_t)flu_npg*9*5*16*16*g_buf.elem;
+   const size_t flut_b = (size_t)flu_npg*8*5*512*g_buf.elem;
+   const size_t dt_b   = (size_t)flu_npg*8*g_buf.elem;
    bool ok = hipMalloc( &g_buf.d_in, in_b ) == hipSuccess &&
              hipMalloc( &g_buf.d_out, out_b ) == hipSuccess &&
-             hipMalloc( &g_buf.d_flux, flux_b ) == hipSuccess;
+             hipMalloc( &g_buf.d_flux, flux_b ) == hipSuccess &&
+             hipMalloc( &g_buf.d_flu_t, flut_b ) == hipSuccess &&
+             hipMalloc( &g_buf.d_dt, dt_b ) == hipSuccess;
    for (int k=0; k<2 && ok; k++)
       ok = hipHostMalloc( &g_buf.h_in[k], in_b, hipHostMallocDefault ) == hipSuccess &&
            hipHostMalloc( &g_buf.h_out[k], out_b, hipHostMallocDefault ) == hipSuccess &&
-           hipHostMalloc( &g_buf.h_flux[k], flux_b, hipHostMallocDefault ) == hipSuccess;
+           hipHostMalloc( &g_buf.h_flux[k], flux_b, hipHostMallocDefault ) == hipSuccess &&
+           hipHostMalloc( &g_buf.h_flu_t[k], flut_b, hipHostMallocDefault ) == hipSuccess &&
+           hipHostMalloc( &g_buf.h_dt[k], dt_b, hipHostMallocDefault ) == hipSuccess;
    if ( !ok )
    {
       (void)hipGetLastError();
@@ -498,6 +512,71 @@ int hydro_host_buffers( int array_id, void **h_in, void **h_out, void **h_flux )
    return HYDRO_OK;
 }
 
+int hydro_host_dt_buffers( int array_id, void **h_flu_t, void **h_dt )
+{
+   if ( g_buf.npg_max == 0 ) return fail( HYDRO_E_STATE, "buffers not allocated" );
+   if ( array_id < 0 || array_id > 1 ) return fail( HYDRO_E_ARG, "array_id must be 0 or 1" );
+   if ( h_flu_t ) *h_flu_t = g_buf.h_flu_t[array_id];
+   if ( h_dt )    *h_dt    = g_buf.h_dt[array_id];
+   return HYDRO_OK;
+}
+int hydro_asyn_dt_solver( const void *h_flu_t, void *h_dt, int npg, double dh, double safety, double gamma,
+                          double min_pres, int gpu_nstream )
+{
+   if ( g_buf.npg_max == 0 )            return fail( HYDRO_E_STATE, "hydro_mem_allocate_fluid was not called" );
+   if ( npg < 0 || npg > g_buf.npg_max ) return fail( HYDRO_E_ARG, "dt solver: npg %d outside 0..%d", npg, g_buf.npg_max );
+   if ( npg > 0 && ( h_flu_t == nullptr || h_dt == nullptr ) ) return fail( HYDRO_E_ARG, "dt solver: NULL host array" );
+   if ( !( dh > 0.0 ) || !( safety > 0.0 ) || !( gamma > 1.0 ) ) return fail( HYDRO_E_ARG, "dt solver: dh/safety/gamma invalid" );
+   if ( gpu_nstream <= 0 || gpu_nstream > g_buf.nstream ) return fail( HYDRO_E_ARG, "gpu_nstream out of range" );
+   const int npatch = 8*npg;
+   const size_t p_b = (size_t)5*512*g_buf.elem;
+   // NPatch/NStream per stream, remainder to the last stream (CUAPI_Asyn_dtSolver.cu:183-197)
+   const int base = npatch / gpu_nstream;
+   int off = 0;
+   for (int s=0; s<gpu_nstream; s++)
+   {
+      const int n = ( s == gpu_nstream - 1 ) ? npatch - off : base;
+      if ( n <= 0 ) continue;
+      hipStream_t st = g_buf.streams[s];
+      char *dflu = (char*)g_buf.d_flu_t + off*p_b, *ddt = (char*)g_buf.d_dt + off*g_buf.elem;
+      HIP_TRY( hipMemcpyAsync( dflu, (const char*)h_flu_t + off*p_b, n*p_b, hipMemcpyHostToDevice, st ) );
+      const int rc = ( g_buf.precision == HYDRO_FP64 )
+                     ? launch_dt_cfl<double>( (const double*)dflu, (double*)ddt, n, dh, safety, gamma, min_pres, st )
+                     : launch_dt_cfl<float> ( (const float*)dflu,  (float*)ddt,  n, dh, safety, gamma, min_pres, st );
+      if ( rc != 0 ) return fail( HYDRO_E_DEVICE, "dt solver launch failed" );
+      HIP_TRY( hipMemcpyAsync( (char*)h_dt + off*g_buf.elem, ddt, n*g_buf.elem, hipMemcpyDeviceToHost, st ) );
+      off += n;
+   }
+   return HYDRO_OK;
+}
+int hydro_device_info( HydroDeviceInfo *info )
+{
+   if ( info == nullptr ) return fail( HYDRO_E_ARG, "info == NULL" );
+   std::memset( info, 0, sizeof( *info ) );
+   int dev = 0, n = 0, drv = 0, rt = 0;
+   HIP_TRY( hipGetDeviceCount( &n ) );
+   HIP_TRY( hipGetDevice( &dev ) );
+   hipDeviceProp_t p;
+   HIP_TRY( hipGetDeviceProperties( &p, dev ) );
+   HIP_TRY( hipDriverGetVersion( &drv ) );
+   HIP_TRY( hipRuntimeGetVersion( &rt ) );
+   std::snprintf( info->name, sizeof( info->name ), "%s", p.name );
+   std::snprintf( info->arch, sizeof( info->arch ), "%s", p.gcnArchName );
+   info->device                = dev;
+   info->device_count          = n;
+   info->compute_units         = p.multiProcessorCount;
+   info->clock_khz             = p.clockRate;
+   info->global_mem_bytes      = p.totalGlobalMem;
+   info->lds_per_block_bytes   = p.sharedMemPerBlock;
+   info->wavefront_size        = p.warpSize;
+   info->max_threads_per_block = p.maxThreadsPerBlock;
+   info->driver_version        = drv;
+   info->runtime_version       = rt;
+   info->ecc                   = p.ECCEnabled;
+   info->async_engines         = p.asyncEngineCount;
+   info->concurrent_kernels    = p.concurrentKernels;
+   return HYDRO_OK;
+}
 int hydro_asyn_fluid_solver( const void *h_in, void *h_out, void *h_flux, const HydroSolveArgs *args, int gpu_nstream )
 {
    HydroSolveArgs a = *args;

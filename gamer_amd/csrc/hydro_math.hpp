@@ -514,10 +514,11 @@ __device__ __forceinline__ IO cell_cfl( const IO u[NV], double gamma, double gm1
 }
 
 // Fused CFL of one patch group: each lane holds the maxima of its output column in the two z-halves; the 8
-// patches (px + 2 py + 4 pz, x fastest) are reduced with wave64 butterflies and LDS across the 4 waves.
+// patches (px + 2 py + 4 pz, x fastest) are reduced with wave64 butterflies and LDS across the 4 waves, in the
+// I/O type (the reference computes dt in `real`, CPU_dtSolver_HydroCFL.cpp:62-209).
 // Called by every lane of the workgroup (waves 0-3 hold the output columns); writes dt[8] = dh_safety / max.
 template <typename IO>
-__device__ __forceinline__ void fused_dt( const IO cfl[2], int pxy, float (*red)[4], IO *dt, double dh_safety )
+__device__ __forceinline__ void fused_dt( const IO cfl[2], int pxy, IO (*red)[4], IO *dt, double dh_safety )
 {
    const int tid = threadIdx.x, w = tid >> 6;
    if ( w < 4 )                      // the output columns live on waves 0-3; further waves hold ghost columns
@@ -526,7 +527,7 @@ __device__ __forceinline__ void fused_dt( const IO cfl[2], int pxy, float (*red)
 #     pragma unroll
       for (int q=0; q<4; q++)
       {
-         float m = ( pxy == q ) ? (float)cfl[pz] : 0.0f;
+         IO m = ( pxy == q ) ? cfl[pz] : (IO)0;
 #        pragma unroll
          for (int o = 32; o > 0; o >>= 1) m = vmax( m, __shfl_xor( m, o, 64 ) );
          if ( ( tid & 63 ) == 0 ) red[q + 4*pz][w] = m;
@@ -534,7 +535,7 @@ __device__ __forceinline__ void fused_dt( const IO cfl[2], int pxy, float (*red)
    __syncthreads();
    if ( tid < 8 )
    {
-      const float m = vmax( vmax( red[tid][0], red[tid][1] ), vmax( red[tid][2], red[tid][3] ) );
+      const IO m = vmax( vmax( red[tid][0], red[tid][1] ), vmax( red[tid][2], red[tid][3] ) );
       dt[tid] = (IO)( dh_safety / (double)m );
    }
 }

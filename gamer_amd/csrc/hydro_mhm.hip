@@ -34,7 +34,7 @@ struct SmemM {
    R u[NV][NC];       // conserved input plane t
    R u2[NV][NT];      // lane-private: input cells of the own output column, plane t-2
    int fail, next;
-   float cfl[8][4];   // fused CFL: per-patch maxima of the 4 waves
+   R cfl[8][4];       // fused CFL: per-patch maxima of the 4 waves (R is the I/O type)
 };
 
 // reconstruction-ring column of lane t (i or j in {1,18}, the corners need nothing); false = none

@@ -208,7 +208,7 @@ struct SmemRP {
    R w[NV][NC];                 // half-step primitive plane read by the reconstruction
    R u3[2][NV][NT];             // lane-private ring: input cells of the own output column, planes t-3 and t-2
    int fail, next;
-   float cfl[8][4];   // fused CFL: per-patch maxima of the 4 waves
+   R cfl[8][4];       // fused CFL: per-patch maxima of the 4 waves (R is the I/O type)
 };
 
 template <typename R, typename IO, int RS, int LIM, bool PERSIST>

@@ -35,7 +35,8 @@ EXPORTS = ("hydro_init", "hydro_mhm_solve_async", "hydro_synchronize", "hydro_la
            "hydro_mhm_tangent_async", "hydro_prepare_uniform_adjoint_async", "hydro_correct_unphysical_async",
            "hydro_store_flux_async", "hydro_correct_flux_async", "hydro_fixup_flux_async",
            "hydro_fixup_restrict_async", "hydro_close_patches_async", "hydro_interpolate_async",
-           "hydro_prepare_patches_async", "hydro_gather_boxes_async", "hydro_assemble_patch_groups_async")
+           "hydro_prepare_patches_async", "hydro_gather_boxes_async", "hydro_assemble_patch_groups_async",
+           "hydro_host_dt_buffers", "hydro_asyn_dt_solver", "hydro_device_info")
 
 
 class HydroError(RuntimeError):
@@ -51,6 +52,17 @@ class HydroSolveArgs(ctypes.Structure):
         ("min_dens", ctypes.c_double), ("min_pres", ctypes.c_double), ("min_eint", ctypes.c_double),
         ("store_flux", ctypes.c_int), ("precision", ctypes.c_int),
         ("dt_patch", ctypes.c_void_p), ("safety", ctypes.c_double),
+    ]
+
+
+class HydroDeviceInfo(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 256), ("arch", ctypes.c_char * 64), ("device", ctypes.c_int),
+        ("device_count", ctypes.c_int), ("compute_units", ctypes.c_int), ("clock_khz", ctypes.c_int),
+        ("global_mem_bytes", ctypes.c_size_t), ("lds_per_block_bytes", ctypes.c_size_t),
+        ("wavefront_size", ctypes.c_int), ("max_threads_per_block", ctypes.c_int), ("driver_version", ctypes.c_int),
+        ("runtime_version", ctypes.c_int), ("ecc", ctypes.c_int), ("async_engines", ctypes.c_int),
+        ("concurrent_kernels", ctypes.c_int),
     ]
 
 
@@ -103,6 +115,10 @@ def lib():
         L.hydro_mem_allocate_fluid.argtypes = [ctypes.c_int] * 4
         L.hydro_host_buffers.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_void_p)] * 3
         L.hydro_asyn_fluid_solver.argtypes = [ctypes.c_void_p] * 3 + [ctypes.POINTER(HydroSolveArgs), ctypes.c_int]
+        L.hydro_host_dt_buffers.argtypes = [ctypes.c_int] + [ctypes.POINTER(ctypes.c_void_p)] * 2
+        L.hydro_asyn_dt_solver.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int] + [ctypes.c_double] * 4 + \
+            [ctypes.c_int]
+        L.hydro_device_info.argtypes = [ctypes.POINTER(HydroDeviceInfo)]
         L.hydro_dt_cfl_async.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                          ctypes.c_void_p]
@@ -151,10 +167,33 @@ def make_args(npg, dt, dh, gamma=5.0 / 3.0, scheme="MHM_RP", rsolver="HLLC", lim
                           PRECISION[precision], dt_patch, float(safety))
 
 
-def _stream(dev, stream):
+def _stream(dev, stream, *tensors):
+    """The launch stream: ``stream`` or torch's current stream.  When the launch goes to another stream than the
+    current one, every tensor it touches is recorded on that stream, so the caching allocator does not hand their
+    memory out again (e.g. a temporary from ``.contiguous()``) while the launch may still use it."""
     import torch
-    s = stream if stream is not None else torch.cuda.current_stream(dev)
+    cur = torch.cuda.current_stream(dev)
+    s = stream if stream is not None else cur
+    if s != cur:
+        for t in tensors:
+            if t is not None:
+                t.record_stream(s)
     return ctypes.c_void_p(s.cuda_stream)
+
+
+def _check_out(t, shape, dtype, device, name):
+    """A caller-supplied output: a contiguous device tensor of exactly this shape and dtype on the input's device
+    (the kernels write through its raw pointer, sized from the shapes they assume)."""
+    if t is None:
+        return
+    if not getattr(t, "is_cuda", False) or t.device != device:
+        raise HydroError(f"{name} must be a device tensor on {device}")
+    if t.dtype != dtype:
+        raise HydroError(f"{name} must have dtype {dtype}, got {t.dtype}")
+    if tuple(t.shape) != tuple(shape):
+        raise HydroError(f"{name} must have shape {tuple(shape)}, got {tuple(t.shape)}")
+    if not t.is_contiguous():
+        raise HydroError(f"{name} must be contiguous")
 
 
 def _prec(t):
@@ -186,18 +225,20 @@ def fluid_solver(u, dt, dh, gamma=5.0 / 3.0, scheme="MHM_RP", rsolver="HLLC", li
         raise HydroError(f"input shape {tuple(u.shape)} != (npg, 5, {n}, {n}, {n})")
     u = u.contiguous()
     npg = u.shape[0]
+    _check_out(out, (npg, 5, 16, 16, 16), u.dtype, u.device, "out")
+    _check_out(flux if store_flux else None, (npg, 9, 5, 16, 16), u.dtype, u.device, "flux")
+    _check_out(dt_patch, (npg, 8), u.dtype, u.device, "dt_patch")
     if out is None:
         out = torch.empty((npg, 5, 16, 16, 16), dtype=u.dtype, device=u.device)
     if store_flux and flux is None:
         flux = torch.empty((npg, 9, 5, 16, 16), dtype=u.dtype, device=u.device)
     prec = "fp64" if u.dtype == torch.float64 else "fp32"
-    if dt_patch is not None and (tuple(dt_patch.shape) != (npg, 8) or dt_patch.dtype != u.dtype):
-        raise HydroError("dt_patch must be a (npg, 8) tensor of the input dtype")
     args = make_args(npg, dt, dh, gamma, scheme, rsolver, limiter, minmod_coeff, minmod_maxiter, min_dens, min_pres,
                      min_eint, store_flux, prec, u.data_ptr(), out.data_ptr(),
                      flux.data_ptr() if store_flux else None,
                      dt_patch.data_ptr() if dt_patch is not None else None, safety)
-    _check(lib().hydro_mhm_solve_async(ctypes.byref(args), _stream(u.device, stream)))
+    _check(lib().hydro_mhm_solve_async(ctypes.byref(args),
+                                       _stream(u.device, stream, u, out, flux if store_flux else None, dt_patch)))
     return out, (flux if store_flux else None)
 
 
@@ -225,11 +266,14 @@ def solve_batch(u, pg_dt_dh=None, out=None, flux=None, queue_counter=None, group
     if queue_counter is not None and (queue_counter.dtype != torch.int32 or queue_counter.numel() < 1
                                       or not queue_counter.is_cuda):
         raise HydroError("queue_counter must be an int32 device tensor with at least one element")
-    if dt_patch is not None and (tuple(dt_patch.shape) != (npg, 8) or dt_patch.dtype != u.dtype):
-        raise HydroError("dt_patch must be a (npg, 8) tensor of the input dtype")
+    store_flux = kw.pop("store_flux", True)
+    _check_out(out, (npg, 5, 16, 16, 16), u.dtype, u.device, "out")
+    _check_out(flux if store_flux else None, (npg, 9, 5, 16, 16), u.dtype, u.device, "flux")
+    _check_out(dt_patch, (npg, 8), u.dtype, u.device, "dt_patch")
+    if pg_dt_dh is not None and pg_dt_dh.device != u.device:
+        raise HydroError("pg_dt_dh must be on the input's device")
     if out is None:
         out = torch.empty((npg, 5, 16, 16, 16), dtype=u.dtype, device=u.device)
-    store_flux = kw.pop("store_flux", True)
     if store_flux and flux is None:
         flux = torch.empty((npg, 9, 5, 16, 16), dtype=u.dtype, device=u.device)
     args = make_args(npg, dt, dh, scheme=scheme, store_flux=store_flux,
@@ -239,20 +283,26 @@ def solve_batch(u, pg_dt_dh=None, out=None, flux=None, queue_counter=None, group
     _check(lib().hydro_mhm_solve_batch_async(ctypes.byref(args),
                                              pg_dt_dh.data_ptr() if pg_dt_dh is not None else None,
                                              queue_counter.data_ptr() if queue_counter is not None else None,
-                                             int(groups_per_cu), int(pgs_per_ticket), _stream(u.device, stream)))
+                                             int(groups_per_cu), int(pgs_per_ticket),
+                                             _stream(u.device, stream, u, out, flux if store_flux else None,
+                                                     dt_patch, pg_dt_dh, queue_counter)))
     return out, (flux if store_flux else None)
 
 
 def dt_cfl(u_patch, dh, safety, gamma=5.0 / 3.0, min_pres=0.0, out=None, stream=None):
     """Per-patch CFL dt of (npatch, 5, 8, 8, 8) device data (CUAPI_Asyn_dtSolver for the hydro CFL)."""
     import torch
+    if not u_patch.is_cuda:
+        raise HydroError("dt_cfl needs a device tensor")
     u_patch = u_patch.contiguous()
     if u_patch.dim() != 5 or tuple(u_patch.shape[1:]) != (5, 8, 8, 8):
         raise HydroError(f"patch shape {tuple(u_patch.shape)} != (npatch, 5, 8, 8, 8)")
+    prec = _prec(u_patch)
+    _check_out(out, (u_patch.shape[0],), u_patch.dtype, u_patch.device, "out")
     if out is None:
         out = torch.empty(u_patch.shape[0], dtype=u_patch.dtype, device=u_patch.device)
     _check(lib().hydro_dt_cfl_async(u_patch.data_ptr(), out.data_ptr(), u_patch.shape[0], dh, safety, gamma, min_pres,
-                                    _prec(u_patch), _stream(u_patch.device, stream)))
+                                    prec, _stream(u_patch.device, stream, u_patch, out)))
     return out
 
 
@@ -263,18 +313,20 @@ def prepare_uniform(level, ghost, out=None, stream=None):
     _, nz, ny, nx = level.shape
     n = 16 + 2 * ghost
     npg = (nx // 16) * (ny // 16) * (nz // 16)
+    _check_out(out, (npg, 5, n, n, n), level.dtype, level.device, "out")
     if out is None:
         out = torch.empty((npg, 5, n, n, n), dtype=level.dtype, device=level.device)
     _check(lib().hydro_prepare_uniform_async(level.data_ptr(), out.data_ptr(), nx, ny, nz, ghost, _prec(level),
-                                             _stream(level.device, stream)))
+                                             _stream(level.device, stream, level, out)))
     return out
 
 
 def close_uniform(pg_out, level, stream=None):
     """Scatter (npg, 5, 16, 16, 16) patch-group outputs back into the device level (5, Z, Y, X) in place."""
     _, nz, ny, nx = level.shape
-    _check(lib().hydro_close_uniform_async(pg_out.contiguous().data_ptr(), level.data_ptr(), nx, ny, nz,
-                                           _prec(level), _stream(level.device, stream)))
+    pg_out = pg_out.contiguous()
+    _check(lib().hydro_close_uniform_async(pg_out.data_ptr(), level.data_ptr(), nx, ny, nz,
+                                           _prec(level), _stream(level.device, stream, pg_out, level)))
     return level
 
 
@@ -307,7 +359,7 @@ def mhm_adjoint(u, lam_out, dt, dh, workspace=None, stream=None, **kw):
     lam_in = torch.empty_like(u)
     args = make_args(npg, dt, dh, store_flux=False, precision="fp64", in_=u.data_ptr(), **kw)
     _check(lib().hydro_mhm_adjoint_async(ctypes.byref(args), lam_out.data_ptr(), lam_in.data_ptr(), ws.data_ptr(),
-                                         _stream(u.device, stream)))
+                                         _stream(u.device, stream, u, lam_out, lam_in, ws)))
     return lam_in
 
 
@@ -329,7 +381,7 @@ def mhm_tangent(u, v, dt, dh, workspace=None, stream=None, **kw):
     args = make_args(npg, dt, dh, store_flux=False, precision="fp64", in_=u.data_ptr(), out=out.data_ptr(), **kw)
     _check(lib().hydro_mhm_tangent_async(ctypes.byref(args), v.data_ptr() if v is not None else None,
                                          jv.data_ptr() if jv is not None else None, ws.data_ptr(),
-                                         _stream(u.device, stream)))
+                                         _stream(u.device, stream, u, v, out, jv, ws)))
     return out, jv
 
 
@@ -340,7 +392,7 @@ def prepare_uniform_adjoint(pg_lam, level_shape, ghost, stream=None):
     _, nz, ny, nx = level_shape
     lam = torch.empty(tuple(level_shape), dtype=pg_lam.dtype, device=pg_lam.device)
     _check(lib().hydro_prepare_uniform_adjoint_async(pg_lam.data_ptr(), lam.data_ptr(), nx, ny, nz, ghost,
-                                                     _prec(pg_lam), _stream(pg_lam.device, stream)))
+                                                     _prec(pg_lam), _stream(pg_lam.device, stream, pg_lam, lam)))
     return lam
 
 
@@ -361,8 +413,12 @@ def correct_unphysical(u_in, out, flux, dt, dh, mode="3D1D", rsolver="HLLE", sch
     n = NX[scheme]
     if tuple(u_in.shape[1:]) != (5, n, n, n) or tuple(out.shape) != (u_in.shape[0], 5, 16, 16, 16):
         raise HydroError("correct_unphysical: shapes must be (npg, 5, N^3) and (npg, 5, 16^3)")
-    if u_in.dtype != out.dtype or (flux is not None and flux.dtype != out.dtype):
+    if not u_in.is_cuda or u_in.device != out.device:
+        raise HydroError("correct_unphysical: u_in and out must be device tensors on one device")
+    if u_in.dtype != out.dtype:
         raise HydroError("correct_unphysical: dtypes differ")
+    _check_out(out, (u_in.shape[0], 5, 16, 16, 16), u_in.dtype, u_in.device, "out")
+    _check_out(flux, (u_in.shape[0], 9, 5, 16, 16), u_in.dtype, u_in.device, "flux")
     if counters is None:
         cnt = torch.zeros(2, dtype=torch.int32, device=out.device)
     elif (counters.dtype != torch.int32 or counters.numel() != 2 or not counters.is_contiguous()
@@ -370,28 +426,31 @@ def correct_unphysical(u_in, out, flux, dt, dh, mode="3D1D", rsolver="HLLE", sch
         raise HydroError("correct_unphysical: counters must be a contiguous int32 tensor of 2 elements on out's device")
     else:
         cnt = counters
-    a = HydroCorrectArgs(u_in.contiguous().data_ptr(), out.data_ptr(), flux.data_ptr() if flux is not None else None,
+    u_in = u_in.contiguous()
+    a = HydroCorrectArgs(u_in.data_ptr(), out.data_ptr(), flux.data_ptr() if flux is not None else None,
                          u_in.shape[0], float(dt), float(dh), float(gamma), SCHEME[scheme], FIRST_FLUX_CORR[mode],
                          RSOLVER[rsolver], int(last_resort_floor), int(auto_reduce_dt), int(fixup_flux),
                          int(check_pres), float(min_dens), float(min_pres), float(min_eint), _prec(out),
                          cnt.data_ptr())
-    _check(lib().hydro_correct_unphysical_async(ctypes.byref(a), _stream(out.device, stream)))
+    _check(lib().hydro_correct_unphysical_async(ctypes.byref(a), _stream(out.device, stream, u_in, out, flux, cnt)))
     return cnt
 
 
 def store_flux(flux, slot, dt, reg, stream=None):
     """StoreFlux: reg[slot[pg, LocalID, s]] = dt * patch-face fluxes (reg: float64 (nreg, 5, 8, 8))."""
     _int_table(slot, (flux.shape[0], 8, 6), "slot")
-    _check(lib().hydro_store_flux_async(flux.contiguous().data_ptr(), slot.data_ptr(), flux.shape[0], float(dt),
-                                        reg.data_ptr(), _prec(flux), _stream(flux.device, stream)))
+    flux = flux.contiguous()
+    _check(lib().hydro_store_flux_async(flux.data_ptr(), slot.data_ptr(), flux.shape[0], float(dt),
+                                        reg.data_ptr(), _prec(flux), _stream(flux.device, stream, flux, slot, reg)))
     return reg
 
 
 def correct_flux(flux, slot, dt, reg, stream=None):
     """CorrectFlux: reg[slot[pg, s]] -= dt/4 * the fine fluxes on patch-group face s."""
     _int_table(slot, (flux.shape[0], 6), "slot")
-    _check(lib().hydro_correct_flux_async(flux.contiguous().data_ptr(), slot.data_ptr(), flux.shape[0], float(dt),
-                                          reg.data_ptr(), _prec(flux), _stream(flux.device, stream)))
+    flux = flux.contiguous()
+    _check(lib().hydro_correct_flux_async(flux.data_ptr(), slot.data_ptr(), flux.shape[0], float(dt),
+                                          reg.data_ptr(), _prec(flux), _stream(flux.device, stream, flux, slot, reg)))
     return reg
 
 
@@ -400,7 +459,7 @@ def fixup_flux(patch, slot, reg, dh, gamma=5.0 / 3.0, min_dens=0.0, min_pres=0.0
     _int_table(slot, (patch.shape[0], 6), "slot")
     _check(lib().hydro_fixup_flux_async(patch.data_ptr(), slot.data_ptr(), reg.data_ptr(), patch.shape[0],
                                         float(dh), float(gamma), float(min_dens), float(min_pres), float(min_eint),
-                                        _prec(patch), _stream(patch.device, stream)))
+                                        _prec(patch), _stream(patch.device, stream, patch, slot, reg)))
     return patch
 
 
@@ -408,17 +467,19 @@ def fixup_restrict(son, fa, son0, fa_idx, min_eint=0.0, stream=None):
     """Flu_FixUp_Restrict: fa[fa_idx[q]] = mean of son[son0[q] + LocalID], then the Eint floor (in place)."""
     _int_table(son0, (son0.shape[0],), "son0")
     _int_table(fa_idx, (son0.shape[0],), "fa_idx")
-    _check(lib().hydro_fixup_restrict_async(son.contiguous().data_ptr(), fa.data_ptr(), son0.shape[0],
+    son = son.contiguous()
+    _check(lib().hydro_fixup_restrict_async(son.data_ptr(), fa.data_ptr(), son0.shape[0],
                                             son0.data_ptr(), fa_idx.data_ptr(), float(min_eint), _prec(fa),
-                                            _stream(fa.device, stream)))
+                                            _stream(fa.device, stream, son, fa, son0, fa_idx)))
     return fa
 
 
 def close_patches(pg_out, patch, pid0, stream=None):
     """Copy (npg, 5, 16^3) patch-group outputs into patches pid0[pg] + LocalID of (npatch, 5, 8, 8, 8)."""
     _int_table(pid0, (pg_out.shape[0],), "pid0")
-    _check(lib().hydro_close_patches_async(pg_out.contiguous().data_ptr(), patch.data_ptr(), pg_out.shape[0],
-                                           pid0.data_ptr(), _prec(patch), _stream(patch.device, stream)))
+    pg_out = pg_out.contiguous()
+    _check(lib().hydro_close_patches_async(pg_out.data_ptr(), patch.data_ptr(), pg_out.shape[0],
+                                           pid0.data_ptr(), _prec(patch), _stream(patch.device, stream, pg_out, patch, pid0)))
     return patch
 
 
@@ -452,16 +513,17 @@ def interpolate(cdata, fdata, cstart, crange, fstart=(0, 0, 0), scheme="CQUAD", 
                 or tuple(need.shape) != (nbox,) + tuple(reversed(crange))):
             raise HydroError("interpolate: need must be a contiguous uint8 device tensor (nbox, crz, cry, crx)")
         a.need = need.data_ptr()
-    _check(lib().hydro_interpolate_async(ctypes.byref(a), _stream(fdata.device, stream)))
+    _check(lib().hydro_interpolate_async(ctypes.byref(a), _stream(fdata.device, stream, cdata, fdata, nf, need)))
     return nf
 
 
 def prepare_patches(patch, pg_in, slot, ghost, stream=None):
     """Same-level patch-group inputs from patches (npatch, 5, 8, 8, 8); slot (npg, 4, 4, 4) int32, -1 = keep."""
     _int_table(slot, (pg_in.shape[0], 4, 4, 4), "slot")
-    _check(lib().hydro_prepare_patches_async(patch.contiguous().data_ptr(), pg_in.data_ptr(), pg_in.shape[0],
+    patch = patch.contiguous()
+    _check(lib().hydro_prepare_patches_async(patch.data_ptr(), pg_in.data_ptr(), pg_in.shape[0],
                                              slot.data_ptr(), int(ghost), _prec(pg_in),
-                                             _stream(pg_in.device, stream)))
+                                             _stream(pg_in.device, stream, patch, pg_in, slot)))
     return pg_in
 
 
@@ -474,10 +536,11 @@ def gather_boxes(patch, box, slot, origin, stream=None):
     if slot.dim() != 4:
         raise HydroError("gather_boxes: slot must be (nbox, tz, ty, tx)")
     I3 = ctypes.c_int * 3
-    _check(lib().hydro_gather_boxes_async(patch.contiguous().data_ptr(), box.data_ptr(), nbox,
+    patch = patch.contiguous()
+    _check(lib().hydro_gather_boxes_async(patch.data_ptr(), box.data_ptr(), nbox,
                                           ctypes.byref(I3(*reversed(box.shape[2:]))), ctypes.byref(I3(*origin)),
                                           slot.data_ptr(), ctypes.byref(I3(*tdim)), _prec(box),
-                                          _stream(box.device, stream)))
+                                          _stream(box.device, stream, patch, box, slot)))
     return box
 
 
@@ -489,11 +552,12 @@ def assemble_patch_groups(patch, fbox, pg_in, slot, ghost, fb_origin, stream=Non
     if fbox.shape[0] != npg or fbox.dtype != pg_in.dtype or not fbox.is_contiguous():
         raise HydroError("assemble_patch_groups: fbox must be a contiguous (npg, 5, fz, fy, fx) tensor of pg_in's dtype")
     I3 = ctypes.c_int * 3
-    _check(lib().hydro_assemble_patch_groups_async(patch.contiguous().data_ptr(), fbox.data_ptr(), pg_in.data_ptr(),
+    patch = patch.contiguous()
+    _check(lib().hydro_assemble_patch_groups_async(patch.data_ptr(), fbox.data_ptr(), pg_in.data_ptr(),
                                                    npg, slot.data_ptr(), int(ghost),
                                                    ctypes.byref(I3(*reversed(fbox.shape[2:]))),
                                                    ctypes.byref(I3(*fb_origin)), _prec(pg_in),
-                                                   _stream(pg_in.device, stream)))
+                                                   _stream(pg_in.device, stream, patch, fbox, pg_in, slot)))
     return pg_in
 
 
@@ -547,6 +611,15 @@ class UniformLevel:
         return self.dt_patch.min()
 
 
+def device_info():
+    """Properties of the current device (HydroDeviceInfo as a dict)."""
+    info = HydroDeviceInfo()
+    _check(lib().hydro_device_info(ctypes.byref(info)))
+    d = {k: getattr(info, k) for k, _ in HydroDeviceInfo._fields_}
+    d["name"], d["arch"] = info.name.decode(), info.arch.decode()
+    return d
+
+
 class GpuFluidApi:
     """The GPU_API host-array path: pinned double buffers [2] + GPU_NStream streams (CUAPI_MemAllocate_Fluid)."""
 
@@ -580,6 +653,19 @@ class GpuFluidApi:
         args = make_args(npg, dt, dh, scheme=self.scheme, precision=self.precision, **kw)
         _check(lib().hydro_asyn_fluid_solver(v[3], v[4], v[5], ctypes.byref(args),
                                              gpu_nstream if gpu_nstream is not None else self.nstream))
+
+    def host_dt_arrays(self, array_id: int):
+        """(h_Flu_Array_T (8 npg, 5, 8, 8, 8), h_dt_Array_T (8 npg,)) pinned numpy views of double buffer ``array_id``."""
+        pf, pd = ctypes.c_void_p(), ctypes.c_void_p()
+        _check(lib().hydro_host_dt_buffers(array_id, ctypes.byref(pf), ctypes.byref(pd)))
+        return self._np(pf, (8 * self.flu_npg, 5, 8, 8, 8)), self._np(pd, (8 * self.flu_npg,))
+
+    def asyn_dt_solver(self, array_id: int, npg: int, dh, safety, gamma=5.0 / 3.0, min_pres=0.0, gpu_nstream=None):
+        """CUAPI_Asyn_dtSolver (DT_FLU_SOLVER) on double buffer ``array_id``; returns before the GPU finishes."""
+        pf, pd = ctypes.c_void_p(), ctypes.c_void_p()
+        _check(lib().hydro_host_dt_buffers(array_id, ctypes.byref(pf), ctypes.byref(pd)))
+        _check(lib().hydro_asyn_dt_solver(pf, pd, int(npg), float(dh), float(safety), float(gamma), float(min_pres),
+                                          gpu_nstream if gpu_nstream is not None else self.nstream))
 
     @staticmethod
     def synchronize():

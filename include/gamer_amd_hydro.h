@@ -233,7 +233,8 @@ const char *hydro_last_error( void );
 /* ---- GAMER GPU_API-shaped entry points over host arrays (pinned double buffers + N streams) ---- */
 
 /* Allocate device in/out/flux buffers for flu_npg patch groups and nstream HIP streams, and pinned host
-   double buffers [2] (replaces CUAPI_MemAllocate_Fluid, src/GPU_API/CUAPI_MemAllocate_Fluid.cu:78-347).
+   double buffers [2]; also the CFL dt solver's buffers for the 8*flu_npg patches of those patch groups
+   (replaces CUAPI_MemAllocate_Fluid, src/GPU_API/CUAPI_MemAllocate_Fluid.cu:78-347).
    Returns HYDRO_E_NOMEM on allocation failure so the caller can retry with fewer streams
    (src/GPU_API/CUAPI_MemAllocate.cu:41-96). */
 int hydro_mem_allocate_fluid( int flu_npg, int nstream, int scheme, int precision );
@@ -244,6 +245,36 @@ int hydro_mem_free_fluid( void );
 /* Pinned host buffers [array_id] for the input, output and flux arrays (the h_Flu_Array_In/Out and
    h_Flux_Array double buffers the caller fills and reads between calls). */
 int hydro_host_buffers( int array_id, void **h_in, void **h_out, void **h_flux );
+
+/* Pinned host buffers [array_id] of the CFL dt solver: h_flu_t [8*flu_npg][5][8^3] (the fluid data of the
+   patches) and h_dt [8*flu_npg] (h_Flu_Array_T / h_dt_Array_T, src/GPU_API/CUAPI_MemAllocate_Fluid.cu:315-316).
+   Element type as the allocation's precision. */
+int hydro_host_dt_buffers( int array_id, void **h_flu_t, void **h_dt );
+
+/* Asynchronous per-patch hydro CFL dt over host arrays on the allocated streams (replaces CUAPI_Asyn_dtSolver for
+   DT_FLU_SOLVER, src/GPU_API/CUAPI_Asyn_dtSolver.cu:88-330): the 8*npg patches are split NPatch/NStream per
+   stream with the remainder on the last; per stream H2D copy of h_flu_t, the dt kernel (hydro_dt_cfl_async), D2H
+   copy into h_dt.  dt = safety*dh / max_cells max_d(|v_d| + c_s).  Returns before the GPU finishes; the results are
+   valid after hydro_synchronize. */
+int hydro_asyn_dt_solver( const void *h_flu_t, void *h_dt, int npg, double dh, double safety, double gamma,
+                          double min_pres, int gpu_nstream );
+
+/* Properties of the current device (the queries of CUAPI_DiagnoseDevice, src/GPU_API/CUAPI_DiagnoseDevice.cu:14-140,
+   and of CUAPI_SetMemSize, src/GPU_API/CUAPI_SetMemSize.cu:19-40). */
+typedef struct HydroDeviceInfo {
+   char   name[256];
+   char   arch[64];                /* e.g. "gfx950:sramecc+:xnack-" */
+   int    device, device_count;
+   int    compute_units;           /* the multiprocessor count GAMER sizes FLU_GPU_NPGROUP by */
+   int    clock_khz;
+   size_t global_mem_bytes;
+   size_t lds_per_block_bytes;
+   int    wavefront_size;
+   int    max_threads_per_block;
+   int    driver_version, runtime_version;
+   int    ecc, async_engines, concurrent_kernels;
+} HydroDeviceInfo;
+int hydro_device_info( HydroDeviceInfo *info );
 
 /* Asynchronous solve over host arrays: per stream H2D copy, kernel, D2H copy of out (and flux), the
    NPG/NStream split with the remainder on the last stream (src/GPU_API/CUAPI_Asyn_FluidSolver.cu:433-722).

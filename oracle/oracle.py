@@ -76,6 +76,8 @@ def ref_lib(scheme: str, rsolver: str, prec: str):
         L = ctypes.CDLL(ref_path(scheme, rsolver, prec))
         L.gamer_ref_fluid_solver.argtypes = _REF_ARGS
         L.gamer_ref_fluid_solver.restype = _I
+        L.gamer_ref_dt_cfl.argtypes = [_P, _P, _I, _D, _D, _D, _D, _I]
+        L.gamer_ref_dt_cfl.restype = _I
         _refs[key] = L
     return _refs[key]
 
@@ -139,6 +141,18 @@ def limit_slope(L, C, R, limiter, coeff=1.5):
     if rc != 0:
         raise ValueError(rc)
     return s
+
+
+def ref_dt_cfl(u_patch, dh, safety, gamma=5.0 / 3.0, min_pres=0.0, nthreads=0):
+    """GAMER's own CPU_dtSolver_HydroCFL (compiled reference, FP64 build) on (npatch, 5, 8, 8, 8) float64,
+    npatch a multiple of 8 (the reference loops over the 8 patches of each patch group)."""
+    u = np.ascontiguousarray(u_patch, dtype=np.float64)
+    dt = np.empty(u.shape[0])
+    rc = ref_lib("MHM_RP", "HLLC", "f64").gamer_ref_dt_cfl(_ptr(u), _ptr(dt), u.shape[0], dh, safety, gamma,
+                                                             min_pres, nthreads)
+    if rc != 0:
+        raise ValueError(f"gamer_ref_dt_cfl returned {rc}")
+    return dt
 
 
 def dt_cfl(u_patch, dh, safety, gamma=5.0 / 3.0, min_pres=0.0, nthreads=0):

@@ -10,7 +10,9 @@
 //   (3) allocates the per-PG host scratch arrays as Init_MemAllocate_Fluid does
 //       (src/Init/Init_MemAllocate_Fluid.cpp:91-95), and
 //   (4) calls CPU_FluidSolver_MHM with the same arguments CPU_FluidSolver passes
-//       (src/Fluid/CPU_FluidSolver.cpp:258-266), non-MHD / no gravity / no passive scalars.
+//       (src/Fluid/CPU_FluidSolver.cpp:258-266), non-MHD / no gravity / no passive scalars, and
+//   (5) calls CPU_dtSolver_HydroCFL as CPU_dtSolver does for DT_FLU_SOLVER (src/Model_Hydro/CPU_Hydro/
+//       CPU_dtSolver_HydroCFL.cpp:57-209, src/SelfGravity/CPU_Poisson/CPU_dtSolver.cpp).
 
 #include "GAMER.h"
 #include "CUFLU.h"
@@ -72,6 +74,33 @@ void CPU_FluidSolver_MHM(
    const bool JeansMinPres, const real JeansMinPres_Coeff,
    const EoS_t EoS, const MicroPhy_t MicroPhy );
 
+void CPU_dtSolver_HydroCFL( real g_dt_Array[], const real g_Flu_Array[][FLU_NIN_T][ CUBE(PS1) ],
+                            const real g_Mag_Array[][NCOMP_MAG][ PS1P1*SQR(PS1) ], const int NPG,
+                            const real dh, const real Safety, const real MinPres,
+                            const long PassiveFloor, const EoS_t EoS, const MicroPhy_t MicroPhy );
+
+static EoS_t make_eos( double gamma )
+{
+   GAMMA = gamma;
+   EoS_Init_Gamma();
+   EoS_t EoS_obj;
+   memset( &EoS_obj, 0, sizeof(EoS_obj) );
+   EoS_obj.DensEint2Pres_FuncPtr = EoS_DensEint2Pres_CPUPtr;
+   EoS_obj.DensPres2Eint_FuncPtr = EoS_DensPres2Eint_CPUPtr;
+   EoS_obj.DensPres2CSqr_FuncPtr = EoS_DensPres2CSqr_CPUPtr;
+   EoS_obj.DensEint2Temp_FuncPtr = EoS_DensEint2Temp_CPUPtr;
+   EoS_obj.DensTemp2Pres_FuncPtr = EoS_DensTemp2Pres_CPUPtr;
+   EoS_obj.DensEint2Entr_FuncPtr = EoS_DensEint2Entr_CPUPtr;
+   EoS_obj.GuessHTilde_FuncPtr   = EoS_GuessHTilde_CPUPtr;
+   EoS_obj.HTilde2Temp_FuncPtr   = EoS_HTilde2Temp_CPUPtr;
+   EoS_obj.Temp2HTilde_FuncPtr   = EoS_Temp2HTilde_CPUPtr;
+   EoS_obj.General_FuncPtr       = EoS_General_CPUPtr;
+   EoS_obj.AuxArrayDevPtr_Flt    = EoS_AuxArray_Flt;
+   EoS_obj.AuxArrayDevPtr_Int    = EoS_AuxArray_Int;
+   EoS_obj.Table                 = h_EoS_Table;
+   return EoS_obj;
+}
+
 extern "C" {
 
 // Build-time identity of this reference object: sizeof(real), FLU_NXT, scheme and Riemann solver ids.
@@ -94,24 +123,7 @@ int gamer_ref_fluid_solver( const void *in, void *out, void *flux, int npg,
 {
    if ( npg <= 0 || in == NULL || out == NULL || ( store_flux && flux == NULL ) )   return 1;
 
-   GAMMA = gamma;
-   EoS_Init_Gamma();
-
-   EoS_t EoS_obj;
-   memset( &EoS_obj, 0, sizeof(EoS_obj) );
-   EoS_obj.DensEint2Pres_FuncPtr = EoS_DensEint2Pres_CPUPtr;
-   EoS_obj.DensPres2Eint_FuncPtr = EoS_DensPres2Eint_CPUPtr;
-   EoS_obj.DensPres2CSqr_FuncPtr = EoS_DensPres2CSqr_CPUPtr;
-   EoS_obj.DensEint2Temp_FuncPtr = EoS_DensEint2Temp_CPUPtr;
-   EoS_obj.DensTemp2Pres_FuncPtr = EoS_DensTemp2Pres_CPUPtr;
-   EoS_obj.DensEint2Entr_FuncPtr = EoS_DensEint2Entr_CPUPtr;
-   EoS_obj.GuessHTilde_FuncPtr   = EoS_GuessHTilde_CPUPtr;
-   EoS_obj.HTilde2Temp_FuncPtr   = EoS_HTilde2Temp_CPUPtr;
-   EoS_obj.Temp2HTilde_FuncPtr   = EoS_Temp2HTilde_CPUPtr;
-   EoS_obj.General_FuncPtr       = EoS_General_CPUPtr;
-   EoS_obj.AuxArrayDevPtr_Flt    = EoS_AuxArray_Flt;
-   EoS_obj.AuxArrayDevPtr_Int    = EoS_AuxArray_Int;
-   EoS_obj.Table                 = h_EoS_Table;
+   const EoS_t EoS_obj = make_eos( gamma );
 
    MicroPhy_t MicroPhy_obj;
    memset( &MicroPhy_obj, 0, sizeof(MicroPhy_obj) );
@@ -143,6 +155,22 @@ int gamer_ref_fluid_solver( const void *in, void *out, void *flux, int npg,
                         EoS_obj, MicroPhy_obj );
 
    delete [] h_PriVar;  delete [] h_FC_Var;  delete [] h_FC_Flux;
+   return 0;
+}
+
+// One call of GAMER's CPU_dtSolver_HydroCFL on npatch = 8*NPG patches: in [npatch][5][8^3] -> dt [npatch].
+// The reference loops over the 8 patches of each of NPG patch groups, so npatch must be a multiple of 8.
+int gamer_ref_dt_cfl( const void *in, void *dt, int npatch, double dh, double safety, double gamma,
+                      double min_pres, int nthreads )
+{
+   if ( npatch <= 0 || npatch % 8 != 0 || in == NULL || dt == NULL )   return 1;
+   const EoS_t EoS_obj = make_eos( gamma );
+   MicroPhy_t MicroPhy_obj;
+   memset( &MicroPhy_obj, 0, sizeof(MicroPhy_obj) );
+   if ( nthreads > 0 )  omp_set_num_threads( nthreads );
+   omp_set_schedule( omp_sched_dynamic, 1 );
+   CPU_dtSolver_HydroCFL( (real *)dt, (const real (*)[FLU_NIN_T][ CUBE(PS1) ])in, NULL, npatch/8,
+                          (real)dh, (real)safety, (real)min_pres, 0L, EoS_obj, MicroPhy_obj );
    return 0;
 }
 

@@ -143,3 +143,41 @@ def test_one_hip_runtime_whatever_the_import_order():
             "print(len({l.split()[-1] for l in maps if 'libamdhip64' in l}))\n") % ROOT
     out = subprocess.run([sys.executable, "-c", code], check=True, capture_output=True, text=True, timeout=600)
     assert out.stdout.strip().splitlines()[-1] == "1", out.stdout + out.stderr
+
+
+class _FakeDev:
+    """Duck-typed stand-in for a device tensor: exercises the wrappers' output checks without a GPU."""
+
+    def __init__(self, shape, dtype, device="cuda:0", cuda=True, contiguous=True):
+        import torch
+        self.shape, self.dtype, self.device = tuple(shape), dtype, torch.device(device)
+        self.is_cuda, self._c = cuda, contiguous
+
+    def is_contiguous(self):
+        return self._c
+
+
+def test_check_out_rejects_bad_outputs():
+    """Caller-supplied out / flux / dt_patch tensors are checked for device, dtype, shape and contiguity before any
+    launch: a wrong one would make the kernels write through a raw pointer past its end."""
+    torch = pytest.importorskip("torch")
+    f64, dev = torch.float64, torch.device("cuda:0")
+    shape = (4, 5, 16, 16, 16)
+    hydro._check_out(None, shape, f64, dev, "out")
+    hydro._check_out(_FakeDev(shape, f64), shape, f64, dev, "out")
+    for bad, frag in [(_FakeDev(shape, torch.float32), "dtype"),
+                      (_FakeDev((4, 5, 16, 16, 17), f64), "shape"),
+                      (_FakeDev(shape, f64, cuda=False, device="cpu"), "device"),
+                      (_FakeDev(shape, f64, device="cuda:1"), "device"),
+                      (_FakeDev(shape, f64, contiguous=False), "contiguous")]:
+        with pytest.raises(hydro.HydroError, match=frag):
+            hydro._check_out(bad, shape, f64, dev, "out")
+
+
+def test_wrappers_reject_host_tensors():
+    torch = pytest.importorskip("torch")
+    u = torch.zeros((1, 5, 22, 22, 22), dtype=torch.float64)
+    for fn in (lambda: hydro.fluid_solver(u, 1e-3, 1e-2), lambda: hydro.solve_batch(u),
+               lambda: hydro.dt_cfl(torch.zeros((1, 5, 8, 8, 8)), 1e-2, 0.5)):
+        with pytest.raises(hydro.HydroError):
+            fn()

@@ -23,12 +23,22 @@ pytestmark = pytest.mark.skipif(not (os.path.isfile(os.path.join(REF, "include",
                                      and shutil.which("nm")),
                                 reason="reference headers or toolchain not present")
 
-# GAMER's CUAPI_MemAllocate.cu declares CUAPI_MemAllocate_Fluid locally (src/GPU_API/CUAPI_MemAllocate.cu:9)
+# GAMER's CUAPI_MemAllocate.cu declares CUAPI_MemAllocate_Fluid and CUAPI_SetMemSize locally
+# (src/GPU_API/CUAPI_MemAllocate.cu:9-11), and CPU_EoS_Gamma.cpp declares EoS_SetGPUFunc_Gamma
+# (src/EoS/Gamma/CPU_EoS_Gamma.cpp:447-449); the rest is include/Prototype.h:646-690
 PROBE = """#include "GAMER.h"
 int CUAPI_MemAllocate_Fluid( const int Flu_NPG, const int Pot_NPG, const int Src_NPG, const int GPU_NStream );
+void CUAPI_SetMemSize( int &GPU_NStream, int &Flu_GPU_NPGroup, int &Pot_GPU_NPGroup, int &Che_GPU_NPGroup,
+                       int &Src_GPU_NPGroup );
+void EoS_SetGPUFunc_Gamma( EoS_DE2P_t &, EoS_DP2E_t &, EoS_DP2C_t &, EoS_DE2T_t &, EoS_DT2P_t &, EoS_DE2S_t &,
+                           EoS_GENE_t & );
 void *probe[] = { (void*)&CUAPI_Asyn_FluidSolver, (void*)&CUAPI_SetDevice, (void*)&CUAPI_SetCache,
-                  (void*)&CUAPI_MemFree_Fluid, (void*)&CUAPI_Synchronize, (void*)&CUAPI_MemAllocate_Fluid };
+                  (void*)&CUAPI_MemFree_Fluid, (void*)&CUAPI_Synchronize, (void*)&CUAPI_MemAllocate_Fluid,
+                  (void*)&CUAPI_Asyn_dtSolver, (void*)&CUAPI_DiagnoseDevice, (void*)&CUAPI_MemAllocate,
+                  (void*)&CUAPI_SetConstMemory, (void*)&CUAPI_SetConstMemory_EoS, (void*)&CUAPI_SetMemSize,
+                  (void*)&EoS_SetGPUFunc_Gamma };
 """
+NPROBE = 13
 
 VARIANTS = [("MHM_RP", "HLLC", False), ("MHM_RP", "ROE", True), ("MHM", "HLLE", False), ("MHM", "HLLC", True)]
 
@@ -53,8 +63,8 @@ def test_shim_defines_gamer_gpu_api(tmp_path, scheme, rsolver, fp64):
     probe_c.write_text(PROBE)
     subprocess.run(["g++"] + flags + [SHIM, "-o", str(shim_o)], check=True)
     subprocess.run(["g++"] + flags + [str(probe_c), "-o", str(probe_o)], check=True)
-    wanted = {s for s in _symbols(probe_o, "U") if "CUAPI_" in s}
-    assert len(wanted) == 6, wanted
+    wanted = {s for s in _symbols(probe_o, "U") if "CUAPI_" in s or "EoS_SetGPUFunc" in s}
+    assert len(wanted) == NPROBE, wanted
     missing = wanted - _symbols(shim_o, "T")
     assert not missing, f"GAMER declares but the adapter does not define: {missing}"
 
@@ -68,3 +78,20 @@ def test_shim_calls_only_exported_library_functions(tmp_path):
     out = subprocess.run(["nm", "-D", "--defined-only", LIB], check=True, capture_output=True, text=True).stdout
     exported = {line.split()[-1] for line in out.splitlines()}
     assert called and called <= exported, called - exported
+
+
+@pytest.mark.parametrize("extra,frag", [
+    (["-ULR_SCHEME", "-DLR_SCHEME=PPM"], "PLM"),
+    (["-UNCOMP_PASSIVE_USER", "-DNCOMP_PASSIVE_USER=1"], "passive"),
+    (["-DDUAL_ENERGY=DE_ENPY"], "dual energy"),
+    (["-DMHD"], "MHD"),
+    (["-UEOS", "-DEOS=EOS_ISOTHERMAL"], "EOS_GAMMA"),
+    (["-DGRAVITY"], "self-gravity"),
+])
+def test_shim_rejects_unsupported_builds(tmp_path, extra, frag):
+    """A GAMER build whose arrays the library's layouts do not fit (PPM's FLU_NXT, passive scalars, dual energy, MHD,
+    another EoS, self-gravity) must not compile against the adapter: it would overflow the pinned buffers."""
+    flags = _flags("MHM_RP", "HLLC", False) + extra
+    r = subprocess.run(["g++"] + flags + [SHIM, "-o", str(tmp_path / "shim.o")], capture_output=True, text=True)
+    assert r.returncode != 0
+    assert frag in r.stderr, r.stderr[-2000:]

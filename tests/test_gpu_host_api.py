@@ -52,3 +52,58 @@ def test_host_array_drop_in_matches_oracle(scheme, precision):
                 assert np.all(rel_l1(o, ro) <= 1e-5)
     finally:
         api.free()
+
+
+def test_allocation_beyond_hbm_returns_nomem_and_library_stays_usable():
+    """An allocation larger than the device memory returns HYDRO_E_NOMEM (GAMER's CUDA_CHECK_MALLOC -> GAMER_FAILED,
+    src/GPU_API/CUAPI_MemAllocate.cu:41-96, on which CUAPI_MemAllocate halves GPU_NSTREAM and retries), frees what
+    it had allocated, and leaves the library usable: a normal allocation and solve afterwards match the oracle."""
+    import oracle
+    from gamer_amd import hydro
+    L = hydro.lib()
+    assert L.hydro_init(0) == 0
+    # 10^6 MHM_RP FP64 patch groups: the input buffer alone is 426 GB, more than the 288 GB of HBM
+    print("[gpu] allocation beyond HBM -> HYDRO_E_NOMEM", flush=True)
+    rc = L.hydro_mem_allocate_fluid(1_000_000, 4, hydro.SCHEME["MHM_RP"], hydro.PRECISION["fp64"])
+    assert rc == hydro.HYDRO_E_NOMEM, (rc, L.hydro_last_error())
+    assert b"failed" in L.hydro_last_error()
+    args = hydro.make_args(1, 1e-3, 1e-2)
+    assert L.hydro_asyn_fluid_solver(None, None, None, args, 1) == hydro.HYDRO_E_STATE
+    w = ic.random_sod((1, 2, 2), seed=3, sedov=True)
+    api = hydro.GpuFluidApi(4, gpu_nstream=2)
+    try:
+        h_in, h_out, h_flux = api.host_arrays(0)
+        h_in[...] = w.u
+        api.asyn_fluid_solver(0, 4, w.dt, w.dh)
+        api.synchronize()
+        ro, _ = oracle.fluid_solver(w.u, w.dt, w.dh)
+        assert np.all(rel_err_cells(np.array(h_out), ro) <= 1e-12)
+    finally:
+        api.free()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_host_array_dt_solver_matches_oracle(precision):
+    """The host-array CFL dt (hydro_asyn_dt_solver, CUAPI_Asyn_dtSolver's DT_FLU_SOLVER path) on the pinned dt buffers,
+    8*npg patches split over 3 streams, against the oracle's dt (bit-exact to GAMER's CPU_dtSolver_HydroCFL)."""
+    import oracle
+    from gamer_amd import hydro
+    print(f"[gpu] host-array dt solver {precision}", flush=True)
+    w = ic.random_sod((2, 2, 2), seed=12, sedov=True)
+    out, _ = oracle.fluid_solver(w.u, w.dt, w.dh)
+    npg = out.shape[0] - 1                                 # 56 patches over 3 streams: a remainder on the last
+    p = out.reshape(-1, 5, 2, 8, 2, 8, 2, 8).transpose(0, 2, 4, 6, 1, 3, 5, 7).reshape(-1, 5, 8, 8, 8)[:8 * npg]
+    ref = oracle.dt_cfl(np.ascontiguousarray(p), w.dh, 0.4)
+    api = hydro.GpuFluidApi(8, gpu_nstream=3, precision=precision)
+    try:
+        h_flu, h_dt = api.host_dt_arrays(1)
+        h_flu[:8 * npg] = p
+        h_dt[...] = np.nan
+        api.asyn_dt_solver(1, npg, w.dh, 0.4)
+        api.synchronize()
+        got = np.array(h_dt[:8 * npg], dtype=np.float64)
+        assert np.isnan(h_dt[8 * npg:]).all()
+        rel = np.abs(got - ref) / ref
+        assert rel.max() <= (1e-14 if precision == "fp64" else 1e-6), rel.max()
+    finally:
+        api.free()

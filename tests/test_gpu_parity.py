@@ -137,3 +137,44 @@ def test_empty_batch():
     out, flux = hydro.fluid_solver(u, 1e-3, 1e-2)
     torch.cuda.synchronize()
     assert out.shape[0] == 0 and flux.shape[0] == 0
+
+
+def test_bad_outputs_rejected_before_launch():
+    """Wrong dtype / shape / device / layout of caller-supplied outputs raise before any kernel runs."""
+    from gamer_amd import hydro
+    dev = torch.device("cuda:0")
+    u = torch.zeros((2, 5, 22, 22, 22), dtype=torch.float64, device=dev)
+    good_out = torch.empty((2, 5, 16, 16, 16), dtype=torch.float64, device=dev)
+    bad = [dict(out=torch.empty((2, 5, 16, 16, 16), dtype=torch.float32, device=dev)),
+           dict(out=torch.empty((1, 5, 16, 16, 16), dtype=torch.float64, device=dev)),
+           dict(out=torch.empty((2, 5, 16, 16, 16), dtype=torch.float64)),
+           dict(out=torch.empty((2, 5, 16, 16, 32), dtype=torch.float64, device=dev)[..., ::2]),
+           dict(out=good_out, flux=torch.empty((2, 9, 5, 16, 16), dtype=torch.float32, device=dev)),
+           dict(out=good_out, dt_patch=torch.empty((2, 8), dtype=torch.float64)),
+           dict(out=good_out, dt_patch=torch.empty((2, 8), dtype=torch.float32, device=dev))]
+    for kw in bad:
+        with pytest.raises(hydro.HydroError):
+            hydro.fluid_solver(u, 1e-3, 1e-2, **kw)
+        with pytest.raises(hydro.HydroError):
+            hydro.solve_batch(u, dt=1e-3, dh=1e-2, **kw)
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("scheme", ["MHM_RP", "MHM"])
+def test_roe_fallback_double_rarefaction(scheme):
+    """Toro's 123 problem (rho 1, u -/+2, p 0.4): Roe's intermediate states are unphysical at the central face, so the
+    solver falls back to HLLE there (CPU_Shared_RiemannSolver_Roe.cpp:607-667; tests/test_oracle_props.py shows the
+    oracle's Roe flux equals HLLE's bit for bit at these states and that the whole step equals GAMER's Roe build).
+    The GPU Roe step must match the oracle: FP64 per cell <= 1e-12, FP32-bulk rel L1 <= 1e-5."""
+    from test_oracle_props import _toro123
+    cons, dt, dh = _toro123()
+    g = GHOST[scheme]
+    u = ic.cut_patch_groups(cons, g)
+    w = ic.Workload(u, dt, dh, (1, 1, 2), g)
+    print(f"[gpu] Roe fallback (Toro 123) {scheme}", flush=True)
+    ro, rf = _oracle(u, w, scheme=scheme, rsolver="ROE")
+    o, f = _gpu(u, w, scheme=scheme, rsolver="ROE")
+    assert np.all(rel_err_cells(o, ro) <= FP64_TOL), rel_err_cells(o, ro)
+    assert flux_rel_err(f, rf, ro) <= FP64_TOL
+    o32, _ = _gpu(u.astype(np.float32), w, scheme=scheme, rsolver="ROE")
+    assert np.all(rel_l1(o32, ro) <= FP32_L1_TOL), rel_l1(o32, ro)

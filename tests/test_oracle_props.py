@@ -167,3 +167,43 @@ def test_cfl_dt_matches_definition():
     p = (G - 1) * (patch[0, 4] - 0.5 * rho * (vel ** 2).sum(axis=0))
     c = np.sqrt(G * p / rho)
     assert dt == pytest.approx(0.5 * w.dh / (np.abs(vel).max(axis=0) + c).max(), rel=1e-14)
+
+
+def _toro123(nx_pg=2):
+    """Toro's "123" problem (two strong rarefactions: rho 1, u -/+2, p 0.4) along x on a periodic grid of nx_pg x 1 x 1
+    patch groups: u = -2 in the left half of each pair of patch groups, +2 in the right half."""
+    nx = 16 * nx_pg
+    shape = (16, 16, nx)
+    x = np.arange(nx)[None, None, :]
+    u = np.broadcast_to(np.where(x < nx // 2, -2.0, 2.0), shape)
+    cons = ic.prim_to_cons(np.ones(shape), u, np.zeros(shape), np.zeros(shape), np.full(shape, 0.4))
+    dh = 1.0 / nx
+    return cons, ic.cfl_dt(cons, dh, 0.5), dh
+
+
+def test_roe_falls_back_to_hlle_on_double_rarefaction():
+    """Roe's linearised intermediate states of Toro's 123 problem have negative density / pressure, so the solver
+    returns the HLLE flux (CPU_Shared_RiemannSolver_Roe.cpp:607-667): bit for bit equal to HLLE there, and not
+    equal on a mild jump where the check passes."""
+    L = cons(1.0, -2.0, 0.0, 0.0, 0.4)
+    R = cons(1.0, 2.0, 0.0, 0.0, 0.4)
+    for d in range(3):
+        Ld, Rd = L.copy(), R.copy()
+        Ld[[1, 1 + d]], Rd[[1, 1 + d]] = Ld[[1 + d, 1]], Rd[[1 + d, 1]]
+        assert np.array_equal(oracle.riemann("ROE", d, Ld, Rd), oracle.riemann("HLLE", d, Ld, Rd))
+    Lm = cons(1.0, 0.1, 0.0, 0.0, 1.0)
+    Rm = cons(0.8, 0.0, 0.0, 0.0, 0.8)
+    assert not np.array_equal(oracle.riemann("ROE", 0, Lm, Rm), oracle.riemann("HLLE", 0, Lm, Rm))
+
+
+def test_roe_fallback_solver_step_vs_reference_build():
+    """A whole MHM_RP / Roe step of Toro's 123 problem: the half-step x-solve at the central face takes the cell
+    states of the input, which make Roe fall back to HLLE; the restatement equals GAMER's own Roe build bit for bit."""
+    if not oracle.ref_available("MHM_RP", "ROE", "f64"):
+        pytest.skip("reference build (oracle/_ref) not present")
+    cons, dt, dh = _toro123()
+    u = ic.cut_patch_groups(cons, 3)
+    o1, f1 = oracle.fluid_solver(u, dt, dh, rsolver="ROE")
+    o2, f2 = oracle.ref_fluid_solver(u, dt, dh, rsolver="ROE")
+    np.testing.assert_array_equal(o1, o2)
+    np.testing.assert_array_equal(f1, f2)

@@ -68,3 +68,23 @@ def test_bit_exact_special_paths(case):
     o2, f2 = oracle.ref_fluid_solver(w.u, w.dt, w.dh, **kw)
     np.testing.assert_array_equal(o1, o2)
     np.testing.assert_array_equal(f1, f2)
+
+
+@pytest.mark.parametrize("min_pres", [0.0, 0.3])
+def test_dt_cfl_bit_exact_vs_reference_build(min_pres):
+    """oracle.dt_cfl against GAMER's own CPU_dtSolver_HydroCFL (FP64 build), bit for bit, on the solver's updated
+    patches of a random Sod+Sedov medium; min_pres = 0.3 sits above many cells' pressure, so the MinPres floor
+    inside Hydro_Con2Pres decides their sound speed (CPU_dtSolver_HydroCFL.cpp:139-146)."""
+    if not oracle.ref_available("MHM_RP", "HLLC", "f64"):
+        pytest.skip("reference build (oracle/_ref) not present")
+    w = ic.random_sod((2, 2, 2), seed=606, sedov=True)
+    out, _ = oracle.fluid_solver(w.u, w.dt, w.dh)
+    npg = out.shape[0]
+    patches = np.ascontiguousarray(
+        out.reshape(npg, 5, 2, 8, 2, 8, 2, 8).transpose(0, 2, 4, 6, 1, 3, 5, 7).reshape(npg * 8, 5, 8, 8, 8))
+    p = (2.0 / 3.0) * (patches[:, 4] - 0.5 * (patches[:, 1:4] ** 2).sum(axis=1) / patches[:, 0])
+    if min_pres > 0:
+        assert (p < min_pres).mean() > 0.1          # the floor is really active in many cells
+    a = oracle.dt_cfl(patches, w.dh, 0.5, min_pres=min_pres)
+    b = oracle.ref_dt_cfl(patches, w.dh, 0.5, min_pres=min_pres)
+    np.testing.assert_array_equal(a, b)
