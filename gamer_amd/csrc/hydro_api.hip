@@ -579,11 +579,13 @@ int hydro_device_info( HydroDeviceInfo *info )
 }
 int hydro_asyn_fluid_solver( const void *h_in, void *h_out, void *h_flux, const HydroSolveArgs *args, int gpu_nstream )
 {
+   // call order first: without buffers no argument can be valid
+   if ( g_buf.npg_max == 0 )            return fail( HYDRO_E_STATE, "hydro_mem_allocate_fluid was not called" );
+   if ( !args )                         return fail( HYDRO_E_ARG, "args is NULL" );
    HydroSolveArgs a = *args;
    a.in = h_in;  a.out = h_out;  a.flux = h_flux;
    int rc = check_args( &a );
    if ( rc != HYDRO_OK ) return rc;
-   if ( g_buf.npg_max == 0 )            return fail( HYDRO_E_STATE, "hydro_mem_allocate_fluid was not called" );
    if ( a.npg > g_buf.npg_max )         return fail( HYDRO_E_ARG, "npg %d > allocated %d", a.npg, g_buf.npg_max );
    if ( a.scheme != g_buf.scheme || a.precision != g_buf.precision )
                                         return fail( HYDRO_E_ARG, "scheme/precision differ from the allocation" );

@@ -333,6 +333,21 @@ __device__ __forceinline__ St<R> cell_state( const R c[NV], R p, R ir, R cs )
    return s;
 }
 
+// cell_state<0> or cell_state<1> chosen per lane (dy = y-normal face) from a cell (conserved, P, 1/rho, c_s)
+template <typename R>
+__device__ __forceinline__ St<R> cell_state_xy( const R q[8], bool dy )
+{
+   St<R> s;
+   s.d  = q[0];
+   s.mn = dy ? q[2] : q[1];
+   s.m1 = dy ? q[3] : q[2];
+   s.m2 = dy ? q[1] : q[3];
+   s.e  = q[4];
+   s.p  = q[5];  s.ir = q[6];  s.cs = q[7];
+   s.u  = s.ir*s.mn;
+   return s;
+}
+
 // whether a flux component is NaN (an infinite component is not)
 template <typename R>
 __device__ __forceinline__ bool any_nan5( const R f[NV] )

@@ -6,12 +6,14 @@
 // scratch per patch group in FP64).  Here the workgroup marches along z, one input plane per step, and keeps
 // every intermediate on chip:
 //
-//   * Lane t owns the output column (i,j) = (3 + t%16, 3 + output_row(t)).  Lanes 0-183 (waves 0-2) also own one
-//     "outer" ghost column for the half step (half_step_column()); lanes 192-255 (wave 3) each reconstruct one
-//     ghost column next to the output block along x or y, and half of them compute the full-step flux on the
-//     face between that column and the block (edge_column()).  Everything that couples planes along z — the
-//     previous input planes, the half-step z-flux, the last three half-step primitive planes, the lower z-face
-//     state and the last full-step z-flux — stays in the owning lane's registers.
+//   * Lane t owns the output column (i,j) = (3 + t%16, 3 + output_row(t)).  Lanes 0-127 (waves 0-1) also own one
+//     "arm" ghost column for the half step (arm_column()), lanes 128-167 (wave 2) one single half-step face per
+//     plane (face_task()); together they update only the half-step cells the reconstruction reads.  Lanes
+//     192-255 (wave 3) each reconstruct one ghost column next to the output block along x or y, and half of them
+//     compute the full-step flux on the face between that column and the block (edge_column()).  Everything that
+//     couples planes along z — the previous input planes, the half-step z-flux, the last three half-step
+//     primitive planes, the lower z-face state and the last full-step z-flux — stays in the owning lane's
+//     registers.
 //   * Only x/y neighbours go through LDS, one plane at a time: the input plane with its pressure, 1/rho and
 //     sound speed, x and y fluxes (half step, then full step), the half-step primitive plane and the upper
 //     x/y face states.  A lane-private LDS ring keeps the input cells of the own output column for the planes
@@ -25,7 +27,7 @@
 //   [H] x/y half-step flux differences of plane t, half-step plane t-1 into LDS
 //   [R] full-step fluxes of plane r (x/y) and at r-1/2 (z)
 //   [F] full-step update and output of plane t-3, input plane t+1 into LDS
-// [A] is the longest interval and waves 0-2 (ghost half-step solves) its critical path; they raise their issue
+// [A] is the longest interval and waves 0-1 (arm half-step solves) its critical path; they raise their issue
 // priority inside it.  All branches that contain a barrier depend only on t, so every barrier is reached by the
 // whole workgroup.
 
@@ -57,31 +59,54 @@ __device__ __forceinline__ int output_row( int t )
    return ( t >> 6 ) + 8*( q & 1 ) + 4*( q >> 1 );
 }
 
-// ghost columns served by lane t besides its output column; -1 = none
-__device__ __forceinline__ void ring2( int k, int &i, int &j )
+// The reconstruction reads half-step cells only in three slabs: the 16^2 output columns over z = 1..20, and the
+// "arm" columns (i or j in {1,2,19,20}, the other in 3..18) over z = 3..18 -- 7,168 of the 8,000 cells the
+// reference updates (CPU_FluidSolver_MHM.cpp:656-666 takes the whole 20^3 block).  Those cells need 22,656 of
+// its 25,200 half-step faces.  The ghost work of waves 0-2 is laid out so that each wave runs one task kind
+// over one z range and skips the rest with wave-uniform branches:
+//   wave 0: the 64 arms next to the output block (i or j in {2,19}): half-step cell + low x and y faces
+//   wave 1: the 64 outer arms (i or j in {1,20}): the same
+//   wave 2: 40 single half-step faces (one x or y face per lane, read by an arm or output cell but owned by a
+//           column that is neither): the low x face of column (21,j) at z = 3..18 and of column (19,j) at the
+//           other planes, where the output cell (18,j) still needs it (and the y analogue); and the low x faces
+//           of (19, {1,2,19,20}) and low y faces of ({1,2,19,20}, 19) at z = 3..18
+// Arm column of lane t (waves 0-1); false = none.
+__device__ __forceinline__ bool arm_column( int t, int &i, int &j )
 {
-   // 76 columns with i or j in {1,20}, 1 <= i,j <= 20
-   if      ( k < 20 ) { i = 1 + k;        j = 1;  }
-   else if ( k < 40 ) { i = 1 + k - 20;   j = 20; }
-   else if ( k < 58 ) { i = 1;            j = 2 + k - 40; }
-   else               { i = 20;           j = 2 + k - 58; }
+   if ( t >= 128 ) { i = j = 0;  return false; }
+   const int k = t & 63, a = 3 + ( k & 15 ), s = ( k >> 4 ) & 1;
+   const int e = ( t < 64 ) ? ( s ? 19 : 2 ) : ( s ? 20 : 1 );
+   if ( k < 32 ) { i = e;  j = a; }
+   else          { i = a;  j = e; }
+   return true;
 }
 
-// ghost column of the half step served by lane t (waves 0-2) besides its output column; false = none
-__device__ __forceinline__ bool half_step_column( int t, int &i, int &j )
+// single half-step face of lane t (wave 2) at plane z: the column whose low face it is and whether the face is
+// y-normal; false = no face at this plane
+__device__ __forceinline__ bool face_task( int t, int z, int &c, bool &dy )
 {
-   if ( t <  16 ) { i = 2;          j = 3 + t;       return true; }     // wave 0
-   if ( t <  32 ) { i = 19;         j = 3 + t - 16;  return true; }
-   if ( t <  64 ) { ring2( t - 32, i, j );           return true; }
-   if ( t <  80 ) { i = 3 + t - 64; j = 2;           return true; }     // wave 1
-   if ( t <  96 ) { i = 3 + t - 80; j = 19;          return true; }
-   if ( t < 128 ) { ring2( t - 96 + 32, i, j );      return true; }
-   if ( t < 132 ) { const int c = t - 128; i = ( c & 1 ) ? 19 : 2; j = ( c & 2 ) ? 19 : 2; return true; }  // wave 2
-   if ( t < 144 ) { ring2( t - 132 + 64, i, j );     return true; }
-   if ( t < 164 ) { i = 21;         j = 1 + t - 144; return true; }     // x half-step face only
-   if ( t < 184 ) { i = 1 + t - 164; j = 21;         return true; }     // y half-step face only
-   i = j = -1;
-   return false;
+   const int k = t - 128;
+   const bool mid = ( z >= 3 && z <= 18 );
+   int i, j;
+   dy = false;
+   if ( k < 0 || k >= 40 ) return false;
+   if ( k < 32 )
+   {
+      const int a = 3 + ( k & 15 ), e = mid ? 21 : 19;
+      dy = ( k >= 16 );
+      i = dy ? a : e;
+      j = dy ? e : a;
+   }
+   else
+   {
+      const int q = k & 3, e = ( q < 2 ) ? 1 + q : 17 + q;     // 1, 2, 19, 20
+      dy = ( k >= 36 );
+      i = dy ? e : 19;
+      j = dy ? 19 : e;
+      if ( !mid ) return false;
+   }
+   c = i + NX*j;
+   return true;
 }
 
 // ghost column next to the output block reconstructed by lane t of wave 3 (along y if dy), and whether the
@@ -223,9 +248,10 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
    const int tid = threadIdx.x;
 
    const Col K = make_col( 3 + tid % PS, 3 + output_row( tid ) );
+   const int wave = __builtin_amdgcn_readfirstlane( tid >> 6 );   // wave-uniform role switch
    int oi, oj;
-   const bool has_o = half_step_column( tid, oi, oj );
-   const Col O = make_col( has_o ? oi : 0, has_o ? oj : 0 );
+   arm_column( tid, oi, oj );
+   const Col O = make_col( oi, oj );
    int ei, ej;
    bool e_dy, e_face;
    const bool has_e = edge_column( tid, ei, ej, e_dy, e_face );
@@ -340,14 +366,19 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
          const int o = t - 3;                    // output plane of this step
          const int ring = ( t + 1 ) & 1;         // u3 slot: input plane t-3 (read), then plane t-1 (written)
          __syncthreads();                        // [A] input plane t visible
-         // waves 0-2 carry the ghost half-step solves of this interval and set its length: with two workgroups per
-         // CU sharing the SIMDs, letting them issue ahead of wave 3 (and of waves in other intervals) shortens it
-         if ( tid < 192 ) __builtin_amdgcn_s_setprio( 3 ); else __builtin_amdgcn_s_setprio( 1 );
+         // waves 0-1 carry the arm half step of this interval and set its length: with two workgroups per CU
+         // sharing the SIMDs, letting them issue ahead of waves 2-3 (and of waves in other intervals) shortens it
+         if ( wave < 2 ) __builtin_amdgcn_s_setprio( 3 ); else __builtin_amdgcn_s_setprio( 1 );
+         // arm half step: cells of planes 3..18 (updated at steps 4..19), their z faces (steps 3..19) and their
+         // low x/y faces (planes 3..18)
+         const bool arm_ld = ( wave < 2 ) && t >= 2 && t <= 19;
+         const bool arm_z  = ( wave < 2 ) && t >= 3 && t <= 19;
+         const bool arm_xy = ( wave < 2 ) && t >= 3 && t <= 18;
 
 #        pragma unroll
          for (int v=0; v<8; v++) { kU1[v] = kU[v];  oU1[v] = oU[v]; }
          ld_cell( S.a, K.c, kU );
-         if ( has_o ) ld_cell( S.a, O.c, oU );
+         if ( arm_ld ) ld_cell( S.a, O.c, oU );
 
          // ---- half-step z-flux at t-1/2 and half-step cell update of plane t-1
          if ( t >= 1 )
@@ -362,14 +393,14 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
             }
 #           pragma unroll
             for (int v=0; v<NV; v++) kHz[v] = fz[v];
-
-            if ( has_o && O.w )
-            {
-               riemann<RS, 2>( st_of<2>( oU1 ), st_of<2>( oU ), Q, fz );
-               if ( t >= 2 ) half_cell( oU1, oDh, oHz, fz, Q, oW );
-#              pragma unroll
-               for (int v=0; v<NV; v++) oHz[v] = fz[v];
-            }
+         }
+         if ( arm_z )
+         {
+            R fz[NV];
+            riemann<RS, 2>( st_of<2>( oU1 ), st_of<2>( oU ), Q, fz );
+            if ( t >= 4 ) half_cell( oU1, oDh, oHz, fz, Q, oW );
+#           pragma unroll
+            for (int v=0; v<NV; v++) oHz[v] = fz[v];
          }
 
          // ---- half-step x and y fluxes of plane t (low face owned by each column)
@@ -377,16 +408,37 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
          if ( t >= 1 && t <= 20 )
          {
             riemann<RS, 0>( st_of<0>( S.a, K.c - 1 ), st_of<0>( kU ), Q, fkx );
-            if ( has_o && O.hx ) riemann<RS, 0>( st_of<0>( S.a, O.c - 1 ), st_of<0>( oU ), Q, fox );
             riemann<RS, 1>( st_of<1>( S.a, K.c - NX ), st_of<1>( kU ), Q, fky );
-            if ( has_o && O.hy ) riemann<RS, 1>( st_of<1>( S.a, O.c - NX ), st_of<1>( oU ), Q, foy );
 #           pragma unroll
             for (int v=0; v<NV; v++)
             {
                S.b[v][K.c] = fkx[v];
-               if ( has_o && O.hx ) S.b[v][O.c] = fox[v];
                S.b[NV + v][K.c] = fky[v];
-               if ( has_o && O.hy ) S.b[NV + v][O.c] = foy[v];
+            }
+         }
+         if ( arm_xy )
+         {
+            riemann<RS, 0>( st_of<0>( S.a, O.c - 1 ), st_of<0>( oU ), Q, fox );
+            riemann<RS, 1>( st_of<1>( S.a, O.c - NX ), st_of<1>( oU ), Q, foy );
+#           pragma unroll
+            for (int v=0; v<NV; v++)
+            {
+               S.b[v][O.c] = fox[v];
+               S.b[NV + v][O.c] = foy[v];
+            }
+         }
+         if ( wave == 2 && t >= 1 && t <= 20 )
+         {
+            int fc;
+            bool fdy;
+            if ( face_task( tid, t, fc, fdy ) )
+            {
+               R ql[8], qr[8], ff[NV];
+               ld_cell( S.a, fc - ( fdy ? NX : 1 ), ql );
+               ld_cell( S.a, fc, qr );
+               riemann_xy<RS>( cell_state_xy( ql, fdy ), cell_state_xy( qr, fdy ), Q, fdy, ff );
+#              pragma unroll
+               for (int v=0; v<NV; v++) S.b[( fdy ? NV : 0 ) + v][fc] = ff[v];
             }
          }
 
@@ -426,21 +478,17 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
 
          if ( t >= 1 && t <= 20 )
 #           pragma unroll
-            for (int v=0; v<NV; v++)
-            {
-               kDh[v] = ( S.b[v][K.c + 1] - fkx[v] ) + ( S.b[NV + v][K.c + NX] - fky[v] );
-               if ( has_o && O.w ) oDh[v] = ( S.b[v][O.c + 1] - fox[v] ) + ( S.b[NV + v][O.c + NX] - foy[v] );
-            }
+            for (int v=0; v<NV; v++) kDh[v] = ( S.b[v][K.c + 1] - fkx[v] ) + ( S.b[NV + v][K.c + NX] - fky[v] );
+         if ( arm_xy )
+#           pragma unroll
+            for (int v=0; v<NV; v++) oDh[v] = ( S.b[v][O.c + 1] - fox[v] ) + ( S.b[NV + v][O.c + NX] - foy[v] );
          // the half-step plane t-1 replaces plane t-2 in LDS for the next reconstruction
          if ( t >= 2 )
-         {
 #           pragma unroll
-            for (int v=0; v<NV; v++)
-            {
-               S.w[v][K.c] = kW1[v];
-               if ( has_o && O.w ) S.w[v][O.c] = oW[v];
-            }
-         }
+            for (int v=0; v<NV; v++) S.w[v][K.c] = kW1[v];
+         if ( arm_z && t >= 4 )
+#           pragma unroll
+            for (int v=0; v<NV; v++) S.w[v][O.c] = oW[v];
 
          // ---- full-step fluxes: z at r-1/2 (lane-local), x and y of plane r through LDS
          R fzn[NV];
