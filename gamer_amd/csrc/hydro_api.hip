@@ -46,6 +46,33 @@ struct FluidBuffers {
 FluidBuffers g_buf;
 std::mutex   g_mtx;
 
+// Work-queue counters of hydro_mhm_solve_batch_noalloc: one 32-bit word per queue slot, in a file-scope device
+// array that every GPU the library runs on holds until the process ends.  Host code never passes `scratch`
+// itself to a kernel (on the host it is only a stand-in symbol): each device's copy is looked up once with
+// hipGetSymbolAddress, with that device current, and kept in g_scratch[device].
+constexpr int MAX_DEVICES = 64;
+__device__ __align__( 256 ) unsigned scratch[HYDRO_NOALLOC_QUEUES];
+unsigned  *g_scratch[MAX_DEVICES] = {};
+std::mutex g_scratch_mtx;
+
+// device address of the current device's scratch array; looked up on the first call per device
+int scratch_ptr( unsigned **p )
+{
+   int dev = 0;
+   HIP_TRY( hipGetDevice( &dev ) );
+   if ( dev < 0 || dev >= MAX_DEVICES ) return fail( HYDRO_E_DEVICE, "device %d beyond the %d the library tracks", dev, MAX_DEVICES );
+   std::lock_guard<std::mutex> lk( g_scratch_mtx );
+   if ( g_scratch[dev] == nullptr )
+   {
+      void *q = nullptr;
+      HIP_TRY( hipGetSymbolAddress( &q, HIP_SYMBOL( scratch ) ) );
+      if ( q == nullptr ) return fail( HYDRO_E_DEVICE, "hipGetSymbolAddress(scratch) returned NULL" );
+      g_scratch[dev] = (unsigned *)q;
+   }
+   *p = g_scratch[dev];
+   return HYDRO_OK;
+}
+
 int nx_of( int scheme ) { return scheme == HYDRO_MHM_RP ? 22 : 20; }
 
 int check_args( const HydroSolveArgs *a )
@@ -143,7 +170,8 @@ int hydro_init( int device )
    HIP_TRY( hipGetDeviceCount( &n ) );
    if ( device < 0 || device >= n ) return fail( HYDRO_E_ARG, "device %d out of range (%d devices)", device, n );
    HIP_TRY( hipSetDevice( device ) );
-   return HYDRO_OK;
+   unsigned *p = nullptr;
+   return scratch_ptr( &p );                 // set-up of the allocation-free entry point on this device
 }
 
 int hydro_mhm_solve_async( const HydroSolveArgs *a, void *stream )
@@ -444,6 +472,18 @@ int hydro_mhm_solve_batch_async( const HydroSolveArgs *a, const double *pg_dt_dh
          return fail( HYDRO_E_DEVICE, "queue reset launch failed" );
    }
    return enqueue( a, a->in, a->out, a->flux, a->dt_patch, a->npg, (hipStream_t)stream, &B, grid );
+}
+
+int hydro_mhm_solve_batch_noalloc( const HydroSolveArgs *a, const double *pg_dt_dh, int queue_slot,
+                                   int groups_per_cu, int pgs_per_ticket, void *stream )
+{
+   if ( queue_slot < 0 || queue_slot >= HYDRO_NOALLOC_QUEUES )
+      return fail( HYDRO_E_ARG, "batch_noalloc: queue_slot %d outside 0..%d", queue_slot, HYDRO_NOALLOC_QUEUES - 1 );
+   unsigned *q = nullptr;
+   const int rc = scratch_ptr( &q );
+   if ( rc != HYDRO_OK ) return rc;
+   // the slot's counter is reset by a kernel on `stream` before the solve (no memset, no allocation)
+   return hydro_mhm_solve_batch_async( a, pg_dt_dh, q + queue_slot, groups_per_cu, pgs_per_ticket, stream );
 }
 
 int hydro_synchronize( void )

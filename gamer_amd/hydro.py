@@ -36,7 +36,7 @@ EXPORTS = ("hydro_init", "hydro_mhm_solve_async", "hydro_synchronize", "hydro_la
            "hydro_store_flux_async", "hydro_correct_flux_async", "hydro_fixup_flux_async",
            "hydro_fixup_restrict_async", "hydro_close_patches_async", "hydro_interpolate_async",
            "hydro_prepare_patches_async", "hydro_gather_boxes_async", "hydro_assemble_patch_groups_async",
-           "hydro_host_dt_buffers", "hydro_asyn_dt_solver", "hydro_device_info")
+           "hydro_host_dt_buffers", "hydro_asyn_dt_solver", "hydro_device_info", "hydro_mhm_solve_batch_noalloc")
 
 
 class HydroError(RuntimeError):
@@ -126,6 +126,8 @@ def lib():
         L.hydro_close_uniform_async.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 4 + [ctypes.c_void_p]
         L.hydro_mhm_solve_batch_async.argtypes = [ctypes.POINTER(HydroSolveArgs), ctypes.c_void_p, ctypes.c_void_p,
                                                   ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+        L.hydro_mhm_solve_batch_noalloc.argtypes = [ctypes.POINTER(HydroSolveArgs), ctypes.c_void_p, ctypes.c_int,
+                                                    ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
         L.hydro_mhm_adjoint_workspace_bytes.argtypes = [ctypes.c_int]
         L.hydro_mhm_adjoint_workspace_bytes.restype = ctypes.c_size_t
         L.hydro_mhm_adjoint_async.argtypes = [ctypes.POINTER(HydroSolveArgs)] + [ctypes.c_void_p] * 4

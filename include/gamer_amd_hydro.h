@@ -78,6 +78,20 @@ int hydro_mhm_solve_async( const HydroSolveArgs *args, void *stream );
 int hydro_mhm_solve_batch_async( const HydroSolveArgs *args, const double *pg_dt_dh, unsigned *queue_counter,
                                  int groups_per_cu, int pgs_per_ticket, void *stream );
 
+/* Number of work-queue counters the library keeps in its own device memory for hydro_mhm_solve_batch_noalloc. */
+#define HYDRO_NOALLOC_QUEUES 64
+
+/* hydro_mhm_solve_batch_async in its persistent mode without a caller-allocated counter: the work-queue
+   counter is word `queue_slot` (0..HYDRO_NOALLOC_QUEUES-1; anything else returns HYDRO_E_ARG) of a fixed
+   256-byte device array that the library holds on every GPU it runs on until the process ends.  The call
+   allocates nothing (no hipMalloc, no library allocation); the array's address is looked up by hydro_init (or
+   by the first call on a device).  The counter is reset by a kernel on `stream` before every solve, so the
+   slot keeps no state between calls; two calls that overlap on one GPU with the same slot race, calls on
+   different slots do not.  No counterpart in the reference: it is for callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops). */
+int hydro_mhm_solve_batch_noalloc( const HydroSolveArgs *args, const double *pg_dt_dh, int queue_slot,
+                                   int groups_per_cu, int pgs_per_ticket, void *stream );
+
 /* Per-patch hydro CFL time step on device arrays: in[npatch][5][8][8][8] -> dt[npatch],
    dt = safety*dh / max_cells max_d(|v_d| + c_s) (replaces CUAPI_Asyn_dtSolver for DT_FLU_SOLVER,
    src/GPU_API/CUAPI_Asyn_dtSolver.cu:88, kernel src/Model_Hydro/CPU_Hydro/CPU_dtSolver_HydroCFL.cpp:57-209). */
