@@ -93,6 +93,9 @@ int check_args( const HydroSolveArgs *a )
                                                              return fail( HYDRO_E_ARG, "unsupported precision %d", a->precision );
    if ( a->npg > ( 1 << 26 ) )                               return fail( HYDRO_E_ARG, "npg too large" );
    if ( a->dt_patch != nullptr && !( a->safety > 0.0 ) )     return fail( HYDRO_E_ARG, "fused dt needs safety > 0" );
+   const int w = a->waves_per_group;
+   if ( w != 0 && w != 4 && ( w != 8 || a->scheme != HYDRO_MHM_RP ) )
+                                                             return fail( HYDRO_E_ARG, "waves_per_group %d: 0, 4 or 8 (MHM: 0 or 4)", w );
    return HYDRO_OK;
 }
 
@@ -122,9 +125,10 @@ Prm<R> make_prm( const HydroSolveArgs *a )
 int enqueue( const HydroSolveArgs *a, const void *in, void *out, void *flux, void *dt, int npg, hipStream_t s,
              const Batch *batch = nullptr, int grid_blocks = 0 )
 {
-   Batch nb;
-   nb.pg_dt_dh = nullptr;  nb.counter = nullptr;  nb.safety = 0.0;  nb.claim = 1;
-   const Batch &B = batch ? *batch : nb;
+   Batch B;
+   if ( batch ) B = *batch;
+   else { B.pg_dt_dh = nullptr;  B.counter = nullptr;  B.safety = 0.0;  B.claim = 1; }
+   B.waves = ( a->waves_per_group == 4 ) ? 4 : 8;
    int rc;
    const bool rp = ( a->scheme == HYDRO_MHM_RP );
    if ( a->precision == HYDRO_FP64 )

@@ -90,12 +90,13 @@ struct Prm {
 // Multi-level batch description: optional per-PG (dt, dh) pairs in device memory (uneven AMR batches, local
 // time steps) and an optional global work-queue counter for the persistent mode (each workgroup pulls tickets of
 // `claim` consecutive patch groups until the queue is exhausted).  pg_dt_dh == nullptr: every PG uses Prm's dt/dh;
-// counter == nullptr: one PG per workgroup.
+// counter == nullptr: one PG per workgroup.  `waves` selects the MHM_RP kernel's workgroup shape.
 struct Batch {
    const double *pg_dt_dh;
    unsigned     *counter;
    double        safety;
    int           claim;
+   int           waves;         // MHM_RP launch shape: 4 or 8 waves per workgroup (host side only)
 };
 
 // an interface state rotated to the face normal: conserved fields + derived quantities

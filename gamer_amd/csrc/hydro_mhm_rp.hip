@@ -31,6 +31,8 @@
 // priority inside it.  All branches that contain a barrier depend only on t, so every barrier is reached by the
 // whole workgroup.
 
+#include <type_traits>
+
 #include "hydro_math.hpp"
 #include "hydro_kernels.hpp"
 
@@ -41,12 +43,18 @@ namespace {
 constexpr int NX  = 22;          // FLU_NXT for MHM_RP
 constexpr int NC  = NX*NX;       // columns per plane
 constexpr int PS  = 16;          // PS2
-constexpr int NT  = 256;         // lanes per workgroup
+constexpr int NK  = 256;         // output columns (lanes of waves 0-3)
 #ifndef WPE32
-#define WPE32 2                  // target waves per SIMD (register budget) of the FP32-bulk kernels
+#define WPE32 2                  // target waves per SIMD (register budget) of the 4-wave FP32-bulk kernels
 #endif
 #ifndef WPE64
-#define WPE64 1                  // ... and of the FP64 kernels
+#define WPE64 1                  // ... and of the 4-wave FP64 kernels
+#endif
+#ifndef WPE32_8
+#define WPE32_8 4                // ... of the 8-wave FP32-bulk kernels (two workgroups per CU)
+#endif
+#ifndef WPE64_8
+#define WPE64_8 2                // ... of the 8-wave FP64 kernels
 #endif
 
 
@@ -231,30 +239,39 @@ struct SmemRP {
    R f[2*NV][NC];               // upper x (0-4) and y (5-9) face states of the reconstructed plane
    R b[2*NV][NC];               // half-step or full-step x (0-4) and y (5-9) fluxes
    R w[NV][NC];                 // half-step primitive plane read by the reconstruction
-   R u3[2][NV][NT];             // lane-private ring: input cells of the own output column, planes t-3 and t-2
+   R u3[3][NV][NK];             // lane-private ring: input cells of the own output column, planes t-3 .. t-1
    int fail, next;
    R cfl[8][4];       // fused CFL: per-patch maxima of the 4 waves (R is the I/O type)
 };
 
-template <typename R, typename IO, int RS, int LIM, bool PERSIST>
-__global__ void __launch_bounds__( NT ) __attribute__(( amdgpu_waves_per_eu( sizeof(R) == 4 ? WPE32 : WPE64 ) ))
+// NW = 4: every lane owns an output column and one ghost role (waves 0-1 arms, wave 2 single faces, wave 3 edge
+// columns), both marched in one loop.  NW = 8: waves 0-3 own the output columns only and waves 4-7 the ghost
+// roles only (in the same wave order), each half in a loop of its own, so a lane keeps only its own half's z
+// history in registers and two waves share each SIMD.  Both layouts run the same per-cell arithmetic in the
+// same order, so their results are bitwise equal.
+template <typename R, typename IO, int RS, int LIM, bool PERSIST, int NW>
+__global__ void __launch_bounds__( 64*NW )
+__attribute__(( amdgpu_waves_per_eu( NW == 8 ? ( sizeof(R) == 4 ? WPE32_8 : WPE64_8 ) : ( sizeof(R) == 4 ? WPE32 : WPE64 ) ) ))
 mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restrict__ g_flux,
                int npg, int store_flux, int minmod_maxiter, R minmod_coeff, Prm<R> P0,
                IO *__restrict__ g_dt, double dh_safety0, Batch B )
 {
    static_assert( sizeof( R ) == sizeof( IO ), "the input ring holds IO values in R storage" );
+   static_assert( NW == 4 || NW == 8, "4 or 8 waves per workgroup" );
    __shared__ SmemRP<R> S;
 
-   const int tid = threadIdx.x;
+   const int tid  = threadIdx.x;
+   const int wave = __builtin_amdgcn_readfirstlane( tid >> 6 );   // wave-uniform role switch
+   const int g    = ( NW == 4 ) ? tid : tid - 256;                // ghost-role index (0..255 on the ghost waves)
+   const int gw   = ( NW == 4 ) ? wave : wave - 4;                // ghost wave: 0-1 arms, 2 single faces, 3 edges
 
    const Col K = make_col( 3 + tid % PS, 3 + output_row( tid ) );
-   const int wave = __builtin_amdgcn_readfirstlane( tid >> 6 );   // wave-uniform role switch
    int oi, oj;
-   arm_column( tid, oi, oj );
+   arm_column( g, oi, oj );
    const Col O = make_col( oi, oj );
    int ei, ej;
    bool e_dy, e_face;
-   const bool has_e = edge_column( tid, ei, ej, e_dy, e_face );
+   const bool has_e = edge_column( g, ei, ej, e_dy, e_face );
    const int  e_c  = ei + NX*ej;
    const int  e_st = e_dy ? NX : 1;             // neighbour stride of the edge column's direction
    const int  e_vo = e_dy ? NV : 0;             // its face states / fluxes in S.f / S.b
@@ -317,257 +334,282 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
       if ( tid == 0 ) S.fail = 0;
       IO cfl[2] = { (IO)0, (IO)0 };            // fused CFL: max(|v_d|+c_s) of this column in patches z=0,1
 
-      // lane state carried along z (output column K, half-step ghost column O)
-      R kU[8], kU1[8];                            // input cell of planes t and t-1 (+ pressure, 1/rho, c_s)
-      R kHz[NV], kDh[NV];                         // half-step z-flux at t-3/2 ; half-step dFx+dFy of plane t-1
-      R kW1[NV], kW2[NV], kW3[NV];                // half-step primitives of planes t-1, t-2, t-3
-      R kFzR[NV], kFz[NV], kDf[NV];               // upper z face of plane r-1 ; full z-flux at r-3/2 ; full dFx+dFy of plane r-1
-      R oU[8], oU1[8], oHz[NV], oDh[NV], oW[NV];  // half-step ghost column
-
-      // prefetch of the next input plane (two cells per lane, coalesced)
-      IO pf[2][NV];
-      auto issue = [&]( int t )
+      // the z-march; DK: this lane runs its output column, DG: its ghost role (both compile-time)
+      auto march = [&]( auto DK, auto DG )
       {
-#        pragma unroll
-         for (int k=0; k<2; k++)
+         constexpr bool doK = decltype( DK )::value, doG = decltype( DG )::value;
+         // lane state carried along z (output column K, half-step arm column O)
+         R kU[8], kU1[8];                            // input cell of planes t and t-1 (+ pressure, 1/rho, c_s)
+         R kHz[NV], kDh[NV];                         // half-step z-flux at t-3/2 ; half-step dFx+dFy of plane t-1
+         R kW1[NV], kW2[NV], kW3[NV];                // half-step primitives of planes t-1, t-2, t-3
+         R kFzR[NV], kFz[NV], kDf[NV];               // upper z face of plane r-1 ; full z-flux at r-3/2 ; full dFx+dFy of plane r-1
+         R oU[8], oU1[8], oHz[NV], oDh[NV], oW[NV];  // arm column
+
+         // prefetch of the next input plane by the ghost-role lanes (two cells per lane, coalesced)
+         IO pf[2][NV];
+         auto issue = [&]( int t )
          {
-            const int c = tid + k*NT;
-            if ( c < NC )
+#           pragma unroll
+            for (int k=0; k<2; k++)
+            {
+               const int c = g + k*256;
+               if ( c < NC )
+#                 pragma unroll
+                  for (int v=0; v<NV; v++) buf_load( b_in, c*SZ, ( v*NC*NX + t*NC )*SZ, pf[k][v] );
+            }
+         };
+         // input plane (prefetched) -> LDS, with pressure, 1/rho and c_s once per cell
+         auto stage = [&]()
+         {
+#           pragma unroll
+            for (int k=0; k<2; k++)
+            {
+               const int c = g + k*256;
+               if ( c < NC )
+               {
+                  R q[8];
+                  derive_cell( pf[k], q, Q );
+#                 pragma unroll
+                  for (int v=0; v<8; v++) S.a[v][c] = q[v];
+               }
+            }
+         };
+         __syncthreads();                            // the previous patch group / iteration is done with S.a
+         if constexpr ( doG ) { issue( 0 );  stage();  issue( 1 ); }
+
+         // two steps per iteration let the compiler rename the plane-to-plane register hand-offs (kU1 = kU, the
+         // z-flux and face carries) instead of copying them; the Roe kernels stay rolled, their code is twice as large
+#        pragma unroll ( RS != 2 ? 2 : 1 )
+         for (int t = 0; t < NX; t++)
+         {
+            const int o = t - 3;                    // output plane of this step
+            const int r = t - 2;                    // reconstructed plane of this step
+            const int ring = t % 3;                 // u3 slot of input plane t-3 (read at the update); plane t-1
+                                                    // goes into slot (t+2) % 3 as soon as it leaves S.a
+            __syncthreads();                        // [A] input plane t visible
+            // the longest work of this interval issues first: with 4 waves the arm waves 0-1 (output column and
+            // arm half step), with 8 the output-column waves 0-3; the others issue behind them
+            if ( NW == 4 ? wave < 2 : wave < 4 ) __builtin_amdgcn_s_setprio( 3 ); else __builtin_amdgcn_s_setprio( 1 );
+            // arm half step: cells of planes 3..18 (updated at steps 4..19), their z faces (steps 3..19) and their
+            // low x/y faces (planes 3..18)
+            const bool arm_ld = doG && ( gw < 2 ) && t >= 2 && t <= 19;
+            const bool arm_z  = doG && ( gw < 2 ) && t >= 3 && t <= 19;
+            const bool arm_xy = doG && ( gw < 2 ) && t >= 3 && t <= 18;
+
+            if constexpr ( doK )
+            {
 #              pragma unroll
-               for (int v=0; v<NV; v++) buf_load( b_in, c*SZ, ( v*NC*NX + t*NC )*SZ, pf[k][v] );
-         }
-      };
-      // input plane (prefetched) -> LDS, with pressure, 1/rho and c_s once per cell
-      auto stage = [&]()
-      {
-#        pragma unroll
-         for (int k=0; k<2; k++)
-         {
-            const int c = tid + k*NT;
-            if ( c < NC )
-            {
-               R q[8];
-               derive_cell( pf[k], q, Q );
+               for (int v=0; v<8; v++) kU1[v] = kU[v];
+               ld_cell( S.a, K.c, kU );
 #              pragma unroll
-               for (int v=0; v<8; v++) S.a[v][c] = q[v];
+               for (int v=0; v<NV; v++) S.u3[( t + 2 ) % 3][v][tid] = kU1[v];     // read back at step t+2
             }
-         }
-      };
-      __syncthreads();                            // the previous patch group / iteration is done with S.a
-      issue( 0 );
-      stage();
-      issue( 1 );
-
-      // two steps per iteration let the compiler rename the plane-to-plane register hand-offs (kU1 = kU, the
-      // z-flux and face carries) instead of copying them; the Roe kernels stay rolled, their code is twice as large
-#     pragma unroll ( RS != 2 ? 2 : 1 )
-      for (int t = 0; t < NX; t++)
-      {
-         const int o = t - 3;                    // output plane of this step
-         const int ring = ( t + 1 ) & 1;         // u3 slot: input plane t-3 (read), then plane t-1 (written)
-         __syncthreads();                        // [A] input plane t visible
-         // waves 0-1 carry the arm half step of this interval and set its length: with two workgroups per CU
-         // sharing the SIMDs, letting them issue ahead of waves 2-3 (and of waves in other intervals) shortens it
-         if ( wave < 2 ) __builtin_amdgcn_s_setprio( 3 ); else __builtin_amdgcn_s_setprio( 1 );
-         // arm half step: cells of planes 3..18 (updated at steps 4..19), their z faces (steps 3..19) and their
-         // low x/y faces (planes 3..18)
-         const bool arm_ld = ( wave < 2 ) && t >= 2 && t <= 19;
-         const bool arm_z  = ( wave < 2 ) && t >= 3 && t <= 19;
-         const bool arm_xy = ( wave < 2 ) && t >= 3 && t <= 18;
-
-#        pragma unroll
-         for (int v=0; v<8; v++) { kU1[v] = kU[v];  oU1[v] = oU[v]; }
-         ld_cell( S.a, K.c, kU );
-         if ( arm_ld ) ld_cell( S.a, O.c, oU );
-
-         // ---- half-step z-flux at t-1/2 and half-step cell update of plane t-1
-         if ( t >= 1 )
-         {
-            R fz[NV], w[NV];
-            riemann<RS, 2>( st_of<2>( kU1 ), st_of<2>( kU ), Q, fz );
-            if ( t >= 2 )
+            if constexpr ( doG )
             {
-               half_cell( kU1, kDh, kHz, fz, Q, w );
 #              pragma unroll
-               for (int v=0; v<NV; v++) { kW3[v] = kW2[v];  kW2[v] = kW1[v];  kW1[v] = w[v]; }
+               for (int v=0; v<8; v++) oU1[v] = oU[v];
+               if ( arm_ld ) ld_cell( S.a, O.c, oU );
             }
-#           pragma unroll
-            for (int v=0; v<NV; v++) kHz[v] = fz[v];
-         }
-         if ( arm_z )
-         {
-            R fz[NV];
-            riemann<RS, 2>( st_of<2>( oU1 ), st_of<2>( oU ), Q, fz );
-            if ( t >= 4 ) half_cell( oU1, oDh, oHz, fz, Q, oW );
-#           pragma unroll
-            for (int v=0; v<NV; v++) oHz[v] = fz[v];
-         }
 
-         // ---- half-step x and y fluxes of plane t (low face owned by each column)
-         R fkx[NV], fox[NV], fky[NV], foy[NV];
-         if ( t >= 1 && t <= 20 )
-         {
-            riemann<RS, 0>( st_of<0>( S.a, K.c - 1 ), st_of<0>( kU ), Q, fkx );
-            riemann<RS, 1>( st_of<1>( S.a, K.c - NX ), st_of<1>( kU ), Q, fky );
-#           pragma unroll
-            for (int v=0; v<NV; v++)
+            // ---- half-step z-flux at t-1/2 and half-step cell update of plane t-1
+            if ( doK && t >= 1 )
             {
-               S.b[v][K.c] = fkx[v];
-               S.b[NV + v][K.c] = fky[v];
-            }
-         }
-         if ( arm_xy )
-         {
-            riemann<RS, 0>( st_of<0>( S.a, O.c - 1 ), st_of<0>( oU ), Q, fox );
-            riemann<RS, 1>( st_of<1>( S.a, O.c - NX ), st_of<1>( oU ), Q, foy );
-#           pragma unroll
-            for (int v=0; v<NV; v++)
-            {
-               S.b[v][O.c] = fox[v];
-               S.b[NV + v][O.c] = foy[v];
-            }
-         }
-         if ( wave == 2 && t >= 1 && t <= 20 )
-         {
-            int fc;
-            bool fdy;
-            if ( face_task( tid, t, fc, fdy ) )
-            {
-               R ql[8], qr[8], ff[NV];
-               ld_cell( S.a, fc - ( fdy ? NX : 1 ), ql );
-               ld_cell( S.a, fc, qr );
-               riemann_xy<RS>( cell_state_xy( ql, fdy ), cell_state_xy( qr, fdy ), Q, fdy, ff );
+               R fz[NV], w[NV];
+               riemann<RS, 2>( st_of<2>( kU1 ), st_of<2>( kU ), Q, fz );
+               if ( t >= 2 )
+               {
+                  half_cell( kU1, kDh, kHz, fz, Q, w );
+#                 pragma unroll
+                  for (int v=0; v<NV; v++) { kW3[v] = kW2[v];  kW2[v] = kW1[v];  kW1[v] = w[v]; }
+               }
 #              pragma unroll
-               for (int v=0; v<NV; v++) S.b[( fdy ? NV : 0 ) + v][fc] = ff[v];
+               for (int v=0; v<NV; v++) kHz[v] = fz[v];
             }
-         }
-
-         // ---- PLM reconstruction of plane r = t-2 (x/y upper faces to LDS, z faces stay in the lane); the
-         //      x/y neighbours come from the half-step plane r in S.w
-         const int r = t - 2;
-         R kFL[3][NV], kFRz[NV], eFL[NV];
-         if ( r >= 2 && r <= 19 )
-         {
-#           pragma unroll
-            for (int v=0; v<NV; v++) plm_face<LIM>( kW3[v], kW2[v], kW1[v], lim, Q.coeff, kFL[2][v], kFRz[v] );
-            if ( r >= 3 && r <= 18 )
+            if ( arm_z )
             {
+               R fz[NV];
+               riemann<RS, 2>( st_of<2>( oU1 ), st_of<2>( oU ), Q, fz );
+               if ( t >= 4 ) half_cell( oU1, oDh, oHz, fz, Q, oW );
+#              pragma unroll
+               for (int v=0; v<NV; v++) oHz[v] = fz[v];
+            }
+
+            // ---- half-step x and y fluxes of plane t (low face owned by each column)
+            R fkx[NV], fox[NV], fky[NV], foy[NV];
+            if ( doK && t >= 1 && t <= 20 )
+            {
+               riemann<RS, 0>( st_of<0>( S.a, K.c - 1 ), st_of<0>( kU ), Q, fkx );
+               riemann<RS, 1>( st_of<1>( S.a, K.c - NX ), st_of<1>( kU ), Q, fky );
 #              pragma unroll
                for (int v=0; v<NV; v++)
                {
-                  R fr;
-                  plm_face<LIM>( S.w[v][K.c - 1], kW2[v], S.w[v][K.c + 1], lim, Q.coeff, kFL[0][v], fr );
-                  S.f[v][K.c] = fr;
-                  plm_face<LIM>( S.w[v][K.c - NX], kW2[v], S.w[v][K.c + NX], lim, Q.coeff, kFL[1][v], fr );
-                  S.f[NV + v][K.c] = fr;
+                  S.b[v][K.c] = fkx[v];
+                  S.b[NV + v][K.c] = fky[v];
                }
-               if ( has_e )
+            }
+            if ( arm_xy )
+            {
+               riemann<RS, 0>( st_of<0>( S.a, O.c - 1 ), st_of<0>( oU ), Q, fox );
+               riemann<RS, 1>( st_of<1>( S.a, O.c - NX ), st_of<1>( oU ), Q, foy );
+#              pragma unroll
+               for (int v=0; v<NV; v++)
                {
+                  S.b[v][O.c] = fox[v];
+                  S.b[NV + v][O.c] = foy[v];
+               }
+            }
+            if ( doG && gw == 2 && t >= 1 && t <= 20 )
+            {
+               int fc;
+               bool fdy;
+               if ( face_task( g, t, fc, fdy ) )
+               {
+                  R ql[8], qr[8], ff[NV];
+                  ld_cell( S.a, fc - ( fdy ? NX : 1 ), ql );
+                  ld_cell( S.a, fc, qr );
+                  riemann_xy<RS>( cell_state_xy( ql, fdy ), cell_state_xy( qr, fdy ), Q, fdy, ff );
+#                 pragma unroll
+                  for (int v=0; v<NV; v++) S.b[( fdy ? NV : 0 ) + v][fc] = ff[v];
+               }
+            }
+
+            // ---- PLM reconstruction of plane r (x/y upper faces to LDS, z faces stay in the lane); the x/y
+            //      neighbours come from the half-step plane r in S.w
+            R kFL[3][NV], kFRz[NV], eFL[NV];
+            if ( doK && r >= 2 && r <= 19 )
+            {
+#              pragma unroll
+               for (int v=0; v<NV; v++) plm_face<LIM>( kW3[v], kW2[v], kW1[v], lim, Q.coeff, kFL[2][v], kFRz[v] );
+               if ( r >= 3 && r <= 18 )
 #                 pragma unroll
                   for (int v=0; v<NV; v++)
                   {
                      R fr;
-                     plm_face<LIM>( S.w[v][e_c - e_st], S.w[v][e_c], S.w[v][e_c + e_st], lim, Q.coeff, eFL[v], fr );
-                     S.f[e_vo + v][e_c] = fr;
+                     plm_face<LIM>( S.w[v][K.c - 1], kW2[v], S.w[v][K.c + 1], lim, Q.coeff, kFL[0][v], fr );
+                     S.f[v][K.c] = fr;
+                     plm_face<LIM>( S.w[v][K.c - NX], kW2[v], S.w[v][K.c + NX], lim, Q.coeff, kFL[1][v], fr );
+                     S.f[NV + v][K.c] = fr;
                   }
-               }
             }
-         }
-         __syncthreads();                        // [H] half-step x/y fluxes visible; S.w plane r no longer read
-         __builtin_amdgcn_s_setprio( 0 );
-
-         if ( t >= 1 && t <= 20 )
-#           pragma unroll
-            for (int v=0; v<NV; v++) kDh[v] = ( S.b[v][K.c + 1] - fkx[v] ) + ( S.b[NV + v][K.c + NX] - fky[v] );
-         if ( arm_xy )
-#           pragma unroll
-            for (int v=0; v<NV; v++) oDh[v] = ( S.b[v][O.c + 1] - fox[v] ) + ( S.b[NV + v][O.c + NX] - foy[v] );
-         // the half-step plane t-1 replaces plane t-2 in LDS for the next reconstruction
-         if ( t >= 2 )
-#           pragma unroll
-            for (int v=0; v<NV; v++) S.w[v][K.c] = kW1[v];
-         if ( arm_z && t >= 4 )
-#           pragma unroll
-            for (int v=0; v<NV; v++) S.w[v][O.c] = oW[v];
-
-         // ---- full-step fluxes: z at r-1/2 (lane-local), x and y of plane r through LDS
-         R fzn[NV];
-         if ( r >= 3 && r <= 19 )
-         {
-            riemann<RS, 2>( face_state<2>( kFzR, Q ), face_state<2>( kFL[2], Q ), Q, fzn );
-            if ( flx && ( r - 3 ) % 8 == 0 )
+            if ( doG && has_e && r >= 3 && r <= 18 )
 #              pragma unroll
                for (int v=0; v<NV; v++)
-                  buf_store( (IO)fzn[v], b_flx, kxy*SZ, ( ( 6 + ( r - 3 )/8 )*NV + v )*PS*PS*SZ );
-         }
-         if ( r >= 2 )
-#           pragma unroll
-            for (int v=0; v<NV; v++) kFzR[v] = kFRz[v];
+               {
+                  R fr;
+                  plm_face<LIM>( S.w[v][e_c - e_st], S.w[v][e_c], S.w[v][e_c + e_st], lim, Q.coeff, eFL[v], fr );
+                  S.f[e_vo + v][e_c] = fr;
+               }
+            __syncthreads();                        // [H] half-step x/y fluxes visible; S.w plane r no longer read
+            __builtin_amdgcn_s_setprio( 0 );
 
-         R dfn[NV];
-         if ( r >= 3 && r <= 18 )
-         {
-            __syncthreads();                     // [R] the half-step fluxes in S.b are read
-            R ffx[NV], ffy[NV], fe[NV];
-            riemann<RS, 0>( st_face<0>( S.f, 0, K.c - 1, Q ), face_state<0>( kFL[0], Q ), Q, ffx );
-            riemann<RS, 1>( st_face<1>( S.f, NV, K.c - NX, Q ), face_state<1>( kFL[1], Q ), Q, ffy );
-            if ( has_e && e_face )
-               riemann_xy<RS>( st_face_xy( S.f, e_vo, e_c - e_st, e_dy, Q ), face_state_xy( eFL, e_dy, Q ), Q, e_dy, fe );
-#           pragma unroll
-            for (int v=0; v<NV; v++)
+            // flux differences; the lane's own low-face fluxes are read back from LDS rather than held in registers
+            // across the reconstruction (the stored values, so both launch shapes round alike)
+            if ( doK && t >= 1 && t <= 20 )
+#              pragma unroll
+               for (int v=0; v<NV; v++) kDh[v] = ( S.b[v][K.c + 1] - S.b[v][K.c] ) + ( S.b[NV + v][K.c + NX] - S.b[NV + v][K.c] );
+            if ( arm_xy )
+#              pragma unroll
+               for (int v=0; v<NV; v++) oDh[v] = ( S.b[v][O.c + 1] - S.b[v][O.c] ) + ( S.b[NV + v][O.c + NX] - S.b[NV + v][O.c] );
+            // the half-step plane t-1 replaces plane t-2 in LDS for the next reconstruction
+            if ( doK && t >= 2 )
+#              pragma unroll
+               for (int v=0; v<NV; v++) S.w[v][K.c] = kW1[v];
+            if ( arm_z && t >= 4 )
+#              pragma unroll
+               for (int v=0; v<NV; v++) S.w[v][O.c] = oW[v];
+
+            // ---- full-step fluxes: z at r-1/2 (lane-local), x and y of plane r through LDS
+            R fzn[NV];
+            if ( doK && r >= 3 && r <= 19 )
             {
-               S.b[v][K.c] = ffx[v];
-               S.b[NV + v][K.c] = ffy[v];
-               if ( has_e && e_face ) S.b[e_vo + v][e_c] = fe[v];
+               riemann<RS, 2>( face_state<2>( kFzR, Q ), face_state<2>( kFL[2], Q ), Q, fzn );
+               if ( flx && ( r - 3 ) % 8 == 0 )
+#                 pragma unroll
+                  for (int v=0; v<NV; v++)
+                     buf_store( (IO)fzn[v], b_flx, kxy*SZ, ( ( 6 + ( r - 3 )/8 )*NV + v )*PS*PS*SZ );
             }
-            if ( flx )
+            if ( doK && r >= 2 )
+#              pragma unroll
+               for (int v=0; v<NV; v++) kFzR[v] = kFRz[v];
+
+            R dfn[NV], ffx[NV], ffy[NV];
+            if ( r >= 3 && r <= 18 )
             {
-               if ( ( K.i - 3 ) % 8 == 0 )
+               __syncthreads();                     // [R] the half-step fluxes in S.b are read
+               if constexpr ( doK )
+               {
+                  riemann<RS, 0>( st_face<0>( S.f, 0, K.c - 1, Q ), face_state<0>( kFL[0], Q ), Q, ffx );
+                  riemann<RS, 1>( st_face<1>( S.f, NV, K.c - NX, Q ), face_state<1>( kFL[1], Q ), Q, ffy );
 #                 pragma unroll
                   for (int v=0; v<NV; v++)
-                     buf_store( (IO)ffx[v], b_flx, fx_off, ( v*PS*PS + PS*( r - 3 ) )*SZ );
-               if ( ( K.j - 3 ) % 8 == 0 )
+                  {
+                     S.b[v][K.c] = ffx[v];
+                     S.b[NV + v][K.c] = ffy[v];
+                  }
+                  if ( flx )
+                  {
+                     if ( ( K.i - 3 ) % 8 == 0 )
+#                       pragma unroll
+                        for (int v=0; v<NV; v++)
+                           buf_store( (IO)ffx[v], b_flx, fx_off, ( v*PS*PS + PS*( r - 3 ) )*SZ );
+                     if ( ( K.j - 3 ) % 8 == 0 )
+#                       pragma unroll
+                        for (int v=0; v<NV; v++)
+                           buf_store( (IO)ffy[v], b_flx, fy_off, ( v*PS*PS + PS*( r - 3 ) )*SZ );
+                  }
+               }
+               if ( doG && has_e && e_face )
+               {
+                  R fe[NV];
+                  riemann_xy<RS>( st_face_xy( S.f, e_vo, e_c - e_st, e_dy, Q ), face_state_xy( eFL, e_dy, Q ), Q, e_dy, fe );
 #                 pragma unroll
-                  for (int v=0; v<NV; v++)
-                     buf_store( (IO)ffy[v], b_flx, fy_off, ( v*PS*PS + PS*( r - 3 ) )*SZ );
-               if ( has_e && e_face )             // planes x = 16 (2) and y = 16 (5)
+                  for (int v=0; v<NV; v++) S.b[e_vo + v][e_c] = fe[v];
+                  if ( flx )                        // planes x = 16 (2) and y = 16 (5)
+#                    pragma unroll
+                     for (int v=0; v<NV; v++)
+                        buf_store( (IO)fe[v], b_flx, fe_off, ( v*PS*PS + PS*( r - 3 ) )*SZ );
+               }
+               __syncthreads();                     // [F] full-step fluxes visible
+               if constexpr ( doK )
 #                 pragma unroll
-                  for (int v=0; v<NV; v++)
-                     buf_store( (IO)fe[v], b_flx, fe_off, ( v*PS*PS + PS*( r - 3 ) )*SZ );
+                  for (int v=0; v<NV; v++) dfn[v] = ( S.b[v][K.c + 1] - ffx[v] ) + ( S.b[NV + v][K.c + NX] - ffy[v] );
             }
-            __syncthreads();                     // [F] full-step fluxes visible
-#           pragma unroll
-            for (int v=0; v<NV; v++) dfn[v] = ( S.b[v][K.c + 1] - ffx[v] ) + ( S.b[NV + v][K.c + NX] - ffy[v] );
-         }
 
-         // ---- full-step update and output of plane o = t-3
-         if ( o >= 3 && o <= 18 )
-         {
-            IO res[NV], uo[NV];
-#           pragma unroll
-            for (int v=0; v<NV; v++) uo[v] = S.u3[ring][v][tid];
-            const bool bad = full_cell( uo, kDf, kFz, fzn, Q, res );
-#           pragma unroll
-            for (int v=0; v<NV; v++) buf_store( res[v], b_out, kxy*SZ, ( v*PS*PS*PS + PS*PS*( o - 3 ) )*SZ );
-            if ( bad ) S.fail = 1;
-            if ( g_dt ) cfl[ ( o - 3 )/8 ] = vmax( cfl[ ( o - 3 )/8 ], cell_cfl( res, (double)Q.gamma, Q.gm1_d, Q.min_pres_d ) );
-         }
-         if ( r >= 3 && r <= 19 )
-#           pragma unroll
-            for (int v=0; v<NV; v++) kFz[v] = fzn[v];
-         if ( r >= 3 && r <= 18 )
-#           pragma unroll
-            for (int v=0; v<NV; v++) kDf[v] = dfn[v];
-#        pragma unroll
-         for (int v=0; v<NV; v++) S.u3[ring][v][tid] = kU1[v];     // input plane t-1, read back at step t+2
+            if constexpr ( doK )
+            {
+               // ---- full-step update and output of plane o = t-3
+               if ( o >= 3 && o <= 18 )
+               {
+                  IO res[NV], uo[NV];
+#                 pragma unroll
+                  for (int v=0; v<NV; v++) uo[v] = S.u3[ring][v][tid];
+                  const bool bad = full_cell( uo, kDf, kFz, fzn, Q, res );
+#                 pragma unroll
+                  for (int v=0; v<NV; v++) buf_store( res[v], b_out, kxy*SZ, ( v*PS*PS*PS + PS*PS*( o - 3 ) )*SZ );
+                  if ( bad ) S.fail = 1;
+                  if ( g_dt ) cfl[ ( o - 3 )/8 ] = vmax( cfl[ ( o - 3 )/8 ], cell_cfl( res, (double)Q.gamma, Q.gm1_d, Q.min_pres_d ) );
+               }
+               if ( r >= 3 && r <= 19 )
+#                 pragma unroll
+                  for (int v=0; v<NV; v++) kFz[v] = fzn[v];
+               if ( r >= 3 && r <= 18 )
+#                 pragma unroll
+                  for (int v=0; v<NV; v++) kDf[v] = dfn[v];
+            }
 
-         // ---- input plane t+1 into LDS: every read of S.a of this step came before barrier [H]
-         if ( t + 1 < NX )
-         {
-            stage();
-            if ( t + 2 < NX ) issue( t + 2 );
-         }
-      } // for t
+            // ---- input plane t+1 into LDS: every read of S.a of this step came before barrier [H]
+            if ( doG && t + 1 < NX )
+            {
+               stage();
+               if ( t + 2 < NX ) issue( t + 2 );
+            }
+         } // for t
+      };
+      using T_ = std::integral_constant<bool, true>;
+      using F_ = std::integral_constant<bool, false>;
+      if constexpr ( NW == 4 ) march( T_(), T_() );
+      else if ( wave < 4 )     march( T_(), F_() );
+      else                     march( F_(), T_() );
 
       if ( g_dt ) fused_dt( cfl, ( K.i - 3 )/8 + 2*( ( K.j - 3 )/8 ), S.cfl, g_dt + (size_t)pg*8, dh_safety );
       __syncthreads();
@@ -584,12 +626,15 @@ template <typename R, typename IO, int RS>
 static void launch_rs( const IO *in, IO *out, IO *flux, int npg, int store_flux, int minmod_maxiter,
                        double minmod_coeff, const Prm<R> &P, IO *dt, double dh_safety, const Batch &B, int grid_blocks, hipStream_t stream )
 {
-   const dim3 grid( grid_blocks > 0 ? grid_blocks : npg ), block( NT );
+   const int nw = ( B.waves == 4 ) ? 4 : 8;
+   const dim3 grid( grid_blocks > 0 ? grid_blocks : npg ), block( 64*nw );
    const bool pers = ( B.counter != nullptr );
-#  define GA_LAUNCH( LIM_, PERS_ ) hipLaunchKernelGGL( ( mhm_rp_kernel<R, IO, RS, LIM_, PERS_> ), grid, block, 0, stream, \
-                                                      in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P, dt, dh_safety, B )
-   if ( P.limiter == 4 ) { if ( pers ) GA_LAUNCH( 4, true ); else GA_LAUNCH( 4, false ); }
-   else                  { if ( pers ) GA_LAUNCH( 0, true ); else GA_LAUNCH( 0, false ); }
+#  define GA_LAUNCH( LIM_, PERS_, NW_ ) hipLaunchKernelGGL( ( mhm_rp_kernel<R, IO, RS, LIM_, PERS_, NW_> ), grid, block, 0, stream, \
+                                                           in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P, dt, dh_safety, B )
+#  define GA_SHAPE( LIM_, PERS_ ) { if ( nw == 4 ) GA_LAUNCH( LIM_, PERS_, 4 ); else GA_LAUNCH( LIM_, PERS_, 8 ); }
+   if ( P.limiter == 4 ) { if ( pers ) GA_SHAPE( 4, true ) else GA_SHAPE( 4, false ) }
+   else                  { if ( pers ) GA_SHAPE( 0, true ) else GA_SHAPE( 0, false ) }
+#  undef GA_SHAPE
 #  undef GA_LAUNCH
 }
 

@@ -51,7 +51,7 @@ class HydroSolveArgs(ctypes.Structure):
         ("minmod_coeff", ctypes.c_double), ("minmod_maxiter", ctypes.c_int),
         ("min_dens", ctypes.c_double), ("min_pres", ctypes.c_double), ("min_eint", ctypes.c_double),
         ("store_flux", ctypes.c_int), ("precision", ctypes.c_int),
-        ("dt_patch", ctypes.c_void_p), ("safety", ctypes.c_double),
+        ("dt_patch", ctypes.c_void_p), ("safety", ctypes.c_double), ("waves_per_group", ctypes.c_int),
     ]
 
 
@@ -162,11 +162,12 @@ def _check(rc: int):
 
 def make_args(npg, dt, dh, gamma=5.0 / 3.0, scheme="MHM_RP", rsolver="HLLC", limiter="VL_GMINMOD",
               minmod_coeff=1.5, minmod_maxiter=0, min_dens=0.0, min_pres=0.0, min_eint=0.0, store_flux=True,
-              precision="fp64", in_=None, out=None, flux=None, dt_patch=None, safety=0.0) -> HydroSolveArgs:
+              precision="fp64", in_=None, out=None, flux=None, dt_patch=None, safety=0.0,
+              waves_per_group=0) -> HydroSolveArgs:
     return HydroSolveArgs(in_, out, flux, int(npg), float(dt), float(dh), float(gamma), SCHEME[scheme],
                           RSOLVER[rsolver], LIMITER[limiter], float(minmod_coeff), int(minmod_maxiter),
                           float(min_dens), float(min_pres), float(min_eint), int(bool(store_flux)),
-                          PRECISION[precision], dt_patch, float(safety))
+                          PRECISION[precision], dt_patch, float(safety), int(waves_per_group))
 
 
 def _stream(dev, stream, *tensors):
@@ -209,7 +210,7 @@ def _prec(t):
 
 def fluid_solver(u, dt, dh, gamma=5.0 / 3.0, scheme="MHM_RP", rsolver="HLLC", limiter="VL_GMINMOD",
                  minmod_coeff=1.5, minmod_maxiter=0, min_dens=0.0, min_pres=0.0, min_eint=0.0, store_flux=True,
-                 out=None, flux=None, stream=None, dt_patch=None, safety=0.5):
+                 out=None, flux=None, stream=None, dt_patch=None, safety=0.5, waves_per_group=0):
     """One patch-group solver call on device tensors.
 
     u: torch tensor on a ROCm device, float64 (FP64 path) or float32 (FP32-bulk path), shape (npg, 5, N, N, N).
@@ -238,7 +239,7 @@ def fluid_solver(u, dt, dh, gamma=5.0 / 3.0, scheme="MHM_RP", rsolver="HLLC", li
     args = make_args(npg, dt, dh, gamma, scheme, rsolver, limiter, minmod_coeff, minmod_maxiter, min_dens, min_pres,
                      min_eint, store_flux, prec, u.data_ptr(), out.data_ptr(),
                      flux.data_ptr() if store_flux else None,
-                     dt_patch.data_ptr() if dt_patch is not None else None, safety)
+                     dt_patch.data_ptr() if dt_patch is not None else None, safety, waves_per_group)
     _check(lib().hydro_mhm_solve_async(ctypes.byref(args),
                                        _stream(u.device, stream, u, out, flux if store_flux else None, dt_patch)))
     return out, (flux if store_flux else None)

@@ -57,6 +57,9 @@ typedef struct HydroSolveArgs {
    void  *dt_patch;        /* optional device pointer [npg][8]: fused CFL dt of the updated patches
                               (index px + 2 py + 4 pz within the PG); NULL = not computed */
    double safety;          /* CFL safety factor of the fused dt (DT__FLUID) */
+   int    waves_per_group; /* launch shape of the MHM_RP kernel: 0 = default (8), 4 or 8 waves per workgroup
+                              (results are bitwise equal); MHM accepts 0 or 4.  No counterpart in the reference,
+                              whose block size is a compile-time constant (include/CUFLU.h:302-371). */
 } HydroSolveArgs;
 
 /* Select the HIP device (replaces CUAPI_SetDevice, src/GPU_API/CUAPI_SetDevice.cu). */

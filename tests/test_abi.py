@@ -181,3 +181,10 @@ def test_wrappers_reject_host_tensors():
                lambda: hydro.dt_cfl(torch.zeros((1, 5, 8, 8, 8)), 1e-2, 0.5)):
         with pytest.raises(hydro.HydroError):
             fn()
+
+
+def test_launch_shape_rejected_before_device_work():
+    """waves_per_group: 0 (default), 4 or 8 for MHM_RP; 0 or 4 for MHM; anything else is HYDRO_E_ARG."""
+    for scheme, w in (("MHM_RP", 2), ("MHM_RP", 16), ("MHM", 8), ("MHM_RP", -4)):
+        rc = _call(npg=4, dt=1e-3, dh=1e-2, in_=16, out=16, flux=16, scheme=scheme, waves_per_group=w)
+        assert rc == -1 and "waves_per_group" in hydro.lib().hydro_last_error().decode(), (scheme, w)
