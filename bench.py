@@ -184,7 +184,9 @@ def amr_batch(args):
     print("[amr] autotune", file=sys.stderr, flush=True)
     tune = amr.autotune(torch, b, groups=(1, 2, 3, 4), tickets=(1, 2, 4), reps=reps)
     best = tune["best"]
-    gpc, tk = (2, 1) if best == "grid" else (int(best.split(":")[1]), int(best.split(":")[2]))
+    # the persistent and graph modes below run the best persistent configuration of the sweep
+    pbest = min((k for k in tune["trials_ms"] if k.startswith("persistent")), key=tune["trials_ms"].get)
+    _, gpc, tk, wpg = (int(x) if x.isdigit() else x for x in pbest.split(":"))
 
     idx = [np.nonzero(b.level == l)[0] for l in range(len(counts))]
     u_lv = [b.u[torch.from_numpy(i).to(dev)].contiguous() for i in idx]
@@ -199,11 +201,11 @@ def amr_batch(args):
     flux = torch.empty((npg, 9, 5, 16, 16), dtype=torch.float32, device=dev)
     cnt = torch.zeros(1, dtype=torch.int32, device=dev)
     print("[amr] timing", file=sys.stderr, flush=True)
-    g = amr.GraphSubstep(torch, b, groups_per_cu=gpc, pgs_per_ticket=tk, persistent=True)
+    g = amr.GraphSubstep(torch, b, groups_per_cu=gpc, pgs_per_ticket=tk, persistent=True, waves_per_group=wpg)
     fns = {"lv": per_level,
            "grid": lambda: hydro.solve_batch(b.u, b.pg_dt_dh, out=out, flux=flux),
            "pers": lambda: hydro.solve_batch(b.u, b.pg_dt_dh, out=out, flux=flux, queue_counter=cnt, groups_per_cu=gpc,
-                                             pgs_per_ticket=tk),
+                                             pgs_per_ticket=tk, waves_per_group=wpg),
            "graph": g.replay}
     best = {k: float("inf") for k in fns}
     for _ in range(3):                                   # interleaved rounds, best of each
@@ -220,7 +222,8 @@ def amr_batch(args):
             "unit": "cell-updates/s",
             "per_level_launches": {"ms": t_lv, "value": rate(t_lv)},
             "one_launch_grid": {"ms": t_grid, "value": rate(t_grid)},
-            "one_launch_persistent": {"ms": t_pers, "value": rate(t_pers), "groups_per_cu": gpc, "pgs_per_ticket": tk},
+            "one_launch_persistent": {"ms": t_pers, "value": rate(t_pers), "groups_per_cu": gpc, "pgs_per_ticket": tk,
+                                      "waves_per_group": wpg},
             "graph_replay": {"ms": t_graph, "value": rate(t_graph)},
             "autotune": tune}
     print(json.dumps(line))
@@ -341,6 +344,11 @@ def _field_scale(per_field, agg):
 
 
 VALU_PEAK_GINST = 1024 * 2.4 / 4       # 256 CUs x 4 SIMDs, 2.4 GHz, one wave64 VALU instruction per 4 cycles
+# the two profile-derived figures of the bench line are not measured in this run: they are read from the counter
+# profile of the same kernel build, and the line says so
+TRAFFIC_SOURCE = ("stored: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE of this kernel (profiles/traffic.json), "
+                  "not measured in this run")
+VALU_SOURCE = ("stored: rocprofv3 --pmc SQ_INSTS_VALU of this kernel (profiles/traffic.json), over this run's kernel time")
 
 
 def load_traffic(name):
@@ -431,13 +439,21 @@ def main():
                          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(prec),
                          "algorithmic_bytes_per_launch": bpl},
         }
+        if results[prec]["roofline"]["traffic"] is not None:
+            results[prec]["roofline"]["traffic_source"] = TRAFFIC_SOURCE
         valu = load_traffic("valu_" + prec)
         if valu:
             # the limit the kernel actually runs into: VALU issue, one wave64 instruction per SIMD every 4 cycles
             # at the nominal clock, with the profiled instruction count of one launch over this run's kernel time
             rate = valu / (kms * 1e-3) / 1e9
+            clk = load_traffic("clock_ghz_" + prec)
             results[prec]["valu_issue"] = {"achieved": round(rate, 1), "peak": VALU_PEAK_GINST, "frac": round(rate / VALU_PEAK_GINST, 4),
-                                           "unit": "G wave64 VALU instructions/s", "instructions_per_launch": valu}
+                                           "unit": "G wave64 VALU instructions/s", "instructions_per_launch": valu,
+                                           "instructions_source": VALU_SOURCE, "peak_clock_ghz": 2.4}
+            if clk:
+                # the same figure against the clock the profiled run measured (SQ_BUSY_CYCLES over kernel time)
+                results[prec]["valu_issue"]["frac_at_profiled_clock"] = round(rate / (1024 * clk / 4), 4)
+                results[prec]["valu_issue"]["profiled_clock_ghz"] = clk
         del out, flux
         torch.cuda.empty_cache()
         if not args.skip_host_arrays:

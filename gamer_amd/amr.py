@@ -120,26 +120,39 @@ def time_launch(torch, fn, reps=5, warmup=1):
 
 
 def launch_kwargs(torch, config, device):
-    """solve_batch keyword arguments of an autotuner configuration name ("grid" or "persistent:g:k")."""
-    if config == "grid":
-        return {}
-    _, g, k = config.split(":")
-    return dict(queue_counter=torch.zeros(1, dtype=torch.int32, device=device), groups_per_cu=int(g),
-                pgs_per_ticket=int(k))
+    """solve_batch keyword arguments of an autotuner configuration name: "grid[:w]" or "persistent:g:k[:w]", with
+    w the workgroup shape in waves (absent: the library default)."""
+    parts = config.split(":")
+    kw = {}
+    if parts[0] == "grid":
+        if len(parts) > 1:
+            kw["waves_per_group"] = int(parts[1])
+        return kw
+    _, g, k = parts[:3]
+    kw = dict(queue_counter=torch.zeros(1, dtype=torch.int32, device=device), groups_per_cu=int(g),
+              pgs_per_ticket=int(k))
+    if len(parts) > 3:
+        kw["waves_per_group"] = int(parts[3])
+    return kw
 
 
-def autotune(torch, batch: LevelBatch, groups=(1, 2, 3, 4), tickets=(1, 2, 4), reps=5, rounds=3, cache=None, **kw):
+def autotune(torch, batch: LevelBatch, groups=(1, 2, 3, 4), tickets=(1, 2, 4), waves=None, reps=5, rounds=3,
+             cache=None, **kw):
     """Measure every launch configuration of a batched substep and return the fastest.
 
-    Configurations: one workgroup per patch group ("grid"), and the persistent work queue ("persistent:g:k")
+    Configurations: one workgroup per patch group ("grid:w"), and the persistent work queue ("persistent:g:k:w")
     with g workgroups per compute unit for g in ``groups``, each taking k consecutive patch groups per queue
-    ticket for k in ``tickets`` (fewer queue atomics and barriers against a coarser tail).  The configurations are timed in ``rounds`` interleaved rounds of
+    ticket for k in ``tickets`` (fewer queue atomics and barriers against a coarser tail); each in every workgroup
+    shape w of ``waves`` (MHM_RP: 4 waves, every lane marching an output column and a ghost role, or 8 waves, the
+    two on separate waves; MHM: 4 only).  The configurations are timed in ``rounds`` interleaved rounds of
     ``reps`` launches and each keeps its best round, so that a clock or thermal drift during the sweep does not
     pick the winner.  Results are cached in ``cache`` (a JSON path) keyed by the batch shape and precision when
     given."""
     npg = batch.u.shape[0]
+    if waves is None:
+        waves = (4, 8) if kw.get("scheme", "MHM_RP") == "MHM_RP" else (4,)
     key = (f"{torch.cuda.get_device_name(0)}|npg={npg}|{batch.u.dtype}|{kw.get('scheme', 'MHM_RP')}|"
-           f"{kw.get('rsolver', 'HLLC')}|groups={tuple(groups)}|tickets={tuple(tickets)}")
+           f"{kw.get('rsolver', 'HLLC')}|groups={tuple(groups)}|tickets={tuple(tickets)}|waves={tuple(waves)}")
     if cache and os.path.exists(cache):
         try:
             with open(cache) as f:
@@ -151,12 +164,15 @@ def autotune(torch, batch: LevelBatch, groups=(1, 2, 3, 4), tickets=(1, 2, 4), r
     out = torch.empty((npg, 5, 16, 16, 16), dtype=batch.u.dtype, device=batch.u.device)
     flux = torch.empty((npg, 9, 5, 16, 16), dtype=batch.u.dtype, device=batch.u.device)
     counter = torch.zeros(1, dtype=torch.int32, device=batch.u.device)
-    configs = {"grid": lambda: hydro.solve_batch(batch.u, batch.pg_dt_dh, out=out, flux=flux, **kw)}
-    for g in groups:
-        for k in tickets:
-            configs[f"persistent:{g}:{k}"] = (lambda g=g, k=k: hydro.solve_batch(
-                batch.u, batch.pg_dt_dh, out=out, flux=flux, queue_counter=counter, groups_per_cu=g,
-                pgs_per_ticket=k, **kw))
+    configs = {}
+    for w in waves:
+        configs[f"grid:{w}"] = (lambda w=w: hydro.solve_batch(batch.u, batch.pg_dt_dh, out=out, flux=flux,
+                                                              waves_per_group=w, **kw))
+        for g in groups:
+            for k in tickets:
+                configs[f"persistent:{g}:{k}:{w}"] = (lambda g=g, k=k, w=w: hydro.solve_batch(
+                    batch.u, batch.pg_dt_dh, out=out, flux=flux, queue_counter=counter, groups_per_cu=g,
+                    pgs_per_ticket=k, waves_per_group=w, **kw))
     trials = {name: float("inf") for name in configs}
     for _ in range(max(1, rounds)):
         for name, fn in configs.items():

@@ -56,5 +56,38 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_host_sanitized(out_dir: str) -> str:
+    """The same library with its host code (C ABI, argument checks, buffer management) built with AddressSanitizer
+    and UndefinedBehaviorSanitizer, every report fatal; device code as in build() but without the scheduler
+    options (they do not change host code).  Written to out_dir/libgamer_amd_hydro.so, for the sanitizer test,
+    which loads it with the compiler's ASan runtime preloaded.  Never the shipped library."""
+    os.makedirs(out_dir, exist_ok=True)
+    san = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+           "-Xarch_host", "-fno-sanitize-recover=all"]
+    flags = [f for f in FLAGS if f != "-shared"] + ["-Xarch_host", "-O1", "-Xarch_host", "-g"] + san
+    objs, procs = [], []
+    for src in SOURCES:
+        obj = os.path.join(out_dir, src.replace(".hip", "").replace(":", "_") + ".o")
+        extra = [f for f in EXTRA.get(src, []) if f.startswith("-D")]
+        cmd = [HIPCC, *flags, *extra, "-c", os.path.join(CSRC, src.split(":")[0]), "-o", obj]
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        objs.append(obj)
+    for cmd, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError(f"hipcc failed: {' '.join(cmd)}\n{out}")
+    lib = os.path.join(out_dir, "libgamer_amd_hydro.so")
+    r = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *san[:4], "-shared-libsan", *objs,
+                        "-o", lib], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{r.stdout}")
+    for o in objs:
+        os.remove(o)
+    return lib
+
+
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    if "--sanitize-host" in sys.argv:
+        print(build_host_sanitized(sys.argv[sys.argv.index("--sanitize-host") + 1]))
+    else:
+        print(build(verbose="-v" in sys.argv))

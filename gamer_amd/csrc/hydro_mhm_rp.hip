@@ -378,8 +378,9 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
          if constexpr ( doG ) { issue( 0 );  stage();  issue( 1 ); }
 
          // two steps per iteration let the compiler rename the plane-to-plane register hand-offs (kU1 = kU, the
-         // z-flux and face carries) instead of copying them; the Roe kernels stay rolled, their code is twice as large
-#        pragma unroll ( RS != 2 ? 2 : 1 )
+         // z-flux and face carries) instead of copying them; the Roe kernels stay rolled, their code is twice as large,
+         // and so does the 8-wave FP32-bulk kernel, whose 128-register budget the unrolled schedule exceeds more
+#        pragma unroll ( RS != 2 && ( NW == 4 || sizeof( R ) == 8 ) ? 2 : 1 )
          for (int t = 0; t < NX; t++)
          {
             const int o = t - 3;                    // output plane of this step
@@ -532,12 +533,13 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
 #              pragma unroll
                for (int v=0; v<NV; v++) kFzR[v] = kFRz[v];
 
-            R dfn[NV], ffx[NV], ffy[NV];
+            R dfn[NV];
             if ( r >= 3 && r <= 18 )
             {
                __syncthreads();                     // [R] the half-step fluxes in S.b are read
                if constexpr ( doK )
                {
+                  R ffx[NV], ffy[NV];
                   riemann<RS, 0>( st_face<0>( S.f, 0, K.c - 1, Q ), face_state<0>( kFL[0], Q ), Q, ffx );
                   riemann<RS, 1>( st_face<1>( S.f, NV, K.c - NX, Q ), face_state<1>( kFL[1], Q ), Q, ffy );
 #                 pragma unroll
@@ -570,9 +572,9 @@ mhm_rp_kernel( const IO *__restrict__ g_in, IO *__restrict__ g_out, IO *__restri
                         buf_store( (IO)fe[v], b_flx, fe_off, ( v*PS*PS + PS*( r - 3 ) )*SZ );
                }
                __syncthreads();                     // [F] full-step fluxes visible
-               if constexpr ( doK )
+               if constexpr ( doK )                 // own fluxes read back, as for the half step
 #                 pragma unroll
-                  for (int v=0; v<NV; v++) dfn[v] = ( S.b[v][K.c + 1] - ffx[v] ) + ( S.b[NV + v][K.c + NX] - ffy[v] );
+                  for (int v=0; v<NV; v++) dfn[v] = ( S.b[v][K.c + 1] - S.b[v][K.c] ) + ( S.b[NV + v][K.c + NX] - S.b[NV + v][K.c] );
             }
 
             if constexpr ( doK )

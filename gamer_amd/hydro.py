@@ -18,7 +18,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgamer_amd_hydro.so")
+# GAMER_AMD_HYDRO_LIB: another build of the same library (the sanitizer test loads one whose host code is built
+# with ASan/UBSan); there is no other library and no CPU fallback
+LIB_PATH = os.environ.get("GAMER_AMD_HYDRO_LIB") or os.path.join(HERE, "libgamer_amd_hydro.so")
 
 HYDRO_OK = 0
 HYDRO_E_ARG, HYDRO_E_STATE, HYDRO_E_DEVICE, HYDRO_E_NOMEM = -1, -2, -3, -4

@@ -32,7 +32,8 @@ def lib():
     """The restatement library (built by ``make -C oracle oracle``)."""
     global _lib
     if _lib is None:
-        path = os.path.join(HERE, "liboracle.so")
+        # ORACLE_LIB: another build of the same restatement (the sanitizer test loads an ASan/UBSan build)
+        path = os.environ.get("ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} missing: run `make -C oracle oracle`")
         L = ctypes.CDLL(path)

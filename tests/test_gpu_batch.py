@@ -56,10 +56,13 @@ def test_persistent_equals_grid_bitwise(dtype, scheme, rsolver):
     kw = dict(scheme=scheme, rsolver=rsolver)
     o1, f1 = hydro.solve_batch(b.u, b.pg_dt_dh, **kw)
     cnt = torch.zeros(1, dtype=torch.int32, device=DEV)
-    for g, k in ((1, 1), (3, 1), (2, 3), (1, 4), (4, 64)):
-        o2, f2 = hydro.solve_batch(b.u, b.pg_dt_dh, queue_counter=cnt, groups_per_cu=g, pgs_per_ticket=k, **kw)
-        torch.cuda.synchronize()
-        assert torch.equal(o1, o2) and torch.equal(f1, f2), (g, k)
+    shapes = (4, 8) if scheme == "MHM_RP" else (4,)       # workgroup shapes of the kernel (waves per group)
+    for w in shapes:
+        for g, k in ((1, 1), (3, 1), (2, 3), (1, 4), (4, 64)):
+            o2, f2 = hydro.solve_batch(b.u, b.pg_dt_dh, queue_counter=cnt, groups_per_cu=g, pgs_per_ticket=k,
+                                       waves_per_group=w, **kw)
+            torch.cuda.synchronize()
+            assert torch.equal(o1, o2) and torch.equal(f1, f2), (w, g, k)
 
 
 def test_batch_uniform_equals_fluid_solver():
@@ -123,7 +126,8 @@ def test_autotune_picks_a_measured_config(tmp_path):
     b = _batch(dtype=torch.float32, counts=(64, 128, 192))
     cache = str(tmp_path / "tune.json")
     res = amr.autotune(torch, b, groups=(1, 2), tickets=(1, 2), reps=2, cache=cache)
-    assert res["best"] in res["trials_ms"] and res["ms"] > 0 and len(res["trials_ms"]) == 5
+    assert res["best"] in res["trials_ms"] and res["ms"] > 0 and len(res["trials_ms"]) == 10    # (grid + 2x2) x 2 shapes
+    assert {k.split(":")[-1] for k in res["trials_ms"]} == {"4", "8"}
     assert amr.autotune(torch, b, groups=(1, 2), tickets=(1, 2), reps=2, cache=cache)["best"] == res["best"]   # cached
     o, _ = hydro.solve_batch(b.u, b.pg_dt_dh, **amr.launch_kwargs(torch, res["best"], DEV))
     ref, _ = hydro.solve_batch(b.u, b.pg_dt_dh)
