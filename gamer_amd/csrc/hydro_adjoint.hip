@@ -39,6 +39,8 @@
 
 #pragma clang fp contract(off)
 
+// the derivative phases and their dual-number twins take the sound speed as sqrt(gamma P / rho) (see sound_speed)
+#define HYDRO_IEEE_SOUND_SPEED
 #include "hydro_dual.hpp"
 #include "hydro_kernels.hpp"
 #include "hydro_reverse.hpp"

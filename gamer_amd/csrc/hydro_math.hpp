@@ -47,6 +47,26 @@ __device__ __forceinline__ double rcp_recover( double x )
 
 template <typename R> __device__ __forceinline__ double rcp_pres( double x ) { return sizeof( R ) == 4 ? rcp_recover( x ) : rcp( x ); }
 
+// sound speed (gamma P / rho)^1/2 from gp = gamma P, rho and ir = 1/rho.  Generic form sqrt(gp ir).  FP64:
+// gp (gp rho)^-1/2 from the hardware reciprocal square root and two Newton steps, about 2 ulp, one transcendental
+// and 9 instructions where the reciprocal's Newton steps and the IEEE square root take about 20; gp == 0 gives 0
+// and NaN passes.  The derivative kernels define HYDRO_IEEE_SOUND_SPEED before including this header and keep the
+// generic form, which their dual-number twins evaluate bit for bit.
+template <typename R> __device__ __forceinline__ R sound_speed( R gp, R rho, R ir ) { (void)rho; return vsqrt( gp*ir ); }
+#ifndef HYDRO_IEEE_SOUND_SPEED
+template <> __device__ __forceinline__ double sound_speed<double>( double gp, double rho, double ir )
+{
+   (void)ir;
+   const double a = gp*rho, h = 0.5*a;
+   double y = __builtin_amdgcn_rsq( a );
+   double r = __builtin_fma( -h*y, y, 0.5 );        // y (1.5 - a y^2 / 2) = y + y (0.5 - h y^2)
+   y = __builtin_fma( y, r, y );
+   r = __builtin_fma( -h*y, y, 0.5 );
+   y = __builtin_fma( y, r, y );
+   return ( gp == 0.0 ) ? 0.0 : gp*y;
+}
+#endif
+
 // pressure floor that lets NaN through (FluUtility.cpp:681-690): a NaN compares false and is returned as is.
 // One compare + select; max() would first have to quiet operands read from memory (IEEE mode).
 template <typename R> __device__ __forceinline__ R floor_pres( R p, R pmin ) { return ( p < pmin ) ? pmin : p; }
@@ -297,7 +317,7 @@ __device__ __forceinline__ St<R> face_state( const R w[NV], const Prm<R> &P )
    s.ir = rcp( w[0] );
    s.p  = floor_pres( w[4], P.min_pres );
    s.e  = fmad( (R)0.5*w[0], fmad( w[3], w[3], fmad( w[2], w[2], w[1]*w[1] ) ), w[4]*P.inv_gm1 );
-   s.cs = vsqrt( P.gamma*s.p*s.ir );
+   s.cs = sound_speed( P.gamma*s.p, s.d, s.ir );
    return s;
 }
 
@@ -315,7 +335,7 @@ __device__ __forceinline__ St<R> face_state_xy( const R w[NV], bool dy, const Pr
    s.ir = rcp( w[0] );
    s.p  = floor_pres( w[4], P.min_pres );
    s.e  = fmad( (R)0.5*w[0], fmad( w[3], w[3], fmad( w[2], w[2], w[1]*w[1] ) ), w[4]*P.inv_gm1 );
-   s.cs = vsqrt( P.gamma*s.p*s.ir );
+   s.cs = sound_speed( P.gamma*s.p, s.d, s.ir );
    return s;
 }
 
@@ -512,7 +532,7 @@ __device__ __forceinline__ St<R> cons_state( const R c[NV], const Prm<R> &P )
    s.p  = floor_pres( (R)( eint*P.gm1_d ), P.min_pres );
    s.ir = rcp( c[0] );
    s.u  = s.ir*s.mn;
-   s.cs = vsqrt( P.gamma*s.p*s.ir );
+   s.cs = sound_speed( P.gamma*s.p, s.d, s.ir );
    return s;
 }
 

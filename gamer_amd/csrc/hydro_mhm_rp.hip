@@ -171,7 +171,7 @@ __device__ __forceinline__ void derive_cell( const IO u[NV], R q[8], const Prm<R
    for (int v=0; v<NV; v++) q[v] = (R)u[v];
    q[5] = p;
    q[6] = ir;
-   q[7] = vsqrt( P.gamma*p*ir );
+   q[7] = sound_speed( P.gamma*p, q[0], ir );
 }
 
 // half-step cell update, floors and conversion to primitive (CPU_FluidSolver_MHM.cpp:846-933).  The update
