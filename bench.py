@@ -388,6 +388,17 @@ def main():
               f"multi-GPU sharding is a design note, not a run.", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration_mib > 0:
+        # a device copy of known size, so that counter profiles of every mode can scale FETCH_SIZE / WRITE_SIZE
+        import torch
+        dev = torch.device("cuda:0")
+        a = torch.ones(args.copy_calibration_mib * 2 ** 18, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
     if args.riemann_sweep:
         riemann_sweep(args)
         return
@@ -408,13 +419,6 @@ def main():
     from gamer_amd import hydro, ic
     hydro.lib()                                      # fails loudly if the gfx950 library is missing
     dev = torch.device("cuda:0")
-    if args.copy_calibration_mib > 0:
-        a = torch.ones(args.copy_calibration_mib * 2 ** 18, dtype=torch.float32, device=dev)
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
     n = args.npg_axis
     print(f"[bench] generating workload {n}^3 patch groups", file=sys.stderr, flush=True)
     cons, dt, dh = ic.random_sod_grid((n, n, n), seed=20260, sedov=True)

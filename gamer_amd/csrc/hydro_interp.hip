@@ -235,7 +235,7 @@ __device__ __forceinline__ bool prim_bad( const T w[NV], double inv_gm1 )
       if ( v >= 1 && v <= 3 ) bad |= ( w[v] < -huge || w[v] > huge );
       else                    bad |= ( w[v] <= (T)0 || w[v] > huge );
    }
-   const T e = (T)( (double)w[4]*inv_gm1 );
+   const T e = w[4]*(T)inv_gm1;                      // P/(gamma-1) in the storage type, as EOS_GAMMA
    return bad || e < (T)0 || e > huge || e != e;
 }
 
@@ -266,7 +266,7 @@ __device__ __forceinline__ bool store_fine( T f[NV][8], bool iterate, bool prim,
       if ( prim )
       {
          // Hydro_Pri2Con with EoS_DensPres2Eint
-         const T ei = (T)( (double)t[4]*p.inv_gm1 );
+         const T ei = t[4]*(T)p.inv_gm1;
          t[1] = t[0]*t[1];  t[2] = t[0]*t[2];  t[3] = t[0]*t[3];
          t[4] = (T)0.5*( t[1]*t[1] + t[2]*t[2] + t[3]*t[3] )/t[0] + ei;
       }

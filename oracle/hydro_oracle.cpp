@@ -1079,7 +1079,7 @@ template <typename T> bool prim_unphysical( const T w[NV], double inv_gm1 )
       if ( v >= 1 && v <= 3 ) { if ( w[v] < -huge || w[v] > huge ) return true; }
       else                    { if ( w[v] <= (T)0.0 || w[v] > huge ) return true; }
    }
-   const T e = (T)( (double)w[4]*inv_gm1 );
+   const T e = w[4]*(T)inv_gm1;                      // Pres*_Gamma_m1 in real (CPU_EoS_Gamma.cpp:151-154)
    return ( e < (T)0.0 || e > huge || e != e );
 }
 
@@ -1137,7 +1137,7 @@ int interpolate( T *C, T *F, const Box &b, int scheme, int mono_mask, double mon
          if ( bad ) { any = true;  if ( it == max_iter ) return 1;  continue; }
          if ( prim )
          {
-            const T w4 = (T)( (double)t[4]*inv_gm1 );          // EoS_DensPres2Eint
+            const T w4 = t[4]*(T)inv_gm1;                      // EoS_DensPres2Eint_Gamma: Pres*_Gamma_m1 in real
             T c[NV];
             c[0] = t[0];  c[1] = t[0]*t[1];  c[2] = t[0]*t[2];  c[3] = t[0]*t[3];
             c[4] = etot_of( c[0], c[1], c[2], c[3], w4 );

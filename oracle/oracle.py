@@ -79,6 +79,8 @@ def ref_lib(scheme: str, rsolver: str, prec: str):
         L.gamer_ref_fluid_solver.restype = _I
         L.gamer_ref_dt_cfl.argtypes = [_P, _P, _I, _D, _D, _D, _D, _I]
         L.gamer_ref_dt_cfl.restype = _I
+        L.gamer_ref_interpolate.argtypes = [_P, _P, _I, _P, _P, _P, _P, _P, _I, _I, _D, _I, _I, _I, _I, _D, _D]
+        L.gamer_ref_interpolate.restype = _I
         _refs[key] = L
     return _refs[key]
 
@@ -154,6 +156,24 @@ def ref_dt_cfl(u_patch, dh, safety, gamma=5.0 / 3.0, min_pres=0.0, nthreads=0):
     if rc != 0:
         raise ValueError(f"gamer_ref_dt_cfl returned {rc}")
     return dt
+
+
+def ref_interpolate(cdata, fdata, cstart, crange, fstart=(0, 0, 0), scheme="CQUAD", monotonic=0x1F, mono_coeff=2.0,
+                    opp_sign_0th=False, int_prim=True, reduce_mono=True, mono_max_iter=10, gamma=5.0 / 3.0,
+                    min_pres=0.0):
+    """GAMER's own Interpolate() (compiled reference, the build of cdata's precision) box by box, with the
+    arguments of ``interpolate``: returns (cdata after the call, fdata).  The reference aborts the process if a
+    box still fails after its last iteration, so only inputs it accepts may be passed."""
+    c, f = np.array(cdata, copy=True), np.array(fdata, copy=True)
+    i3 = lambda t: np.ascontiguousarray(t, dtype=np.int32)
+    L = ref_lib("MHM_RP", "HLLC", "f64" if c.dtype == np.float64 else "f32")
+    rc = L.gamer_ref_interpolate(_ptr(c), _ptr(f), c.shape[0], _ptr(i3(c.shape[:1:-1])), _ptr(i3(cstart)),
+                                 _ptr(i3(crange)), _ptr(i3(f.shape[:1:-1])), _ptr(i3(fstart)), _INT[scheme],
+                                 int(monotonic), mono_coeff, int(opp_sign_0th), int(int_prim), int(reduce_mono),
+                                 int(mono_max_iter), gamma, min_pres)
+    if rc != 0:
+        raise ValueError(f"gamer_ref_interpolate returned {rc}")
+    return c, f
 
 
 def dt_cfl(u_patch, dh, safety, gamma=5.0 / 3.0, min_pres=0.0, nthreads=0):

@@ -12,7 +12,9 @@
 //   (4) calls CPU_FluidSolver_MHM with the same arguments CPU_FluidSolver passes
 //       (src/Fluid/CPU_FluidSolver.cpp:258-266), non-MHD / no gravity / no passive scalars, and
 //   (5) calls CPU_dtSolver_HydroCFL as CPU_dtSolver does for DT_FLU_SOLVER (src/Model_Hydro/CPU_Hydro/
-//       CPU_dtSolver_HydroCFL.cpp:57-209, src/SelfGravity/CPU_Poisson/CPU_dtSolver.cpp).
+//       CPU_dtSolver_HydroCFL.cpp:57-209, src/SelfGravity/CPU_Poisson/CPU_dtSolver.cpp), and
+//   (6) calls Interpolate (src/Interpolation/Interpolate.cpp) box by box with the interpolation run-time
+//       parameters it reads (INT_MONO_COEFF, MONO_MAX_ITER, MIN_PRES) set from its arguments.
 
 #include "GAMER.h"
 #include "CUFLU.h"
@@ -27,6 +29,17 @@ double MOLECULAR_WEIGHT = 0.6;
 double MU_NORM          = 1.0;
 double UNIT_L = 1.0, UNIT_M = 1.0, UNIT_T = 1.0, UNIT_V = 1.0, UNIT_D = 1.0, UNIT_E = 1.0, UNIT_P = 1.0;
 bool   OPT__UNIT        = false;
+
+// run-time parameters read by Interpolate() (src/Interpolation/Interpolate.cpp; include/Global.h:44-125);
+// the interpolation entry point below sets the ones it takes as arguments before each call
+int    MPI_Rank                 = 0;
+long   PassiveFloorMask         = 0L;
+int    PassiveIntFrac_NVar      = 0;
+int    PassiveIntFrac_VarIdx[NCOMP_PASSIVE];
+bool   OPT__INT_FRAC_PASSIVE_LR = false;
+double INT_MONO_COEFF           = 2.0;
+int    MONO_MAX_ITER            = 10;
+double MIN_PRES                 = 0.0;
 
 double EoS_AuxArray_Flt[EOS_NAUX_MAX];
 int    EoS_AuxArray_Int[EOS_NAUX_MAX];
@@ -171,6 +184,34 @@ int gamer_ref_dt_cfl( const void *in, void *dt, int npatch, double dh, double sa
    omp_set_schedule( omp_sched_dynamic, 1 );
    CPU_dtSolver_HydroCFL( (real *)dt, (const real (*)[FLU_NIN_T][ CUBE(PS1) ])in, NULL, npatch/8,
                           (real)dh, (real)safety, (real)min_pres, 0L, EoS_obj, MicroPhy_obj );
+   return 0;
+}
+
+// One call of GAMER's Interpolate() per box (src/Interpolation/Interpolate.cpp:90-330, with the schemes of
+// src/Interpolation/Int_*.cpp), hydro: NComp = 5 conserved fields, AllCons, no phase unwrapping, no magnetic field.
+//   cdata [nbox][5][CSize]  (Interpolate may convert it in place)   fdata [nbox][5][FSize]
+//   monotonic: bit v = Monotonic[v]; int_prim / reduce_mono: IntPrim_t / ReduceOrFixMonoCoeff_t values;
+//   mono_coeff, mono_max_iter, min_pres: INT_MONO_COEFF, MONO_MAX_ITER and MIN_PRES of the run.
+// A box whose interpolation still fails after the last iteration makes the reference abort the process
+// (MPI_Exit), so callers pass only inputs it accepts.
+int gamer_ref_interpolate( void *cdata, void *fdata, int nbox, const int csize[3], const int cstart[3],
+                           const int crange[3], const int fsize[3], const int fstart[3], int scheme, int monotonic,
+                           double mono_coeff, int opp_sign0, int int_prim, int reduce_mono, int mono_max_iter,
+                           double gamma, double min_pres )
+{
+   if ( nbox <= 0 || cdata == NULL || fdata == NULL )   return 1;
+   make_eos( gamma );
+   INT_MONO_COEFF = mono_coeff;
+   MONO_MAX_ITER  = mono_max_iter;
+   MIN_PRES       = min_pres;
+   bool Monotonic[NCOMP_TOTAL];
+   for (int v=0; v<NCOMP_TOTAL; v++)   Monotonic[v] = ( monotonic >> v ) & 1;
+   const long nc = (long)NCOMP_TOTAL*csize[0]*csize[1]*csize[2];
+   const long nf = (long)NCOMP_TOTAL*fsize[0]*fsize[1]*fsize[2];
+   for (int b=0; b<nbox; b++)
+      Interpolate( (real *)cdata + b*nc, csize, cstart, crange, (real *)fdata + b*nf, fsize, fstart, NCOMP_TOTAL,
+                   (IntScheme_t)scheme, false, Monotonic, opp_sign0 != 0, true, (IntPrim_t)int_prim,
+                   (ReduceOrFixMonoCoeff_t)reduce_mono, NULL, NULL );
    return 0;
 }
 

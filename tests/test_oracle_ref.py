@@ -1,6 +1,8 @@
 """The oracle restatement against GAMER's own CPU_FluidSolver compiled from the reference sources
 (oracle/_ref, built by `make -C oracle ref`): bit-for-bit equality over every compile-time variant
 (scheme x Riemann solver x precision) and every PLM limiter, on fresh seeded inputs."""
+import zlib
+
 import numpy as np
 import pytest
 
@@ -88,3 +90,62 @@ def test_dt_cfl_bit_exact_vs_reference_build(min_pres):
     a = oracle.dt_cfl(patches, w.dh, 0.5, min_pres=min_pres)
     b = oracle.ref_dt_cfl(patches, w.dh, 0.5, min_pres=min_pres)
     np.testing.assert_array_equal(a, b)
+
+
+# ---- Interpolate(): the oracle restatement against GAMER's src/Interpolation compiled from the reference ----
+
+_G = 5.0 / 3.0
+INT_SCHEMES = ["MINMOD3D", "MINMOD1D", "VANLEER", "CQUAD", "QUAD", "CQUAR", "QUAR"]
+
+
+def _interp_box(rng, nbox, cs, jump=False):
+    """nbox coarse boxes of random gas (conserved), optionally with a pressure / velocity jump through the middle."""
+    shp = (nbox,) + tuple(cs[::-1])
+    rho = np.exp(0.3 * rng.standard_normal(shp))
+    v = 0.3 * rng.standard_normal((3,) + shp)
+    p = np.exp(0.3 * rng.standard_normal(shp))
+    if jump:
+        half = cs[0] // 2
+        p[..., half:] *= 100.0
+        rho[..., half:] *= 0.125
+        v[0, ..., :half] += 2.0
+    e = p / (_G - 1) + 0.5 * rho * (v ** 2).sum(0)
+    return np.stack([rho, rho * v[0], rho * v[1], rho * v[2], e], axis=1)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("scheme", INT_SCHEMES)
+@pytest.mark.parametrize("case", ["smooth_cons", "jump_prim_reduce", "jump_unlimited_cons", "opp_sign"])
+def test_interpolate_bit_exact_vs_reference_build(prec, scheme, case):
+    """Every interpolation scheme, with and without monotonic limiting, in conserved and in primitive variables,
+    with the monotonic-coefficient reduction and its retries (Interpolate_Iterate): identical bits to
+    GAMER's own Interpolate() for the fine box and for the coarse box after the call."""
+    if not oracle.ref_available("MHM_RP", "HLLC", prec):
+        pytest.skip("reference build (oracle/_ref) not present")
+    rng = np.random.default_rng(zlib.crc32(f"{prec}:{scheme}:{case}".encode()))
+    g = 2 if scheme in ("CQUAR", "QUAR") else 1
+    cr = (5, 4, 6)
+    cs = tuple(c + 2 * g for c in cr)
+    kw = dict(scheme=scheme, gamma=_G)
+    if case == "smooth_cons":
+        c = _interp_box(rng, 3, cs)
+        kw.update(monotonic=0x1F, int_prim=False, reduce_mono=False, mono_coeff=2.0)
+    elif case == "jump_prim_reduce":
+        c = _interp_box(rng, 3, cs, jump=True)
+        # the reduction retries are defined for the schemes that take a monotonic coefficient
+        reduce = scheme not in ("MINMOD3D", "MINMOD1D", "VANLEER")
+        kw.update(monotonic=0x1F, int_prim=True, reduce_mono=reduce, mono_max_iter=10 if reduce else 0,
+                  mono_coeff=2.0)
+    elif case == "jump_unlimited_cons":
+        c = _interp_box(rng, 3, cs, jump=True)
+        kw.update(monotonic=0x0A, int_prim=False, reduce_mono=False, mono_coeff=1.5)
+    else:
+        c = _interp_box(rng, 3, cs, jump=True)
+        kw.update(monotonic=0x1F, int_prim=False, reduce_mono=False, mono_coeff=2.0, opp_sign_0th=True)
+    c = c.astype(np.float64 if prec == "f64" else np.float32)
+    f0 = np.zeros((c.shape[0], 5) + tuple(2 * x for x in cr[::-1]), dtype=c.dtype)
+    c1, f1, nf = oracle.interpolate(c, f0, (g,) * 3, cr, **kw)
+    assert nf == 0
+    c2, f2 = oracle.ref_interpolate(c, f0, (g,) * 3, cr, **kw)
+    np.testing.assert_array_equal(f1, f2)
+    np.testing.assert_array_equal(c1, c2)
