@@ -128,7 +128,7 @@ int enqueue( const HydroSolveArgs *a, const void *in, void *out, void *flux, voi
    Batch B;
    if ( batch ) B = *batch;
    else { B.pg_dt_dh = nullptr;  B.counter = nullptr;  B.safety = 0.0;  B.claim = 1; }
-   B.waves = ( a->waves_per_group == 4 ) ? 4 : 8;
+   B.waves = a->waves_per_group;                             // 0: each kernel's default shape
    int rc;
    const bool rp = ( a->scheme == HYDRO_MHM_RP );
    if ( a->precision == HYDRO_FP64 )

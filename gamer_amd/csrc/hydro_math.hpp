@@ -116,7 +116,7 @@ struct Batch {
    unsigned     *counter;
    double        safety;
    int           claim;
-   int           waves;         // MHM_RP launch shape: 4 or 8 waves per workgroup (host side only)
+   int           waves;         // launch shape: 4 or 8 waves per workgroup, 0 = the kernel's default (host side only)
 };
 
 // an interface state rotated to the face normal: conserved fields + derived quantities

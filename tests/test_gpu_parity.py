@@ -180,26 +180,29 @@ def test_roe_fallback_double_rarefaction(scheme):
     assert np.all(rel_l1(o32, ro) <= FP32_L1_TOL), rel_l1(o32, ro)
 
 
+@pytest.mark.parametrize("scheme", ["MHM_RP", "MHM"])
 @pytest.mark.parametrize("dtype", ["fp64", "fp32"])
 @pytest.mark.parametrize("rsolver,limiter", [("HLLC", "VL_GMINMOD"), ("HLLE", "VANLEER"), ("ROE", "GMINMOD"),
                                              ("HLLC", "ALBADA")])
-def test_launch_shapes_bitwise(dtype, rsolver, limiter):
+def test_launch_shapes_bitwise(dtype, rsolver, limiter, scheme):
     """The MHM_RP kernel's two workgroup shapes (4 waves: every lane marches an output column and a ghost role;
     8 waves: output columns and ghost roles on separate waves) run the same arithmetic in the same order, so
-    outputs, coarse-fine fluxes and the fused dt are bitwise equal; the default shape (0) is 8 waves."""
+    outputs, coarse-fine fluxes and the fused dt are bitwise equal, and equal to the default shape (0); MHM has
+    the 4-wave shape, which must equal its default."""
     from gamer_amd import hydro
-    sod = sod_for("MHM_RP")
-    print(f"[gpu] launch shapes {dtype} {rsolver} {limiter}", flush=True)
+    sod = sod_for(scheme)
+    print(f"[gpu] launch shapes {scheme} {dtype} {rsolver} {limiter}", flush=True)
     dev = torch.device("cuda:0")
     ut = torch.from_numpy(np.ascontiguousarray(sod.u)).to(dev, dtype=torch.float64 if dtype == "fp64" else torch.float32)
     res = {}
-    for w in (0, 4, 8):
+    shapes = (0, 4, 8) if scheme == "MHM_RP" else (0, 4)    # MHM has the 4-wave shape only
+    for w in shapes:
         dtp = torch.zeros((ut.shape[0], 8), dtype=ut.dtype, device=dev)
-        o, f = hydro.fluid_solver(ut, sod.dt, sod.dh, rsolver=rsolver, limiter=limiter, minmod_maxiter=2,
-                                  dt_patch=dtp, waves_per_group=w)
+        o, f = hydro.fluid_solver(ut, sod.dt, sod.dh, scheme=scheme, rsolver=rsolver, limiter=limiter,
+                                  minmod_maxiter=2, dt_patch=dtp, waves_per_group=w)
         torch.cuda.synchronize()
         res[w] = [x.cpu().numpy() for x in (o, f, dtp)]
-    for w in (4, 8):
+    for w in shapes[1:]:
         for a, b in zip(res[0], res[w]):
             assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), w
     assert np.isfinite(res[0][0]).all()
