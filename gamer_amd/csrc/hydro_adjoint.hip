@@ -567,7 +567,7 @@ int launch_mhm_rp_adjoint( const double *in, const double *lam_out, double *lam_
 {
    if ( npg <= 0 ) return 0;
    const dim3 grid( npg ), block( NT );
-#  define GA_ADJ( RS_ ) do { if ( P.limiter == 4 ) hipLaunchKernelGGL( ( adjoint_kernel<RS_, 4> ), grid, block, 0, stream, in, lam_out, lam_in, ws, npg, P, minmod_coeff ); \
+#  define GA_ADJ( RS_ ) do { if ( kernel_limiter( P.limiter, minmod_coeff ) == 4 ) hipLaunchKernelGGL( ( adjoint_kernel<RS_, 4> ), grid, block, 0, stream, in, lam_out, lam_in, ws, npg, P, minmod_coeff ); \
                              else                  hipLaunchKernelGGL( ( adjoint_kernel<RS_, 0> ), grid, block, 0, stream, in, lam_out, lam_in, ws, npg, P, minmod_coeff ); } while ( 0 )
    switch ( rsolver )
    {
@@ -585,7 +585,7 @@ int launch_mhm_rp_tangent( const double *in, const double *tin, double *out, dou
 {
    if ( npg <= 0 ) return 0;
    const dim3 grid( npg ), block( NT );
-#  define GA_TAN( RS_, T_ ) do { if ( P.limiter == 4 ) hipLaunchKernelGGL( ( tangent_kernel<RS_, 4, T_> ), grid, block, 0, stream, in, tin, out, tout, ws, npg, P, minmod_coeff ); \
+#  define GA_TAN( RS_, T_ ) do { if ( kernel_limiter( P.limiter, minmod_coeff ) == 4 ) hipLaunchKernelGGL( ( tangent_kernel<RS_, 4, T_> ), grid, block, 0, stream, in, tin, out, tout, ws, npg, P, minmod_coeff ); \
                                  else                  hipLaunchKernelGGL( ( tangent_kernel<RS_, 0, T_> ), grid, block, 0, stream, in, tin, out, tout, ws, npg, P, minmod_coeff ); } while ( 0 )
    const bool tl = ( tin != nullptr );
    switch ( rsolver )

@@ -10,6 +10,7 @@
 // evaluated once (per cell for the half step, per face for the full step) instead of inside every solve.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 namespace gamer_amd {
 
@@ -279,24 +280,55 @@ __device__ __forceinline__ double clamp_between( double x, double a, double b ) 
 
 // van Leer + generalised MinMod slope without branches:
 //   sign(sc) * min( theta|sl|, theta|sr|, |sc|, |2 sl sr/(sl+sr)| )  if sl*sr > 0, else 0.
-// h = 2 sl sr / |sl+sr| equals the harmonic-mean magnitude when sl*sr > 0 and is <= 0 (or NaN) otherwise,
-// so max(min(...), 0) realises the sign test; |sc| >= |harmonic mean| is implied and dropped.
+// h = 2 sl sr / |sl+sr| equals the harmonic-mean magnitude when sl*sr > 0 and is <= 0 otherwise, or NaN when
+// sl*sr underflows to zero while 1/|sl+sr| overflows (both slopes below about 1e-38 in FP32); max(h, 0) maps
+// both cases to 0, which realises the sign test.  |sc| >= |harmonic mean| is implied and dropped.
+template <typename R>
+__device__ __forceinline__ R gminmod_mag( R h, R x ) { return vmin( x, vmax( h, (R)0 ) ); }
+// FP32, x >= 0: med3(h, 0, x) is 0, h or x for h < 0, 0 <= h <= x or h > x, and 0 for a NaN h (with a NaN
+// operand v_med3_f32 returns the minimum of the other two)
+__device__ __forceinline__ float gminmod_mag( float h, float x ) { return __builtin_amdgcn_fmed3f( h, 0.0f, x ); }
+
 template <typename R>
 __device__ __forceinline__ R slope_vl_gminmod( R sl, R sr, R coeff )
 {
    const R t = sl + sr;
    const R h = (R)2.0*( sl*sr )*rcp( vabs( t ) );
-   R m = vmin( vmin( vabs( sl )*coeff, vabs( sr )*coeff ), h );
-   m = vmax( m, (R)0 );
-   return vcopysign( m, t );
+   return vcopysign( gminmod_mag( h, vmin( vabs( sl )*coeff, vabs( sr )*coeff ) ), t );
 }
 
+// Coefficients up to this value leave the PLM face clamps inactive under VL_GMINMOD (plm_face).
+constexpr double VLGM_NOCLAMP_COEFF = 1.9999;
+
+// Limiter id to instantiate a kernel with: 4 (VL_GMINMOD, resolved at compile time, face clamps left out) only
+// for a MinMod coefficient <= VLGM_NOCLAMP_COEFF; otherwise 0, the run-time dispatch that keeps the clamps.
+// The MinMod retries only lower the coefficient.
+inline int kernel_limiter( int limiter, double minmod_coeff )
+{
+   return ( limiter == 4 && minmod_coeff <= VLGM_NOCLAMP_COEFF ) ? 4 : 0;
+}
+
+// Faces C -/+ s/2, the left one clamped between C and L, the right one 2C - (left face) clamped between C and R
+// (DataReconstruction.cpp:415-430).  With LIM == 4 and plain FP32/FP64 arithmetic the clamps never act, and are
+// left out: the slope is at most theta*|C-L|*(1+u)^2 <= 2|C-L| for theta <= 1.9999, so the exact C - s/2 lies
+// in [min(C,L), max(C,L)], whose ends are representable, and rounding keeps it there; likewise s/2 <= |R-C|
+// puts 2C - (left face) between C and R before its rounding.  Same bits as the clamped form; dual numbers keep
+// the clamps for their derivative tie rules.
 template <int LIM, typename R>
 __device__ __forceinline__ void plm_face( R L, R C, R Rr, int lim, R coeff, R &fL, R &fR )
 {
    const R s = ( LIM == 4 ) ? slope_vl_gminmod( C - L, Rr - C, coeff ) : plm_slope<LIM>( C - L, Rr - C, (R)0, lim, coeff );
-   R fl = clamp_between( C - (R)0.5*s, C, L );
-   R fr = clamp_between( (R)2.0*C - fl, C, Rr );
+   R fl, fr;
+   if constexpr ( LIM == 4 && std::is_floating_point<R>::value )
+   {
+      fl = C - (R)0.5*s;
+      fr = (R)2.0*C - fl;
+   }
+   else
+   {
+      fl = clamp_between( C - (R)0.5*s, C, L );
+      fr = clamp_between( (R)2.0*C - fl, C, Rr );
+   }
    fL = (R)2.0*C - fr;
    fR = fr;
 }

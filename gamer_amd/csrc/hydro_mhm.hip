@@ -329,7 +329,7 @@ void launch_rs( const IO *in, IO *out, IO *flux, int npg, int store_flux, int mi
    const bool pers = ( B.counter != nullptr );
 #  define GA_LAUNCH( LIM_, PERS_ ) hipLaunchKernelGGL( ( mhm_kernel<R, IO, RS, LIM_, PERS_> ), grid, block, 0, stream, \
                                                       in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P, dt, dh_safety, B )
-   if ( P.limiter == 4 ) { if ( pers ) GA_LAUNCH( 4, true ); else GA_LAUNCH( 4, false ); }
+   if ( kernel_limiter( P.limiter, minmod_coeff ) == 4 ) { if ( pers ) GA_LAUNCH( 4, true ); else GA_LAUNCH( 4, false ); }
    else                  { if ( pers ) GA_LAUNCH( 0, true ); else GA_LAUNCH( 0, false ); }
 #  undef GA_LAUNCH
 }

@@ -634,7 +634,7 @@ static void launch_rs( const IO *in, IO *out, IO *flux, int npg, int store_flux,
 #  define GA_LAUNCH( LIM_, PERS_, NW_ ) hipLaunchKernelGGL( ( mhm_rp_kernel<R, IO, RS, LIM_, PERS_, NW_> ), grid, block, 0, stream, \
                                                            in, out, flux, npg, store_flux, minmod_maxiter, (R)minmod_coeff, P, dt, dh_safety, B )
 #  define GA_SHAPE( LIM_, PERS_ ) { if ( nw == 4 ) GA_LAUNCH( LIM_, PERS_, 4 ); else GA_LAUNCH( LIM_, PERS_, 8 ); }
-   if ( P.limiter == 4 ) { if ( pers ) GA_SHAPE( 4, true ) else GA_SHAPE( 4, false ) }
+   if ( kernel_limiter( P.limiter, minmod_coeff ) == 4 ) { if ( pers ) GA_SHAPE( 4, true ) else GA_SHAPE( 4, false ) }
    else                  { if ( pers ) GA_SHAPE( 0, true ) else GA_SHAPE( 0, false ) }
 #  undef GA_SHAPE
 #  undef GA_LAUNCH

@@ -68,6 +68,23 @@ def test_fp64_limiters(scheme, limiter):
 
 
 @pytest.mark.parametrize("scheme", ["MHM_RP", "MHM"])
+@pytest.mark.parametrize("coeff", [1.0, 1.9999, 2.0])
+def test_vl_gminmod_coefficients(scheme, coeff):
+    """VL_GMINMOD up to coefficient 1.9999 runs the kernel whose PLM leaves the face clamps out (they cannot act
+    there); 2.0 runs the run-time-dispatch kernel that keeps them.  Both against the oracle, which always clamps."""
+    w = ic.random_sod((2, 2, 2), seed=91, sedov=True, scheme_ghost=GHOST[scheme])
+    print(f"[gpu] VL_GMINMOD scheme={scheme} coeff={coeff}", flush=True)
+    ro, rf = _oracle(w.u, w, scheme=scheme, minmod_coeff=coeff)
+    o, f = _gpu(w.u, w, scheme=scheme, minmod_coeff=coeff)
+    err = rel_err_cells(o, ro)
+    assert np.all(err <= FP64_TOL), err
+    assert flux_rel_err(f, rf, ro) <= FP64_TOL
+    o32, _ = _gpu(w.u.astype(np.float32), w, scheme=scheme, minmod_coeff=coeff)
+    l1 = rel_l1(o32, ro)
+    assert np.all(l1 <= FP32_L1_TOL), l1
+
+
+@pytest.mark.parametrize("scheme", ["MHM_RP", "MHM"])
 @pytest.mark.parametrize("rsolver", ["HLLC", "HLLE", "ROE"])
 def test_fp32_bulk_vs_fp64_oracle(scheme, rsolver):
     sod = sod_for(scheme)
