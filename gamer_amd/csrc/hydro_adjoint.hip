@@ -2,7 +2,8 @@
 //
 // The step is the map U_in (22^3 x 5 per patch group) -> U_out (16^3 x 5) of the forward kernel
 // (src/Model_Hydro/CPU_Hydro/CPU_FluidSolver_MHM.cpp:263-589 with the half-step prediction of :615-949).
-// One 256-lane workgroup per patch group runs explicit phases over a per-PG global workspace:
+// Explicit phases over a per-PG global workspace, each phase one kernel over all patch groups with one face or
+// cell per lane:
 //
 //   forward   F1 half-step fluxes Fh (Hydro_RiemannPredict_Flux)   -> ws
 //             F2 half-step update, floors, primitive W (Hydro_RiemannPredict)   -> ws
@@ -61,6 +62,7 @@ constexpr int WS_LU  = 10;           // [5]       lambda of the half-step conser
 constexpr int WS_LFC = 15;           // [3][2][5] lambda of the faces
 constexpr int WS_LS  = 45;           // [3][2][5] half-step face contributions to lambda_in (low / high cell);
                                      //           the first 15 arrays hold Fh [3][5] during the forward phases
+constexpr int WS_H   = 60;           // [5]       half-step conserved state, F2 -> R2 (free slots of WS_LS until R1)
 constexpr int WS_ADJ = 75;
 
 __device__ __forceinline__ void coords( int c, int &i, int &j, int &k ) { i = c % NX;  j = ( c / NX ) % NX;  k = c / N2; }
@@ -194,7 +196,7 @@ __device__ __forceinline__ void plm( const T &L, const T &C, const T &R, const P
 }
 
 // ---------------- workspace views ----------------
-// The adjoint keeps Fh in the lambda-side slots (read until R2, overwritten in R1); the tangent kernel stores
+// The adjoint keeps Fh (read by F2) and h (read by R2) in the lambda-side slots, which R1 overwrites; the tangent kernel stores
 // dual numbers (two doubles each) and packs Fh right after W: 20 duals = 40 doubles per cell.
 constexpr int WS_FH_TAN = 5;
 static_assert( 2*( WS_FH_TAN + 15 ) <= WS_ADJ, "tangent workspace must fit the per-PG slot" );
@@ -226,44 +228,29 @@ template <> struct Load< Dual<1> >
    }
 };
 
-// F1-F3: forward phases into the workspace
-template <int RS, int LIM, typename T>
-__device__ void forward_phases( const double *in, const double *tin, Fwd<T> &F, const Prm<T> &P )
+// ---------------- phases: one kernel each, one element per lane ----------------
+// Every phase is a gather over the faces or cells of each patch group that reads only what earlier phases
+// wrote, so the phases run as separate kernels in stream order, each over a flat grid of bpp(count) workgroups
+// per patch group: element e of patch group pg is lane e % NT of workgroup pg*bpp + e / NT.  One workgroup
+// marching through all phases of its patch group kept only 4 waves per patch group busy; spread over the grid,
+// a phase keeps as many patch groups' workspace traffic in flight as the registers of its own kernel allow.
+constexpr int bpp( int count ) { return ( count + NT - 1 ) / NT; }
+constexpr int N_F1 = 3*NHF, N_F2 = 8000, N_F4 = PS*PS*PS, N_R4 = 3*NFF, N_R2 = 8000, N_R1 = 3*NHF, N_R0 = N3;
+__host__ __device__ constexpr int r3_tasks( int d ) { return NV*( ( d == 0 ) ? 400 : 256 ); }
+
+struct Elem { int pg, e; };
+__device__ __forceinline__ Elem elem( int count )
 {
-   const int tid = threadIdx.x;
-   // F1 half-step fluxes at the 25,200 faces (stored at the low cell)
-   for (int e = tid; e < 3*NHF; e += NT)
-   {
-      const int d = e / NHF;
-      int i, j, k, n, a, b;
-      const int c = box_cell( face_box( d, 0, 21, 1, 20 ), e - d*NHF, i, j, k );
-      split( d, i, j, k, n, a, b );
-      T uL[NV], uR[NV], f[NV];
-      Load<T>::cell( in, tin, c, uL );
-      Load<T>::cell( in, tin, c + STRIDE[d], uR );
-      half_flux<RS>( d, uL, uR, P, f );
-#     pragma unroll
-      for (int v=0; v<NV; v++) F.fh( d, v, c ) = f[v];
-   }
-   __syncthreads();
-   // F2 half-step update of the 20^3 cells (x and y flux differences, then z, as the production kernel)
-   for (int e = tid; e < 8000; e += NT)
-   {
-      int i, j, k;
-      const int c = box_cell( HALF_CELLS, e, i, j, k );
-      T u[NV], h[NV], w[NV];
-      Load<T>::cell( in, tin, c, u );
-#     pragma unroll
-      for (int v=0; v<NV; v++)
-      {
-         const T dxy = ( F.fh( 0, v, c ) - F.fh( 0, v, c - 1 ) ) + ( F.fh( 1, v, c ) - F.fh( 1, v, c - NX ) );
-         h[v] = u[v] - P.dt_dh2*( dxy + ( F.fh( 2, v, c ) - F.fh( 2, v, c - N2 ) ) );
-      }
-      con2pri_floor( h, P, w );
-#     pragma unroll
-      for (int v=0; v<NV; v++) F.w( v, c ) = w[v];
-   }
-   __syncthreads();
+   const int b = bpp( count );
+   return { (int)blockIdx.x / b, ( (int)blockIdx.x % b )*NT + (int)threadIdx.x };
+}
+
+template <typename T>
+__device__ __forceinline__ Prm<T> phase_prm( const Prm<double> &P0, double minmod_coeff )
+{
+   Prm<T> P = prm_as<T>( P0 );
+   P.coeff = vmax( (T)minmod_coeff, (T)0.0 );
+   return P;
 }
 
 template <typename T> struct Store;
@@ -276,58 +263,105 @@ template <> struct Store< Dual<1> >
    __device__ static void put( double *out, double *tout, size_t o, const Dual<1> &x ) { if ( out ) out[o] = x.v;  tout[o] = x.d[0]; }
 };
 
-// F4: full-step update of the 16^3 output cells.  The PLM faces of each face are recomputed from the half-step
-// primitives W (centres c - e_d, c, c + e_d), as the adjoint's R4 does, instead of being stored and re-read.
-template <int RS, int LIM, typename T>
-__device__ void full_update( const double *in, const double *tin, Fwd<T> &F, const Prm<T> &P, double *out, double *tout )
+constexpr size_t NIN = (size_t)NV*N3, NOUT = (size_t)NV*PS*PS*PS;
+
+// F1 half-step fluxes at the 25,200 faces (stored at the low cell); fh_off: workspace slot of Fh
+template <int RS, typename T>
+__global__ void __launch_bounds__( NT )
+f1_kernel( const double *__restrict__ g_in, const double *__restrict__ g_tin, double *__restrict__ g_ws, int fh_off,
+           Prm<double> P0, double minmod_coeff )
 {
-   for (int e = threadIdx.x; e < PS*PS*PS; e += NT)
-   {
-      int i, j, k;
-      const int c = box_cell( CORE_CELLS, e, i, j, k );
-      T u[NV], dF[3][NV];
-      Load<T>::cell( in, tin, c, u );
-      for (int d=0; d<3; d++)
-      {
-         const int s = STRIDE[d];
-         T rm[NV], l0[NV], r0[NV], lp[NV], fm[NV], fp[NV];
-#        pragma unroll
-         for (int v=0; v<NV; v++)
-         {
-            const T w2 = F.w( v, c - 2*s ), w1 = F.w( v, c - s ), w0 = F.w( v, c ), p1 = F.w( v, c + s ), p2 = F.w( v, c + 2*s );
-            T dummy;
-            plm<LIM>( w2, w1, w0, P, dummy, rm[v] );       // R face of c - e_d
-            plm<LIM>( w1, w0, p1, P, l0[v], r0[v] );       // both faces of c
-            plm<LIM>( w0, p1, p2, P, lp[v], dummy );       // L face of c + e_d
-         }
-         full_flux<RS>( d, rm, l0, P, fm );
-         full_flux<RS>( d, r0, lp, P, fp );
-#        pragma unroll
-         for (int v=0; v<NV; v++) dF[d][v] = fp[v] - fm[v];
-      }
-      const size_t o = out_index( i, j, k );
-#     pragma unroll
-      for (int v=0; v<NV; v++)
-         Store<T>::put( out, tout, (size_t)v*PS*PS*PS + o, u[v] - P.dt_dh*( ( dF[0][v] + dF[1][v] ) + dF[2][v] ) );
-   }
+   const Elem x = elem( N_F1 );
+   if ( x.e >= N_F1 ) return;
+   const Prm<T> P = phase_prm<T>( P0, minmod_coeff );
+   const double *in = g_in + x.pg*NIN, *tin = g_tin ? g_tin + x.pg*NIN : nullptr;
+   Fwd<T> F( reinterpret_cast<T*>( g_ws + (size_t)x.pg*WS_ADJ*N3 ), fh_off );
+   const int d = x.e / NHF;
+   int i, j, k;
+   const int c = box_cell( face_box( d, 0, 21, 1, 20 ), x.e - d*NHF, i, j, k );
+   T uL[NV], uR[NV], f[NV];
+   Load<T>::cell( in, tin, c, uL );
+   Load<T>::cell( in, tin, c + STRIDE[d], uR );
+   half_flux<RS>( d, uL, uR, P, f );
+#  pragma unroll
+   for (int v=0; v<NV; v++) F.fh( d, v, c ) = f[v];
 }
 
-// Tangent-linear step (T = Dual<1>: out = U_out, tout = J v) or plain phase-wise forward (T = double).
+// F2 half-step update of the 20^3 cells (x and y flux differences, then z, as the production kernel), floors,
+// primitive variables
+// (h_off >= 0, FP64 only: the half-step state h is also stored, for R2)
+template <typename T>
+__global__ void __launch_bounds__( NT )
+f2_kernel( const double *__restrict__ g_in, const double *__restrict__ g_tin, double *__restrict__ g_ws, int fh_off,
+           int h_off, Prm<double> P0, double minmod_coeff )
+{
+   const Elem x = elem( N_F2 );
+   if ( x.e >= N_F2 ) return;
+   const Prm<T> P = phase_prm<T>( P0, minmod_coeff );
+   const double *in = g_in + x.pg*NIN, *tin = g_tin ? g_tin + x.pg*NIN : nullptr;
+   Fwd<T> F( reinterpret_cast<T*>( g_ws + (size_t)x.pg*WS_ADJ*N3 ), fh_off );
+   int i, j, k;
+   const int c = box_cell( HALF_CELLS, x.e, i, j, k );
+   T u[NV], h[NV], w[NV];
+   Load<T>::cell( in, tin, c, u );
+#  pragma unroll
+   for (int v=0; v<NV; v++)
+   {
+      const T dxy = ( F.fh( 0, v, c ) - F.fh( 0, v, c - 1 ) ) + ( F.fh( 1, v, c ) - F.fh( 1, v, c - NX ) );
+      h[v] = u[v] - P.dt_dh2*( dxy + ( F.fh( 2, v, c ) - F.fh( 2, v, c - N2 ) ) );
+   }
+   con2pri_floor( h, P, w );
+#  pragma unroll
+   for (int v=0; v<NV; v++) F.w( v, c ) = w[v];
+   if constexpr ( std::is_same<T, double>::value )
+      if ( h_off >= 0 )
+      {
+         double *H = g_ws + (size_t)x.pg*WS_ADJ*N3 + (size_t)h_off*N3;
+#        pragma unroll
+         for (int v=0; v<NV; v++) H[ v*N3 + c ] = h[v];
+      }
+}
+
+// F4 (tangent / plain forward): full-step update of the 16^3 output cells.  The PLM faces of each face are
+// recomputed from the half-step primitives W (centres c - e_d, c, c + e_d), as the adjoint's R4 does, instead of
+// being stored and re-read.  T = Dual<1>: out = U_out (optional), tout = J v; T = double: out = U_out.
 template <int RS, int LIM, typename T>
 __global__ void __launch_bounds__( NT )
-tangent_kernel( const double *__restrict__ g_in, const double *__restrict__ g_tin, double *__restrict__ g_out,
-                double *__restrict__ g_tout, double *__restrict__ g_ws, int npg, Prm<double> P0, double minmod_coeff )
+f4_kernel( const double *__restrict__ g_in, const double *__restrict__ g_tin, double *__restrict__ g_out,
+           double *__restrict__ g_tout, double *__restrict__ g_ws, Prm<double> P0, double minmod_coeff )
 {
-   const int pg = blockIdx.x;
-   if ( pg >= npg ) return;
-   Prm<T> P = prm_as<T>( P0 );
-   P.coeff = vmax( (T)minmod_coeff, (T)0.0 );
-   const size_t nin = (size_t)NV*N3, nout = (size_t)NV*PS*PS*PS;
-   const double *in  = g_in + pg*nin;
-   const double *tin = g_tin ? g_tin + pg*nin : nullptr;
-   Fwd<T> F( reinterpret_cast<T*>( g_ws + (size_t)pg*WS_ADJ*N3 ), WS_FH_TAN );
-   forward_phases<RS, LIM, T>( in, tin, F, P );
-   full_update<RS, LIM, T>( in, tin, F, P, g_out ? g_out + pg*nout : nullptr, g_tout ? g_tout + pg*nout : nullptr );
+   const Elem x = elem( N_F4 );
+   if ( x.e >= N_F4 ) return;
+   const Prm<T> P = phase_prm<T>( P0, minmod_coeff );
+   const double *in = g_in + x.pg*NIN, *tin = g_tin ? g_tin + x.pg*NIN : nullptr;
+   double *out = g_out ? g_out + x.pg*NOUT : nullptr, *tout = g_tout ? g_tout + x.pg*NOUT : nullptr;
+   Fwd<T> F( reinterpret_cast<T*>( g_ws + (size_t)x.pg*WS_ADJ*N3 ), WS_FH_TAN );
+   int i, j, k;
+   const int c = box_cell( CORE_CELLS, x.e, i, j, k );
+   T u[NV], dF[3][NV];
+   Load<T>::cell( in, tin, c, u );
+   for (int d=0; d<3; d++)
+   {
+      const int s = STRIDE[d];
+      T rm[NV], l0[NV], r0[NV], lp[NV], fm[NV], fp[NV];
+#     pragma unroll
+      for (int v=0; v<NV; v++)
+      {
+         const T w2 = F.w( v, c - 2*s ), w1 = F.w( v, c - s ), w0 = F.w( v, c ), p1 = F.w( v, c + s ), p2 = F.w( v, c + 2*s );
+         T dummy;
+         plm<LIM>( w2, w1, w0, P, dummy, rm[v] );       // R face of c - e_d
+         plm<LIM>( w1, w0, p1, P, l0[v], r0[v] );       // both faces of c
+         plm<LIM>( w0, p1, p2, P, lp[v], dummy );       // L face of c + e_d
+      }
+      full_flux<RS>( d, rm, l0, P, fm );
+      full_flux<RS>( d, r0, lp, P, fp );
+#     pragma unroll
+      for (int v=0; v<NV; v++) dF[d][v] = fp[v] - fm[v];
+   }
+   const size_t o = out_index( i, j, k );
+#  pragma unroll
+   for (int v=0; v<NV; v++)
+      Store<T>::put( out, tout, (size_t)v*PS*PS*PS + o, u[v] - P.dt_dh*( ( dF[0][v] + dF[1][v] ) + dF[2][v] ) );
 }
 
 // ---------------- reverse phases ----------------
@@ -342,176 +376,263 @@ struct Adj {
 
 typedef Dual<1> D1;
 
+// R4 full-step faces: lambda_F = dt/dh (lambda_out(c+e_d) - lambda_out(c)), then J_RS^T per side.  Only the
+// faces that feed a full-step flux receive a value; R3 reads the others as zero.
 template <int RS, int LIM>
 __global__ void __launch_bounds__( NT )
-adjoint_kernel( const double *__restrict__ g_in, const double *__restrict__ g_lout, double *__restrict__ g_lin,
-                double *__restrict__ g_ws, int npg, Prm<double> P0, double minmod_coeff )
+r4_kernel( const double *__restrict__ g_lout, double *__restrict__ g_ws, Prm<double> P0, double minmod_coeff )
 {
-   const int pg = blockIdx.x, tid = threadIdx.x;
-   if ( pg >= npg ) return;
-   Prm<double> P = P0;
-   P.coeff = vmax( minmod_coeff, 0.0 );
-   Prm<D1> PD = prm_as<D1>( P );
-   const size_t nin = (size_t)NV*N3, nout = (size_t)NV*PS*PS*PS;
-   const double *in   = g_in + pg*nin;
-   const double *lout = g_lout + pg*nout;
-   double *lin = g_lin + pg*nin;
+   const Elem x = elem( N_R4 );
+   if ( x.e >= N_R4 ) return;
+   const Prm<double> P = phase_prm<double>( P0, minmod_coeff );
+   const double *lout = g_lout + x.pg*NOUT;
+   double *ws = g_ws + (size_t)x.pg*WS_ADJ*N3;
+   Fwd<double> F( ws, WS_LS );
+   Adj A( ws );
+   const int d = x.e / NFF;
+   int i, j, k;
+   const int c = box_cell( face_box( d, 2, 17, 3, 16 ), x.e - d*NFF, i, j, k );
+   const int s = STRIDE[d];
+   int i2 = i, j2 = j, k2 = k;
+   if ( d == 0 ) i2++; else if ( d == 1 ) j2++; else k2++;
+   double lf[NV];
+   const bool lo = core3( i, j, k ), hi = core3( i2, j2, k2 );
+#  pragma unroll
+   for (int v=0; v<NV; v++)
+   {
+      const double ll = lo ? lout[ (size_t)v*PS*PS*PS + out_index( i, j, k ) ] : 0.0;
+      const double lh = hi ? lout[ (size_t)v*PS*PS*PS + out_index( i2, j2, k2 ) ] : 0.0;
+      lf[v] = P.dt_dh*( lh - ll );
+   }
+   // the R face of c and the L face of c + e_d (F3's values, recomputed from the half-step primitives)
+   double wl[NV], wr[NV];
+#  pragma unroll
+   for (int v=0; v<NV; v++)
+   {
+      const double wm = F.w( v, c - s ), w0 = F.w( v, c ), w1 = F.w( v, c + s ), w2 = F.w( v, c + 2*s );
+      double fl, fr;
+      plm<LIM>( wm, w0, w1, P, fl, wl[v] );
+      plm<LIM>( w0, w1, w2, P, wr[v], fr );
+   }
+   // one reverse sweep for both sides
+   double gl[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 }, gr[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+   full_flux_rev<RS>( d, wl, wr, P, lf, gl, gr );
+#  pragma unroll
+   for (int v=0; v<NV; v++) { A.lfc( d, 1, v, c ) = gl[v];  A.lfc( d, 0, v, c + s ) = gr[v]; }
+}
+
+// R3 lambda_W of the 20^3 half-step cells along direction d (one launch per direction, in order 0, 1, 2).
+// Task (v, line): the lane marches the line through the reconstruction stencils (centres 2..19) of field v and
+// reverses each one (plm_face_rev), keeping a window of the three cell adjoints the stencil touches; a cell is
+// complete once the stencil above it is done.  Direction 0 writes every cell of the 20^3 block (zero on lines no
+// full-step face reads), directions 1 and 2 add.  The faces of a centre that R4 wrote: the L face when the
+// centre's normal coordinate is 3..19, the R face when it is 2..18, with both tangential coordinates in 3..18.
+template <int LIM>
+__global__ void __launch_bounds__( NT )
+r3_kernel( double *__restrict__ g_ws, Prm<double> P0, double minmod_coeff, int d )
+{
+   const int ntask = r3_tasks( d ), nline = ntask/NV;
+   const Elem x = elem( ntask );
+   if ( x.e >= ntask ) return;
+   const Prm<double> P = phase_prm<double>( P0, minmod_coeff );
+   double *ws = g_ws + (size_t)x.pg*WS_ADJ*N3;
+   Fwd<double> F( ws, WS_LS );
+   Adj A( ws );
+   const int  s = STRIDE[d], v = x.e / nline, ln = x.e - v*nline;
+   const int  a = ( d == 0 ) ? 1 + ln % 20 : 3 + ln % 16, b = ( d == 0 ) ? 1 + ln / 20 : 3 + ln / 16;
+   const bool live = in_rng( a, 3, 18 ) && in_rng( b, 3, 18 );
+   const int  c0 = ( d == 0 ) ? NX*a + N2*b : ( d == 1 ) ? a + N2*b : a + NX*b;     // n = 0 of the line
+   if ( !live )
+   {
+      for (int n=1; n<=20; n++) A.lw( v, c0 + n*s ) = 0.0;
+      return;
+   }
+   double wm = F.w( v, c0 + s ), w0 = F.w( v, c0 + 2*s ), am = 0.0, a0 = 0.0;
+   for (int n=2; n<=19; n++)
+   {
+      const int    cc = c0 + n*s;
+      const double wp = F.w( v, cc + s );
+      const double gl = ( n >= 3 )  ? A.lfc( d, 0, v, cc ) : 0.0;
+      const double gr = ( n <= 18 ) ? A.lfc( d, 1, v, cc ) : 0.0;
+      double br = 0.0;
+      plm_face_rev<LIM>( wm, w0, wp, P.limiter, P.coeff, gl, gr, am, a0, br );
+      if ( d == 0 ) A.lw( v, cc - s ) = am; else A.lw( v, cc - s ) += am;
+      am = a0;  a0 = br;  wm = w0;  w0 = wp;
+   }
+   if ( d == 0 ) { A.lw( v, c0 + 19*s ) = am;   A.lw( v, c0 + 20*s ) = a0; }
+   else          { A.lw( v, c0 + 19*s ) += am;  A.lw( v, c0 + 20*s ) += a0; }
+}
+
+// R3 along x (d = 0) with the lines staged through LDS.  The x lines of one (z, field) pair are the 20 rows
+// j = 1..20 of one z plane, 440 consecutive doubles of each workspace array, so a workgroup loads W and the two
+// face-adjoint arrays of R3X_PAIRS pairs coalesced, lane t marches row 1 + t % 20 of pair t / 20 in LDS (the
+// adjoint of row cell n - 1 replaces its W, which the march has passed), and the rows go back coalesced.
+// Marching lanes over x lines directly puts 64 different rows under every load instruction.
+constexpr int R3X_PAIRS = 6, R3X_NT = 128, R3X_RL = 23, R3X_PAIR = 20*R3X_RL, R3X_NPAIR = 20*NV;
+constexpr int R3X_BPP = ( R3X_NPAIR + R3X_PAIRS - 1 )/R3X_PAIRS;
+static_assert( R3X_PAIRS*20 <= R3X_NT, "one lane per staged row" );
+
+template <int LIM>
+__global__ void __launch_bounds__( R3X_NT )
+r3x_kernel( double *__restrict__ g_ws, Prm<double> P0, double minmod_coeff )
+{
+   __shared__ double sW[R3X_PAIRS*R3X_PAIR], sG0[R3X_PAIRS*R3X_PAIR], sG1[R3X_PAIRS*R3X_PAIR];
+   const int tid = threadIdx.x, pg = blockIdx.x / R3X_BPP, q0 = ( blockIdx.x % R3X_BPP )*R3X_PAIRS;
+   const int np = min( R3X_PAIRS, R3X_NPAIR - q0 );
+   const Prm<double> P = phase_prm<double>( P0, minmod_coeff );
    double *ws = g_ws + (size_t)pg*WS_ADJ*N3;
    Fwd<double> F( ws, WS_LS );
    Adj A( ws );
-
-   // F1, F2 (the PLM faces are recomputed where they are needed instead of stored)
-   forward_phases<RS, LIM, double>( in, nullptr, F, P );
-
-   // R4 full-step faces: lambda_F = dt/dh (lambda_out(c+e_d) - lambda_out(c)), then J_RS^T per side.  Only the
-   // faces that feed a full-step flux receive a value; R3 reads the others as zero.
-   for (int e = tid; e < 3*NFF; e += NT)
+   // pair q: field q / 20, z = 1 + q % 20; its rows j = 1..20 start at cell NX + N2 z, a 16-byte aligned offset
+   // (NX and N2 are even), so the block moves as 220 double2 per array, R3X_LD of them per lane in flight
+   constexpr int U = 20*NX/2, R3X_LD = 4;
+   const int nu = np*U;
+   for (int u0 = tid; u0 < nu; u0 += R3X_LD*R3X_NT)
    {
-      const int d = e / NFF;
-      int i, j, k, n, a, b;
-      const int c = box_cell( face_box( d, 2, 17, 3, 16 ), e - d*NFF, i, j, k );
-      split( d, i, j, k, n, a, b );
-      const int s = STRIDE[d];
-      int i2 = i, j2 = j, k2 = k;
-      if ( d == 0 ) i2++; else if ( d == 1 ) j2++; else k2++;
-      double lf[NV];
-      const bool lo = core3( i, j, k ), hi = core3( i2, j2, k2 );
+      double2 a[R3X_LD], b[R3X_LD], g[R3X_LD];
 #     pragma unroll
-      for (int v=0; v<NV; v++)
+      for (int t=0; t<R3X_LD; t++)
       {
-         const double ll = lo ? lout[ (size_t)v*PS*PS*PS + out_index( i, j, k ) ] : 0.0;
-         const double lh = hi ? lout[ (size_t)v*PS*PS*PS + out_index( i2, j2, k2 ) ] : 0.0;
-         lf[v] = P.dt_dh*( lh - ll );
+         const int u = u0 + t*R3X_NT;
+         if ( u >= nu ) continue;
+         const int p = u / U, q = q0 + p, v = q / 20, c = NX + N2*( 1 + q % 20 ) + 2*( u - p*U );
+         a[t] = *reinterpret_cast<const double2*>( &F.w( v, c ) );
+         b[t] = *reinterpret_cast<const double2*>( &A.lfc( 0, 0, v, c ) );
+         g[t] = *reinterpret_cast<const double2*>( &A.lfc( 0, 1, v, c ) );
       }
-      // the R face of c and the L face of c + e_d (F3's values, recomputed from the half-step primitives)
-      double wl[NV], wr[NV];
 #     pragma unroll
-      for (int v=0; v<NV; v++)
+      for (int t=0; t<R3X_LD; t++)
       {
-         const double wm = F.w( v, c - s ), w0 = F.w( v, c ), w1 = F.w( v, c + s ), w2 = F.w( v, c + 2*s );
-         double fl, fr;
-         plm<LIM>( wm, w0, w1, P, fl, wl[v] );
-         plm<LIM>( w0, w1, w2, P, wr[v], fr );
+         const int u = u0 + t*R3X_NT;
+         if ( u >= nu ) continue;
+         const int p = u / U, r = 2*( u - p*U ), l = p*R3X_PAIR + ( r / NX )*R3X_RL + r % NX;   // r, r + 1: one row
+         sW[l] = a[t].x;   sW[l + 1] = a[t].y;
+         sG0[l] = b[t].x;  sG0[l + 1] = b[t].y;
+         sG1[l] = g[t].x;  sG1[l + 1] = g[t].y;
       }
-      // one reverse sweep for both sides
-      double gl[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 }, gr[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
-      full_flux_rev<RS>( d, wl, wr, P, lf, gl, gr );
-#     pragma unroll
-      for (int v=0; v<NV; v++) { A.lfc( d, 1, v, c ) = gl[v];  A.lfc( d, 0, v, c + s ) = gr[v]; }
    }
    __syncthreads();
-
-   // R3 lambda_W of the 20^3 half-step cells, one direction at a time.  Lane t marches line t along d through the
-   // reconstruction stencils (centres 2..19) and reverses each one (plm_face_rev), keeping a window of the three
-   // cell adjoints the stencil touches; a cell is complete once the stencil above it is done.  Direction 0 writes
-   // every cell of the 20^3 block (zero on lines no full-step face reads), directions 1 and 2 add.  The faces of
-   // a centre that R4 wrote: the L face when the centre's normal coordinate is 3..19, the R face when it is 2..18,
-   // with both tangential coordinates in 3..18.
-   for (int d=0; d<3; d++)
+   if ( tid < np*20 )
    {
-      const int s = STRIDE[d];
-      const int nline = ( d == 0 ) ? 400 : 256;
-      for (int ln = tid; ln < nline; ln += NT)
+      const int p = tid / 20, j = 1 + tid % 20, k = 1 + ( q0 + p ) % 20;
+      double *row = sW + p*R3X_PAIR + ( j - 1 )*R3X_RL;
+      const double *g0 = sG0 + p*R3X_PAIR + ( j - 1 )*R3X_RL, *g1 = sG1 + p*R3X_PAIR + ( j - 1 )*R3X_RL;
+      if ( !( in_rng( j, 3, 18 ) && in_rng( k, 3, 18 ) ) )
       {
-         const int  a = ( d == 0 ) ? 1 + ln % 20 : 3 + ln % 16, b = ( d == 0 ) ? 1 + ln / 20 : 3 + ln / 16;
-         const bool live = in_rng( a, 3, 18 ) && in_rng( b, 3, 18 );
-         const int  c0 = ( d == 0 ) ? NX*a + N2*b : ( d == 1 ) ? a + N2*b : a + NX*b;     // n = 0 of the line
-         for (int v=0; v<NV; v++)
+         for (int n=1; n<=20; n++) row[n] = 0.0;
+      }
+      else
+      {
+         double wm = row[1], w0 = row[2], am = 0.0, a0 = 0.0;
+         for (int n=2; n<=19; n++)
          {
-            if ( !live )
-            {
-               for (int n=1; n<=20; n++) A.lw( v, c0 + n*s ) = 0.0;
-               continue;
-            }
-            double wm = F.w( v, c0 + s ), w0 = F.w( v, c0 + 2*s ), am = 0.0, a0 = 0.0;
-            for (int n=2; n<=19; n++)
-            {
-               const int    cc = c0 + n*s;
-               const double wp = F.w( v, cc + s );
-               const double gl = ( n >= 3 )  ? A.lfc( d, 0, v, cc ) : 0.0;
-               const double gr = ( n <= 18 ) ? A.lfc( d, 1, v, cc ) : 0.0;
-               double br = 0.0;
-               plm_face_rev<LIM>( wm, w0, wp, P.limiter, P.coeff, gl, gr, am, a0, br );
-               if ( d == 0 ) A.lw( v, cc - s ) = am; else A.lw( v, cc - s ) += am;
-               am = a0;  a0 = br;  wm = w0;  w0 = wp;
-            }
-            if ( d == 0 ) { A.lw( v, c0 + 19*s ) = am;   A.lw( v, c0 + 20*s ) = a0; }
-            else          { A.lw( v, c0 + 19*s ) += am;  A.lw( v, c0 + 20*s ) += a0; }
+            const double wp = row[n + 1];
+            const double gl = ( n >= 3 )  ? g0[n] : 0.0;
+            const double gr = ( n <= 18 ) ? g1[n] : 0.0;
+            double br = 0.0;
+            plm_face_rev<LIM>( wm, w0, wp, P.limiter, P.coeff, gl, gr, am, a0, br );
+            row[n - 1] = am;
+            am = a0;  a0 = br;  wm = w0;  w0 = wp;
          }
-      }
-      __syncthreads();
-   }
-
-   // R2 lambda of the half-step conserved state: J_Con2Pri^T lambda_W (floors included)
-   for (int e = tid; e < 8000; e += NT)
-   {
-      int i, j, k;
-      const int c = box_cell( HALF_CELLS, e, i, j, k );
-      double h[NV], lw[NV];
-#     pragma unroll
-      for (int v=0; v<NV; v++)
-      {
-         const double dxy = ( F.fh( 0, v, c ) - F.fh( 0, v, c - 1 ) ) + ( F.fh( 1, v, c ) - F.fh( 1, v, c - NX ) );
-         h[v]  = in[ v*N3 + c ] - P.dt_dh2*( dxy + ( F.fh( 2, v, c ) - F.fh( 2, v, c - N2 ) ) );
-         lw[v] = A.lw( v, c );
-      }
-      for (int q=0; q<NV; q++)
-      {
-         D1 hd[NV], w[NV];
-#        pragma unroll
-         for (int v=0; v<NV; v++) { hd[v] = D1( h[v] );  hd[v].d[0] = ( q == v ) ? 1.0 : 0.0; }
-         con2pri_floor( hd, PD, w );
-         double g = 0.0;
-#        pragma unroll
-         for (int v=0; v<NV; v++) g += lw[v]*w[v].d[0];
-         A.lu( q, c ) = g;
+         row[19] = am;  row[20] = a0;
       }
    }
    __syncthreads();
-
-   // R1 half-step faces: lambda_Fh = dt/(2dh) (lambda_U(c+e_d) - lambda_U(c)), then J_RS^T per side.
-   // (The Fh arrays alias the low-side slots written here; every reader of Fh has passed the barrier above.)
-   for (int e = tid; e < 3*NHF; e += NT)
+   for (int e = tid; e < np*400; e += R3X_NT)
    {
-      const int d = e / NHF;
-      int i, j, k, n, a, b;
-      const int c = box_cell( face_box( d, 0, 21, 1, 20 ), e - d*NHF, i, j, k );
-      split( d, i, j, k, n, a, b );
-      const int s = STRIDE[d];
-      const bool lo = in_rng( n, 1, 20 ), hi = in_rng( n + 1, 1, 20 );   // tangential range is 1..20 already
-      double lf[NV];
-#     pragma unroll
-      for (int v=0; v<NV; v++) lf[v] = P.dt_dh2*( ( hi ? A.lu( v, c + s ) : 0.0 ) - ( lo ? A.lu( v, c ) : 0.0 ) );
-      double uL[NV], uR[NV];
-#     pragma unroll
-      for (int v=0; v<NV; v++) { uL[v] = in[ v*N3 + c ];  uR[v] = in[ v*N3 + c + s ]; }
-      double gl[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 }, gr[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
-      half_flux_rev<RS>( d, uL, uR, P, lf, gl, gr );
-#     pragma unroll
-      for (int v=0; v<NV; v++) { A.ls( d, 0, v, c ) = gl[v];  A.ls( d, 1, v, c + s ) = gr[v]; }
+      const int p = e / 400, r = e - p*400, q = q0 + p, v = q / 20, j = 1 + r / 20, n = 1 + r % 20;
+      A.lw( v, n + NX*j + N2*( 1 + q % 20 ) ) = sW[ p*R3X_PAIR + ( j - 1 )*R3X_RL + n ];
    }
-   __syncthreads();
+}
 
-   // R0 lambda_in: direct dependence of the output and of the half-step state on U_in, plus the half-step faces
-   for (int c = tid; c < N3; c += NT)
+// R2 lambda of the half-step conserved state: J_Con2Pri^T lambda_W (floors included), at the state F2 stored
+__global__ void __launch_bounds__( NT )
+r2_kernel( double *__restrict__ g_ws, Prm<double> P0, double minmod_coeff )
+{
+   const Elem x = elem( N_R2 );
+   if ( x.e >= N_R2 ) return;
+   const Prm<double> P = phase_prm<double>( P0, minmod_coeff );
+   const Prm<D1> PD = prm_as<D1>( P );
+   double *ws = g_ws + (size_t)x.pg*WS_ADJ*N3;
+   const double *H = ws + (size_t)WS_H*N3;
+   Adj A( ws );
+   int i, j, k;
+   const int c = box_cell( HALF_CELLS, x.e, i, j, k );
+   double h[NV], lw[NV];
+#  pragma unroll
+   for (int v=0; v<NV; v++)
    {
-      int i, j, k;
-      coords( c, i, j, k );
-      const bool core = core3( i, j, k ), half = in_rng( i, 1, 20 ) && in_rng( j, 1, 20 ) && in_rng( k, 1, 20 );
+      h[v]  = H[ v*N3 + c ];
+      lw[v] = A.lw( v, c );
+   }
+   for (int q=0; q<NV; q++)
+   {
+      D1 hd[NV], w[NV];
 #     pragma unroll
-      for (int v=0; v<NV; v++)
+      for (int v=0; v<NV; v++) { hd[v] = D1( h[v] );  hd[v].d[0] = ( q == v ) ? 1.0 : 0.0; }
+      con2pri_floor( hd, PD, w );
+      double g = 0.0;
+#     pragma unroll
+      for (int v=0; v<NV; v++) g += lw[v]*w[v].d[0];
+      A.lu( q, c ) = g;
+   }
+}
+
+// R1 half-step faces: lambda_Fh = dt/(2dh) (lambda_U(c+e_d) - lambda_U(c)), then J_RS^T per side.
+// (Fh and h alias the slots written here; F2 and R2, their last readers, have completed.)
+template <int RS>
+__global__ void __launch_bounds__( NT )
+r1_kernel( const double *__restrict__ g_in, double *__restrict__ g_ws, Prm<double> P0, double minmod_coeff )
+{
+   const Elem x = elem( N_R1 );
+   if ( x.e >= N_R1 ) return;
+   const Prm<double> P = phase_prm<double>( P0, minmod_coeff );
+   const double *in = g_in + x.pg*NIN;
+   Adj A( g_ws + (size_t)x.pg*WS_ADJ*N3 );
+   const int d = x.e / NHF;
+   int i, j, k, n, a, b;
+   const int c = box_cell( face_box( d, 0, 21, 1, 20 ), x.e - d*NHF, i, j, k );
+   split( d, i, j, k, n, a, b );
+   const int s = STRIDE[d];
+   const bool lo = in_rng( n, 1, 20 ), hi = in_rng( n + 1, 1, 20 );   // tangential range is 1..20 already
+   double lf[NV];
+#  pragma unroll
+   for (int v=0; v<NV; v++) lf[v] = P.dt_dh2*( ( hi ? A.lu( v, c + s ) : 0.0 ) - ( lo ? A.lu( v, c ) : 0.0 ) );
+   double uL[NV], uR[NV];
+#  pragma unroll
+   for (int v=0; v<NV; v++) { uL[v] = in[ v*N3 + c ];  uR[v] = in[ v*N3 + c + s ]; }
+   double gl[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 }, gr[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+   half_flux_rev<RS>( d, uL, uR, P, lf, gl, gr );
+#  pragma unroll
+   for (int v=0; v<NV; v++) { A.ls( d, 0, v, c ) = gl[v];  A.ls( d, 1, v, c + s ) = gr[v]; }
+}
+
+// R0 lambda_in: direct dependence of the output and of the half-step state on U_in, plus the half-step faces
+__global__ void __launch_bounds__( NT )
+r0_kernel( const double *__restrict__ g_lout, double *__restrict__ g_lin, double *__restrict__ g_ws )
+{
+   const Elem x = elem( N_R0 );
+   if ( x.e >= N_R0 ) return;
+   const double *lout = g_lout + x.pg*NOUT;
+   double *lin = g_lin + x.pg*NIN;
+   Adj A( g_ws + (size_t)x.pg*WS_ADJ*N3 );
+   const int c = x.e;
+   int i, j, k;
+   coords( c, i, j, k );
+   const bool core = core3( i, j, k ), half = in_rng( i, 1, 20 ) && in_rng( j, 1, 20 ) && in_rng( k, 1, 20 );
+#  pragma unroll
+   for (int v=0; v<NV; v++)
+   {
+      double acc = core ? lout[ (size_t)v*PS*PS*PS + out_index( i, j, k ) ] : 0.0;
+      if ( half ) acc += A.lu( v, c );
+      for (int d=0; d<3; d++)
       {
-         double acc = core ? lout[ (size_t)v*PS*PS*PS + out_index( i, j, k ) ] : 0.0;
-         if ( half ) acc += A.lu( v, c );
-         for (int d=0; d<3; d++)
-         {
-            int n, a, b;
-            split( d, i, j, k, n, a, b );
-            if ( half_face( n, a, b ) )     acc += A.ls( d, 0, v, c );      // c is the low cell of face c+1/2
-            if ( half_face( n - 1, a, b ) ) acc += A.ls( d, 1, v, c );      // c is the high cell of face c-1/2
-         }
-         lin[ v*N3 + c ] = acc;
+         int n, a, b;
+         split( d, i, j, k, n, a, b );
+         if ( half_face( n, a, b ) )     acc += A.ls( d, 0, v, c );      // c is the low cell of face c+1/2
+         if ( half_face( n - 1, a, b ) ) acc += A.ls( d, 1, v, c );      // c is the high cell of face c-1/2
       }
+      lin[ v*N3 + c ] = acc;
    }
 }
 
@@ -562,41 +683,95 @@ __global__ void prepare_adjoint_kernel( const IO *__restrict__ pg_lam, IO *__res
 
 size_t adjoint_workspace_bytes( int npg ) { return (size_t)( npg > 0 ? npg : 0 )*WS_ADJ*N3*sizeof( double ); }
 
+namespace {
+
+// one flat grid of bpp(count) workgroups per patch group; false when npg patch groups would not fit one launch
+bool phase_grid( int npg, int count, dim3 &grid )
+{
+   const long blocks = (long)npg*bpp( count );
+   if ( blocks*NT >= ( 1L << 32 ) ) return false;
+   grid = dim3( (unsigned)blocks );
+   return true;
+}
+
+template <int RS, int LIM>
+int adjoint_phases( const double *in, const double *lam_out, double *lam_in, double *ws, int npg, double minmod_coeff,
+                    const Prm<double> &P, hipStream_t stream )
+{
+   const dim3 block( NT );
+   dim3 g;
+   // forward: F1, F2 (the PLM faces are recomputed where they are needed instead of stored)
+   if ( !phase_grid( npg, N_F1, g ) ) return -3;
+   hipLaunchKernelGGL( ( f1_kernel<RS, double> ), g, block, 0, stream, in, nullptr, ws, WS_LS, P, minmod_coeff );
+   phase_grid( npg, N_F2, g );
+   hipLaunchKernelGGL( ( f2_kernel<double> ), g, block, 0, stream, in, nullptr, ws, WS_LS, WS_H, P, minmod_coeff );
+   // reverse: R4, R3 (three directions in order), R2, R1, R0
+   phase_grid( npg, N_R4, g );
+   hipLaunchKernelGGL( ( r4_kernel<RS, LIM> ), g, block, 0, stream, lam_out, ws, P, minmod_coeff );
+   hipLaunchKernelGGL( ( r3x_kernel<LIM> ), dim3( (unsigned)( npg*R3X_BPP ) ), dim3( R3X_NT ), 0, stream, ws, P, minmod_coeff );
+   for (int d=1; d<3; d++)
+   {
+      phase_grid( npg, r3_tasks( d ), g );
+      hipLaunchKernelGGL( ( r3_kernel<LIM> ), g, block, 0, stream, ws, P, minmod_coeff, d );
+   }
+   phase_grid( npg, N_R2, g );
+   hipLaunchKernelGGL( r2_kernel, g, block, 0, stream, ws, P, minmod_coeff );
+   phase_grid( npg, N_R1, g );
+   hipLaunchKernelGGL( ( r1_kernel<RS> ), g, block, 0, stream, in, ws, P, minmod_coeff );
+   phase_grid( npg, N_R0, g );
+   hipLaunchKernelGGL( r0_kernel, g, block, 0, stream, lam_out, lam_in, ws );
+   return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
+}
+
+template <int RS, int LIM, typename T>
+int tangent_phases( const double *in, const double *tin, double *out, double *tout, double *ws, int npg,
+                    double minmod_coeff, const Prm<double> &P, hipStream_t stream )
+{
+   const dim3 block( NT );
+   dim3 g;
+   if ( !phase_grid( npg, N_F1, g ) ) return -3;
+   hipLaunchKernelGGL( ( f1_kernel<RS, T> ), g, block, 0, stream, in, tin, ws, WS_FH_TAN, P, minmod_coeff );
+   phase_grid( npg, N_F2, g );
+   hipLaunchKernelGGL( ( f2_kernel<T> ), g, block, 0, stream, in, tin, ws, WS_FH_TAN, -1, P, minmod_coeff );
+   phase_grid( npg, N_F4, g );
+   hipLaunchKernelGGL( ( f4_kernel<RS, LIM, T> ), g, block, 0, stream, in, tin, out, tout, ws, P, minmod_coeff );
+   return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
+}
+
+} // namespace
+
 int launch_mhm_rp_adjoint( const double *in, const double *lam_out, double *lam_in, double *ws, int npg, int rsolver,
                            double minmod_coeff, const Prm<double> &P, hipStream_t stream )
 {
    if ( npg <= 0 ) return 0;
-   const dim3 grid( npg ), block( NT );
-#  define GA_ADJ( RS_ ) do { if ( kernel_limiter( P.limiter, minmod_coeff ) == 4 ) hipLaunchKernelGGL( ( adjoint_kernel<RS_, 4> ), grid, block, 0, stream, in, lam_out, lam_in, ws, npg, P, minmod_coeff ); \
-                             else                  hipLaunchKernelGGL( ( adjoint_kernel<RS_, 0> ), grid, block, 0, stream, in, lam_out, lam_in, ws, npg, P, minmod_coeff ); } while ( 0 )
+   const bool l4 = ( kernel_limiter( P.limiter, minmod_coeff ) == 4 );
+#  define GA_ADJ( RS_ ) ( l4 ? adjoint_phases<RS_, 4>( in, lam_out, lam_in, ws, npg, minmod_coeff, P, stream ) \
+                             : adjoint_phases<RS_, 0>( in, lam_out, lam_in, ws, npg, minmod_coeff, P, stream ) )
    switch ( rsolver )
    {
-      case 4: GA_ADJ( 4 ); break;
-      case 3: GA_ADJ( 3 ); break;
-      case 2: GA_ADJ( 2 ); break;
+      case 4: return GA_ADJ( 4 );
+      case 3: return GA_ADJ( 3 );
+      case 2: return GA_ADJ( 2 );
       default: return -1;
    }
 #  undef GA_ADJ
-   return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
 }
 
 int launch_mhm_rp_tangent( const double *in, const double *tin, double *out, double *tout, double *ws, int npg,
                            int rsolver, double minmod_coeff, const Prm<double> &P, hipStream_t stream )
 {
    if ( npg <= 0 ) return 0;
-   const dim3 grid( npg ), block( NT );
-#  define GA_TAN( RS_, T_ ) do { if ( kernel_limiter( P.limiter, minmod_coeff ) == 4 ) hipLaunchKernelGGL( ( tangent_kernel<RS_, 4, T_> ), grid, block, 0, stream, in, tin, out, tout, ws, npg, P, minmod_coeff ); \
-                                 else                  hipLaunchKernelGGL( ( tangent_kernel<RS_, 0, T_> ), grid, block, 0, stream, in, tin, out, tout, ws, npg, P, minmod_coeff ); } while ( 0 )
-   const bool tl = ( tin != nullptr );
+   const bool l4 = ( kernel_limiter( P.limiter, minmod_coeff ) == 4 ), tl = ( tin != nullptr );
+#  define GA_TAN( RS_, T_ ) ( l4 ? tangent_phases<RS_, 4, T_>( in, tin, out, tout, ws, npg, minmod_coeff, P, stream ) \
+                                 : tangent_phases<RS_, 0, T_>( in, tin, out, tout, ws, npg, minmod_coeff, P, stream ) )
    switch ( rsolver )
    {
-      case 4: if ( tl ) GA_TAN( 4, D1 ); else GA_TAN( 4, double ); break;
-      case 3: if ( tl ) GA_TAN( 3, D1 ); else GA_TAN( 3, double ); break;
-      case 2: if ( tl ) GA_TAN( 2, D1 ); else GA_TAN( 2, double ); break;
+      case 4: return tl ? GA_TAN( 4, D1 ) : GA_TAN( 4, double );
+      case 3: return tl ? GA_TAN( 3, D1 ) : GA_TAN( 3, double );
+      case 2: return tl ? GA_TAN( 2, D1 ) : GA_TAN( 2, double );
       default: return -1;
    }
 #  undef GA_TAN
-   return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
 }
 
 template <typename IO>
