@@ -5,8 +5,8 @@
 // Explicit phases over a per-PG global workspace, each phase one kernel over all patch groups with one face or
 // cell per lane:
 //
-//   forward   F1 half-step fluxes Fh (Hydro_RiemannPredict_Flux)   -> ws
-//             F2 half-step update, floors, primitive W (Hydro_RiemannPredict)   -> ws
+//   forward   F1 + F2, one z plane per workgroup: half-step fluxes (Hydro_RiemannPredict_Flux) in LDS, then the
+//                half-step update, floors and primitive W (Hydro_RiemannPredict) -> ws
 //             F4 (tangent / plain forward only) PLM faces (Hydro_DataReconstruction), full-step fluxes and update
 //                of the 16^3 output cells
 //   reverse   R4 lambda_F of each full-step face from lambda_out (transpose of Hydro_FullStepUpdate),
@@ -196,17 +196,15 @@ __device__ __forceinline__ void plm( const T &L, const T &C, const T &R, const P
 }
 
 // ---------------- workspace views ----------------
-// The adjoint keeps Fh (read by F2) and h (read by R2) in the lambda-side slots, which R1 overwrites; the tangent kernel stores
-// dual numbers (two doubles each) and packs Fh right after W: 20 duals = 40 doubles per cell.
-constexpr int WS_FH_TAN = 5;
-static_assert( 2*( WS_FH_TAN + 15 ) <= WS_ADJ, "tangent workspace must fit the per-PG slot" );
+// The adjoint keeps h (read by R2) in lambda-side slots that R1 overwrites; the tangent kernels store W as dual
+// numbers (two doubles each).
+static_assert( 2*NV <= WS_ADJ, "tangent workspace must fit the per-PG slot" );
 
 template <typename T>
 struct Fwd {
-   T *W, *Fh;
-   __device__ Fwd( T *base, int fh_off ) : W( base + WS_W*N3 ), Fh( base + (size_t)fh_off*N3 ) {}
-   __device__ T &w( int v, int c )                { return W[ v*N3 + c ]; }
-   __device__ T &fh( int d, int v, int c )        { return Fh[ ( d*NV + v )*N3 + c ]; }
+   T *W;
+   __device__ Fwd( T *base ) : W( base + WS_W*N3 ) {}
+   __device__ T &w( int v, int c ) { return W[ v*N3 + c ]; }
 };
 
 // load cell c of the input (value, and tangent for Dual<1>)
@@ -235,7 +233,7 @@ template <> struct Load< Dual<1> >
 // marching through all phases of its patch group kept only 4 waves per patch group busy; spread over the grid,
 // a phase keeps as many patch groups' workspace traffic in flight as the registers of its own kernel allow.
 constexpr int bpp( int count ) { return ( count + NT - 1 ) / NT; }
-constexpr int N_F1 = 3*NHF, N_F2 = 8000, N_F4 = PS*PS*PS, N_R4 = 3*NFF, N_R2 = 8000, N_R1 = 3*NHF, N_R0 = N3;
+constexpr int N_F4 = PS*PS*PS, N_R4 = 3*NFF, N_R2 = 8000, N_R1 = 3*NHF, N_R0 = N3;
 __host__ __device__ constexpr int r3_tasks( int d ) { return NV*( ( d == 0 ) ? 400 : 256 ); }
 
 struct Elem { int pg, e; };
@@ -265,61 +263,69 @@ template <> struct Store< Dual<1> >
 
 constexpr size_t NIN = (size_t)NV*N3, NOUT = (size_t)NV*PS*PS*PS;
 
-// F1 half-step fluxes at the 25,200 faces (stored at the low cell); fh_off: workspace slot of Fh
+// F1 + F2 for one z plane k per workgroup: the half-step fluxes of the plane's x and y faces and of the z faces
+// below and above it (Hydro_RiemannPredict_Flux) go to LDS, then the plane's 20^2 cells take the half-step update
+// (x and y flux differences, then z, as the production kernel), the floors and the primitive conversion
+// (Hydro_RiemannPredict).  The fluxes never reach memory; the price is that each interior z face is solved by
+// both planes it separates (32,800 solves per PG instead of 25,200).  h_off >= 0 (FP64 only): the half-step
+// state h is stored too, for R2.
+constexpr int F12_NT = 512, F12_NXF = 21*20, F12_NZF = 20*20, F12_NF = 2*F12_NXF + 2*F12_NZF;
 template <int RS, typename T>
-__global__ void __launch_bounds__( NT )
-f1_kernel( const double *__restrict__ g_in, const double *__restrict__ g_tin, double *__restrict__ g_ws, int fh_off,
-           Prm<double> P0, double minmod_coeff )
+__global__ void __launch_bounds__( F12_NT )
+f12_kernel( const double *__restrict__ g_in, const double *__restrict__ g_tin, double *__restrict__ g_ws, int h_off,
+            Prm<double> P0, double minmod_coeff )
 {
-   const Elem x = elem( N_F1 );
-   if ( x.e >= N_F1 ) return;
+   __shared__ double s_raw[NV*F12_NF*( sizeof( T )/sizeof( double ) )];
+   T *sF = reinterpret_cast<T*>( s_raw );                    // [v][face]
+   const int tid = threadIdx.x, pg = blockIdx.x / 20, k = 1 + blockIdx.x % 20;
    const Prm<T> P = phase_prm<T>( P0, minmod_coeff );
-   const double *in = g_in + x.pg*NIN, *tin = g_tin ? g_tin + x.pg*NIN : nullptr;
-   Fwd<T> F( reinterpret_cast<T*>( g_ws + (size_t)x.pg*WS_ADJ*N3 ), fh_off );
-   const int d = x.e / NHF;
-   int i, j, k;
-   const int c = box_cell( face_box( d, 0, 21, 1, 20 ), x.e - d*NHF, i, j, k );
-   T uL[NV], uR[NV], f[NV];
-   Load<T>::cell( in, tin, c, uL );
-   Load<T>::cell( in, tin, c + STRIDE[d], uR );
-   half_flux<RS>( d, uL, uR, P, f );
-#  pragma unroll
-   for (int v=0; v<NV; v++) F.fh( d, v, c ) = f[v];
-}
-
-// F2 half-step update of the 20^3 cells (x and y flux differences, then z, as the production kernel), floors,
-// primitive variables
-// (h_off >= 0, FP64 only: the half-step state h is also stored, for R2)
-template <typename T>
-__global__ void __launch_bounds__( NT )
-f2_kernel( const double *__restrict__ g_in, const double *__restrict__ g_tin, double *__restrict__ g_ws, int fh_off,
-           int h_off, Prm<double> P0, double minmod_coeff )
-{
-   const Elem x = elem( N_F2 );
-   if ( x.e >= N_F2 ) return;
-   const Prm<T> P = phase_prm<T>( P0, minmod_coeff );
-   const double *in = g_in + x.pg*NIN, *tin = g_tin ? g_tin + x.pg*NIN : nullptr;
-   Fwd<T> F( reinterpret_cast<T*>( g_ws + (size_t)x.pg*WS_ADJ*N3 ), fh_off );
-   int i, j, k;
-   const int c = box_cell( HALF_CELLS, x.e, i, j, k );
-   T u[NV], h[NV], w[NV];
-   Load<T>::cell( in, tin, c, u );
-#  pragma unroll
-   for (int v=0; v<NV; v++)
+   const double *in = g_in + pg*NIN, *tin = g_tin ? g_tin + pg*NIN : nullptr;
+   Fwd<T> F( reinterpret_cast<T*>( g_ws + (size_t)pg*WS_ADJ*N3 ) );
+   // faces, stored at their low cell: x (normal i = 0..20, j = 1..20), y (i = 1..20, j = 0..20), then the z
+   // faces with low cell in plane k - 1 and in plane k (i, j = 1..20); lanes run along x
+   for (int f = tid; f < F12_NF; f += F12_NT)
    {
-      const T dxy = ( F.fh( 0, v, c ) - F.fh( 0, v, c - 1 ) ) + ( F.fh( 1, v, c ) - F.fh( 1, v, c - NX ) );
-      h[v] = u[v] - P.dt_dh2*( dxy + ( F.fh( 2, v, c ) - F.fh( 2, v, c - N2 ) ) );
-   }
-   con2pri_floor( h, P, w );
-#  pragma unroll
-   for (int v=0; v<NV; v++) F.w( v, c ) = w[v];
-   if constexpr ( std::is_same<T, double>::value )
-      if ( h_off >= 0 )
+      int d, c;
+      if ( f < F12_NXF )        { d = 0;  c = f % 21 + NX*( 1 + f / 21 ) + N2*k; }
+      else if ( f < 2*F12_NXF ) { d = 1;  const int r = f - F12_NXF;  c = 1 + r % 20 + NX*( r / 20 ) + N2*k; }
+      else
       {
-         double *H = g_ws + (size_t)x.pg*WS_ADJ*N3 + (size_t)h_off*N3;
-#        pragma unroll
-         for (int v=0; v<NV; v++) H[ v*N3 + c ] = h[v];
+         d = 2;
+         const int r = f - 2*F12_NXF, zz = r / F12_NZF, q = r - zz*F12_NZF;
+         c = 1 + q % 20 + NX*( 1 + q / 20 ) + N2*( k - 1 + zz );
       }
+      T uL[NV], uR[NV], fl[NV];
+      Load<T>::cell( in, tin, c, uL );
+      Load<T>::cell( in, tin, c + STRIDE[d], uR );
+      half_flux<RS>( d, uL, uR, P, fl );
+#     pragma unroll
+      for (int v=0; v<NV; v++) sF[ v*F12_NF + f ] = fl[v];
+   }
+   __syncthreads();
+   for (int e = tid; e < F12_NZF; e += F12_NT)
+   {
+      const int i = 1 + e % 20, j = 1 + e / 20, c = i + NX*j + N2*k;
+      const int xh = i + 21*( j - 1 ), yh = F12_NXF + ( i - 1 ) + 20*j, zh = 2*F12_NXF + F12_NZF + ( i - 1 ) + 20*( j - 1 );
+      T u[NV], h[NV], w[NV];
+      Load<T>::cell( in, tin, c, u );
+#     pragma unroll
+      for (int v=0; v<NV; v++)
+      {
+         const T *fv = sF + v*F12_NF;
+         const T dxy = ( fv[xh] - fv[xh - 1] ) + ( fv[yh] - fv[yh - 20] );
+         h[v] = u[v] - P.dt_dh2*( dxy + ( fv[zh] - fv[zh - F12_NZF] ) );
+      }
+      con2pri_floor( h, P, w );
+#     pragma unroll
+      for (int v=0; v<NV; v++) F.w( v, c ) = w[v];
+      if constexpr ( std::is_same<T, double>::value )
+         if ( h_off >= 0 )
+         {
+            double *H = g_ws + (size_t)pg*WS_ADJ*N3 + (size_t)h_off*N3;
+#           pragma unroll
+            for (int v=0; v<NV; v++) H[ v*N3 + c ] = h[v];
+         }
+   }
 }
 
 // F4 (tangent / plain forward): full-step update of the 16^3 output cells.  The PLM faces of each face are
@@ -335,7 +341,7 @@ f4_kernel( const double *__restrict__ g_in, const double *__restrict__ g_tin, do
    const Prm<T> P = phase_prm<T>( P0, minmod_coeff );
    const double *in = g_in + x.pg*NIN, *tin = g_tin ? g_tin + x.pg*NIN : nullptr;
    double *out = g_out ? g_out + x.pg*NOUT : nullptr, *tout = g_tout ? g_tout + x.pg*NOUT : nullptr;
-   Fwd<T> F( reinterpret_cast<T*>( g_ws + (size_t)x.pg*WS_ADJ*N3 ), WS_FH_TAN );
+   Fwd<T> F( reinterpret_cast<T*>( g_ws + (size_t)x.pg*WS_ADJ*N3 ) );
    int i, j, k;
    const int c = box_cell( CORE_CELLS, x.e, i, j, k );
    T u[NV], dF[3][NV];
@@ -387,7 +393,7 @@ r4_kernel( const double *__restrict__ g_lout, double *__restrict__ g_ws, Prm<dou
    const Prm<double> P = phase_prm<double>( P0, minmod_coeff );
    const double *lout = g_lout + x.pg*NOUT;
    double *ws = g_ws + (size_t)x.pg*WS_ADJ*N3;
-   Fwd<double> F( ws, WS_LS );
+   Fwd<double> F( ws );
    Adj A( ws );
    const int d = x.e / NFF;
    int i, j, k;
@@ -436,7 +442,7 @@ r3_kernel( double *__restrict__ g_ws, Prm<double> P0, double minmod_coeff, int d
    if ( x.e >= ntask ) return;
    const Prm<double> P = phase_prm<double>( P0, minmod_coeff );
    double *ws = g_ws + (size_t)x.pg*WS_ADJ*N3;
-   Fwd<double> F( ws, WS_LS );
+   Fwd<double> F( ws );
    Adj A( ws );
    const int  s = STRIDE[d], v = x.e / nline, ln = x.e - v*nline;
    const int  a = ( d == 0 ) ? 1 + ln % 20 : 3 + ln % 16, b = ( d == 0 ) ? 1 + ln / 20 : 3 + ln / 16;
@@ -481,7 +487,7 @@ r3x_kernel( double *__restrict__ g_ws, Prm<double> P0, double minmod_coeff )
    const int np = min( R3X_PAIRS, R3X_NPAIR - q0 );
    const Prm<double> P = phase_prm<double>( P0, minmod_coeff );
    double *ws = g_ws + (size_t)pg*WS_ADJ*N3;
-   Fwd<double> F( ws, WS_LS );
+   Fwd<double> F( ws );
    Adj A( ws );
    // pair q: field q / 20, z = 1 + q % 20; its rows j = 1..20 start at cell NX + N2 z, a 16-byte aligned offset
    // (NX and N2 are even), so the block moves as 220 double2 per array, R3X_LD of them per lane in flight
@@ -700,11 +706,9 @@ int adjoint_phases( const double *in, const double *lam_out, double *lam_in, dou
 {
    const dim3 block( NT );
    dim3 g;
-   // forward: F1, F2 (the PLM faces are recomputed where they are needed instead of stored)
-   if ( !phase_grid( npg, N_F1, g ) ) return -3;
-   hipLaunchKernelGGL( ( f1_kernel<RS, double> ), g, block, 0, stream, in, nullptr, ws, WS_LS, P, minmod_coeff );
-   phase_grid( npg, N_F2, g );
-   hipLaunchKernelGGL( ( f2_kernel<double> ), g, block, 0, stream, in, nullptr, ws, WS_LS, WS_H, P, minmod_coeff );
+   // forward: F1 + F2 (the PLM faces are recomputed where they are needed instead of stored)
+   if ( !phase_grid( npg, N_R1, g ) ) return -3;
+   hipLaunchKernelGGL( ( f12_kernel<RS, double> ), dim3( (unsigned)( 20*npg ) ), dim3( F12_NT ), 0, stream, in, nullptr, ws, WS_H, P, minmod_coeff );
    // reverse: R4, R3 (three directions in order), R2, R1, R0
    phase_grid( npg, N_R4, g );
    hipLaunchKernelGGL( ( r4_kernel<RS, LIM> ), g, block, 0, stream, lam_out, ws, P, minmod_coeff );
@@ -729,10 +733,8 @@ int tangent_phases( const double *in, const double *tin, double *out, double *to
 {
    const dim3 block( NT );
    dim3 g;
-   if ( !phase_grid( npg, N_F1, g ) ) return -3;
-   hipLaunchKernelGGL( ( f1_kernel<RS, T> ), g, block, 0, stream, in, tin, ws, WS_FH_TAN, P, minmod_coeff );
-   phase_grid( npg, N_F2, g );
-   hipLaunchKernelGGL( ( f2_kernel<T> ), g, block, 0, stream, in, tin, ws, WS_FH_TAN, -1, P, minmod_coeff );
+   if ( !phase_grid( npg, N_R1, g ) ) return -3;
+   hipLaunchKernelGGL( ( f12_kernel<RS, T> ), dim3( (unsigned)( 20*npg ) ), dim3( F12_NT ), 0, stream, in, tin, ws, -1, P, minmod_coeff );
    phase_grid( npg, N_F4, g );
    hipLaunchKernelGGL( ( f4_kernel<RS, LIM, T> ), g, block, 0, stream, in, tin, out, tout, ws, P, minmod_coeff );
    return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
