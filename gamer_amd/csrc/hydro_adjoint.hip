@@ -55,15 +55,16 @@ constexpr int PS = 16;
 constexpr int NT = 256;
 constexpr int STRIDE[3] = { 1, NX, N2 };
 
-// workspace layout per patch group, in units of N3 doubles
-constexpr int WS_W   = 0;            // [5]       half-step primitives (cells 1..20)
-constexpr int WS_LW  = 5;            // [5]       lambda_W
-constexpr int WS_LU  = 10;           // [5]       lambda of the half-step conserved state
-constexpr int WS_LFC = 15;           // [3][2][5] lambda of the faces
-constexpr int WS_LS  = 45;           // [3][2][5] half-step face contributions to lambda_in (low / high cell);
-                                     //           the first 15 arrays hold Fh [3][5] during the forward phases
-constexpr int WS_H   = 60;           // [5]       half-step conserved state, F2 -> R2 (free slots of WS_LS until R1)
-constexpr int WS_ADJ = 75;
+// workspace layout per patch group, in units of N3 doubles.  Written / last read by:
+constexpr int WS_W   = 0;            // [5]       half-step primitives (cells 1..20)            F1+F2 / R3
+constexpr int WS_LFC = 5;            // [3][2][5] lambda of the full-step faces                 R4 / R3
+constexpr int WS_LW  = 35;           // [5]       lambda_W                                      R3 / R2
+constexpr int WS_LU  = 40;           // [5]       lambda of the half-step conserved state       R2 / R0
+constexpr int WS_H   = 45;           // [5]       half-step conserved state                     F1+F2 / R2
+constexpr int WS_LS  = 0;            // [3][2][5] half-step face contributions to lambda_in     R1 / R0
+                                     //           (low / high cell), over W and LFC, which are dead after R3
+constexpr int WS_ADJ = 50;
+static_assert( WS_LS + 30 <= WS_LW && WS_LFC + 30 <= WS_LW && WS_H + NV <= WS_ADJ, "workspace slots overlap" );
 
 __device__ __forceinline__ void coords( int c, int &i, int &j, int &k ) { i = c % NX;  j = ( c / NX ) % NX;  k = c / N2; }
 __device__ __forceinline__ bool in_rng( int x, int lo, int hi ) { return x >= lo && x <= hi; }
