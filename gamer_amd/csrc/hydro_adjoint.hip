@@ -15,8 +15,9 @@
 //             R3 lambda_W = sum over the three stencil cells of J_PLM^T lambda_FC (transpose of the limiter,
 //                face clamp and mirror), one line march per direction
 //             R2 lambda_U_half = J_Con2Pri^T lambda_W (transpose of the floors and primitive conversion)
-//             R1 lambda of every half-step face, then J_RS^T of the cell-centred half-step solve per side
-//             R0 lambda_in = lambda_out (core) + lambda_U_half + the half-step face contributions
+//             R1 + R0, one z plane per workgroup: lambda of every half-step face, J_RS^T of the cell-centred
+//                half-step solve per side, and lambda_in = lambda_out (core) + lambda_U_half + the half-step face
+//                contributions (the z faces' upper sides added by a second, small kernel)
 //
 // The reverse sweep is written out by hand.  The Riemann solves (HLLC, HLLE and Roe) run backwards as hand-written
 // reverse-mode sweeps (hydro_reverse.hpp: solver + interface-state builders, one pass per face for all ten
@@ -61,10 +62,10 @@ constexpr int WS_LFC = 5;            // [3][2][5] lambda of the full-step faces 
 constexpr int WS_LW  = 35;           // [5]       lambda_W                                      R3 / R2
 constexpr int WS_LU  = 40;           // [5]       lambda of the half-step conserved state       R2 / R0
 constexpr int WS_H   = 45;           // [5]       half-step conserved state                     F1+F2 / R2
-constexpr int WS_LS  = 0;            // [3][2][5] half-step face contributions to lambda_in     R1 / R0
-                                     //           (low / high cell), over W and LFC, which are dead after R3
+constexpr int WS_ZGR = 0;            // [5]       upper-side contributions of the half-step z faces  R1 / R0
+                                     //           (over W, dead after R3)
 constexpr int WS_ADJ = 50;
-static_assert( WS_LS + 30 <= WS_LW && WS_LFC + 30 <= WS_LW && WS_H + NV <= WS_ADJ, "workspace slots overlap" );
+static_assert( WS_LFC + 30 <= WS_LW && WS_H + NV <= WS_ADJ, "workspace slots overlap" );
 
 __device__ __forceinline__ void coords( int c, int &i, int &j, int &k ) { i = c % NX;  j = ( c / NX ) % NX;  k = c / N2; }
 __device__ __forceinline__ bool in_rng( int x, int lo, int hi ) { return x >= lo && x <= hi; }
@@ -197,7 +198,7 @@ __device__ __forceinline__ void plm( const T &L, const T &C, const T &R, const P
 }
 
 // ---------------- workspace views ----------------
-// The adjoint keeps h (read by R2) in lambda-side slots that R1 overwrites; the tangent kernels store W as dual
+// The adjoint keeps h (read by R2) in its own slots; the tangent kernels store W as dual
 // numbers (two doubles each).
 static_assert( 2*NV <= WS_ADJ, "tangent workspace must fit the per-PG slot" );
 
@@ -234,7 +235,7 @@ template <> struct Load< Dual<1> >
 // marching through all phases of its patch group kept only 4 waves per patch group busy; spread over the grid,
 // a phase keeps as many patch groups' workspace traffic in flight as the registers of its own kernel allow.
 constexpr int bpp( int count ) { return ( count + NT - 1 ) / NT; }
-constexpr int N_F4 = PS*PS*PS, N_R4 = 3*NFF, N_R2 = 8000, N_R1 = 3*NHF, N_R0 = N3;
+constexpr int N_F4 = PS*PS*PS, N_R4 = 3*NFF, N_R2 = 8000;
 __host__ __device__ constexpr int r3_tasks( int d ) { return NV*( ( d == 0 ) ? 400 : 256 ); }
 
 struct Elem { int pg, e; };
@@ -373,12 +374,11 @@ f4_kernel( const double *__restrict__ g_in, const double *__restrict__ g_tin, do
 
 // ---------------- reverse phases ----------------
 struct Adj {
-   double *LW, *LU, *LFC, *LS;
-   __device__ Adj( double *base ) : LW( base + WS_LW*N3 ), LU( base + WS_LU*N3 ), LFC( base + WS_LFC*N3 ), LS( base + WS_LS*N3 ) {}
+   double *LW, *LU, *LFC;
+   __device__ Adj( double *base ) : LW( base + WS_LW*N3 ), LU( base + WS_LU*N3 ), LFC( base + WS_LFC*N3 ) {}
    __device__ double &lw( int v, int c )                { return LW[ v*N3 + c ]; }
    __device__ double &lu( int v, int c )                { return LU[ v*N3 + c ]; }
    __device__ double &lfc( int d, int s, int v, int c ) { return LFC[ ( ( d*2 + s )*NV + v )*N3 + c ]; }
-   __device__ double &ls( int d, int s, int v, int c )  { return LS[ ( ( d*2 + s )*NV + v )*N3 + c ]; }
 };
 
 typedef Dual<1> D1;
@@ -585,62 +585,94 @@ r2_kernel( double *__restrict__ g_ws, Prm<double> P0, double minmod_coeff )
    }
 }
 
-// R1 half-step faces: lambda_Fh = dt/(2dh) (lambda_U(c+e_d) - lambda_U(c)), then J_RS^T per side.
-// (Fh and h alias the slots written here; F2 and R2, their last readers, have completed.)
+// R1 + R0, one z plane k = 0..21 per workgroup.  R1: lambda_Fh = dt/(2dh) (lambda_U(c+e_d) - lambda_U(c)) of
+// every half-step face, then J_RS^T per side; the plane's x and y faces and the z face above it (normal k)
+// leave their side contributions in LDS, the z face's upper-side one goes to the workspace (zgr).  R0: lambda_in of the plane's cells = lambda_out (output cells) +
+// lambda_U_half (half-step cells) + the face contributions in the order x low, x high, y low, y high, z low;
+// r0z_kernel then adds the z high contribution.  Each face is solved once.
+// 768 lanes (3 waves per SIMD, <= 170 VGPRs); Roe's reverse solve needs about 210, so it runs 512
+constexpr int R10_NXF = 21*20, R10_NZF = 20*20, R10_NF = 2*R10_NXF + R10_NZF;
+constexpr int N_R0Z = 21*R10_NZF;
+constexpr int r10_nt( int rs ) { return rs == 2 ? 512 : 768; }
+
 template <int RS>
-__global__ void __launch_bounds__( NT )
-r1_kernel( const double *__restrict__ g_in, double *__restrict__ g_ws, Prm<double> P0, double minmod_coeff )
+__global__ void __launch_bounds__( r10_nt( RS ) )
+r10_kernel( const double *__restrict__ g_in, const double *__restrict__ g_lout, double *__restrict__ g_lin,
+            double *__restrict__ g_ws, Prm<double> P0, double minmod_coeff )
 {
-   const Elem x = elem( N_R1 );
-   if ( x.e >= N_R1 ) return;
+   constexpr int R10_NT = r10_nt( RS );
+   __shared__ double sX[2][NV][R10_NXF], sY[2][NV][R10_NXF], sZ[NV][R10_NZF];     // [low/high side][v][face]
+   const int tid = threadIdx.x, pg = blockIdx.x / 22, k = blockIdx.x % 22;
    const Prm<double> P = phase_prm<double>( P0, minmod_coeff );
-   const double *in = g_in + x.pg*NIN;
-   Adj A( g_ws + (size_t)x.pg*WS_ADJ*N3 );
-   const int d = x.e / NHF;
-   int i, j, k, n, a, b;
-   const int c = box_cell( face_box( d, 0, 21, 1, 20 ), x.e - d*NHF, i, j, k );
-   split( d, i, j, k, n, a, b );
-   const int s = STRIDE[d];
-   const bool lo = in_rng( n, 1, 20 ), hi = in_rng( n + 1, 1, 20 );   // tangential range is 1..20 already
-   double lf[NV];
-#  pragma unroll
-   for (int v=0; v<NV; v++) lf[v] = P.dt_dh2*( ( hi ? A.lu( v, c + s ) : 0.0 ) - ( lo ? A.lu( v, c ) : 0.0 ) );
-   double uL[NV], uR[NV];
-#  pragma unroll
-   for (int v=0; v<NV; v++) { uL[v] = in[ v*N3 + c ];  uR[v] = in[ v*N3 + c + s ]; }
-   double gl[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 }, gr[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
-   half_flux_rev<RS>( d, uL, uR, P, lf, gl, gr );
-#  pragma unroll
-   for (int v=0; v<NV; v++) { A.ls( d, 0, v, c ) = gl[v];  A.ls( d, 1, v, c + s ) = gr[v]; }
+   const double *in = g_in + pg*NIN, *lout = g_lout + pg*NOUT;
+   double *lin = g_lin + pg*NIN, *ws = g_ws + (size_t)pg*WS_ADJ*N3, *zgr = ws + (size_t)WS_ZGR*N3;
+   Adj A( ws );
+   // faces with low cell in plane k: x (normal i = 0..20, j = 1..20), y (i = 1..20, j = 0..20), z (i, j = 1..20);
+   // plane 0 has only its z faces, plane 21 none
+   const int nf = ( k >= 1 && k <= 20 ) ? R10_NF : ( k == 0 ) ? R10_NZF : 0, f0 = ( k == 0 ) ? 2*R10_NXF : 0;
+   for (int t = tid; t < nf; t += R10_NT)
+   {
+      const int f = f0 + t;
+      int d, c, r;
+      if ( f < R10_NXF )        { d = 0;  r = f;                c = f % 21 + NX*( 1 + f / 21 ) + N2*k; }
+      else if ( f < 2*R10_NXF ) { d = 1;  r = f - R10_NXF;      c = 1 + r % 20 + NX*( r / 20 ) + N2*k; }
+      else                      { d = 2;  r = f - 2*R10_NXF;    c = 1 + r % 20 + NX*( 1 + r / 20 ) + N2*k; }
+      int i, j, kk, n, a, b;
+      coords( c, i, j, kk );
+      split( d, i, j, kk, n, a, b );
+      const int s = STRIDE[d];
+      const bool lo = in_rng( n, 1, 20 ), hi = in_rng( n + 1, 1, 20 );   // tangential range is 1..20 already
+      double lf[NV];
+#     pragma unroll
+      for (int v=0; v<NV; v++) lf[v] = P.dt_dh2*( ( hi ? A.lu( v, c + s ) : 0.0 ) - ( lo ? A.lu( v, c ) : 0.0 ) );
+      double uL[NV], uR[NV];
+#     pragma unroll
+      for (int v=0; v<NV; v++) { uL[v] = in[ v*N3 + c ];  uR[v] = in[ v*N3 + c + s ]; }
+      double gl[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 }, gr[NV] = { 0.0, 0.0, 0.0, 0.0, 0.0 };
+      half_flux_rev<RS>( d, uL, uR, P, lf, gl, gr );
+#     pragma unroll
+      for (int v=0; v<NV; v++)
+      {
+         if      ( d == 0 ) { sX[0][v][r] = gl[v];  sX[1][v][r] = gr[v]; }
+         else if ( d == 1 ) { sY[0][v][r] = gl[v];  sY[1][v][r] = gr[v]; }
+         else               { sZ[v][r] = gl[v];  zgr[ v*N3 + k*R10_NZF + r ] = gr[v]; }
+      }
+   }
+   __syncthreads();
+   for (int e = tid; e < NX*NX; e += R10_NT)
+   {
+      const int i = e % NX, j = e / NX, c = i + NX*j + N2*k;
+      const bool core = core3( i, j, k ), half = in_rng( i, 1, 20 ) && in_rng( j, 1, 20 ) && in_rng( k, 1, 20 );
+      const bool xl = half_face( i, j, k ), xh = half_face( i - 1, j, k ), yl = half_face( j, i, k ),
+                 yh = half_face( j - 1, i, k ), zl = half_face( k, i, j );
+      const int rx = i + 21*( j - 1 ), ry = ( i - 1 ) + 20*j, rz = ( i - 1 ) + 20*( j - 1 );
+#     pragma unroll
+      for (int v=0; v<NV; v++)
+      {
+         double acc = core ? lout[ (size_t)v*PS*PS*PS + out_index( i, j, k ) ] : 0.0;
+         if ( half ) acc += A.lu( v, c );
+         if ( xl ) acc += sX[0][v][rx];
+         if ( xh ) acc += sX[1][v][rx - 1];
+         if ( yl ) acc += sY[0][v][ry];
+         if ( yh ) acc += sY[1][v][ry - 20];
+         if ( zl ) acc += sZ[v][rz];
+         lin[ v*N3 + c ] = acc;
+      }
+   }
 }
 
-// R0 lambda_in: direct dependence of the output and of the half-step state on U_in, plus the half-step faces
+// the z high contribution: cell (i, j, k), k = 1..21, i, j = 1..20, is the upper side of z face k - 1/2
 __global__ void __launch_bounds__( NT )
-r0_kernel( const double *__restrict__ g_lout, double *__restrict__ g_lin, double *__restrict__ g_ws )
+r0z_kernel( double *__restrict__ g_lin, const double *__restrict__ g_ws )
 {
-   const Elem x = elem( N_R0 );
-   if ( x.e >= N_R0 ) return;
-   const double *lout = g_lout + x.pg*NOUT;
+   const Elem x = elem( N_R0Z );
+   if ( x.e >= N_R0Z ) return;
    double *lin = g_lin + x.pg*NIN;
-   Adj A( g_ws + (size_t)x.pg*WS_ADJ*N3 );
-   const int c = x.e;
-   int i, j, k;
-   coords( c, i, j, k );
-   const bool core = core3( i, j, k ), half = in_rng( i, 1, 20 ) && in_rng( j, 1, 20 ) && in_rng( k, 1, 20 );
+   const double *zgr = g_ws + (size_t)x.pg*WS_ADJ*N3 + (size_t)WS_ZGR*N3;
+   const int n = x.e / R10_NZF, q = x.e - n*R10_NZF;                  // face normal n = 0..20
+   const int c = 1 + q % 20 + NX*( 1 + q / 20 ) + N2*( n + 1 );
 #  pragma unroll
-   for (int v=0; v<NV; v++)
-   {
-      double acc = core ? lout[ (size_t)v*PS*PS*PS + out_index( i, j, k ) ] : 0.0;
-      if ( half ) acc += A.lu( v, c );
-      for (int d=0; d<3; d++)
-      {
-         int n, a, b;
-         split( d, i, j, k, n, a, b );
-         if ( half_face( n, a, b ) )     acc += A.ls( d, 0, v, c );      // c is the low cell of face c+1/2
-         if ( half_face( n - 1, a, b ) ) acc += A.ls( d, 1, v, c );      // c is the high cell of face c-1/2
-      }
-      lin[ v*N3 + c ] = acc;
-   }
+   for (int v=0; v<NV; v++) lin[ v*N3 + c ] += zgr[ v*N3 + n*R10_NZF + q ];
 }
 
 // ---------------- transpose of the periodic ghost-zone preparation ----------------
@@ -692,14 +724,10 @@ size_t adjoint_workspace_bytes( int npg ) { return (size_t)( npg > 0 ? npg : 0 )
 
 namespace {
 
-// one flat grid of bpp(count) workgroups per patch group; false when npg patch groups would not fit one launch
-bool phase_grid( int npg, int count, dim3 &grid )
-{
-   const long blocks = (long)npg*bpp( count );
-   if ( blocks*NT >= ( 1L << 32 ) ) return false;
-   grid = dim3( (unsigned)blocks );
-   return true;
-}
+// one flat grid of bpp(count) workgroups per patch group
+void phase_grid( int npg, int count, dim3 &grid ) { grid = dim3( (unsigned)( (long)npg*bpp( count ) ) ); }
+// whether every phase's grid of npg patch groups fits one launch (the per-plane R1 + R0 kernel has the most lanes)
+bool grids_fit( int npg ) { return (long)npg*22*r10_nt( 4 ) < ( 1L << 32 ) && (long)npg*bpp( N_R4 )*NT < ( 1L << 32 ); }
 
 template <int RS, int LIM>
 int adjoint_phases( const double *in, const double *lam_out, double *lam_in, double *ws, int npg, double minmod_coeff,
@@ -708,7 +736,7 @@ int adjoint_phases( const double *in, const double *lam_out, double *lam_in, dou
    const dim3 block( NT );
    dim3 g;
    // forward: F1 + F2 (the PLM faces are recomputed where they are needed instead of stored)
-   if ( !phase_grid( npg, N_R1, g ) ) return -3;
+   if ( !grids_fit( npg ) ) return -3;
    hipLaunchKernelGGL( ( f12_kernel<RS, double> ), dim3( (unsigned)( 20*npg ) ), dim3( F12_NT ), 0, stream, in, nullptr, ws, WS_H, P, minmod_coeff );
    // reverse: R4, R3 (three directions in order), R2, R1, R0
    phase_grid( npg, N_R4, g );
@@ -721,10 +749,9 @@ int adjoint_phases( const double *in, const double *lam_out, double *lam_in, dou
    }
    phase_grid( npg, N_R2, g );
    hipLaunchKernelGGL( r2_kernel, g, block, 0, stream, ws, P, minmod_coeff );
-   phase_grid( npg, N_R1, g );
-   hipLaunchKernelGGL( ( r1_kernel<RS> ), g, block, 0, stream, in, ws, P, minmod_coeff );
-   phase_grid( npg, N_R0, g );
-   hipLaunchKernelGGL( r0_kernel, g, block, 0, stream, lam_out, lam_in, ws );
+   hipLaunchKernelGGL( ( r10_kernel<RS> ), dim3( (unsigned)( 22*npg ) ), dim3( r10_nt( RS ) ), 0, stream, in, lam_out, lam_in, ws, P, minmod_coeff );
+   phase_grid( npg, N_R0Z, g );
+   hipLaunchKernelGGL( r0z_kernel, g, block, 0, stream, lam_in, ws );
    return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
 }
 
@@ -734,7 +761,7 @@ int tangent_phases( const double *in, const double *tin, double *out, double *to
 {
    const dim3 block( NT );
    dim3 g;
-   if ( !phase_grid( npg, N_R1, g ) ) return -3;
+   if ( !grids_fit( npg ) ) return -3;
    hipLaunchKernelGGL( ( f12_kernel<RS, T> ), dim3( (unsigned)( 20*npg ) ), dim3( F12_NT ), 0, stream, in, tin, ws, -1, P, minmod_coeff );
    phase_grid( npg, N_F4, g );
    hipLaunchKernelGGL( ( f4_kernel<RS, LIM, T> ), g, block, 0, stream, in, tin, out, tout, ws, P, minmod_coeff );
