@@ -7,8 +7,8 @@
 //
 //   forward   F1 + F2, one z plane per workgroup: half-step fluxes (Hydro_RiemannPredict_Flux) in LDS, then the
 //                half-step update, floors and primitive W (Hydro_RiemannPredict) -> ws
-//             F4 (tangent / plain forward only) PLM faces (Hydro_DataReconstruction), full-step fluxes and update
-//                of the 16^3 output cells
+//             F4 (tangent / plain forward only) PLM faces (Hydro_DataReconstruction) and the full-step flux of
+//                each face, then the update of the 16^3 output cells
 //   reverse   R4 lambda_F of each full-step face from lambda_out (transpose of Hydro_FullStepUpdate),
 //                lambda_FC = J_RS^T lambda_F (transpose of Hydro_ComputeFlux), with the two PLM faces of the
 //                face recomputed from W
@@ -330,41 +330,71 @@ f12_kernel( const double *__restrict__ g_in, const double *__restrict__ g_tin, d
    }
 }
 
-// F4 (tangent / plain forward): full-step update of the 16^3 output cells.  The PLM faces of each face are
-// recomputed from the half-step primitives W (centres c - e_d, c, c + e_d), as the adjoint's R4 does, instead of
-// being stored and re-read.  T = Dual<1>: out = U_out (optional), tout = J v; T = double: out = U_out.
+// F4 (tangent / plain forward), in two kernels.  f4_face_kernel: the full-step flux of each of the 13,056 faces
+// (3 x 17 x 16 x 16, low cell 2..18 along the normal, tangential 3..18), once per face, with its two PLM faces
+// recomputed from the half-step primitives W as the adjoint's R4 does; the fluxes go to the workspace.
+// f4_update_kernel: the full-step update of the 16^3 output cells from the fluxes of their faces.
+// T = Dual<1>: out = U_out (optional), tout = J v; T = double: out = U_out.
+constexpr int WS_TF = 2*NV;          // tangent flux slots: [3][5][NFF] values of T, after W (5 values of T per cell)
+static_assert( WS_TF*N3 + 3*NV*NFF*2 <= WS_ADJ*N3, "tangent fluxes must fit the per-PG workspace" );
+
+// index of the d-normal face with low cell (i, j, k) in face_box(d, 2, 17, 3, 16)
+__device__ __forceinline__ int full_face_index( int d, int i, int j, int k )
+{
+   return ( d == 0 ) ? ( i - 2 ) + 17*( ( j - 3 ) + 16*( k - 3 ) )
+        : ( d == 1 ) ? ( i - 3 ) + 16*( ( j - 2 ) + 17*( k - 3 ) )
+        :              ( i - 3 ) + 16*( ( j - 3 ) + 16*( k - 2 ) );
+}
+
 template <int RS, int LIM, typename T>
 __global__ void __launch_bounds__( NT )
-f4_kernel( const double *__restrict__ g_in, const double *__restrict__ g_tin, double *__restrict__ g_out,
-           double *__restrict__ g_tout, double *__restrict__ g_ws, Prm<double> P0, double minmod_coeff )
+f4_face_kernel( double *__restrict__ g_ws, Prm<double> P0, double minmod_coeff )
+{
+   const Elem x = elem( 3*NFF );
+   if ( x.e >= 3*NFF ) return;
+   const Prm<T> P = phase_prm<T>( P0, minmod_coeff );
+   T *ws = reinterpret_cast<T*>( g_ws + (size_t)x.pg*WS_ADJ*N3 );
+   Fwd<T> F( ws );
+   T *tf = ws + (size_t)WS_TF*N3/( sizeof( T )/sizeof( double ) );
+   const int d = x.e / NFF, q = x.e - d*NFF;
+   int i, j, k;
+   const int c = box_cell( face_box( d, 2, 17, 3, 16 ), q, i, j, k );
+   const int s = STRIDE[d];
+   T wl[NV], wr[NV], f[NV];
+#  pragma unroll
+   for (int v=0; v<NV; v++)
+   {
+      const T wm = F.w( v, c - s ), w0 = F.w( v, c ), w1 = F.w( v, c + s ), w2 = F.w( v, c + 2*s );
+      T dummy;
+      plm<LIM>( wm, w0, w1, P, dummy, wl[v] );      // R face of the low cell
+      plm<LIM>( w0, w1, w2, P, wr[v], dummy );      // L face of the high cell
+   }
+   full_flux<RS>( d, wl, wr, P, f );
+#  pragma unroll
+   for (int v=0; v<NV; v++) tf[ ( d*NV + v )*NFF + q ] = f[v];
+}
+
+template <typename T>
+__global__ void __launch_bounds__( NT )
+f4_update_kernel( const double *__restrict__ g_in, const double *__restrict__ g_tin, double *__restrict__ g_out,
+                  double *__restrict__ g_tout, double *__restrict__ g_ws, Prm<double> P0, double minmod_coeff )
 {
    const Elem x = elem( N_F4 );
    if ( x.e >= N_F4 ) return;
    const Prm<T> P = phase_prm<T>( P0, minmod_coeff );
    const double *in = g_in + x.pg*NIN, *tin = g_tin ? g_tin + x.pg*NIN : nullptr;
    double *out = g_out ? g_out + x.pg*NOUT : nullptr, *tout = g_tout ? g_tout + x.pg*NOUT : nullptr;
-   Fwd<T> F( reinterpret_cast<T*>( g_ws + (size_t)x.pg*WS_ADJ*N3 ) );
+   const T *tf = reinterpret_cast<const T*>( g_ws + (size_t)x.pg*WS_ADJ*N3 ) + (size_t)WS_TF*N3/( sizeof( T )/sizeof( double ) );
    int i, j, k;
    const int c = box_cell( CORE_CELLS, x.e, i, j, k );
    T u[NV], dF[3][NV];
    Load<T>::cell( in, tin, c, u );
    for (int d=0; d<3; d++)
    {
-      const int s = STRIDE[d];
-      T rm[NV], l0[NV], r0[NV], lp[NV], fm[NV], fp[NV];
+      const int qp = full_face_index( d, i, j, k );                                 // face c + 1/2
+      const int qm = full_face_index( d, i - ( d == 0 ), j - ( d == 1 ), k - ( d == 2 ) );   // face c - 1/2
 #     pragma unroll
-      for (int v=0; v<NV; v++)
-      {
-         const T w2 = F.w( v, c - 2*s ), w1 = F.w( v, c - s ), w0 = F.w( v, c ), p1 = F.w( v, c + s ), p2 = F.w( v, c + 2*s );
-         T dummy;
-         plm<LIM>( w2, w1, w0, P, dummy, rm[v] );       // R face of c - e_d
-         plm<LIM>( w1, w0, p1, P, l0[v], r0[v] );       // both faces of c
-         plm<LIM>( w0, p1, p2, P, lp[v], dummy );       // L face of c + e_d
-      }
-      full_flux<RS>( d, rm, l0, P, fm );
-      full_flux<RS>( d, r0, lp, P, fp );
-#     pragma unroll
-      for (int v=0; v<NV; v++) dF[d][v] = fp[v] - fm[v];
+      for (int v=0; v<NV; v++) dF[d][v] = tf[ ( d*NV + v )*NFF + qp ] - tf[ ( d*NV + v )*NFF + qm ];
    }
    const size_t o = out_index( i, j, k );
 #  pragma unroll
@@ -763,8 +793,10 @@ int tangent_phases( const double *in, const double *tin, double *out, double *to
    dim3 g;
    if ( !grids_fit( npg ) ) return -3;
    hipLaunchKernelGGL( ( f12_kernel<RS, T> ), dim3( (unsigned)( 20*npg ) ), dim3( F12_NT ), 0, stream, in, tin, ws, -1, P, minmod_coeff );
+   phase_grid( npg, 3*NFF, g );
+   hipLaunchKernelGGL( ( f4_face_kernel<RS, LIM, T> ), g, block, 0, stream, ws, P, minmod_coeff );
    phase_grid( npg, N_F4, g );
-   hipLaunchKernelGGL( ( f4_kernel<RS, LIM, T> ), g, block, 0, stream, in, tin, out, tout, ws, P, minmod_coeff );
+   hipLaunchKernelGGL( ( f4_update_kernel<T> ), g, block, 0, stream, in, tin, out, tout, ws, P, minmod_coeff );
    return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
 }
 
