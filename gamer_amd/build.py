@@ -18,9 +18,9 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
 ]
-# per-source additions.  The MHM_RP kernels run two waves per SIMD with four barriers per z-plane; LLVM's
-# iterative ILP scheduler gives them a schedule that is 1.3% (FP32) and 1.9% (FP64) faster than the default one
-# (interleaved A/B on the 4096-PG workload).  The MHM file is compiled once per precision: its FP32 kernels run
+# per-source additions.  The MHM_RP kernels march z-planes with four barriers per plane; LLVM's iterative ILP
+# scheduler gives the 8-wave kernels a schedule that is 5.1% (FP32) and 9.1% (FP64) faster than the default one
+# and 11% / 4% faster than the max-ILP one (interleaved A/B on the 4096-PG workload).  The MHM file is compiled once per precision: its FP32 kernels run
 # 4.0% faster with the max-ILP scheduler, its FP64 kernels 1.3% slower, so FP64 keeps the default.  The
 # adjoint, interpolation and sync sources crash the compiler under these strategies and keep the default.
 EXTRA = {"hydro_mhm_rp.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
