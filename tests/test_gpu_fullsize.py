@@ -6,6 +6,8 @@
   sums of the five conserved fields equal to the oracle's (and to the input's: the domain is periodic) to
   round-off, and the fused per-patch CFL dt equal to the oracle's dt solver on the updated cells to 1e-14.
   FP32-bulk: per-field relative L1 <= 1e-5 against the FP64 oracle; per-field L1 and Linf are printed.
+* The same grid through the MHM scheme (20^3 inputs): FP64 per cell within 1e-12 with conserved sums to round-off,
+  FP32-bulk per-field relative L1 <= 1e-5.
 * The uneven 3-level batch at its stated counts 512 / 2048 / 6144 (8704 PGs in one launch, per-PG dt and dh), in
   FP32-bulk, level by level against the FP64 oracle at that level's dt, dh: outputs and coarse-fine flux arrays.
 * Every golden fixture written from GAMER's own CPU_FluidSolver (tests/golden/golden.json, the NPG = 64 C1
@@ -106,6 +108,40 @@ def test_bench_workload_fp32_bulk(bench_workload):
     assert np.all(l1 <= FP32_L1_TOL), l1
     assert np.all(linf <= 1e-4), linf
     assert np.all(fl1 <= FP32_L1_TOL), fl1
+
+
+@pytest.fixture(scope="module")
+def bench_workload_mhm():
+    """The same 4096-PG grid cut for the MHM scheme (ghost 2, 20^3 inputs) and the oracle's FP64 MHM result."""
+    import oracle
+    cons, dt, dh = ic.random_sod_grid((16, 16, 16), seed=20260, sedov=True)
+    u = ic.cut_patch_groups(cons, 2)
+    print(f"[gpu] full-size MHM workload: npg={u.shape[0]}", flush=True)
+    ref_out, ref_flux = oracle.fluid_solver(u, dt, dh, scheme="MHM")
+    return cons, u, dt, dh, ref_out, ref_flux
+
+
+def test_bench_workload_mhm(bench_workload_mhm):
+    """MHM (Hancock half step) on the 4096-PG workload: FP64 per cell <= 1e-12 with conserved sums to round-off,
+    FP32-bulk per-field rel L1 <= 1e-5, both against the FP64 oracle."""
+    from gamer_amd import hydro
+    cons, u, dt, dh, ref_out, ref_flux = bench_workload_mhm
+    print(f"[gpu] MHM full-size parity npg={u.shape[0]}", flush=True)
+    out, flux = hydro.fluid_solver(torch.from_numpy(u).to(DEV), dt, dh, scheme="MHM")
+    torch.cuda.synchronize()
+    o, f = out.cpu().numpy(), flux.cpu().numpy()
+    err, ferr = rel_err_cells(o, ref_out), flux_rel_err(f, ref_flux, ref_out)
+    print(f"  fp64 per-field max rel err {err}, flux {ferr:.3e}", flush=True)
+    assert np.all(err <= FP64_TOL), err
+    assert ferr <= FP64_TOL
+    tot = np.abs(cons).reshape(5, -1).sum(axis=1)
+    assert np.all(np.abs(o.sum(axis=(0, 2, 3, 4)) - ref_out.sum(axis=(0, 2, 3, 4))) <= 1e-13 * tot)
+    out32, _ = hydro.fluid_solver(torch.from_numpy(u).to(DEV, dtype=torch.float32), dt, dh, scheme="MHM")
+    torch.cuda.synchronize()
+    o32 = out32.cpu().numpy()
+    l1, linf = rel_l1(o32, ref_out), rel_linf(o32, ref_out)
+    print(f"  fp32-bulk per-field rel L1 {l1}, rel Linf {linf}", flush=True)
+    assert np.all(l1 <= FP32_L1_TOL), l1
 
 
 def test_amr_batch_fp32_full_counts():
