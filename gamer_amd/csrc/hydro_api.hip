@@ -249,6 +249,7 @@ int hydro_mhm_adjoint_async( const HydroSolveArgs *a, const void *lam_out, void 
    if ( a->npg > 0 && ( lam_out == nullptr || lam_in == nullptr ) ) return fail( HYDRO_E_ARG, "adjoint: lam_out/lam_in == NULL" );
    rc = launch_mhm_rp_adjoint( (const double*)a->in, (const double*)lam_out, (double*)lam_in, (double*)workspace, a->npg,
                                a->rsolver, a->minmod_coeff, make_prm<double>( a ), (hipStream_t)stream );
+   if ( rc == -3 ) return fail( HYDRO_E_ARG, "adjoint: npg %d does not fit one launch per phase", a->npg );
    return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "adjoint launch failed: %s", hipGetErrorString( hipGetLastError() ) );
 }
 
@@ -261,6 +262,7 @@ int hydro_mhm_tangent_async( const HydroSolveArgs *a, const void *v_in, void *jv
    rc = launch_mhm_rp_tangent( (const double*)a->in, (const double*)v_in, (double*)a->out, (double*)jv_out,
                                (double*)workspace, a->npg, a->rsolver, a->minmod_coeff, make_prm<double>( a ),
                                (hipStream_t)stream );
+   if ( rc == -3 ) return fail( HYDRO_E_ARG, "tangent: npg %d does not fit one launch per phase", a->npg );
    return rc == 0 ? HYDRO_OK : fail( HYDRO_E_DEVICE, "tangent launch failed: %s", hipGetErrorString( hipGetLastError() ) );
 }
 

@@ -44,6 +44,13 @@ def test_adjoint_entry_points_reject_bad_arguments():
     assert L.hydro_mhm_adjoint_async(ctypes.byref(a), None, 16, 16, None) == hydro.HYDRO_E_ARG
     assert L.hydro_mhm_tangent_async(ctypes.byref(a), None, None, 16, None) == hydro.HYDRO_E_ARG
     assert L.hydro_prepare_uniform_adjoint_async(16, 16, 20, 16, 16, 3, 0, None) == hydro.HYDRO_E_ARG
+    # a batch whose per-phase grids would exceed one launch is an argument error, found before any launch
+    big = hydro.make_args(**{**ok, "npg": 1 << 19})
+    assert L.hydro_mhm_adjoint_async(ctypes.byref(big), 16, 16, 16, None) == hydro.HYDRO_E_ARG
+    assert "does not fit" in L.hydro_last_error().decode()
+    big = hydro.make_args(**{**ok, "npg": 1 << 19, "out": 16})
+    assert L.hydro_mhm_tangent_async(ctypes.byref(big), 16, 16, 16, None) == hydro.HYDRO_E_ARG
+    assert "does not fit" in L.hydro_last_error().decode()
 
 
 def _call(**kw):
