@@ -236,7 +236,6 @@ template <> struct Load< Dual<1> >
 // a phase keeps as many patch groups' workspace traffic in flight as the registers of its own kernel allow.
 constexpr int bpp( int count ) { return ( count + NT - 1 ) / NT; }
 constexpr int N_F4 = PS*PS*PS, N_R4 = 3*NFF, N_R2 = 8000;
-__host__ __device__ constexpr int r3_tasks( int d ) { return NV*( ( d == 0 ) ? 400 : 256 ); }
 
 struct Elem { int pg, e; };
 __device__ __forceinline__ Elem elem( int count )
@@ -458,32 +457,26 @@ r4_kernel( const double *__restrict__ g_lout, double *__restrict__ g_ws, Prm<dou
    for (int v=0; v<NV; v++) { A.lfc( d, 1, v, c ) = gl[v];  A.lfc( d, 0, v, c + s ) = gr[v]; }
 }
 
-// R3 lambda_W of the 20^3 half-step cells along direction d (one launch per direction, in order 0, 1, 2).
+// R3 along y (d = 1) and z (d = 2), after r3x_kernel has written lambda_W along x: one launch per direction.
 // Task (v, line): the lane marches the line through the reconstruction stencils (centres 2..19) of field v and
-// reverses each one (plm_face_rev), keeping a window of the three cell adjoints the stencil touches; a cell is
-// complete once the stencil above it is done.  Direction 0 writes every cell of the 20^3 block (zero on lines no
-// full-step face reads), directions 1 and 2 add.  The faces of a centre that R4 wrote: the L face when the
-// centre's normal coordinate is 3..19, the R face when it is 2..18, with both tangential coordinates in 3..18.
+// reverses each one (plm_face_rev), keeping a window of the three cell adjoints the stencil touches, and adds each
+// cell's adjoint to lambda_W once the stencil above it is done.  Only the 16^2 lines with both tangential
+// coordinates in 3..18 carry face adjoints (the L face of a centre at normal coordinate 3..19, the R face at
+// 2..18); lanes run along x, so the loads of a march step are coalesced.
+constexpr int R3_LINES = 16*16;
 template <int LIM>
 __global__ void __launch_bounds__( NT )
 r3_kernel( double *__restrict__ g_ws, Prm<double> P0, double minmod_coeff, int d )
 {
-   const int ntask = r3_tasks( d ), nline = ntask/NV;
-   const Elem x = elem( ntask );
-   if ( x.e >= ntask ) return;
+   const Elem x = elem( NV*R3_LINES );
+   if ( x.e >= NV*R3_LINES ) return;
    const Prm<double> P = phase_prm<double>( P0, minmod_coeff );
    double *ws = g_ws + (size_t)x.pg*WS_ADJ*N3;
    Fwd<double> F( ws );
    Adj A( ws );
-   const int  s = STRIDE[d], v = x.e / nline, ln = x.e - v*nline;
-   const int  a = ( d == 0 ) ? 1 + ln % 20 : 3 + ln % 16, b = ( d == 0 ) ? 1 + ln / 20 : 3 + ln / 16;
-   const bool live = in_rng( a, 3, 18 ) && in_rng( b, 3, 18 );
-   const int  c0 = ( d == 0 ) ? NX*a + N2*b : ( d == 1 ) ? a + N2*b : a + NX*b;     // n = 0 of the line
-   if ( !live )
-   {
-      for (int n=1; n<=20; n++) A.lw( v, c0 + n*s ) = 0.0;
-      return;
-   }
+   const int s = STRIDE[d], v = x.e / R3_LINES, ln = x.e - v*R3_LINES;
+   const int a = 3 + ln % 16, b = 3 + ln / 16;                                  // i, and k (d = 1) or j (d = 2)
+   const int c0 = ( d == 1 ) ? a + N2*b : a + NX*b;                             // n = 0 of the line
    double wm = F.w( v, c0 + s ), w0 = F.w( v, c0 + 2*s ), am = 0.0, a0 = 0.0;
    for (int n=2; n<=19; n++)
    {
@@ -493,11 +486,11 @@ r3_kernel( double *__restrict__ g_ws, Prm<double> P0, double minmod_coeff, int d
       const double gr = ( n <= 18 ) ? A.lfc( d, 1, v, cc ) : 0.0;
       double br = 0.0;
       plm_face_rev<LIM>( wm, w0, wp, P.limiter, P.coeff, gl, gr, am, a0, br );
-      if ( d == 0 ) A.lw( v, cc - s ) = am; else A.lw( v, cc - s ) += am;
+      A.lw( v, cc - s ) += am;
       am = a0;  a0 = br;  wm = w0;  w0 = wp;
    }
-   if ( d == 0 ) { A.lw( v, c0 + 19*s ) = am;   A.lw( v, c0 + 20*s ) = a0; }
-   else          { A.lw( v, c0 + 19*s ) += am;  A.lw( v, c0 + 20*s ) += a0; }
+   A.lw( v, c0 + 19*s ) += am;
+   A.lw( v, c0 + 20*s ) += a0;
 }
 
 // R3 along x (d = 0) with the lines staged through LDS.  The x lines of one (z, field) pair are the 20 rows
@@ -768,13 +761,13 @@ int adjoint_phases( const double *in, const double *lam_out, double *lam_in, dou
    // forward: F1 + F2 (the PLM faces are recomputed where they are needed instead of stored)
    if ( !grids_fit( npg ) ) return -3;
    hipLaunchKernelGGL( ( f12_kernel<RS, double> ), dim3( (unsigned)( 20*npg ) ), dim3( F12_NT ), 0, stream, in, nullptr, ws, WS_H, P, minmod_coeff );
-   // reverse: R4, R3 (three directions in order), R2, R1, R0
+   // reverse: R4, R3 (x staged through LDS, then y and z), R2, R1 + R0 per z plane, the z faces' upper sides
    phase_grid( npg, N_R4, g );
    hipLaunchKernelGGL( ( r4_kernel<RS, LIM> ), g, block, 0, stream, lam_out, ws, P, minmod_coeff );
    hipLaunchKernelGGL( ( r3x_kernel<LIM> ), dim3( (unsigned)( npg*R3X_BPP ) ), dim3( R3X_NT ), 0, stream, ws, P, minmod_coeff );
    for (int d=1; d<3; d++)
    {
-      phase_grid( npg, r3_tasks( d ), g );
+      phase_grid( npg, NV*R3_LINES, g );
       hipLaunchKernelGGL( ( r3_kernel<LIM> ), g, block, 0, stream, ws, P, minmod_coeff, d );
    }
    phase_grid( npg, N_R2, g );
