@@ -20,12 +20,13 @@ FLAGS = [
 ]
 # per-source additions.  The MHM_RP kernels march z-planes with four barriers per plane; LLVM's iterative ILP
 # scheduler gives the 8-wave kernels a schedule that is 5.1% (FP32) and 9.1% (FP64) faster than the default one
-# and 11% / 4% faster than the max-ILP one (interleaved A/B on the 4096-PG workload).  The MHM file is compiled once per precision: its FP32 kernels run
-# 4.0% faster with the max-ILP scheduler, its FP64 kernels 1.3% slower, so FP64 keeps the default.  The
-# adjoint, interpolation and sync sources crash the compiler under these strategies and keep the default.
+# and 11% / 4% faster than the max-ILP one (interleaved A/B on the 4096-PG workload).  The MHM file is compiled
+# once per precision: its FP32 kernels run fastest with the max-ILP scheduler (2.3% ahead of iterative ILP), its
+# FP64 kernels with iterative ILP (4.1% ahead of the default).  The adjoint, interpolation and sync sources crash
+# the compiler under these strategies and keep the default.
 EXTRA = {"hydro_mhm_rp.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
          "hydro_mhm.hip:32": ["-DMHM_PRECISION=32", "-mllvm", "-amdgpu-sched-strategy=max-ilp"],
-         "hydro_mhm.hip:64": ["-DMHM_PRECISION=64"]}
+         "hydro_mhm.hip:64": ["-DMHM_PRECISION=64", "-mllvm", "-amdgpu-sched-strategy=iterative-ilp"]}
 
 
 def build(verbose: bool = False) -> str:
