@@ -370,7 +370,8 @@ class TwoLevel:
 
     def _close(self, pg_in, dt, dh):
         out, flux = hydro.fluid_solver(pg_in, dt, dh, rsolver=self.rsolver, gamma=self.gamma, **self.kw)
-        hydro.correct_unphysical(pg_in, out, flux, dt, dh, gamma=self.gamma, counters=self.counters)
+        hydro.correct_unphysical(pg_in, out, flux, dt, dh, gamma=self.gamma, counters=self.counters,
+                                 main_rsolver=self.rsolver)
         return out, flux
 
     def step(self, dt_base):

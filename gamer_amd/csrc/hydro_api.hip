@@ -289,10 +289,13 @@ int hydro_correct_unphysical_async( const HydroCorrectArgs *a, void *stream )
    if ( a->mode < 0 || a->mode > 2 ) return fail( HYDRO_E_ARG, "correct: mode %d (0 none, 1 3D, 2 3D1D)", a->mode );
    if ( a->rsolver != HYDRO_HLLC && a->rsolver != HYDRO_HLLE && a->rsolver != HYDRO_ROE )
       return fail( HYDRO_E_ARG, "correct: unsupported Riemann solver %d", a->rsolver );
+   if ( a->main_rsolver != 0 && a->main_rsolver != HYDRO_HLLC && a->main_rsolver != HYDRO_HLLE && a->main_rsolver != HYDRO_ROE )
+      return fail( HYDRO_E_ARG, "correct: unsupported main Riemann solver %d", a->main_rsolver );
    if ( !( a->gamma > 1.0 ) || !( a->dh > 0.0 ) || !( a->dt >= 0.0 ) ) return fail( HYDRO_E_ARG, "correct: gamma/dt/dh invalid" );
    CorrPrm c;
    c.mode = a->mode;  c.rsolver = a->rsolver;  c.last_resort_floor = a->last_resort_floor;
    c.auto_reduce = a->auto_reduce_dt;  c.fixup_flux = a->fixup_flux;  c.check_pres = a->check_pres;
+   c.roe_check = ( ( a->main_rsolver == 0 ) ? a->rsolver : a->main_rsolver ) == HYDRO_ROE;
    c.NX = ( a->scheme == HYDRO_MHM_RP ) ? 22 : 20;  c.ghost = ( c.NX - 16 )/2;
    c.min_dens = a->min_dens;  c.min_pres = a->min_pres;  c.min_eint = a->min_eint;  c.gm1 = a->gamma - 1.0;
    HydroSolveArgs s = {};                     // solver constants for the first-order interface solves

@@ -31,11 +31,12 @@ int launch_scatter( const IO *pg_out, IO *grid, int nx, int ny, int nz, hipStrea
 // set the work-queue counter to 0 (a kernel, so the reset can live inside a captured graph)
 int launch_reset_counter( unsigned *counter, hipStream_t stream );
 
-// options of the device CorrectUnphysical (src/Fluid/Flu_Close.cpp:542-1275)
+// options of the device CorrectUnphysical (src/Fluid/Flu_Close.cpp:542-1297)
 struct CorrPrm {
    int    mode;                // OPT__1ST_FLUX_CORR: 0 none, 1 3D, 2 3D1D
    int    rsolver;             // first-order solver: HYDRO_ROE / HYDRO_HLLE / HYDRO_HLLC
    int    last_resort_floor, auto_reduce, fixup_flux, check_pres;
+   int    roe_check;           // a first-order Roe solve checks its intermediate states (the fluid solver is Roe)
    int    NX, ghost;           // input patch-group width and ghost zones of the solver call
    double min_dens, min_pres, min_eint, gm1;
 };

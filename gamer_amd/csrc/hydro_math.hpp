@@ -198,8 +198,10 @@ __device__ __forceinline__ void rs_hllc( const St<R> &L, const St<R> &Q, R pmin,
    f[4] = cf*F[4] + cs*vs;
 }
 
-// Roe solver with the intermediate-state positivity check falling back to HLLE (Roe.cpp:87-711)
-template <typename R>
+// Roe solver with the intermediate-state positivity check falling back to HLLE (Roe.cpp:87-711).  GAMER compiles
+// that check only into builds whose RSOLVER is ROE (include/CUFLU.h:178-186, CHECK_INTERMEDIATE); CHECK = false is
+// the Roe solve of the other builds, which CorrectUnphysical's first-order correction can still select.
+template <typename R, bool CHECK = true>
 __device__ __forceinline__ void rs_roe( const St<R> &L, const St<R> &Q, const Prm<R> &P, R f[NV] )
 {
    const R Z = (R)0, HALF = (R)0.5;
@@ -231,6 +233,7 @@ __device__ __forceinline__ void rs_roe( const St<R> &L, const St<R> &Q, const Pr
    const R a4_ = j0 - a0_ - a1_;
 
    // intermediate states after the 1st and 4th waves (the degenerate middle waves are skipped)
+   if constexpr ( CHECK )
    {
       const R d1 = L.d + a0_, m1 = L.mn + a0_*e0, t1 = L.m1 + a0_*v, t2 = L.m2 + a0_*w, en1 = L.e + a0_*( H - u*a );
       const R ie1 = en1 - HALF*( m1*m1 + t1*t1 + t2*t2 )*rcp( d1 );

@@ -1,6 +1,6 @@
 // Flu_Close and the AMR level synchronisation on the device (the steps after the patch-group solver):
 //
-//   correct_unphysical_kernel   CorrectUnphysical (src/Fluid/Flu_Close.cpp:542-1275) with the Unphysical() check of
+//   correct_unphysical_kernel   CorrectUnphysical (src/Fluid/Flu_Close.cpp:542-1297) with the Unphysical() check of
 //                               :412-535: one lane per output cell; cells that pass the check leave at once, the rare
 //                               failing cells are recomputed from the solver input with first-order fluxes (unsplit 3D,
 //                               then the x->y->z split sweep over the 3^3 neighbourhood for 3D1D), floored, re-checked,
@@ -69,6 +69,7 @@ __device__ __forceinline__ void rs1( const R L[NV], const R Rr[NV], const Prm<R>
    R f[NV];
    if      ( RS == 4 ) rs_hllc( a, b, P.min_pres, f );
    else if ( RS == 3 ) rs_hlle( a, b, f );
+   else if ( RS == 1 ) rs_roe<R, false>( a, b, P, f );     // Roe of a build whose RSOLVER is not ROE
    else                rs_roe( a, b, P, f );
    out[0] = f[0];
    out[4] = f[4];
@@ -332,11 +333,12 @@ int launch_correct_unphysical( const R *in, R *out, R *flux, int npg, const Corr
 {
    if ( npg <= 0 ) return 0;
    const dim3 g( blocks_for( (long)npg*PS*PS*PS ) ), b( 256 );
-   switch ( c.rsolver )
+   switch ( ( c.rsolver == 2 && !c.roe_check ) ? 1 : c.rsolver )
    {
       case 4: hipLaunchKernelGGL( ( correct_unphysical_kernel<R, 4> ), g, b, 0, stream, in, out, flux, npg, c, P, dt_dh, counters ); break;
       case 3: hipLaunchKernelGGL( ( correct_unphysical_kernel<R, 3> ), g, b, 0, stream, in, out, flux, npg, c, P, dt_dh, counters ); break;
       case 2: hipLaunchKernelGGL( ( correct_unphysical_kernel<R, 2> ), g, b, 0, stream, in, out, flux, npg, c, P, dt_dh, counters ); break;
+      case 1: hipLaunchKernelGGL( ( correct_unphysical_kernel<R, 1> ), g, b, 0, stream, in, out, flux, npg, c, P, dt_dh, counters ); break;
       default: return -1;
    }
    return ( hipGetLastError() == hipSuccess ) ? 0 : -2;

@@ -76,6 +76,7 @@ class HydroCorrectArgs(ctypes.Structure):
         ("last_resort_floor", ctypes.c_int), ("auto_reduce_dt", ctypes.c_int), ("fixup_flux", ctypes.c_int),
         ("check_pres", ctypes.c_int), ("min_dens", ctypes.c_double), ("min_pres", ctypes.c_double),
         ("min_eint", ctypes.c_double), ("precision", ctypes.c_int), ("counters", ctypes.c_void_p),
+        ("main_rsolver", ctypes.c_int),
     ]
 
 
@@ -410,10 +411,12 @@ def _int_table(t, shape, name):
 
 def correct_unphysical(u_in, out, flux, dt, dh, mode="3D1D", rsolver="HLLE", scheme="MHM_RP", gamma=5.0 / 3.0,
                        last_resort_floor=True, auto_reduce_dt=False, fixup_flux=True, check_pres=False,
-                       min_dens=0.0, min_pres=0.0, min_eint=0.0, stream=None, counters=None):
+                       min_dens=0.0, min_pres=0.0, min_eint=0.0, stream=None, counters=None, main_rsolver=None):
     """CorrectUnphysical on device: corrects ``out`` (and the coarse-fine planes of ``flux``) in place.
     Returns a 2-element int32 device tensor: [corrected cells, cells still unphysical].  With ``counters`` (an
-    int32 device tensor of 2 elements) the kernel adds its counts to it and returns it, with no extra launches."""
+    int32 device tensor of 2 elements) the kernel adds its counts to it and returns it, with no extra launches.
+    ``main_rsolver`` is the fluid solver's Riemann solver (GAMER's compile-time RSOLVER; None: ``rsolver``): a
+    first-order Roe correction checks its intermediate states only when it is "ROE"."""
     import torch
     n = NX[scheme]
     if tuple(u_in.shape[1:]) != (5, n, n, n) or tuple(out.shape) != (u_in.shape[0], 5, 16, 16, 16):
@@ -436,7 +439,7 @@ def correct_unphysical(u_in, out, flux, dt, dh, mode="3D1D", rsolver="HLLE", sch
                          u_in.shape[0], float(dt), float(dh), float(gamma), SCHEME[scheme], FIRST_FLUX_CORR[mode],
                          RSOLVER[rsolver], int(last_resort_floor), int(auto_reduce_dt), int(fixup_flux),
                          int(check_pres), float(min_dens), float(min_pres), float(min_eint), _prec(out),
-                         cnt.data_ptr())
+                         cnt.data_ptr(), 0 if main_rsolver is None else RSOLVER[main_rsolver])
     _check(lib().hydro_correct_unphysical_async(ctypes.byref(a), _stream(out.device, stream, u_in, out, flux, cnt)))
     return cnt
 

@@ -117,7 +117,7 @@ int hydro_close_uniform_async( const void *pg_out, void *level, int nx, int ny, 
    (0,0,0) (1,0,0) (0,1,0) (0,0,1) (1,1,0) (0,1,1) (1,0,1) (1,1,1).  Flux registers (the patch->flux[s] buffers of
    coarse patches on coarse-fine faces) are double reg[nreg][5][8][8] in both precisions. */
 
-/* Options of CorrectUnphysical (src/Fluid/Flu_Close.cpp:542-1275, run-time parameters of the same names). */
+/* Options of CorrectUnphysical (src/Fluid/Flu_Close.cpp:542-1297, run-time parameters of the same names). */
 typedef struct HydroCorrectArgs {
    const void *in;            /* the solver input of the call, [npg][5][N^3] */
    void       *out;           /* the solver output [npg][5][16^3], corrected in place */
@@ -134,9 +134,13 @@ typedef struct HydroCorrectArgs {
    double min_dens, min_pres, min_eint;
    int    precision;
    int   *counters;           /* device int[2]: [0] += corrected cells, [1] += cells still unphysical (failure) */
+   int    main_rsolver;       /* the fluid solver's Riemann solver, GAMER's compile-time RSOLVER (0: same as rsolver).
+                                 A first-order Roe solve checks its intermediate states, falling back to HLLE, only
+                                 when this is HYDRO_ROE: CHECK_INTERMEDIATE exists only in RSOLVER == ROE builds
+                                 (include/CUFLU.h:178-186, CPU_Shared_RiemannSolver_Roe.cpp:606-667). */
 } HydroCorrectArgs;
 
-/* Replaces CorrectUnphysical (src/Fluid/Flu_Close.cpp:542-1275) and its Unphysical test (:412-535). */
+/* Replaces CorrectUnphysical (src/Fluid/Flu_Close.cpp:542-1297) and its Unphysical test (:412-535). */
 int hydro_correct_unphysical_async( const HydroCorrectArgs *args, void *stream );
 
 /* StoreFlux (src/Fluid/Flu_Close.cpp:234-307): reg[slot[pg][LocalID][s]] = dt * flux of patch face s

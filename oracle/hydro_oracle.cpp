@@ -196,7 +196,11 @@ template <typename T> void rs_hllc_x( const Gas<T> &g, const T L[NV], const T R[
    out[4] += cs*vs;
 }
 
-template <typename T> void rs_roe_x( const Gas<T> &g, const T L[NV], const T R[NV], T pmin, T out[NV] )
+// check_intermediate: GAMER compiles the intermediate-state check into Hydro_RiemannSolver_Roe only in builds whose
+// RSOLVER (or RSOLVER_RESCUE) is ROE (include/CUFLU.h:178-186, CHECK_INTERMEDIATE); in other builds the Roe solve
+// of CorrectUnphysical's first-order correction runs without it
+template <typename T> void rs_roe_x( const Gas<T> &g, const T L[NV], const T R[NV], T pmin, T out[NV],
+                                     bool check_intermediate = true )
 {
    const T Z = (T)0.0, ONE = (T)1.0, HALF = (T)0.5;
    const T rl = ONE / L[0], rr = ONE / R[0];
@@ -238,7 +242,7 @@ template <typename T> void rs_roe_x( const Gas<T> &g, const T L[NV], const T R[N
    // intermediate-state positivity check, falling back to HLLE (Roe.cpp:607-667; CUFLU.h CHECK_INTERMEDIATE)
    T is[NV];
    for (int k=0; k<NV; k++) is[k] = L[k];
-   for (int t=0; t<NV-1; t++)
+   for (int t=0; t<NV-1 && check_intermediate; t++)
    {
       for (int k=0; k<NV; k++) is[k] += amp[t]*rev[k][t];
       if ( ev[t+1] > ev[t] )
@@ -622,6 +626,7 @@ const int LOCAL_XYZ[8][3] = { {0,0,0}, {1,0,0}, {0,1,0}, {0,0,1}, {1,1,0}, {0,1,
 
 struct CorrOpt {
    int    mode, rsolver, last_resort_floor, auto_reduce, fixup_flux, check_pres;
+   int    roe_check;                              // the build's Roe solver checks intermediate states (RSOLVER == ROE)
    double min_dens, min_pres, min_eint, gm1;      // gm1: EoS aux constant gamma-1, a double (CPU_EoS_Gamma.cpp:54-66)
 };
 
@@ -646,8 +651,9 @@ template <typename T> bool close_unphysical( const Gas<T> &g, const T f[NV], boo
 }
 
 // the first-order interface solve of CorrectUnphysical: the solver itself, without the NaN rescue
-// (src/Fluid/Flu_Close.cpp:647-745)
-template <typename T> void riemann_1st( const Gas<T> &g, int rsolver, int d, const T Lin[NV], const T Rin[NV], T pmin, T out[NV] )
+// (src/Fluid/Flu_Close.cpp:647-745); roe_check as in rs_roe_x
+template <typename T> void riemann_1st( const Gas<T> &g, int rsolver, int d, const T Lin[NV], const T Rin[NV], T pmin, T out[NV],
+                                        bool roe_check )
 {
    T L[NV], R[NV];
    for (int v=0; v<NV; v++) { L[v] = Lin[v];  R[v] = Rin[v]; }
@@ -657,12 +663,12 @@ template <typename T> void riemann_1st( const Gas<T> &g, int rsolver, int d, con
    {
       case ORC_HLLC: rs_hllc_x( g, L, R, pmin, out ); break;
       case ORC_HLLE: rs_hlle_x( g, L, R, pmin, out ); break;
-      default:       rs_roe_x ( g, L, R, pmin, out ); break;
+      default:       rs_roe_x ( g, L, R, pmin, out, roe_check ); break;
    }
    rotate( out, d, false );
 }
 
-// CorrectUnphysical() for one patch group (src/Fluid/Flu_Close.cpp:542-1275): cells that fail the
+// CorrectUnphysical() for one patch group (src/Fluid/Flu_Close.cpp:542-1297): cells that fail the
 // Eint-mode check are recomputed from the input with first-order fluxes (unsplit 3D, then the x->y->z split
 // 3D1D sweep over the 3^3 neighbourhood if still unphysical), floored, re-checked, stored, and their first-order
 // fluxes replace the coarse-fine flux planes they touch.  cnt[0] counts corrected cells, cnt[1] failures.
@@ -694,8 +700,8 @@ void correct_unphysical_pg( const Gas<T> &g, const CorrOpt &c, const T *in, T *o
          }
          for (int d=0; d<3; d++)
          {
-            riemann_1st( g, c.rsolver, d, L[d], C, pmin, FL[d] );
-            riemann_1st( g, c.rsolver, d, C, R[d], pmin, FR[d] );
+            riemann_1st( g, c.rsolver, d, L[d], C, pmin, FL[d], c.roe_check != 0 );
+            riemann_1st( g, c.rsolver, d, C, R[d], pmin, FR[d], c.roe_check != 0 );
          }
          for (int d=0; d<3; d++) for (int v=0; v<NV; v++) dF[d][v] = FR[d][v] - FL[d][v];
          for (int v=0; v<NV; v++) Update[v] = C[v] - dt_dh*( dF[0][v] + dF[1][v] + dF[2][v] );
@@ -714,8 +720,8 @@ void correct_unphysical_pg( const Gas<T> &g, const CorrOpt &c, const T *in, T *o
             {
                const int ox = ( d == 0 ), oy = ( d == 1 ), oz = ( d == 2 );
                T fl[NV], fr[NV];
-               riemann_1st( g, c.rsolver, d, blk[z-oz][y-oy][x-ox], blk[z][y][x], pmin, fl );
-               riemann_1st( g, c.rsolver, d, blk[z][y][x], blk[z+oz][y+oy][x+ox], pmin, fr );
+               riemann_1st( g, c.rsolver, d, blk[z-oz][y-oy][x-ox], blk[z][y][x], pmin, fl, c.roe_check != 0 );
+               riemann_1st( g, c.rsolver, d, blk[z][y][x], blk[z+oz][y+oy][x+ox], pmin, fr, c.roe_check != 0 );
                for (int v=0; v<NV; v++) blk[z][y][x][v] -= dt_dh*( fr[v] - fl[v] );
                if ( x == 1 && y == 1 && z == 1 )
                   for (int v=0; v<NV; v++) { FL[d][v] = fl[v];  FR[d][v] = fr[v]; }
@@ -1290,14 +1296,15 @@ int oracle_dt_cfl_f64( const double *in, double *dt, int npatch, double dh, doub
 int oracle_correct_unphysical_##SUF( const T *in, T *out, T *flux, int npg, int scheme, double dt, double dh,          \
                                      double gamma, int mode, int rsolver, int last_resort_floor, int auto_reduce,      \
                                      int fixup_flux, int check_pres, double min_dens, double min_pres,               \
-                                     double min_eint, long counters[2] )                                               \
+                                     double min_eint, int main_rsolver, long counters[2] )                             \
 {                                                                                                                      \
    if ( scheme != ORC_MHM && scheme != ORC_MHM_RP ) return 2;                                                          \
    if ( mode < 0 || mode > 2 ) return 3;                                                                               \
    const int NX = ( scheme == ORC_MHM_RP ) ? 22 : 20, ghost = ( NX - PS )/2;                                           \
    Gas<T> g( gamma );                                                                                                  \
-   CorrOpt c = { mode, rsolver, last_resort_floor, auto_reduce, fixup_flux, check_pres, min_dens, min_pres, min_eint, \
-                 gamma - 1.0 };                                                                                        \
+   const int roe_check = ( ( main_rsolver == 0 ) ? rsolver : main_rsolver ) == ORC_ROE;                                \
+   CorrOpt c = { mode, rsolver, last_resort_floor, auto_reduce, fixup_flux, check_pres, roe_check,                    \
+                 min_dens, min_pres, min_eint, gamma - 1.0 };                                                          \
    const T dt_dh = (T)dt/(T)dh;                                                                                        \
    counters[0] = counters[1] = 0;                                                                                      \
    for (int p=0; p<npg; p++)                                                                                           \

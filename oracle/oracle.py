@@ -45,7 +45,7 @@ def lib():
         L.oracle_dt_cfl_f64.argtypes = [_P, _P, _I, _D, _D, _D, _D, _I]
         for suf in ("f64", "f32"):
             getattr(L, "oracle_correct_unphysical_" + suf).argtypes = [_P, _P, _P, _I, _I, _D, _D, _D, _I, _I, _I, _I,
-                                                                         _I, _I, _D, _D, _D, _P]
+                                                                         _I, _I, _D, _D, _D, _I, _P]
             getattr(L, "oracle_store_flux_" + suf).argtypes = [_P, _I, _P, _D, _P]
             getattr(L, "oracle_correct_flux_" + suf).argtypes = [_P, _I, _P, _D, _P]
             getattr(L, "oracle_fixup_flux_" + suf).argtypes = [_P, _I, _P, _P, _D, _D, _D, _D, _D]
@@ -81,6 +81,9 @@ def ref_lib(scheme: str, rsolver: str, prec: str):
         L.gamer_ref_dt_cfl.restype = _I
         L.gamer_ref_interpolate.argtypes = [_P, _P, _I, _P, _P, _P, _P, _P, _I, _I, _D, _I, _I, _I, _I, _D, _D]
         L.gamer_ref_interpolate.restype = _I
+        L.gamer_ref_correct_unphysical.argtypes = [_P, _P, _P, _I, _D, _D, _D, _I, _I, _I, _I, _I, _I, _D, _D, _D,
+                                                   _P, _I]
+        L.gamer_ref_correct_unphysical.restype = _I
         _refs[key] = L
     return _refs[key]
 
@@ -196,8 +199,10 @@ def _suffix(a):
 
 def correct_unphysical(u_in, out, flux, dt, dh, mode="3D1D", rsolver="HLLE", scheme="MHM_RP", gamma=5.0 / 3.0,
                        last_resort_floor=True, auto_reduce_dt=False, fixup_flux=True, check_pres=False,
-                       min_dens=0.0, min_pres=0.0, min_eint=0.0):
-    """Restatement of CorrectUnphysical; returns (out, flux, counters) as new arrays."""
+                       min_dens=0.0, min_pres=0.0, min_eint=0.0, main_rsolver=None):
+    """Restatement of CorrectUnphysical; returns (out, flux, counters) as new arrays.  main_rsolver is the
+    fluid solver's Riemann solver (GAMER's compile-time RSOLVER; None: the same as rsolver): a first-order Roe
+    correction checks its intermediate states only when it is "ROE"."""
     u_in = np.ascontiguousarray(u_in)
     out = np.array(out, copy=True)
     flux = None if flux is None else np.array(flux, copy=True)
@@ -206,9 +211,37 @@ def correct_unphysical(u_in, out, flux, dt, dh, mode="3D1D", rsolver="HLLE", sch
     rc = fn(_ptr(u_in), _ptr(out), _ptr(flux) if flux is not None else None, out.shape[0], SCHEME[scheme],
             ctypes.c_double(dt), ctypes.c_double(dh), ctypes.c_double(gamma), _FFC[mode], RSOLVER[rsolver],
             int(last_resort_floor), int(auto_reduce_dt), int(fixup_flux), int(check_pres),
-            ctypes.c_double(min_dens), ctypes.c_double(min_pres), ctypes.c_double(min_eint), _ptr(cnt))
+            ctypes.c_double(min_dens), ctypes.c_double(min_pres), ctypes.c_double(min_eint),
+            0 if main_rsolver is None else RSOLVER[main_rsolver], _ptr(cnt))
     if rc != 0:
         raise ValueError(f"oracle_correct_unphysical returned {rc}")
+    return out, flux, cnt
+
+
+_RSOLVER_1ST = {"ROE": 1, "HLLC": 2, "HLLE": 3}     # OptRSolver1st_t (include/Typedef.h:469-474)
+
+
+def ref_correct_unphysical(u_in, out, flux, dt, dh, mode="3D1D", rsolver="HLLE", scheme="MHM_RP", gamma=5.0 / 3.0,
+                           last_resort_floor=True, auto_reduce_dt=False, fixup_flux=True, check_pres=False,
+                           min_dens=0.0, min_pres=0.0, min_eint=0.0, main_rsolver="HLLC", nthreads=0):
+    """GAMER's own CorrectUnphysical (src/Fluid/Flu_Close.cpp, compiled reference of out's precision, the given
+    scheme and RSOLVER = main_rsolver) on the same arguments as correct_unphysical.  Returns (out, flux, (ncorr, failed)): ncorr is what
+    the call adds to NCorrUnphy (0 when a cell failed), failed is 1 when FluStatus_ThisRank was set to
+    GAMER_FAILED.  A cell that stays unphysical with auto_reduce_dt False makes the reference abort the process,
+    so callers check such inputs with the restatement first."""
+    u_in = np.ascontiguousarray(u_in)
+    out = np.array(out, copy=True)
+    flux = None if flux is None else np.array(flux, copy=True)
+    cnt = np.zeros(2, dtype=np.int64)
+    # CorrectUnphysical reads the first-order solver from OPT__1ST_FLUX_CORR_SCHEME at run time; the build's
+    # RSOLVER decides whether its Roe solver checks intermediate states (include/CUFLU.h:178-186)
+    L = ref_lib(scheme, main_rsolver, _suffix(out))
+    rc = L.gamer_ref_correct_unphysical(_ptr(u_in), _ptr(out), _ptr(flux), out.shape[0], dt, dh, gamma, _FFC[mode],
+                                        _RSOLVER_1ST[rsolver], int(last_resort_floor), int(auto_reduce_dt),
+                                        int(fixup_flux), int(check_pres), min_dens, min_pres, min_eint,
+                                        _ptr(cnt), nthreads)
+    if rc != 0:
+        raise ValueError(f"gamer_ref_correct_unphysical returned {rc}")
     return out, flux, cnt
 
 

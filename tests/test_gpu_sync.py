@@ -40,13 +40,15 @@ def _corrupted_case(seed, dtype):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("mode", ["NONE", "3D", "3D1D"])
-@pytest.mark.parametrize("rsolver", ["HLLE", "HLLC", "ROE"])
-def test_correct_unphysical_matches_oracle(dtype, mode, rsolver):
+@pytest.mark.parametrize("rsolver,main", [("HLLE", None), ("HLLC", None), ("ROE", None), ("ROE", "HLLC")])
+def test_correct_unphysical_matches_oracle(dtype, mode, rsolver, main):
+    """main: the fluid solver's Riemann solver (None: rsolver); a first-order Roe solve checks its intermediate
+    states only when it is ROE, as in GAMER builds with RSOLVER == ROE."""
     import oracle
     from gamer_amd import hydro
-    print(f"[gpu] correct_unphysical {np.dtype(dtype).name} {mode} {rsolver}", flush=True)
+    print(f"[gpu] correct_unphysical {np.dtype(dtype).name} {mode} {rsolver} main={main}", flush=True)
     u, out, flux, dt, dh = _corrupted_case(11, dtype)
-    kw = dict(mode=mode, rsolver=rsolver, min_dens=1e-6, min_pres=1e-8, min_eint=1e-8)
+    kw = dict(mode=mode, rsolver=rsolver, min_dens=1e-6, min_pres=1e-8, min_eint=1e-8, main_rsolver=main)
     ro, rf, rc = oracle.correct_unphysical(u, out, flux, dt, dh, **kw)
     go, gf = _dev(out), _dev(flux)
     cnt = hydro.correct_unphysical(_dev(u), go, gf, dt, dh, **kw)

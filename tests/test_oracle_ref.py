@@ -149,3 +149,104 @@ def test_interpolate_bit_exact_vs_reference_build(prec, scheme, case):
     c2, f2 = oracle.ref_interpolate(c, f0, (g,) * 3, cr, **kw)
     np.testing.assert_array_equal(f1, f2)
     np.testing.assert_array_equal(c1, c2)
+
+
+# ---------------- CorrectUnphysical (src/Fluid/Flu_Close.cpp:542-1297) ----------------
+def _damaged_closure_case(scheme, prec, seed, with_nan):
+    """A solver output with ~10% of its cells made unphysical in four ways (negative density, negative internal
+    energy, density below MIN_DENS, NaN momentum), spread over every patch-group plane so the first-order fluxes
+    replace coarse-fine flux entries on all nine face planes."""
+    g = 3 if scheme == "MHM_RP" else 2
+    w = ic.random_sod((1, 2, 2), seed=seed, scheme_ghost=g, sedov=True, cfl=0.9, sigma_v=1.0)
+    u = w.u if prec == "f64" else w.u.astype(np.float32)
+    out, flux = oracle.fluid_solver(u, w.dt, w.dh, scheme=scheme, rsolver="HLLC")
+    rng = np.random.default_rng(seed)
+    bad = out.copy().reshape(out.shape[0], 5, -1)
+    damaged = []
+    for p in range(bad.shape[0]):
+        cells = rng.choice(bad.shape[2], size=410, replace=False)
+        damaged.append(cells)
+        kinds = rng.integers(0, 4 if with_nan else 3, size=cells.size)
+        for c, k in zip(cells, kinds):
+            rho, m = bad[p, 0, c], bad[p, 1:4, c]
+            if k == 0:
+                bad[p, 0, c] = -abs(rho)
+            elif k == 1:
+                bad[p, 4, c] = 0.25 * 0.5 * np.sum(m * m) / rho - 0.01
+            elif k == 2:
+                bad[p, 0, c] = rho * 1e-7
+            else:
+                bad[p, 1 + int(rng.integers(3)), c] = np.nan
+    eint = u[:, 4] - 0.5 * np.sum(u[:, 1:4] ** 2, axis=1) / u[:, 0]
+    meint = float(np.quantile(eint, 0.25))
+    if with_nan:                    # a NaN input cell under a damaged output cell: no correction can succeed
+        u = u.copy()
+        for p in range(u.shape[0]):
+            k, j, i = np.unravel_index(damaged[p][:3], (16, 16, 16))
+            u[p, 0, k + g, j + g, i + g] = np.nan
+    return u, bad.reshape(out.shape), flux, w.dt, w.dh, meint
+
+
+# (last_resort_floor, auto_reduce_dt, fixup_flux, check_pres, min_dens, min_pres factor, NaN damage)
+_CLOSURE_OPTS = {
+    "floors": (True, False, True, False, 1e-3, 0.0, False),
+    "auto_reduce": (False, True, True, True, 1e-3, 1.0, True),
+    "floors_pres": (True, False, False, True, 1e-4, 0.5, False),
+}
+
+
+@pytest.mark.parametrize("scheme,prec", [(s, p) for s in ("MHM_RP", "MHM") for p in ("f64", "f32")])
+@pytest.mark.parametrize("rsolver,main", [("ROE", "HLLC"), ("ROE", "ROE"), ("HLLC", "HLLE"), ("HLLE", "ROE")])
+@pytest.mark.parametrize("mode", ["NONE", "3D", "3D1D"])
+def test_correct_unphysical_bit_exact_vs_reference_build(scheme, prec, rsolver, main, mode):
+    """oracle.correct_unphysical against GAMER's own CorrectUnphysical (Flu_Close.cpp compiled in place into the
+    build with RSOLVER = main), bit for bit on the corrected output and the flux array, and on the outcome: the
+    number of corrected cells, or the AUTO_REDUCE_DT failure flag.  The first-order Roe solve checks its
+    intermediate states only in the RSOLVER = ROE build.  Three option sets: last-resort floors
+    (OPT__LAST_RESORT_FLOOR, MIN_DENS, MIN_EINT at the input's 25% Eint quantile); AUTO_REDUCE_DT with the pressure
+    check (no floors, NaN damage, so cells fail); floors with the pressure check at MIN_PRES = (gamma-1) MIN_EINT / 2
+    and no flux fix-up."""
+    if not oracle.ref_available(scheme, main, prec):
+        pytest.skip("reference build (oracle/_ref) not present")
+    seed = zlib.crc32(f"{scheme}{prec}{rsolver}{mode}".encode()) % 100000
+    for name, (floor, auto, fix, chk, mdens, pfac, nan) in _CLOSURE_OPTS.items():
+        u, bad, flux, dt, dh, meint = _damaged_closure_case(scheme, prec, seed, nan)
+        kw = dict(mode=mode, rsolver=rsolver, scheme=scheme, last_resort_floor=floor, auto_reduce_dt=auto,
+                  fixup_flux=fix, check_pres=chk, min_dens=mdens, min_pres=pfac * (2.0 / 3.0) * meint,
+                  min_eint=meint, main_rsolver=main)
+        o1, f1, c1 = oracle.correct_unphysical(u, bad, flux, dt, dh, **kw)
+        if not auto and c1[1]:
+            # without AUTO_REDUCE_DT a failed cell aborts the reference process: cells that the restatement leaves
+            # unwritten and unphysical (e.g. a Roe solve without the intermediate check returning NaN) get the
+            # input state of their position, which needs no correction
+            same = np.all((o1 == bad) | (np.isnan(o1) & np.isnan(bad)), axis=1, keepdims=True)
+            with np.errstate(all="ignore"):
+                e = bad[:, 4:5] - 0.5 * np.sum(bad[:, 1:4] ** 2, axis=1, keepdims=True) / bad[:, 0:1]
+            flag = ~np.all(np.isfinite(bad), axis=1, keepdims=True) | (bad[:, 0:1] < mdens) | (e < meint)
+            g = (u.shape[-1] - 16) // 2
+            bad = np.where(same & flag, u[:, :, g:g + 16, g:g + 16, g:g + 16], bad)
+            o1, f1, c1 = oracle.correct_unphysical(u, bad, flux, dt, dh, **kw)
+        assert auto or c1[1] == 0, (name, c1)
+        o2, f2, c2 = oracle.ref_correct_unphysical(u, bad, flux, dt, dh, **kw)
+        np.testing.assert_array_equal(o1, o2, err_msg=name)
+        np.testing.assert_array_equal(f1, f2, err_msg=name)
+        assert c2.tolist() == ([c1[0], 0] if c1[1] == 0 else [0, 1]), (name, c1, c2)
+        if auto:
+            assert c1[1] > 0, (name, c1)
+        if not (auto and mode == "NONE"):          # with neither a correction nor floors every cell fails
+            assert c1[0] > 1000, (name, c1)
+
+
+def test_correct_unphysical_split_sweep_runs_in_the_reference():
+    """With MIN_EINT at the input's 25% quantile the x->y->z split sweep (FIRST_FLUX_CORR_3D1D) changes cells that
+    the unsplit correction leaves unphysical: the reference's 3D and 3D1D outputs differ, and the restatement
+    matches both (so the bit-exact test above covers the split branch, not only the unsplit one)."""
+    if not oracle.ref_available("MHM_RP", "HLLC", "f64"):
+        pytest.skip("reference build (oracle/_ref) not present")
+    u, bad, flux, dt, dh, meint = _damaged_closure_case("MHM_RP", "f64", 4242, False)
+    kw = dict(rsolver="HLLE", min_dens=1e-3, min_eint=meint)
+    r3, _, _ = oracle.ref_correct_unphysical(u, bad, flux, dt, dh, mode="3D", **kw)
+    r31, _, _ = oracle.ref_correct_unphysical(u, bad, flux, dt, dh, mode="3D1D", **kw)
+    o31, _, _ = oracle.correct_unphysical(u, bad, flux, dt, dh, mode="3D1D", **kw)
+    assert np.count_nonzero(np.any(r3 != r31, axis=1)) > 10
+    np.testing.assert_array_equal(o31, r31)
