@@ -84,6 +84,13 @@ def ref_lib(scheme: str, rsolver: str, prec: str):
         L.gamer_ref_correct_unphysical.argtypes = [_P, _P, _P, _I, _D, _D, _D, _I, _I, _I, _I, _I, _I, _D, _D, _D,
                                                    _P, _I]
         L.gamer_ref_correct_unphysical.restype = _I
+        L.gamer_ref_store_flux.argtypes = [_P, _I, _P, _D, _P]
+        L.gamer_ref_correct_flux.argtypes = [_P, _I, _P, _D, _P]
+        L.gamer_ref_fixup_flux.argtypes = [_P, _I, _P, _P, _D, _D, _D, _D, _D]
+        L.gamer_ref_fixup_restrict.argtypes = [_P, _P, _I, _P, _P, _I, _D]
+        L.gamer_ref_close_patches.argtypes = [_P, _P, _I, _P, _I]
+        for f in ("store_flux", "correct_flux", "fixup_flux", "fixup_restrict", "close_patches"):
+            getattr(L, "gamer_ref_" + f).restype = _I
         _refs[key] = L
     return _refs[key]
 
@@ -283,6 +290,52 @@ def fixup_restrict(son, fa, son0, fa_idx, min_eint=0.0):
 def close_patches(pg_out, patch, pid0):
     pg_out, patch, pid0 = np.ascontiguousarray(pg_out), np.array(patch, copy=True), _i32(pid0)
     getattr(lib(), "oracle_close_patches_" + _suffix(patch))(_ptr(pg_out), _ptr(patch), pg_out.shape[0], _ptr(pid0))
+    return patch
+
+
+# GAMER's own StoreFlux / CorrectFlux / Flu_Close copy-back (src/Fluid/Flu_Close.cpp) and Flu_FixUp_Flux /
+# Flu_FixUp_Restrict (src/Fluid/Flu_FixUp_*.cpp) on small patch trees built from the same slot tables, in the
+# compiled reference of the arrays' precision (its flux arrays are `real`: registers pass through that type)
+def _ref_sync(a):
+    return ref_lib("MHM_RP", "HLLC", _suffix(a))
+
+
+def _rc(name, rc):
+    if rc != 0:
+        raise ValueError(f"gamer_ref_{name} returned {rc}")
+
+
+def ref_store_flux(flux, slot, dt, reg):
+    flux, slot, reg = np.ascontiguousarray(flux), _i32(slot), np.array(reg, dtype=np.float64, copy=True)
+    _rc("store_flux", _ref_sync(flux).gamer_ref_store_flux(_ptr(flux), flux.shape[0], _ptr(slot), dt, _ptr(reg)))
+    return reg
+
+
+def ref_correct_flux(flux, slot, dt, reg):
+    flux, slot, reg = np.ascontiguousarray(flux), _i32(slot), np.array(reg, dtype=np.float64, copy=True)
+    _rc("correct_flux", _ref_sync(flux).gamer_ref_correct_flux(_ptr(flux), flux.shape[0], _ptr(slot), dt, _ptr(reg)))
+    return reg
+
+
+def ref_fixup_flux(patch, slot, reg, dh, gamma=5.0 / 3.0, min_dens=0.0, min_pres=0.0, min_eint=0.0):
+    patch, slot, reg = np.array(patch, copy=True), _i32(slot), np.ascontiguousarray(reg, dtype=np.float64)
+    _rc("fixup_flux", _ref_sync(patch).gamer_ref_fixup_flux(_ptr(patch), patch.shape[0], _ptr(slot), _ptr(reg), dh,
+                                                            gamma, min_dens, min_pres, min_eint))
+    return patch
+
+
+def ref_fixup_restrict(son, fa, son0, fa_idx, min_eint=0.0):
+    son, fa = np.ascontiguousarray(son), np.array(fa, copy=True)
+    son0, fa_idx = _i32(son0), _i32(fa_idx)
+    _rc("fixup_restrict", _ref_sync(fa).gamer_ref_fixup_restrict(_ptr(son), _ptr(fa), son0.shape[0], _ptr(son0),
+                                                                 _ptr(fa_idx), fa.shape[0], min_eint))
+    return fa
+
+
+def ref_close_patches(pg_out, patch, pid0):
+    pg_out, patch, pid0 = np.ascontiguousarray(pg_out), np.array(patch, copy=True), _i32(pid0)
+    _rc("close_patches", _ref_sync(patch).gamer_ref_close_patches(_ptr(pg_out), _ptr(patch), pg_out.shape[0],
+                                                                  _ptr(pid0), patch.shape[0]))
     return patch
 
 

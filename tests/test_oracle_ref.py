@@ -250,3 +250,109 @@ def test_correct_unphysical_split_sweep_runs_in_the_reference():
     o31, _, _ = oracle.correct_unphysical(u, bad, flux, dt, dh, mode="3D1D", **kw)
     assert np.count_nonzero(np.any(r3 != r31, axis=1)) > 10
     np.testing.assert_array_equal(o31, r31)
+
+
+# ---------------- StoreFlux / CorrectFlux / Flu_Close copy-back / Flu_FixUp_Flux / Flu_FixUp_Restrict ----------------
+# GAMER keeps flux registers in `real`; this build keeps them in FP64 in both precisions (the FP32-bulk path
+# accumulates its flux corrections in FP64).  FP64: bit for bit.  FP32: within the rounding of one FP32 register
+# operation per term, which is the whole difference between the two.
+_SYNC_PRECS = [np.float64, np.float32]
+
+
+def _sync_ref_or_skip(dtype):
+    prec = "f64" if dtype == np.float64 else "f32"
+    if not oracle.ref_available("MHM_RP", "HLLC", prec):
+        pytest.skip("reference build (oracle/_ref) not present")
+
+
+def _slots(rng, shape, frac):
+    """A slot table of the given shape: about `frac` of the entries get distinct register indices, the rest -1."""
+    use = rng.random(shape) < frac
+    slot = -np.ones(shape, dtype=np.int32)
+    slot[use] = rng.permutation(int(use.sum())).astype(np.int32)
+    return slot, int(use.sum())
+
+
+@pytest.mark.parametrize("dtype", _SYNC_PRECS)
+def test_store_and_correct_flux_vs_reference_build(dtype):
+    """oracle.store_flux / correct_flux against GAMER's StoreFlux / CorrectFlux (Flu_Close.cpp:234-395) on patch
+    trees built from the same slot tables: every (LocalID, face) of 5 patch groups with a register, and fine patch
+    groups whose faces border coarse registers, including faces at a non-periodic boundary (sibling -2)."""
+    _sync_ref_or_skip(dtype)
+    rng = np.random.default_rng(31)
+    npg, dt = 5, 0.0173
+    flux = rng.standard_normal((npg, 9, 5, 16, 16)).astype(dtype)
+    slot, nreg = _slots(rng, (npg, 8, 6), 0.6)
+    r1 = oracle.store_flux(flux, slot, dt, np.zeros((nreg, 5, 8, 8)))
+    r2 = oracle.ref_store_flux(flux, slot, dt, np.zeros((nreg, 5, 8, 8)))
+    cslot, ncreg = _slots(rng, (npg, 6), 0.7)
+    reg0 = rng.standard_normal((ncreg, 5, 8, 8)) * dt
+    c1 = oracle.correct_flux(flux, cslot, 0.5 * dt, reg0)
+    c2 = oracle.ref_correct_flux(flux, cslot, 0.5 * dt, reg0)
+    assert nreg > 100 and ncreg > 10 and not np.array_equal(c1, reg0)
+    if dtype == np.float64:
+        np.testing.assert_array_equal(r1, r2)
+        np.testing.assert_array_equal(c1, c2)
+    else:
+        eps = np.finfo(np.float32).eps
+        np.testing.assert_allclose(r1, r2, rtol=eps, atol=0)
+        np.testing.assert_allclose(c1, c2, rtol=0, atol=8 * eps * (np.abs(reg0).max() + dt * np.abs(flux).max()))
+
+
+def _physical_patches(rng, n, dtype):
+    p = np.empty((n, 5, 8, 8, 8))
+    p[:, 0] = np.exp(0.4 * rng.standard_normal((n, 8, 8, 8)))
+    p[:, 1:4] = 0.5 * rng.standard_normal((n, 3, 8, 8, 8)) * p[:, :1]
+    p[:, 4] = 0.5 * np.sum(p[:, 1:4] ** 2, axis=1) / p[:, 0] + np.exp(0.4 * rng.standard_normal((n, 8, 8, 8))) / (2.0 / 3.0)
+    return p.astype(dtype)
+
+
+@pytest.mark.parametrize("dtype", _SYNC_PRECS)
+def test_fixup_flux_vs_reference_build(dtype):
+    """oracle.fixup_flux against GAMER's Flu_FixUp_Flux: 12 patches with registers on ~half their faces (edge and
+    corner cells get two or three corrections in face order); 5% of the register entries are large enough that
+    the corrected cell fails the MIN_DENS / MIN_EINT / MIN_PRES test and is left as it was."""
+    _sync_ref_or_skip(dtype)
+    rng = np.random.default_rng(32)
+    patch = _physical_patches(rng, 12, dtype)
+    slot, nreg = _slots(rng, (12, 6), 0.5)
+    dh = 1.0 / 128
+    reg = 0.02 * dh * rng.standard_normal((nreg, 5, 8, 8))
+    big = rng.random(reg.shape[:1] + reg.shape[2:]) < 0.05
+    reg[:, 0][big] = 50.0 * dh
+    kw = dict(dh=dh, min_dens=1e-3, min_pres=1e-3, min_eint=1e-3)
+    p1 = oracle.fixup_flux(patch, slot, reg, **kw)
+    p2 = oracle.ref_fixup_flux(patch, slot, reg, **kw)
+    changed = np.any(p1 != patch, axis=1)
+    assert changed.sum() > 500 and (~changed).sum() > 0
+    if dtype == np.float64:
+        np.testing.assert_array_equal(p1, p2)
+    else:
+        eps = np.finfo(np.float32).eps              # the fields are O(1)
+        np.testing.assert_allclose(p1, p2, rtol=4 * eps, atol=4 * eps)
+
+
+@pytest.mark.parametrize("dtype", _SYNC_PRECS)
+def test_fixup_restrict_and_close_patches_bit_exact_vs_reference_build(dtype):
+    """oracle.fixup_restrict against GAMER's Flu_FixUp_Restrict (8-son average, then the MIN_EINT floor on every
+    father cell, which the MIN_EINT here makes fire) and oracle.close_patches against the copy-back of GAMER's
+    Flu_Close (Flu_Close.cpp:125-212): bit for bit in both precisions, since neither uses a flux register."""
+    _sync_ref_or_skip(dtype)
+    rng = np.random.default_rng(33)
+    son = _physical_patches(rng, 40, dtype)
+    fa = _physical_patches(rng, 7, dtype)
+    son0 = np.array([8, 24, 0, 32, 16])
+    fa_idx = np.array([3, 0, 6, 1, 4])
+    f1 = oracle.fixup_restrict(son, fa, son0, fa_idx, min_eint=1.6)
+    f2 = oracle.ref_fixup_restrict(son, fa, son0, fa_idx, min_eint=1.6)
+    np.testing.assert_array_equal(f1, f2)
+    assert np.array_equal(f1[[2, 5]], fa[[2, 5]]) and not np.array_equal(f1[3], fa[3])
+    floored = np.count_nonzero(f1[:, 4] != oracle.fixup_restrict(son, fa, son0, fa_idx, min_eint=0.0)[:, 4])
+    assert floored > 100, floored
+    pg_out = rng.standard_normal((3, 5, 16, 16, 16)).astype(dtype)
+    patch0 = rng.standard_normal((32, 5, 8, 8, 8)).astype(dtype)
+    pid0 = np.array([16, 0, 24])
+    c1 = oracle.close_patches(pg_out, patch0, pid0)
+    c2 = oracle.ref_close_patches(pg_out, patch0, pid0)
+    np.testing.assert_array_equal(c1, c2)
+    np.testing.assert_array_equal(c1[8:16], patch0[8:16])
