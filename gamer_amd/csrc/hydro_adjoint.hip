@@ -237,23 +237,12 @@ template <> struct Load< Dual<1> >
 constexpr int bpp( int count ) { return ( count + NT - 1 ) / NT; }
 constexpr int N_F4 = PS*PS*PS, N_R4 = 3*NFF, N_R2 = 8000;
 
-// The dispatcher deals workgroups to the 8 XCDs round-robin (workgroup b runs on XCD b % 8), and each XCD has its
-// own L2.  Consecutive logical blocks of a phase share data: the planes k-1..k+1 of a plane-per-workgroup phase,
-// the W stencils of neighbouring face blocks.  The logical block index is therefore laid out XCD-major: XCD x
-// runs the logical blocks [off_x, off_x + n_x) in dispatch order, n_x = G/8 or G/8 + 1.  A bijection of
-// [0, gridDim.x): every logical block runs once, and no phase depends on which workgroup runs which block.
-__device__ __forceinline__ int xcd_block()
-{
-   constexpr int X = 8;
-   const int G = (int)gridDim.x, b = (int)blockIdx.x;
-   const int q = G / X, r = G % X, x = b % X, i = b / X;
-   return ( ( x < r ) ? x*( q + 1 ) : r*( q + 1 ) + ( x - r )*q ) + i;
-}
-
+// Consecutive logical blocks of a phase share data (the planes k-1..k+1 of a plane-per-workgroup phase, the W
+// stencils of neighbouring face blocks), so every phase takes its block index from xcd_block() (hydro_kernels.hpp).
 struct Elem { int pg, e; };
 __device__ __forceinline__ Elem elem( int count )
 {
-   const int b = bpp( count ), l = xcd_block();
+   const int b = bpp( count ), l = (int)xcd_block();       // grids_fit keeps the grid below 2^31 blocks
    return { l / b, ( l % b )*NT + (int)threadIdx.x };
 }
 
@@ -291,7 +280,7 @@ f12_kernel( const double *__restrict__ g_in, const double *__restrict__ g_tin, d
 {
    __shared__ double s_raw[NV*F12_NF*( sizeof( T )/sizeof( double ) )];
    T *sF = reinterpret_cast<T*>( s_raw );                    // [v][face]
-   const int tid = threadIdx.x, lb = xcd_block(), pg = lb / 20, k = 1 + lb % 20;
+   const int tid = threadIdx.x, lb = (int)xcd_block(), pg = lb / 20, k = 1 + lb % 20;
    const Prm<T> P = phase_prm<T>( P0, minmod_coeff );
    const double *in = g_in + pg*NIN, *tin = g_tin ? g_tin + pg*NIN : nullptr;
    Fwd<T> F( reinterpret_cast<T*>( g_ws + (size_t)pg*WS_ADJ*N3 ) );
@@ -520,7 +509,7 @@ __global__ void __launch_bounds__( R3X_NT )
 r3x_kernel( double *__restrict__ g_ws, Prm<double> P0, double minmod_coeff )
 {
    __shared__ double sW[R3X_PAIRS*R3X_PAIR], sG0[R3X_PAIRS*R3X_PAIR], sG1[R3X_PAIRS*R3X_PAIR];
-   const int tid = threadIdx.x, lb = xcd_block(), pg = lb / R3X_BPP, q0 = ( lb % R3X_BPP )*R3X_PAIRS;
+   const int tid = threadIdx.x, lb = (int)xcd_block(), pg = lb / R3X_BPP, q0 = ( lb % R3X_BPP )*R3X_PAIRS;
    const int np = min( R3X_PAIRS, R3X_NPAIR - q0 );
    const Prm<double> P = phase_prm<double>( P0, minmod_coeff );
    double *ws = g_ws + (size_t)pg*WS_ADJ*N3;
@@ -638,7 +627,7 @@ r10_kernel( const double *__restrict__ g_in, const double *__restrict__ g_lout, 
 {
    constexpr int R10_NT = r10_nt( RS );
    __shared__ double sX[2][NV][R10_NXF], sY[2][NV][R10_NXF], sZ[NV][R10_NZF];     // [low/high side][v][face]
-   const int tid = threadIdx.x, lb = xcd_block(), pg = lb / 22, k = lb % 22;
+   const int tid = threadIdx.x, lb = (int)xcd_block(), pg = lb / 22, k = lb % 22;
    const Prm<double> P = phase_prm<double>( P0, minmod_coeff );
    const double *in = g_in + pg*NIN, *lout = g_lout + pg*NOUT;
    double *lin = g_lin + pg*NIN, *ws = g_ws + (size_t)pg*WS_ADJ*N3, *zgr = ws + (size_t)WS_ZGR*N3;

@@ -5,6 +5,21 @@
 
 namespace gamer_amd {
 
+// The dispatcher deals the workgroups of a launch to the 8 XCDs round-robin in dispatch order (x, then y, then z:
+// workgroup b runs on XCD b % 8), and each XCD has its own L2.  Kernels whose neighbouring workgroups read the same
+// data (planes k-1..k+1 of the adjoint's plane-per-workgroup phases, their neighbouring face stencils) take
+// their logical block index from xcd_block(), laid out XCD-major: XCD x runs the logical blocks
+// [off_x, off_x + n_x) in dispatch order, n_x = G/8 or G/8 + 1.  It is a bijection of [0, G), so every logical
+// block runs exactly once and only the placement changes.  64-bit, as a 3-D grid may hold more than 2^31 blocks.
+__device__ __forceinline__ long xcd_block()
+{
+   constexpr long X = 8;
+   const long G = (long)gridDim.x*gridDim.y*gridDim.z;
+   const long b = blockIdx.x + (long)gridDim.x*( blockIdx.y + (long)gridDim.y*blockIdx.z );
+   const long q = G / X, r = G % X, x = b % X, i = b / X;
+   return ( ( x < r ) ? x*( q + 1 ) : r*( q + 1 ) + ( x - r )*q ) + i;
+}
+
 // MHM_RP patch-group update: in[npg][5][22^3] -> out[npg][5][16^3] (+ flux[npg][9][5][16^2] if store_flux)
 template <typename R, typename IO>
 int launch_mhm_rp( const IO *in, IO *out, IO *flux, int npg, int rsolver, int store_flux, int minmod_maxiter,
