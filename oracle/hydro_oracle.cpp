@@ -14,11 +14,17 @@
 //   conversions and checks     src/Model_Hydro/CPU_Hydro/CPU_Shared_FluUtility.cpp:93-551, 681-1330
 //   EOS_GAMMA                  src/EoS/Gamma/CPU_EoS_Gamma.cpp:54-192
 //   CFL time step              src/Model_Hydro/CPU_Hydro/CPU_dtSolver_HydroCFL.cpp:57-209
+//   closure and AMR sync       src/Fluid/Flu_Close.cpp:83-1297 (CorrectUnphysical, StoreFlux, CorrectFlux, the
+//                              patch copy), src/Fluid/Flu_FixUp_Flux.cpp, src/Fluid/Flu_FixUp_Restrict.cpp
+//   interpolation              src/Interpolation/Interpolate.cpp, Int_*.cpp
 //
-// Pinning: tests/test_oracle_ref.py (bit-for-bit against oracle/_ref on seeded inputs),
-// tests/test_oracle_golden.py (committed fixtures made from the reference build) and
+// Pinning: tests/test_oracle_ref.py (bit-for-bit against oracle/_ref on seeded inputs: the solver, the CFL
+// time step, Interpolate, and the closure / sync functions, whose FP32 flux registers this restatement keeps in
+// FP64 and which therefore agree in FP32 to the reference's float rounding),
+// tests/test_oracle_golden.py (committed fixtures made from the reference build),
 // tests/test_oracle_props.py (uniform state, Riemann consistency, upwinding, limiter closed forms,
-// conservation telescoping, axis-permutation symmetry).
+// conservation telescoping, axis-permutation symmetry), tests/test_oracle_sync.py and test_oracle_interp.py
+// (closed forms and invariants of the closure, sync and interpolation functions).  Every function has a pin.
 
 #include "hydro_oracle.h"
 #include <cmath>
