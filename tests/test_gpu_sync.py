@@ -61,6 +61,33 @@ def test_correct_unphysical_matches_oracle(dtype, mode, rsolver, main):
     np.testing.assert_allclose(gf, rf, rtol=tol, atol=tol * np.nanmax(np.abs(rf)))
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("mode", ["3D", "3D1D"])
+def test_correct_unphysical_mhm_scheme_matches_oracle(dtype, mode):
+    """The MHM scheme's closure: the solver input has 2 ghost zones (20^3 per patch group), so the first-order
+    stencils and the 3D1D neighbourhood index a 20^3 array."""
+    import oracle
+    from gamer_amd import hydro
+    print(f"[gpu] correct_unphysical MHM {np.dtype(dtype).name} {mode}", flush=True)
+    w = ic.random_sod((2, 1, 2), seed=23, scheme_ghost=2, sedov=True)
+    u = w.u.astype(dtype)
+    out, flux = oracle.fluid_solver(u, w.dt, w.dh, scheme="MHM")
+    rng = np.random.default_rng(23)
+    o = out.reshape(out.shape[0], 5, -1)
+    for c in rng.choice(o.shape[0] * 4096, size=o.shape[0] * 60, replace=False):
+        p, q = divmod(int(c), 4096)
+        o[p, 0, q] = -abs(o[p, 0, q])
+    kw = dict(mode=mode, rsolver="HLLC", scheme="MHM", min_dens=1e-6, min_pres=1e-8, min_eint=0.3)
+    ro, rf, rc = oracle.correct_unphysical(u, out, flux, w.dt, w.dh, **kw)
+    go, gf = _dev(out), _dev(flux)
+    cnt = hydro.correct_unphysical(_dev(u), go, gf, w.dt, w.dh, **kw)
+    torch.cuda.synchronize()
+    assert cnt.cpu().tolist() == rc.tolist() and rc[0] >= out.shape[0] * 60
+    tol = 1e-13 if dtype == np.float64 else 2e-5
+    np.testing.assert_allclose(go.cpu().numpy(), ro, rtol=tol, atol=tol * np.nanmax(np.abs(ro)))
+    np.testing.assert_allclose(gf.cpu().numpy(), rf, rtol=tol, atol=tol * np.nanmax(np.abs(rf)))
+
+
 def test_correct_unphysical_accumulates_into_given_counters():
     """With ``counters`` the kernel adds to the caller's tensor (the AMR step keeps one running count): two calls
     leave the sum of the oracle's counts, and the outputs equal those of the call without it."""
