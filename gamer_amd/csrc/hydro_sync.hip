@@ -86,8 +86,12 @@ __device__ __forceinline__ void rs1_dir( int d, const R L[NV], const R Rr[NV], c
    else               rs1<RS, 2>( L, Rr, P, out );
 }
 
+// Four waves per SIMD (128 VGPRs): nearly every lane only loads and tests its cell, and the rare correction path
+// (the 3^3 split-sweep block) spills instead of holding the whole kernel at the 2 waves per SIMD its 246 VGPRs
+// allowed.  On the 4096-PG bench output with 4 damaged cells per patch group (3D1D, HLLE) the kernel went from
+// 0.328 to 0.219 ms (FP32) and 0.271 to 0.242 ms (FP64), bitwise equal; 8 waves per SIMD (64 VGPRs) was slower in FP64.
 template <typename R, int RS>
-__global__ void __launch_bounds__( 256 )
+__global__ void __launch_bounds__( 256 ) __attribute__(( amdgpu_waves_per_eu( 4 ) ))
 correct_unphysical_kernel( const R *__restrict__ g_in, R *__restrict__ g_out, R *__restrict__ g_flux, int npg,
                            CorrPrm c, Prm<R> P, R dt_dh, int *counters )
 {
