@@ -12,7 +12,8 @@
 //                               of registered patch faces, faces applied in s order per cell, with the positivity test
 //   fixup_restrict_kernel       Flu_FixUp_Restrict (src/Fluid/Flu_FixUp_Restrict.cpp): father cell = mean of its 8 son
 //                               cells, then the internal-energy floor
-//   close_patches_kernel        the copy of patch-group outputs into their 8 patches (Flu_Close.cpp:133-160)
+//   close_patches_kernel        the copy of patch-group outputs into their 8 patches (Flu_Close.cpp:133-160), one
+//                               aligned 4-vector per thread
 //
 // Patch data use GAMER's per-patch layout [npatch][5][8][8][8]; patch LocalID offsets follow TABLE_02
 // (src/Tables/Table_02.cpp:22-70).  Flux registers are FP64 [nreg][5][8][8] in both precisions (the reference keeps
@@ -309,18 +310,20 @@ __global__ void fixup_restrict_kernel( const IO *__restrict__ son, IO *__restric
 }
 
 template <typename IO>
-__global__ void close_patches_kernel( const IO *__restrict__ pg_out, IO *__restrict__ patch, int npg,
-                                      const int *__restrict__ pid0 )
+__global__ void __launch_bounds__( 256 )
+close_patches_kernel( const IO *__restrict__ pg_out, IO *__restrict__ patch, const int *__restrict__ pid0, bool vec )
 {
-   const long total = (long)npg*NV*PS*PS*PS;
-   for (long e = blockIdx.x*(long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x*blockDim.x)
-   {
-      const int c = (int)( e % ( PS*PS*PS ) ), v = (int)( ( e/( PS*PS*PS ) ) % NV );
-      const long p = e/( (long)NV*PS*PS*PS );
-      const int I = c % PS, J = ( c/PS ) % PS, K = c/( PS*PS );
-      const int lid = local_id( I/H, J/H, K/H );
-      patch[ ( (long)( pid0[p] + lid )*NV + v )*P3 + ( ( K % H )*H + J % H )*H + I % H ] = pg_out[e];
-   }
+   // block (PG p, field v, planes K = 4 kq .. 4 kq + 3); thread: 4 consecutive cells I = 4 q .. 4 q + 3 of row J of
+   // plane K, which stay inside one patch (H = 8), so both sides move as one aligned 4-vector
+   typedef IO V4 __attribute__(( ext_vector_type( 4 ) ));
+   const int p = blockIdx.x, v = blockIdx.y >> 2, kq = blockIdx.y & 3, t = threadIdx.x;
+   const int I = 4*( t & 3 ), J = ( t >> 2 ) & 15, K = 4*kq + ( t >> 6 );
+   const IO *src = pg_out + ( (size_t)p*NV + v )*PS*PS*PS + ( K*PS + J )*PS + I;
+   IO *dst = patch + ( (size_t)( pid0[p] + local_id( I/H, J/H, K/H ) )*NV + v )*P3 + ( ( K % H )*H + J % H )*H + I % H;
+   if ( vec ) *reinterpret_cast<V4*>( dst ) = *reinterpret_cast<const V4*>( src );
+   else
+#     pragma unroll
+      for (int k = 0; k < 4; k++) dst[k] = src[k];
 }
 
 int blocks_for( long total )
@@ -387,7 +390,9 @@ template <typename IO>
 int launch_close_patches( const IO *pg_out, IO *patch, int npg, const int *pid0, hipStream_t stream )
 {
    if ( npg <= 0 ) return 0;
-   hipLaunchKernelGGL( close_patches_kernel<IO>, dim3( blocks_for( (long)npg*NV*PS*PS*PS ) ), dim3( 256 ), 0, stream, pg_out, patch, npg, pid0 );
+   const size_t al = 4*sizeof( IO );
+   const bool vec = ( reinterpret_cast<size_t>( pg_out ) % al == 0 ) && ( reinterpret_cast<size_t>( patch ) % al == 0 );
+   hipLaunchKernelGGL( close_patches_kernel<IO>, dim3( npg, NV*4 ), dim3( 256 ), 0, stream, pg_out, patch, pid0, vec );
    return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
 }
 
