@@ -451,13 +451,17 @@ prepare_patches_kernel( const IO *__restrict__ patch, const IO *__restrict__ fbo
                         const int *__restrict__ slot, int ghost, BoxGeom fb )
 {
    const int p = blockIdx.x, v = blockIdx.y, n = PS + 2*ghost, n2 = n*n, n3 = n2*n;
+   // the patch group's 4^3 slot table in LDS: each cell's patch lookup is then an LDS read, not a second
+   // dependent global load in front of the data load
+   __shared__ int sl[64];
+   if ( threadIdx.x < 64 ) sl[ threadIdx.x ] = slot[ (size_t)p*64 + threadIdx.x ];
+   __syncthreads();
    const int e0 = ( blockIdx.z*NT + threadIdx.x )*EPT;
    if ( e0 >= n3 ) return;
    int z = (int)( ( (float)e0 + 0.5f )*( 1.0f/(float)n2 ) );
    const int r = e0 - z*n2;
    int y = (int)( ( (float)r + 0.5f )*( 1.0f/(float)n ) );
    int x = r - y*n;
-   const int *sl = slot + (size_t)p*64;
    const IO  *pv = patch + (size_t)v*P3;
    const IO  *fv = fbox ? fbox + ( (size_t)p*NV + v )*fb.size[2]*fb.size[1]*fb.size[0] : nullptr;
    IO   val[EPT];
