@@ -193,27 +193,24 @@ correct_unphysical_kernel( const R *__restrict__ g_in, R *__restrict__ g_out, R 
 }
 
 template <typename IO>
-__global__ void store_flux_kernel( const IO *__restrict__ flux, const int *__restrict__ slot, int npg, double dt,
-                                   double *__restrict__ reg )
+__global__ void __launch_bounds__( H*H )
+store_flux_kernel( const IO *__restrict__ flux, const int *__restrict__ slot, double dt, double *__restrict__ reg )
 {
-   #pragma clang fp contract(off)      // separate multiply and add, as the reference accumulates
-   const long total = (long)npg*8*6*NV*H*H;
-   for (long e = blockIdx.x*(long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x*blockDim.x)
-   {
-      const int n = (int)( e % H ), m = (int)( ( e/H ) % H ), v = (int)( ( e/( H*H ) ) % NV );
-      const long q = e/( H*H*NV );                          // (pg*8 + LocalID)*6 + s
-      const int s = (int)( q % 6 ), lid = (int)( ( q/6 ) % 8 );
-      const long p = q/48;
-      const int r = slot[q];
-      if ( r < 0 ) continue;
-      int o[3];
-      local_xyz( lid, o );
-      int face, dmm, dnn;
-      if      ( s < 2 ) { face = o[0] + s;      dmm = o[2]*H;  dnn = o[1]*H; }
-      else if ( s < 4 ) { face = o[1] + 1 + s;  dmm = o[2]*H;  dnn = o[0]*H; }
-      else              { face = o[2] + 2 + s;  dmm = o[1]*H;  dnn = o[0]*H; }
+   // block (PG p, LocalID*6 + s): one patch face, its 8 x 8 entries on the threads; most faces have no register,
+   // and those blocks leave after one uniform slot read
+   const long p = blockIdx.x;
+   const int ls = blockIdx.y, s = ls % 6, lid = ls / 6, m = threadIdx.x / H, n = threadIdx.x % H;
+   const int r = slot[ p*48 + ls ];
+   if ( r < 0 ) return;
+   int o[3];
+   local_xyz( lid, o );
+   int face, dmm, dnn;
+   if      ( s < 2 ) { face = o[0] + s;      dmm = o[2]*H;  dnn = o[1]*H; }
+   else if ( s < 4 ) { face = o[1] + 1 + s;  dmm = o[2]*H;  dnn = o[0]*H; }
+   else              { face = o[2] + 2 + s;  dmm = o[1]*H;  dnn = o[0]*H; }
+#  pragma unroll
+   for (int v = 0; v < NV; v++)
       reg[ ( ( (long)r*NV + v )*H + m )*H + n ] = dt*(double)flux[ ( ( p*9 + face )*NV + v )*PS*PS + ( m + dmm )*PS + n + dnn ];
-   }
 }
 
 template <typename IO>
@@ -355,7 +352,7 @@ template <typename IO>
 int launch_store_flux( const IO *flux, const int *slot, int npg, double dt, double *reg, hipStream_t stream )
 {
    if ( npg <= 0 ) return 0;
-   hipLaunchKernelGGL( store_flux_kernel<IO>, dim3( blocks_for( (long)npg*8*6*NV*H*H ) ), dim3( 256 ), 0, stream, flux, slot, npg, dt, reg );
+   hipLaunchKernelGGL( store_flux_kernel<IO>, dim3( npg, 48 ), dim3( H*H ), 0, stream, flux, slot, dt, reg );
    return ( hipGetLastError() == hipSuccess ) ? 0 : -2;
 }
 
